@@ -1,0 +1,300 @@
+"""ctypes binding of the C ABI declared in include/hytgraph_amd.h.
+
+Only plain pointers and sizes cross the boundary. Host arrays are numpy
+arrays; device outputs may be any object exposing a device address
+(e.g. ``torch.Tensor.data_ptr()``) together with ``out_on_device=1``.
+The library is loaded from hytgraph_amd/lib; a missing library is an error
+(there is no fallback path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libhytgraph_amd.so"
+
+HYT_OK = 0
+HYT_ERR_INVALID_ARGUMENT = 1
+HYT_ERR_NO_DEVICE = 2
+HYT_ERR_HIP = 3
+HYT_ERR_OUT_OF_MEMORY = 4
+HYT_ERR_OVERFLOW = 5
+HYT_ERR_UNSUPPORTED = 6
+HYT_ERR_INTERNAL = 7
+
+HYT_GEN_RMAT = 0
+HYT_GEN_ER = 1
+HYT_HOST_PAGEABLE = 0
+HYT_HOST_PINNED = 1
+HYT_GRAPH_WEIGHTS_U32 = 0
+HYT_GRAPH_WEIGHTS_F32 = 1
+HYT_SSSP_SYNC = 0
+HYT_SSSP_ASYNC = 1
+
+U32_INF = 0xFFFFFFFF
+F32_INF_BITS = 0x7F800000
+
+
+class GenParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("scale", C.c_uint32),
+        ("edge_factor", C.c_uint32),
+        ("a", C.c_double),
+        ("b", C.c_double),
+        ("c", C.c_double),
+        ("seed", C.c_uint64),
+        ("weight_min", C.c_uint32),
+        ("weight_max", C.c_uint32),
+        ("threads", C.c_int32),
+        ("memory", C.c_int32),
+    ]
+
+
+class RunOptions(C.Structure):
+    _fields_ = [
+        ("mode", C.c_uint32),
+        ("out_on_device", C.c_uint32),
+        ("max_rounds", C.c_uint64),
+        ("time_kernels", C.c_uint32),
+        ("light_max_degree", C.c_uint32),
+        ("heavy_chunk", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class RunStats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_uint64),
+        ("converged", C.c_int32),
+        ("overflow", C.c_int32),
+        ("frontier_total", C.c_uint64),
+        ("edges_total", C.c_uint64),
+        ("updates_total", C.c_uint64),
+        ("expand_ms", C.c_double),
+        ("expand_launches", C.c_uint64),
+        ("wall_ms", C.c_double),
+        ("residual", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class HytError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[status {status}] {message}")
+        self.status = status
+
+
+class InvalidArgument(HytError, ValueError):
+    pass
+
+
+# Every exported symbol and its signature: (restype, argtypes).
+_P = C.c_void_p
+_u64p = C.POINTER(C.c_uint64)
+_u32p = C.POINTER(C.c_uint32)
+SIGNATURES = {
+    "hyt_last_error": (C.c_char_p, []),
+    "hyt_version": (C.c_char_p, []),
+    "hyt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "hyt_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
+    "hyt_host_graph_view": (
+        C.c_int,
+        [_P, _u64p, _u64p, C.POINTER(_u64p), C.POINTER(_u32p), C.POINTER(_u32p)],
+    ),
+    "hyt_host_graph_free": (C.c_int, [_P]),
+    "hyt_sample_sources": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
+    "hyt_graph_upload": (
+        C.c_int,
+        [C.c_uint64, C.c_uint64, _P, _P, _P, C.c_uint32, C.POINTER(_P)],
+    ),
+    "hyt_graph_free": (C.c_int, [_P]),
+    "hyt_graph_info": (C.c_int, [_P, _u64p, _u64p, _u64p]),
+    "hyt_sssp": (
+        C.c_int,
+        [_P, _P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
+    ),
+    "hyt_pagerank": (
+        C.c_int,
+        [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
+    ),
+    "hyt_ppr": (
+        C.c_int,
+        [_P, _P, C.c_uint32, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
+    ),
+}
+
+_lib = None
+
+
+def load(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Loads the built library (once). Raises FileNotFoundError when absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(
+            f"{p} is missing: build it with `python -m hytgraph_amd.build` (no fallback exists)"
+        )
+    lib = C.CDLL(str(p))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status == HYT_OK:
+        return
+    msg = load().hyt_last_error().decode(errors="replace")
+    if status == HYT_ERR_INVALID_ARGUMENT:
+        raise InvalidArgument(status, msg)
+    raise HytError(status, msg)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    check(load().hyt_device_count(C.byref(n)))
+    return n.value
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return C.c_void_p(a.ctypes.data)
+
+
+class HostGraph:
+    """Generator output owned by the library (views stay valid while alive)."""
+
+    def __init__(self, kind=HYT_GEN_RMAT, scale=16, edge_factor=16, a=0.57, b=0.19, c=0.19,
+                 seed=1, weight_min=1, weight_max=64, threads=0, memory=HYT_HOST_PAGEABLE):
+        p = GenParams(kind, scale, edge_factor, a, b, c, seed, weight_min, weight_max, threads, memory)
+        h = C.c_void_p()
+        check(load().hyt_generate(C.byref(p), C.byref(h)))
+        self._h = h
+        V, E = C.c_uint64(), C.c_uint64()
+        off, col, w = _u64p(), _u32p(), _u32p()
+        check(load().hyt_host_graph_view(h, C.byref(V), C.byref(E), C.byref(off), C.byref(col), C.byref(w)))
+        self.V, self.E = V.value, E.value
+        self.offsets = np.ctypeslib.as_array(off, shape=(self.V + 1,))
+        self.columns = np.ctypeslib.as_array(col, shape=(max(self.E, 1),))[: self.E]
+        self.weights = (np.ctypeslib.as_array(w, shape=(max(self.E, 1),))[: self.E]
+                        if bool(w) else None)
+
+    def free(self):
+        if self._h:
+            check(load().hyt_host_graph_free(self._h))
+            self._h = None
+            self.offsets = self.columns = self.weights = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def sample_sources(offsets: np.ndarray, seed: int, count: int) -> np.ndarray:
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+    out = np.zeros(count, dtype=np.uint32)
+    check(load().hyt_sample_sources(_ptr(offsets), len(offsets) - 1, seed, count, _ptr(out)))
+    return out
+
+
+def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
+             light_max_degree=0, heavy_chunk=0) -> RunOptions:
+    return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk, 0)
+
+
+class DeviceGraph:
+    """A CSR graph resident in HBM (hyt_graph_t)."""
+
+    def __init__(self, offsets, columns, weights=None, weights_f32=False):
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        columns = np.ascontiguousarray(columns, dtype=np.uint32)
+        if weights is not None:
+            if weights_f32:
+                weights = np.ascontiguousarray(weights, dtype=np.float32).view(np.uint32)
+            else:
+                weights = np.ascontiguousarray(weights, dtype=np.uint32)
+        self.V = len(offsets) - 1
+        self.E = int(offsets[-1]) if self.V >= 0 and len(offsets) else 0
+        self.weights_f32 = weights_f32
+        h = C.c_void_p()
+        flags = HYT_GRAPH_WEIGHTS_F32 if weights_f32 else HYT_GRAPH_WEIGHTS_U32
+        check(load().hyt_graph_upload(self.V, len(columns), _ptr(offsets), _ptr(columns),
+                                      _ptr(weights), flags, C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def from_host(cls, hg: HostGraph, weights_f32=False):
+        w = hg.weights
+        if w is not None and weights_f32:
+            w = w.astype(np.float32)
+        return cls(hg.offsets, hg.columns, w, weights_f32=weights_f32)
+
+    def info(self) -> dict:
+        V, E, B = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(load().hyt_graph_info(self._h, C.byref(V), C.byref(E), C.byref(B)))
+        return {"V": V.value, "E": E.value, "device_bytes": B.value}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(load().hyt_graph_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    # -- SSSP ---------------------------------------------------------------
+    def sssp(self, sources, mode=HYT_SSSP_SYNC, out=None, out_ptr=None, **opt):
+        """Returns (dist u32[V] or [V][K], stats dict). With out_ptr (device
+        address) the result stays on the device and None is returned."""
+        src = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+        K = len(src)
+        o = _options(mode=mode, out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = RunStats()
+        if out_ptr is not None:
+            check(load().hyt_sssp(self._h, _ptr(src), K, C.c_void_p(out_ptr), C.byref(o), C.byref(st)))
+            return None, st.as_dict()
+        if out is None:
+            out = np.empty((self.V, K) if K > 1 else self.V, dtype=np.uint32)
+        check(load().hyt_sssp(self._h, _ptr(src), K, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    # -- PageRank -------------------------------------------------------------
+    def pagerank(self, damping=0.85, epsilon=1e-7, out_ptr=None, **opt):
+        o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = RunStats()
+        if out_ptr is not None:
+            check(load().hyt_pagerank(self._h, damping, epsilon, C.c_void_p(out_ptr), C.byref(o), C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty(self.V, dtype=np.float32)
+        check(load().hyt_pagerank(self._h, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    def ppr(self, seeds, damping=0.85, epsilon=1e-6, out_ptr=None, **opt):
+        sd = np.ascontiguousarray(np.atleast_1d(seeds), dtype=np.uint32)
+        K = len(sd)
+        o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = RunStats()
+        if out_ptr is not None:
+            check(load().hyt_ppr(self._h, _ptr(sd), K, damping, epsilon, C.c_void_p(out_ptr),
+                                 C.byref(o), C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty((self.V, K) if K > 1 else self.V, dtype=np.float32)
+        check(load().hyt_ppr(self._h, _ptr(sd), K, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()

@@ -1,0 +1,96 @@
+// Device-resident CSR and the per-graph traversal workspace.
+//
+// HBM layout (structure of arrays, 256-byte aligned by hipMalloc):
+//   off  u32[V+1] when E < 2^32, else u64[V+1]
+//   col  u32[E]   destination ids, row-sorted
+//   w    u32[E]   integer weights or float bit patterns (absent if unweighted)
+// Workspace (single-source traversals):
+//   dist/res u32[V] (distance or residual bits), rank f32[V] (PageRank),
+//   q[2]  u32[V]  frontier queues (current / next), qd u32[V] snapshot values,
+//   mark  u32[V]  round stamps deduplicating next-frontier appends,
+//   heavy HeavyChunk[cap] chunks of high-degree rows, counters.
+#pragma once
+
+#include "hip_util.hpp"
+
+#include <cstdint>
+#include <memory>
+
+namespace hyt
+{
+    enum WeightMode : int
+    {
+        kWU32 = 0,     // integer weights, u32 sums (overflow detected)
+        kWF32 = 1,     // float weights, float sums (bit-order min on >= +0)
+        kWUnitU32 = 2, // unweighted, u32 sums of 1
+        kWUnitF32 = 3  // unweighted, float sums of 1.0f
+    };
+
+    // One chunk of a high-degree row, expanded by one wavefront.
+    struct HeavyChunk
+    {
+        std::uint32_t value;  // snapshot distance / residual share (bits)
+        std::uint32_t count;  // edges in this chunk
+        std::uint64_t begin;  // first edge index
+    };
+
+    struct Counters
+    {
+        std::uint32_t next_count;  // appends to the next frontier
+        std::uint32_t heavy_count; // heavy chunks queued this round
+        std::uint32_t overflow;    // sticky: a u32 sum overflowed
+        std::uint32_t heavy_overflow; // sticky: heavy chunk queue overflowed
+        unsigned long long edges;  // edges expanded this round
+    };
+
+    struct Workspace
+    {
+        DeviceBuffer dist, rank, q0, q1, qd, mark, heavy, ctr, partial;
+        std::uint64_t heavy_cap = 0;
+        std::uint32_t epoch = 0; // last round stamp written into mark
+        bool ready = false;
+    };
+
+    struct DeviceGraph
+    {
+        std::uint64_t V = 0, E = 0;
+        bool off64 = false;
+        bool weighted = false;
+        bool weights_f32 = false;
+        int device = 0;
+        DeviceBuffer off, col, w;
+        hipStream_t stream = nullptr;
+        Workspace ws;
+        std::unique_ptr<PinnedBuffer> host_ctr;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+        ~DeviceGraph();
+        std::uint64_t device_bytes() const
+        {
+            return off.bytes + col.bytes + w.bytes + ws.dist.bytes + ws.rank.bytes + ws.q0.bytes +
+                   ws.q1.bytes + ws.qd.bytes + ws.mark.bytes + ws.heavy.bytes + ws.ctr.bytes +
+                   ws.partial.bytes;
+        }
+        int weight_mode() const
+        {
+            if (!weighted)
+                return weights_f32 ? kWUnitF32 : kWUnitU32;
+            return weights_f32 ? kWF32 : kWU32;
+        }
+    };
+
+    // Default work-split thresholds (degree binning).
+    constexpr std::uint32_t kDefaultLightMaxDegree = 256;
+    constexpr std::uint32_t kDefaultHeavyChunk = 1024;
+
+    // Allocates the single-source workspace if needed (sized for the graph).
+    void ensure_workspace(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
+
+    // Next epoch stamp for mark[] (clears mark on wrap-around).
+    std::uint32_t next_epoch(DeviceGraph &g);
+} // namespace hyt
+
+struct hyt_graph
+{
+    hyt::DeviceGraph g;
+};

@@ -1,0 +1,171 @@
+// Frontier expansion shared by SSSP and delta-push PageRank.
+//
+// Degree binning for 64-lane wavefronts:
+//   * light rows (out-degree <= light_max): a wavefront takes 64 consecutive
+//     frontier entries, prefix-sums their degrees across lanes, and walks the
+//     union of their edge lists 64 edges at a time, each lane finding its row
+//     by binary search over the 64 prefix values in LDS. Consecutive lanes
+//     read consecutive column/weight entries of a row (coalesced streaming),
+//     and every lane has work regardless of the degree mix.
+//   * heavy rows (out-degree > light_max): split into `chunk`-edge pieces
+//     appended to a chunk queue; a second kernel gives each piece to one
+//     wavefront, so a hub's edges spread over the whole GPU.
+// Next-frontier appends are wavefront-aggregated (ballot + one atomic).
+//
+// Op supplies the per-algorithm parts:
+//   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
+//   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
+//   bool apply(uint32_t v, uint32_t c)            update; true = append v
+// load returns false to drop the edge (e.g. u32 overflow, flagged by Op).
+#pragma once
+
+#include "kern_common.hpp"
+
+namespace hyt
+{
+    struct ExpandArgs
+    {
+        const std::uint32_t *q;   // current frontier
+        std::uint32_t F;          // its length
+        std::uint32_t *qn;        // next frontier
+        Counters *ctr;
+        HeavyChunk *heavy;
+        std::uint64_t heavy_cap;
+        std::uint32_t light_max;
+        std::uint32_t chunk;
+    };
+
+    // Processes `kUnroll` x 64 edges per iteration: all loads first, then all
+    // updates, so each lane has kUnroll independent memory chains in flight.
+    template <class Op, class PosFn>
+    __device__ __forceinline__ void walk_edges(const Op &op, ExpandArgs &a, std::uint32_t total, PosFn pos_of)
+    {
+        const int lane = lane_id();
+        for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
+        {
+            std::uint32_t vv[kUnroll], cc[kUnroll];
+            bool ok[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k)
+            {
+                const std::uint32_t e = base + static_cast<std::uint32_t>(k * kWave + lane);
+                ok[k] = false;
+                vv[k] = 0;
+                cc[k] = 0;
+                if (e < total)
+                {
+                    std::uint64_t pos;
+                    std::uint32_t value;
+                    pos_of(e, pos, value);
+                    ok[k] = op.load(pos, value, vv[k], cc[k]);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k)
+            {
+                const bool want = ok[k] && op.apply(vv[k], cc[k]);
+                std::uint64_t mask;
+                const std::uint32_t slot = wave_append_slot(want, &a.ctr->next_count, mask);
+                if (want)
+                    a.qn[slot] = vv[k];
+            }
+        }
+    }
+
+    template <class OffT, class Op>
+    __global__ __launch_bounds__(kBlock) void expand_light(const OffT *__restrict__ off, Op op, ExpandArgs a)
+    {
+        __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
+        __shared__ OffT s_begin[kWavesPerBlock][kWave];
+        __shared__ std::uint32_t s_val[kWavesPerBlock][kWave];
+
+        const int lane = lane_id();
+        const int wid = static_cast<int>(threadIdx.x) / kWave;
+        const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+        const bool valid = i < a.F;
+
+        std::uint32_t deg = 0, value = 0, nch = 0;
+        OffT begin = 0, full = 0;
+        if (valid)
+        {
+            const std::uint32_t u = a.q[i];
+            begin = off[u];
+            full = off[u + 1] - begin;
+            value = op.row_value(i, u, static_cast<std::uint64_t>(full));
+            if (full > a.light_max)
+                nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
+            else
+                deg = static_cast<std::uint32_t>(full);
+        }
+
+        // Heavy rows: queue their chunks (one atomic per wavefront).
+        const std::uint32_t hincl = wave_incl_scan(nch);
+        const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
+        if (htotal)
+        {
+            std::uint32_t hbase = 0;
+            if (lane == 0)
+                hbase = atomicAdd(&a.ctr->heavy_count, htotal);
+            hbase = __shfl(hbase, 0, kWave);
+            if (static_cast<std::uint64_t>(hbase) + htotal > a.heavy_cap)
+            {
+                if (lane == 0)
+                    atomicOr(&a.ctr->heavy_overflow, 1U);
+            }
+            else if (nch)
+            {
+                std::uint32_t slot = hbase + hincl - nch;
+                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                for (std::uint32_t c = 0; c < nch; ++c)
+                {
+                    const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+                    const std::uint64_t n = (full64 - lo < a.chunk) ? (full64 - lo) : a.chunk;
+                    HeavyChunk h;
+                    h.value = value;
+                    h.count = static_cast<std::uint32_t>(n);
+                    h.begin = static_cast<std::uint64_t>(begin) + lo;
+                    a.heavy[slot + c] = h;
+                }
+            }
+        }
+
+        const std::uint32_t incl = wave_incl_scan(deg);
+        const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+        s_ex[wid][lane] = incl - deg;
+        s_begin[wid][lane] = begin;
+        s_val[wid][lane] = value;
+        __syncthreads();
+        if (total == 0)
+            return;
+        if (lane == 0)
+            atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
+
+        const std::uint32_t *ex = s_ex[wid];
+        const OffT *bg = s_begin[wid];
+        const std::uint32_t *vl = s_val[wid];
+        walk_edges(op, a, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
+            const int j = owner_of(ex, e);
+            pos = static_cast<std::uint64_t>(bg[j]) + (e - ex[j]);
+            val = vl[j];
+        });
+    }
+
+    template <class Op>
+    __global__ __launch_bounds__(kBlock) void expand_heavy(Op op, ExpandArgs a)
+    {
+        const int lane = lane_id();
+        const std::uint32_t nh = a.ctr->heavy_count;
+        const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
+        const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+        for (std::uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kWave; c < n; c += waves)
+        {
+            const HeavyChunk h = a.heavy[c];
+            if (lane == 0)
+                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+            walk_edges(op, a, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
+                pos = h.begin + e;
+                val = h.value;
+            });
+        }
+    }
+} // namespace hyt

@@ -1,0 +1,52 @@
+// Kernel building blocks shared by the traversal kernels: fills, the
+// wavefront load-balanced edge walk, and launch helpers.
+#pragma once
+
+#include "dgraph.hpp"
+
+namespace hyt
+{
+    constexpr int kBlock = 256;                 // 4 wavefronts per workgroup
+    constexpr int kWavesPerBlock = kBlock / kWave;
+    constexpr int kUnroll = 4;                  // edges in flight per lane
+    constexpr unsigned kHeavyGrid = 2048;       // workgroups of the chunk kernels
+
+    template <class T>
+    __global__ void fill_kernel(T *p, std::uint64_t n, T value)
+    {
+        std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+        const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+        for (; i < n; i += stride)
+            p[i] = value;
+    }
+
+    template <class T>
+    inline void launch_fill(T *p, std::uint64_t n, T value, hipStream_t s)
+    {
+        if (n == 0)
+            return;
+        std::uint64_t blocks = (n + kBlock - 1) / kBlock;
+        if (blocks > 65536)
+            blocks = 65536;
+        fill_kernel<T><<<static_cast<unsigned>(blocks), kBlock, 0, s>>>(p, n, value);
+        hip_check(hipGetLastError(), "launch fill");
+    }
+
+    inline unsigned blocks_for(std::uint64_t items)
+    {
+        std::uint64_t b = (items + kBlock - 1) / kBlock;
+        return static_cast<unsigned>(b == 0 ? 1 : b);
+    }
+
+    // Largest j in [0, 64) with ex[j] <= e, for a non-decreasing ex with
+    // ex[0] = 0 <= e (binary search in LDS).
+    __device__ __forceinline__ int owner_of(const std::uint32_t *ex, std::uint32_t e)
+    {
+        int lo = 0;
+#pragma unroll
+        for (int s = 32; s > 0; s >>= 1)
+            if (ex[lo + s] <= e)
+                lo += s;
+        return lo;
+    }
+} // namespace hyt

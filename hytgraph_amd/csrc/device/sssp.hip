@@ -1,0 +1,291 @@
+// Frontier-relaxation SSSP (single source) on gfx950.
+//
+// Semantics follow the original synchronous Bellman-Ford
+// (src/algorithms/sssp.cpp:109-234, cuda/sssp.cu:100-154): in round k every
+// vertex improved in round k-1 offers dist_{k-1}(u) + w(u,v) to each
+// out-neighbour v, v keeps the minimum, and v joins the next frontier when its
+// distance dropped. The original pulls over a transposed CSR and scans every
+// vertex each round; here the frontier pushes over the forward CSR, so a
+// round costs only the frontier's out-edges:
+//   * dist is u32 (integer weights, or float bit patterns: for values >= +0
+//     the IEEE order equals the unsigned order, so one atomicMin serves both);
+//   * SYNC mode snapshots dist(u) of the frontier before the round (the
+//     Jacobi values), which reproduces the original iteration count;
+//     ASYNC mode reads live distances;
+//   * a round-stamped mark[] deduplicates next-frontier appends with no
+//     per-round clearing pass.
+#include "expand.hpp"
+
+#include <chrono>
+#include <string>
+
+namespace hyt
+{
+    namespace
+    {
+        template <int WM>
+        struct Weights;
+
+        template <>
+        struct Weights<kWU32>
+        {
+            static constexpr std::uint32_t kInf = 0xFFFFFFFFU;
+            __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+            __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
+            {
+                const std::uint64_t s = static_cast<std::uint64_t>(d) + w;
+                out = static_cast<std::uint32_t>(s);
+                return s < 0xFFFFFFFFULL;
+            }
+        };
+        template <>
+        struct Weights<kWUnitU32> : Weights<kWU32>
+        {
+            __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 1U; }
+        };
+        template <>
+        struct Weights<kWF32>
+        {
+            static constexpr std::uint32_t kInf = 0x7F800000U;
+            __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+            __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
+            {
+                out = __float_as_uint(__fadd_rn(__uint_as_float(d), __uint_as_float(w)));
+                return true;
+            }
+        };
+        template <>
+        struct Weights<kWUnitF32> : Weights<kWF32>
+        {
+            __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 0x3F800000U; }
+        };
+
+        template <int WM, bool SYNC>
+        struct SsspOp
+        {
+            const std::uint32_t *__restrict__ col;
+            const std::uint32_t *__restrict__ w;
+            std::uint32_t *dist;
+            std::uint32_t *mark;
+            const std::uint32_t *qd; // snapshot values (SYNC)
+            std::uint32_t *overflow;
+            std::uint32_t epoch;
+
+            __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t) const
+            {
+                return SYNC ? qd[i] : dist[u];
+            }
+            __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
+            {
+                v = col[pos];
+                if (Weights<WM>::add(du, Weights<WM>::load(w, pos), c))
+                    return true;
+                atomicOr(overflow, 1U);
+                return false;
+            }
+            __device__ bool apply(std::uint32_t v, std::uint32_t c) const
+            {
+                // Plain loads may see an older (larger) value; both dist and
+                // mark only move one way within a round, so a stale read only
+                // costs an atomic and never skips a needed one.
+                if (c >= dist[v])
+                    return false;
+                if (c >= atomicMin(&dist[v], c))
+                    return false;
+                if (mark[v] == epoch)
+                    return false;
+                return atomicExch(&mark[v], epoch) != epoch;
+            }
+        };
+
+        __global__ void snapshot_kernel(const std::uint32_t *q, std::uint32_t F, const std::uint32_t *dist,
+                                        std::uint32_t *qd)
+        {
+            const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+            if (i < F)
+                qd[i] = dist[q[i]];
+        }
+
+        __global__ void seed_kernel(std::uint32_t *dist, std::uint32_t *q, std::uint32_t src, std::uint32_t zero)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+            {
+                dist[src] = zero;
+                q[0] = src;
+            }
+        }
+
+        struct Launch
+        {
+            DeviceGraph &g;
+            const hyt_run_options &o;
+            std::uint32_t light_max, chunk;
+        };
+
+        template <class OffT, int WM, bool SYNC>
+        void run_rounds(Launch &L, std::uint32_t src, hyt_run_stats &st)
+        {
+            DeviceGraph &g = L.g;
+            Workspace &ws = g.ws;
+            hipStream_t s = g.stream;
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            Counters *d_ctr = ws.ctr.as<Counters>();
+
+            launch_fill<std::uint32_t>(ws.dist.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
+            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), ws.q0.as<std::uint32_t>(), src, 0U);
+            hip_check(hipGetLastError(), "launch seed");
+            hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
+
+            std::uint32_t *qc = ws.q0.as<std::uint32_t>();
+            std::uint32_t *qn = ws.q1.as<std::uint32_t>();
+            std::uint32_t F = 1;
+            const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
+
+            SsspOp<WM, SYNC> op;
+            op.col = g.col.as<std::uint32_t>();
+            op.w = g.w.as<std::uint32_t>();
+            op.dist = ws.dist.as<std::uint32_t>();
+            op.mark = ws.mark.as<std::uint32_t>();
+            op.qd = ws.qd.as<std::uint32_t>();
+            op.overflow = &d_ctr->overflow;
+
+            ExpandArgs a;
+            a.ctr = d_ctr;
+            a.heavy = ws.heavy.as<HeavyChunk>();
+            a.heavy_cap = ws.heavy_cap;
+            a.light_max = L.light_max;
+            a.chunk = L.chunk;
+
+            st.converged = 0;
+            for (std::uint64_t round = 0; round < max_rounds; ++round)
+            {
+                op.epoch = next_epoch(g);
+                if (SYNC)
+                {
+                    snapshot_kernel<<<blocks_for(F), kBlock, 0, s>>>(qc, F, op.dist, ws.qd.as<std::uint32_t>());
+                    hip_check(hipGetLastError(), "launch snapshot");
+                }
+                a.q = qc;
+                a.F = F;
+                a.qn = qn;
+                if (L.o.time_kernels)
+                    hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
+                expand_light<OffT, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                hip_check(hipGetLastError(), "launch sssp_expand_light");
+                expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                hip_check(hipGetLastError(), "launch sssp_expand_heavy");
+                if (L.o.time_kernels)
+                    hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
+                hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s),
+                          "read counters");
+                hip_check(hipStreamSynchronize(s), "sssp round");
+                if (L.o.time_kernels)
+                {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, g.ev0, g.ev1), "hipEventElapsedTime");
+                    st.expand_ms += ms;
+                    st.expand_launches += 1;
+                }
+                const Counters c = *h_ctr;
+                if (c.heavy_overflow)
+                    fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
+                st.frontier_total += F;
+                st.edges_total += c.edges;
+                st.updates_total += c.next_count;
+                st.rounds = round + 1;
+                if (c.overflow)
+                {
+                    st.overflow = 1;
+                    fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
+                }
+                if (c.next_count == 0)
+                {
+                    st.converged = 1;
+                    break;
+                }
+                F = c.next_count;
+                std::swap(qc, qn);
+                hip_check(hipMemsetAsync(d_ctr, 0, offsetof(Counters, overflow), s), "reset counters");
+                hip_check(hipMemsetAsync(&d_ctr->edges, 0, sizeof(unsigned long long), s), "reset counters");
+            }
+        }
+
+        template <class OffT, int WM>
+        void dispatch_mode(Launch &L, std::uint32_t src, hyt_run_stats &st)
+        {
+            if (L.o.mode == HYT_SSSP_ASYNC)
+                run_rounds<OffT, WM, false>(L, src, st);
+            else
+                run_rounds<OffT, WM, true>(L, src, st);
+        }
+
+        template <class OffT>
+        void dispatch_weights(Launch &L, std::uint32_t src, hyt_run_stats &st)
+        {
+            switch (L.g.weight_mode())
+            {
+            case kWU32: dispatch_mode<OffT, kWU32>(L, src, st); break;
+            case kWF32: dispatch_mode<OffT, kWF32>(L, src, st); break;
+            case kWUnitU32: dispatch_mode<OffT, kWUnitU32>(L, src, st); break;
+            default: dispatch_mode<OffT, kWUnitF32>(L, src, st); break;
+            }
+        }
+    } // namespace
+
+    void sssp_single(DeviceGraph &g, std::uint32_t src, std::uint32_t *dist_out, const hyt_run_options &o,
+                     hyt_run_stats &st)
+    {
+        Launch L{g, o, o.light_max_degree ? o.light_max_degree : kDefaultLightMaxDegree,
+                 o.heavy_chunk ? o.heavy_chunk : kDefaultHeavyChunk};
+        require(L.light_max <= (1U << 20), "light_max_degree must be <= 2^20");
+        require(L.chunk >= 64 && L.chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
+        ensure_workspace(g, L.light_max, L.chunk);
+        if (g.off64)
+            dispatch_weights<std::uint64_t>(L, src, st);
+        else
+            dispatch_weights<std::uint32_t>(L, src, st);
+        hip_check(hipMemcpyAsync(dist_out, g.ws.dist.ptr, g.V * 4,
+                                 o.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g.stream),
+                  "copy distances");
+        hip_check(hipStreamSynchronize(g.stream), "sssp output");
+    }
+
+    void sssp_batched(DeviceGraph &g, const std::uint32_t *sources, std::uint32_t K, std::uint32_t *dist_out,
+                      const hyt_run_options &o, hyt_run_stats &st);
+} // namespace hyt
+
+extern "C" int hyt_sssp(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources, uint32_t *dist,
+                        const hyt_run_options *opts, hyt_run_stats *stats)
+{
+    return hyt::guarded("hyt_sssp", [&] {
+        using namespace hyt;
+        require(g != nullptr, "null graph");
+        require(sources != nullptr && dist != nullptr, "null sources or dist");
+        require(num_sources >= 1 && num_sources <= 64, "num_sources must be in [1, 64]");
+        for (std::uint32_t k = 0; k < num_sources; ++k)
+            require(sources[k] < g->g.V, "SSSP source vertex is out of range");
+        hyt_run_options o{};
+        if (opts)
+            o = *opts;
+        require(o.mode == HYT_SSSP_SYNC || o.mode == HYT_SSSP_ASYNC, "unknown SSSP mode");
+        hyt_run_stats st{};
+        hip_check(hipSetDevice(g->g.device), "hipSetDevice");
+        const auto t0 = std::chrono::steady_clock::now();
+        try
+        {
+            if (num_sources == 1)
+                sssp_single(g->g, sources[0], dist, o, st);
+            else
+                sssp_batched(g->g, sources, num_sources, dist, o, st);
+        }
+        catch (...)
+        {
+            if (stats)
+                *stats = st;
+            throw;
+        }
+        st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats)
+            *stats = st;
+    });
+}

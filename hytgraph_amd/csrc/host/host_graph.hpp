@@ -1,0 +1,36 @@
+// Host-resident CSR owned by the library (generator output, or the
+// host-side graph of the hybrid transfer engine). Arrays are either pageable
+// (std::malloc) or pinned and device-mapped (hipHostMalloc), the latter so the
+// transfer engine can DMA from them and zero-copy kernels can read them.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdint>
+#include <memory>
+
+namespace hyt
+{
+    struct HostGraph
+    {
+        std::uint64_t V = 0, E = 0;
+        bool weighted = false;
+        int memory = HYT_HOST_PAGEABLE;
+        std::uint64_t *off = nullptr;
+        std::uint32_t *col = nullptr;
+        std::uint32_t *w = nullptr;
+
+        HostGraph(std::uint64_t v, std::uint64_t e, bool weights, int mem);
+        ~HostGraph();
+        HostGraph(const HostGraph &) = delete;
+        HostGraph &operator=(const HostGraph &) = delete;
+    };
+
+    // Allocation helpers for host memory of the given kind.
+    void *host_alloc(std::size_t bytes, int memory);
+    void host_free(void *p, int memory) noexcept;
+
+    std::unique_ptr<HostGraph> generate(const hyt_gen_params &p);
+    void sample_sources(const std::uint64_t *off, std::uint64_t V, std::uint64_t seed,
+                        std::uint32_t count, std::uint32_t *out);
+} // namespace hyt

@@ -1,0 +1,178 @@
+/*
+ * hytgraph_amd.h - C ABI of the MI355X (gfx950) graph engine.
+ *
+ * Plain C types only: pointers, sizes and POD option/stat structs. Every
+ * entry point returns an int status (HYT_OK on success); on failure
+ * hyt_last_error() returns a thread-local message naming the operation.
+ *
+ * The C++ drop-in layer (include/algorithms/{sssp,pagerank}.hpp) is built on
+ * these calls and maps status codes to the exception types the original
+ * project throws (std::invalid_argument / std::runtime_error).
+ *
+ * Threading: thread-compatible. A hyt_graph_t owns a HIP stream and a
+ * workspace; concurrent calls on the same graph handle race, calls on
+ * different handles do not.
+ */
+#ifndef HYTGRAPH_AMD_H
+#define HYTGRAPH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum hyt_status {
+    HYT_OK = 0,
+    HYT_ERR_INVALID_ARGUMENT = 1, /* maps to std::invalid_argument */
+    HYT_ERR_NO_DEVICE = 2,        /* no HIP device visible */
+    HYT_ERR_HIP = 3,              /* a HIP runtime call failed */
+    HYT_ERR_OUT_OF_MEMORY = 4,    /* allocation failed or budget exceeded */
+    HYT_ERR_OVERFLOW = 5,         /* a u32 distance sum exceeded 2^32-2 */
+    HYT_ERR_UNSUPPORTED = 6,
+    HYT_ERR_INTERNAL = 7
+};
+
+/* Thread-local message of the last failing call on this thread. */
+const char *hyt_last_error(void);
+
+/* Library version string. */
+const char *hyt_version(void);
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int hyt_device_count(int *count);
+
+/* --------------------------------------------------- host graph generator */
+/*
+ * Synthetic graph generator (no counterpart in the original project, whose
+ * datasets/ directory is empty). Counter-based: edge i of a graph is a pure
+ * function of (seed, i), so the output is identical for any thread count.
+ * Self-loops and duplicate (src,dst) pairs are removed; rows are sorted by
+ * destination; weights are drawn per final CSR position.
+ */
+enum hyt_gen_kind { HYT_GEN_RMAT = 0, HYT_GEN_ER = 1 };
+enum hyt_host_memory { HYT_HOST_PAGEABLE = 0, HYT_HOST_PINNED = 1 };
+
+typedef struct hyt_gen_params {
+    int32_t kind;          /* hyt_gen_kind */
+    uint32_t scale;        /* V = 2^scale, 1..32 */
+    uint32_t edge_factor;  /* generated edges = edge_factor * V (before cleanup) */
+    double a, b, c;        /* RMAT quadrant probabilities (d = 1-a-b-c) */
+    uint64_t seed;
+    uint32_t weight_min;   /* weights uniform in [weight_min, weight_max]; */
+    uint32_t weight_max;   /* both 0 = unweighted graph */
+    int32_t threads;       /* 0 = min(hardware threads, 16) */
+    int32_t memory;        /* hyt_host_memory: pinned uses hipHostMalloc */
+} hyt_gen_params;
+
+typedef struct hyt_host_graph *hyt_host_graph_t;
+
+int hyt_generate(const hyt_gen_params *params, hyt_host_graph_t *out);
+
+/* Borrowed views of the host CSR: row_offsets u64[V+1], columns u32[E],
+ * weights u32[E] (NULL when unweighted). Valid until hyt_host_graph_free. */
+int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices,
+                        uint64_t *num_edges, const uint64_t **row_offsets,
+                        const uint32_t **columns, const uint32_t **weights);
+int hyt_host_graph_free(hyt_host_graph_t g);
+
+/* Degree-weighted source sampling: pick edge index uniformly with a
+ * counter-based stream keyed by seed and return its source vertex. Vertices
+ * with no out-edges are never picked. */
+int hyt_sample_sources(const uint64_t *row_offsets, uint64_t num_vertices,
+                       uint64_t seed, uint32_t count, uint32_t *out_sources);
+
+/* ------------------------------------------------- device-resident graph */
+enum hyt_graph_flags {
+    HYT_GRAPH_WEIGHTS_U32 = 0,   /* weights are unsigned integers */
+    HYT_GRAPH_WEIGHTS_F32 = 1    /* weights are IEEE float bit patterns (>= +0) */
+};
+
+typedef struct hyt_graph *hyt_graph_t;
+
+/*
+ * Upload a CSR graph into HBM on the current device. row_offsets is u64[V+1]
+ * (the original CSRGraph offset type); the device copy uses u32 offsets when
+ * E < 2^32 and u64 otherwise. weights may be NULL (unweighted: every edge
+ * weighs 1). The CSR invariants are validated on the host first
+ * (offsets[0]=0, monotone, offsets[V]=E, columns < V).
+ */
+int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges,
+                     const uint64_t *row_offsets, const uint32_t *columns,
+                     const uint32_t *weights, uint32_t flags,
+                     hyt_graph_t *out);
+int hyt_graph_free(hyt_graph_t g);
+int hyt_graph_info(hyt_graph_t g, uint64_t *num_vertices, uint64_t *num_edges,
+                   uint64_t *device_bytes);
+
+/* ------------------------------------------------------------------ SSSP */
+enum hyt_sssp_mode {
+    /* Synchronous rounds: a round relaxes the out-edges of the vertices
+     * improved in the previous round using the distances they held at the
+     * end of that round. Reproduces the original sssp_cpu/sssp_gpu
+     * iteration count exactly. */
+    HYT_SSSP_SYNC = 0,
+    /* Rounds read live distances (relaxations of this round are visible);
+     * same final distances, fewer rounds and relaxations. */
+    HYT_SSSP_ASYNC = 1
+};
+
+typedef struct hyt_run_options {
+    uint32_t mode;           /* hyt_sssp_mode (SSSP only) */
+    uint32_t out_on_device;  /* 1: output pointer is device memory */
+    uint64_t max_rounds;     /* 0: |V| (the original default) */
+    uint32_t time_kernels;   /* 1: time the expand kernels with HIP events */
+    uint32_t light_max_degree; /* 0: default; vertices above are chunked */
+    uint32_t heavy_chunk;      /* 0: default edges per wave chunk */
+    uint32_t reserved;
+} hyt_run_options;
+
+typedef struct hyt_run_stats {
+    uint64_t rounds;          /* iterations in the original's sense */
+    int32_t converged;        /* 1 when the last round changed nothing */
+    int32_t overflow;         /* 1 when a u32 sum overflowed (status OVERFLOW) */
+    uint64_t frontier_total;  /* sum over rounds of frontier rows (F) */
+    uint64_t edges_total;     /* sum over rounds of expanded edges (R) */
+    uint64_t updates_total;   /* sum over rounds of next-frontier appends (U) */
+    double expand_ms;         /* sum of expand kernel times (time_kernels=1) */
+    uint64_t expand_launches; /* number of timed expand launches */
+    double wall_ms;           /* host wall time of the call */
+    double residual;          /* PageRank: normalised L1 of leftover residual */
+} hyt_run_stats;
+
+/*
+ * Single- or multi-source SSSP. sources: K vertex ids (host memory).
+ * dist: u32[V][K] (row-major, K contiguous per vertex); UINT32_MAX marks an
+ * unreachable vertex. On an F32-weight graph the u32 values are float bit
+ * patterns (+inf = 0x7F800000), i.e. the float distances of the original.
+ * K = 1 runs the single-source kernels; 2 <= K <= 64 runs the [V][K] batched
+ * kernels, which use async semantics (rounds = rounds of the batch).
+ */
+int hyt_sssp(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources,
+             uint32_t *dist, const hyt_run_options *opts, hyt_run_stats *stats);
+
+/* ------------------------------------------------- PageRank / PPR (FP32) */
+/*
+ * Delta-push PageRank. Residuals start at (1-damping) per vertex; a round
+ * pushes every vertex whose residual exceeds epsilon. Dangling mass is
+ * dropped and the result is normalised to sum 1, which is the fixed point of
+ * the original's uniform-dangling Jacobi iteration. ranks: float[V].
+ */
+int hyt_pagerank(hyt_graph_t g, float damping, float epsilon, float *ranks,
+                 const hyt_run_options *opts, hyt_run_stats *stats);
+
+/*
+ * Personalized PageRank for K seeds, [V][K] FP32 state; residual (1-damping)
+ * at each seed; result columns normalised to sum 1. out: float[V][K].
+ */
+int hyt_ppr(hyt_graph_t g, const uint32_t *seeds, uint32_t num_seeds,
+            float damping, float epsilon, float *out,
+            const hyt_run_options *opts, hyt_run_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYTGRAPH_AMD_H */

@@ -1,0 +1,72 @@
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+if str(ROOT) not in sys.path:
+    sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+
+
+def _ensure_built():
+    """Tests load what is built; they build only when nothing is."""
+    from hytgraph_amd import abi
+
+    if not abi.LIB_PATH.exists():
+        from hytgraph_amd import build as b
+
+        b.build()
+    import oracle
+
+    if not oracle.ORC_LIB.exists():
+        oracle.build(with_ref=True)
+
+
+_ensure_built()
+
+
+@pytest.fixture(scope="session")
+def abi():
+    from hytgraph_amd import abi as a
+
+    a.load()
+    return a
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def gpu(abi):
+    """The HIP path is the product: on a GPU test a missing device is a failure."""
+    n = abi.device_count()
+    assert n >= 1, "no HIP device visible to a gpu-marked test"
+    return n
+
+
+def csr_from_edges(V, edges, weights=None):
+    """CSR (u64 offsets) from (src, dst) pairs, keeping the given edge order per row."""
+    edges = list(edges)
+    order = sorted(range(len(edges)), key=lambda i: edges[i][0])
+    off = np.zeros(V + 1, dtype=np.uint64)
+    for s, _ in edges:
+        off[s + 1] += 1
+    off = np.cumsum(off, dtype=np.uint64)
+    col = np.array([edges[i][1] for i in order], dtype=np.uint32)
+    w = None if weights is None else np.array([weights[i] for i in order])
+    return off, col, w
+
+
+@pytest.fixture
+def rng():
+    return np.random.default_rng(1234)

@@ -1,0 +1,184 @@
+"""GPU parity: frontier SSSP through the C ABI vs the oracle.
+
+Bar: distances bit-exact (u32 for integer weights; float bit patterns for
+float weights), and in SYNC mode the round count equals the original
+sssp_cpu iteration count.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import csr_from_edges
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _say(msg):
+    print(f"[gpu-sssp] {msg}", flush=True)
+
+
+def rmat(abi, scale, seed, wmin=1, wmax=64, ef=16, kind=0):
+    return abi.HostGraph(kind=kind, scale=scale, edge_factor=ef, seed=seed, weight_min=wmin, weight_max=wmax)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reference_cuda_test_graph(gpu, abi, orc, mode):
+    # cuda_algorithm_tests.cu:58-87 (same graph, float weights)
+    off, col, w = np.array([0, 2, 4, 5, 5], np.uint64), np.array([1, 2, 2, 3, 3], np.uint32), \
+        np.array([1, 4, 2, 6, 1], np.float32)
+    g = abi.DeviceGraph(off, col, w, weights_f32=True)
+    d, st = g.sssp(0, mode=mode)
+    ref, it, conv = orc.sssp_jacobi_f32(off, col, w, 0)
+    assert d.view(np.float32).tolist() == [0.0, 1.0, 3.0, 4.0]
+    assert d.tolist() == ref.view(np.uint32).tolist()
+    assert st["converged"] == 1
+    if mode == 0:
+        assert st["rounds"] == it
+
+
+def test_golden_sssp_cases(gpu, abi):
+    data = json.loads((GOLDEN / "reference_cases.json").read_text())
+    for case in data["sssp"]:
+        if case["tolerance"] != 0.0:
+            continue  # order-dependent tolerance rule: covered by the drop-in pull path
+        _say(case["name"])
+        w = None if case["weights"] is None else np.array(case["weights"], np.float32)
+        g = abi.DeviceGraph(np.array(case["offsets"], np.uint64), np.array(case["columns"], np.uint32), w,
+                            weights_f32=True)
+        d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"])
+        assert d.tolist() == case["distance_bits"], case["name"]
+        assert st["rounds"] == case["iterations"] and bool(st["converged"]) == case["converged"], case["name"]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_rmat16_config1_bitexact(gpu, abi, orc, seed):
+    """RMAT scale 16, ef 16, u32 weights in [1,64], source 0 (the CPU-runnable case)."""
+    hg = rmat(abi, 16, seed)
+    g = abi.DeviceGraph.from_host(hg)
+    dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0)
+    ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 0)
+    assert np.array_equal(orc.f32_to_u32_dist(ref), dj)
+    for mode in (0, 1):
+        _say(f"seed {seed} mode {mode}")
+        d, st = g.sssp(0, mode=mode)
+        assert np.array_equal(d, dj)
+        assert st["converged"] == 1
+        if mode == 0:
+            assert st["rounds"] == it
+    # float-weight graph of the same topology: float bit patterns equal the original's
+    gf = abi.DeviceGraph.from_host(hg, weights_f32=True)
+    d, st = gf.sssp(0, mode=0)
+    assert np.array_equal(d, ref.view(np.uint32)) and st["rounds"] == it
+
+
+def test_fractional_float_weights_bitexact(gpu, abi, orc, rng):
+    hg = rmat(abi, 12, 5)
+    w = (hg.weights.astype(np.float32) * rng.choice(np.array([0.1, 0.37, 1.75], np.float32),
+                                                    size=hg.E)).astype(np.float32)
+    g = abi.DeviceGraph(hg.offsets, hg.columns, w, weights_f32=True)
+    for src in [0, 3, 777]:
+        ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, w, src)
+        for mode in (0, 1):
+            d, st = g.sssp(src, mode=mode)
+            assert np.array_equal(d, ref.view(np.uint32)), (src, mode)
+            if mode == 0:
+                assert st["rounds"] == it
+
+
+@pytest.mark.parametrize("max_rounds", [1, 2, 3, 5])
+def test_truncated_rounds_match_original_state(gpu, abi, orc, max_rounds):
+    """SYNC mode is round-for-round Jacobi: the state after k rounds equals the
+    original's state after max_iterations = k."""
+    hg = rmat(abi, 12, 9)
+    g = abi.DeviceGraph.from_host(hg, weights_f32=True)
+    ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 1, 0.0, max_rounds)
+    d, st = g.sssp(1, mode=0, max_rounds=max_rounds)
+    assert np.array_equal(d, ref.view(np.uint32))
+    assert st["rounds"] == it and bool(st["converged"]) == conv
+
+
+def test_unweighted_and_er(gpu, abi, orc):
+    hg = rmat(abi, 13, 3, wmin=0, wmax=0)
+    g = abi.DeviceGraph.from_host(hg)
+    d, st = g.sssp(0)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, None, 0))
+    er = rmat(abi, 13, 4, kind=1, wmax=255)
+    g = abi.DeviceGraph.from_host(er)
+    for mode in (0, 1):
+        d, st = g.sssp(11, mode=mode)
+        assert np.array_equal(d, orc.sssp_dijkstra(er.offsets, er.columns, er.weights, 11))
+
+
+@pytest.mark.parametrize("light_max,chunk", [(0, 0), (4, 64), (64, 128), (1 << 20, 1 << 16)])
+def test_degree_binning_thresholds(gpu, abi, orc, light_max, chunk):
+    """Every work split (light rows only, tiny chunks, huge chunks) gives the
+    same distances; hub rows span many chunks."""
+    hg = rmat(abi, 14, 6, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0)
+    for mode in (0, 1):
+        d, st = g.sssp(0, mode=mode, light_max_degree=light_max, heavy_chunk=chunk)
+        assert np.array_equal(d, dj)
+
+
+def test_star_hub_and_edge_cases(gpu, abi, orc):
+    # A hub with 5000 out-edges (chunked), a chain behind it, isolated vertices.
+    V = 6000
+    edges = [(0, v) for v in range(1, 5001)] + [(5000 + i, 5001 + i) for i in range(998)]
+    weights = [((v * 7919) % 50) + 1 for v in range(1, 5001)] + [3] * 998
+    off, col, w = csr_from_edges(V, edges, weights)
+    g = abi.DeviceGraph(off, col, w.astype(np.uint32))
+    dj = orc.sssp_dijkstra(off, col, w.astype(np.uint32), 0)
+    ref, it, _ = orc.sssp_jacobi_f32(off, col, w.astype(np.float32), 0)
+    d, st = g.sssp(0, mode=0)
+    assert np.array_equal(d, dj) and st["rounds"] == it
+    # source with no out-edges: only itself reached, one quiet round
+    d, st = g.sssp(5999)
+    assert d[5999] == 0 and np.all(np.delete(d, 5999) == 0xFFFFFFFF) and st["rounds"] == 1
+    # single vertex, no edges
+    g1 = abi.DeviceGraph(np.array([0, 0], np.uint64), np.array([], np.uint32))
+    d, st = g1.sssp(0)
+    assert d.tolist() == [0] and st["converged"] == 1
+
+
+def test_invalid_arguments(gpu, abi):
+    g = abi.DeviceGraph(np.array([0, 1, 1], np.uint64), np.array([1], np.uint32), np.array([5], np.uint32))
+    with pytest.raises(abi.InvalidArgument):
+        g.sssp(2)
+    with pytest.raises(abi.InvalidArgument):
+        abi.DeviceGraph(np.array([0, 1, 1], np.uint64), np.array([1], np.uint32), np.array([-1.0], np.float32),
+                        weights_f32=True)
+
+
+def test_u32_overflow_is_reported(gpu, abi):
+    off, col, w = csr_from_edges(3, [(0, 1), (1, 2)], [0xFFFFFFF0, 0x20])
+    g = abi.DeviceGraph(off, col, np.asarray(w, np.uint32))
+    with pytest.raises(abi.HytError) as ei:
+        g.sssp(0)
+    assert ei.value.status == abi.HYT_ERR_OVERFLOW
+
+
+def test_scale20_invariants(gpu, abi, orc):
+    """Size-independent properties at a larger scale: d[src] = 0, no edge can
+    improve any distance, every reached vertex has a tight in-edge; plus exact
+    agreement with Dijkstra."""
+    hg = rmat(abi, 20, 11, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = int(abi.sample_sources(hg.offsets, 12, 1)[0])
+    d, st = g.sssp(src, mode=1)
+    off, col, w = hg.offsets, hg.columns, hg.weights
+    assert d[src] == 0
+    u = np.repeat(np.arange(hg.V, dtype=np.int64), np.diff(off.astype(np.int64)))
+    du = d[u].astype(np.uint64)
+    fin = du != 0xFFFFFFFF
+    cand = du[fin] + w[fin]
+    assert np.all(d[col[fin]].astype(np.uint64) <= cand)
+    tight = np.zeros(hg.V, bool)
+    tight[col[fin][d[col[fin]].astype(np.uint64) == cand]] = True
+    reached = d != 0xFFFFFFFF
+    reached[src] = False
+    assert np.all(tight[reached])
+    assert np.array_equal(d, orc.sssp_dijkstra(off, col, w, src))
