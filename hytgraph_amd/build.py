@@ -20,6 +20,9 @@ CSRC = PKG / "csrc"
 LIBDIR = PKG / "lib"
 OBJDIR = PKG / "_build"
 LIBNAME = "libhytgraph_amd.so"
+CPPLIB = "libhytgraph_cpp.so"
+CPPSRC = PKG / "cpp"
+TESTS_CPP = ROOT / "tests" / "cpp"
 
 ARCH = "gfx950"
 COMMON = [
@@ -75,6 +78,33 @@ def build(jobs: int = 8, debug: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, out)
+    build_cpp(debug)
+    return out
+
+
+def build_cpp(debug: bool = False) -> Path:
+    """The C++ API (include/graph, include/algorithms, ...) as libhytgraph_cpp.so,
+    host-only C++ built with g++ and linked against libhytgraph_amd.so, plus the
+    C++ test executables under tests/cpp."""
+    cxx = os.environ.get("CXX", "g++")
+    opt = ["-O0", "-g"] if debug else ["-O2"]
+    srcs = sorted(CPPSRC.rglob("*.cpp"))
+    out = LIBDIR / CPPLIB
+    tmp = LIBDIR / (CPPLIB + ".tmp")
+    cmd = [cxx, "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-pthread", *opt,
+           f"-I{ROOT / 'include'}", *map(str, srcs), f"-L{LIBDIR}", "-lhytgraph_amd",
+           "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C++ library build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, out)
+    for test_src in sorted(TESTS_CPP.glob("*.cpp")):
+        exe = LIBDIR / test_src.stem
+        cmd = [cxx, "-std=c++17", "-Wall", "-Wextra", *opt, f"-I{ROOT / 'include'}", str(test_src),
+               f"-L{LIBDIR}", "-lhytgraph_cpp", "-lhytgraph_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"C++ test build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return out
 
 

@@ -1,17 +1,674 @@
-// Multi-source [V][K] SSSP and personalized PageRank.
-#include "dgraph.hpp"
+// Multi-source [V][K] SSSP and personalized PageRank (2 <= K <= 64).
+//
+// State is row-major [V][K]: the K query values of a vertex are contiguous,
+// so one edge (u, v) is handled by a group of KP = next_pow2(K) lanes that
+// read and update K consecutive words of row u and row v - coalesced, and
+// the column/weight of the edge are read once for all K queries. A wavefront
+// holds G = 64 / KP such groups, i.e. walks G edges per step.
+//
+// Frontier: a queue of rows plus, per row, the u64 mask of queries for which
+// the row is active. During a round, the lanes that improve v OR their bits
+// into mark[v] (non-returning atomic, one per edge group); the compaction pass
+// scans mark[], moves every non-zero mask into the next queue and clears it.
+//
+// SSSP-K reads live distances (async rounds; final distances equal the
+// single-source results): a lane marks v when its candidate beats a plain load
+// of dist[v][k], which cannot miss an improvement (the first lowering atomic
+// of a round comes from a lane whose load passed). PPR-K needs the exact
+// eps-crossing, so it uses returning float atomics, and snapshots the active
+// residuals of the frontier before each round like the single-source form.
+#include "compact.hpp"
+#include "expand.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 namespace hyt
 {
-    void sssp_batched(DeviceGraph &, const std::uint32_t *, std::uint32_t, std::uint32_t *, const hyt_run_options &,
-                      hyt_run_stats &)
+    namespace
     {
-        fail(HYT_ERR_UNSUPPORTED, "batched SSSP (K > 1) is not available in this build");
+        struct HeavyChunkK
+        {
+            std::uint32_t row;   // SSSP: vertex id; PPR: queue index
+            std::uint32_t count; // edges in the chunk
+            std::uint64_t begin; // first edge
+            std::uint64_t mask;  // active lanes of the row
+            std::uint32_t scale; // PPR: damping / outdeg (float bits)
+            std::uint32_t pad;
+        };
+
+        struct KArgs
+        {
+            const std::uint32_t *q;
+            const unsigned long long *qmask;
+            std::uint32_t F;
+            Counters *ctr;
+            HeavyChunkK *heavy;
+            std::uint64_t heavy_cap;
+            std::uint32_t light_max;
+            std::uint32_t chunk;
+            std::uint32_t K;
+            unsigned long long *mark;
+        };
+
+        template <int KP>
+        __device__ __forceinline__ std::uint64_t group_bits(std::uint64_t ballot, int group)
+        {
+            if constexpr (KP == 64)
+                return ballot;
+            else
+                return (ballot >> (group * KP)) & ((1ULL << KP) - 1ULL);
+        }
+
+        // Walks `total` edges, G per step, kUnroll steps in flight.
+        template <int KP, class Op, class PosFn>
+        __device__ __forceinline__ void walk_edges_k(const Op &op, KArgs &a, std::uint32_t total, PosFn pos_of)
+        {
+            constexpr int G = kWave / KP;
+            const int lane = lane_id();
+            const int group = lane / KP;
+            const int kl = lane % KP;
+            for (std::uint32_t base = 0; base < total; base += G * kUnroll)
+            {
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll];
+                std::uint64_t m[kUnroll];
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const std::uint32_t e = base + static_cast<std::uint32_t>(k * G + group);
+                    m[k] = 0;
+                    vv[k] = ww[k] = row[k] = scale[k] = 0;
+                    if (e < total)
+                    {
+                        std::uint64_t pos;
+                        pos_of(e, pos, row[k], m[k], scale[k]);
+                        op.load(pos, vv[k], ww[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const bool act = static_cast<std::uint32_t>(kl) < a.K && ((m[k] >> kl) & 1ULL);
+                    const bool imp = act && op.update(row[k], scale[k], vv[k], ww[k], kl);
+                    const std::uint64_t gb = group_bits<KP>(__ballot(imp), group);
+                    if (kl == 0 && gb != 0)
+                        atomicOr(&a.mark[vv[k]], static_cast<unsigned long long>(gb));
+                }
+            }
+        }
+
+        template <class OffT, int KP, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_light(const OffT *__restrict__ off, Op op, KArgs a)
+        {
+            __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
+            __shared__ OffT s_begin[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
+            __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
+
+            const int lane = lane_id();
+            const int wid = static_cast<int>(threadIdx.x) / kWave;
+            const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+            std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
+            std::uint64_t mask = 0;
+            OffT begin = 0, full = 0;
+            if (i < a.F)
+            {
+                const std::uint32_t u = a.q[i];
+                begin = off[u];
+                full = off[u + 1] - begin;
+                mask = a.qmask[i];
+                row = Op::kRowIsQueueIndex ? i : u;
+                scale = op.row_scale(static_cast<std::uint64_t>(full));
+                if (full > a.light_max)
+                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
+                else
+                    deg = static_cast<std::uint32_t>(full);
+            }
+            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                HeavyChunkK h;
+                h.row = row;
+                h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+                h.begin = static_cast<std::uint64_t>(begin) + lo;
+                h.mask = mask;
+                h.scale = scale;
+                h.pad = 0;
+                return h;
+            });
+            const std::uint32_t incl = wave_incl_scan(deg);
+            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+            s_ex[wid][lane] = incl - deg;
+            s_begin[wid][lane] = begin;
+            s_row[wid][lane] = row;
+            s_scale[wid][lane] = scale;
+            s_mask[wid][lane] = mask;
+            __syncthreads();
+            if (total == 0)
+                return;
+            if (lane == 0)
+                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
+            const std::uint32_t *ex = s_ex[wid];
+            walk_edges_k<KP>(op, a, total,
+                             [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
+                                 std::uint32_t &sc) {
+                                 const int j = owner_of(ex, e);
+                                 pos = static_cast<std::uint64_t>(s_begin[wid][j]) + (e - ex[j]);
+                                 r = s_row[wid][j];
+                                 m = s_mask[wid][j];
+                                 sc = s_scale[wid][j];
+                             });
+        }
+
+        template <int KP, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_heavy(Op op, KArgs a)
+        {
+            const int lane = lane_id();
+            const std::uint32_t nh = a.ctr->heavy_count;
+            const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
+            const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+            for (std::uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kWave; c < n; c += waves)
+            {
+                const HeavyChunkK h = a.heavy[c];
+                if (lane == 0)
+                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                walk_edges_k<KP>(op, a, h.count,
+                                 [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
+                                     std::uint32_t &sc) {
+                                     pos = h.begin + e;
+                                     r = h.row;
+                                     m = h.mask;
+                                     sc = h.scale;
+                                 });
+            }
+        }
+
+        // ---------------------------------------------------------- SSSP-K
+        template <int WM>
+        struct SsspKOp
+        {
+            static constexpr bool kRowIsQueueIndex = false;
+            const std::uint32_t *__restrict__ col;
+            const std::uint32_t *__restrict__ w;
+            std::uint32_t *dist;
+            std::uint32_t K;
+            std::uint32_t *overflow;
+
+            __device__ std::uint32_t row_scale(std::uint64_t) const { return 0U; }
+            __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
+            {
+                v = col[pos];
+                if constexpr (WM == kWU32 || WM == kWF32)
+                    wt = w[pos];
+                else if constexpr (WM == kWUnitU32)
+                    wt = 1U;
+                else
+                    wt = 0x3F800000U;
+            }
+            __device__ bool update(std::uint32_t u, std::uint32_t, std::uint32_t v, std::uint32_t wt, int kl) const
+            {
+                const std::uint32_t du = dist[static_cast<std::uint64_t>(u) * K + kl];
+                std::uint32_t c;
+                if constexpr (WM == kWU32 || WM == kWUnitU32)
+                {
+                    const std::uint64_t s = static_cast<std::uint64_t>(du) + wt;
+                    if (s >= 0xFFFFFFFFULL)
+                    {
+                        atomicOr(overflow, 1U);
+                        return false;
+                    }
+                    c = static_cast<std::uint32_t>(s);
+                }
+                else
+                {
+                    c = __float_as_uint(__fadd_rn(__uint_as_float(du), __uint_as_float(wt)));
+                }
+                std::uint32_t *p = dist + static_cast<std::uint64_t>(v) * K + kl;
+                if (c >= *p)
+                    return false;
+                atomicMin(p, c);
+                return true;
+            }
+        };
+
+        // ----------------------------------------------------------- PPR-K
+        struct PprKOp
+        {
+            static constexpr bool kRowIsQueueIndex = true;
+            const std::uint32_t *__restrict__ col;
+            float *res;
+            const float *snap; // [F][K] snapshot residuals
+            std::uint32_t K;
+            float damping;
+            float eps;
+
+            __device__ std::uint32_t row_scale(std::uint64_t deg) const
+            {
+                return deg ? __float_as_uint(damping / static_cast<float>(deg)) : 0U;
+            }
+            __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
+            {
+                v = col[pos];
+                wt = 0;
+            }
+            __device__ bool update(std::uint32_t i, std::uint32_t scale, std::uint32_t v, std::uint32_t, int kl) const
+            {
+                const float add = __uint_as_float(scale) * snap[static_cast<std::uint64_t>(i) * K + kl];
+                if (!(add > 0.0F))
+                    return false;
+                const float old = unsafeAtomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl], add);
+                return old <= eps && old + add > eps;
+            }
+        };
+
+        // Moves the active residuals of the frontier rows into rank.
+        __global__ void pprk_snapshot(const std::uint32_t *q, std::uint32_t F, std::uint32_t K,
+                                      const unsigned long long *qmask, float *res, float *rank, float *snap)
+        {
+            const std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            if (t >= static_cast<std::uint64_t>(F) * K)
+                return;
+            const std::uint32_t i = static_cast<std::uint32_t>(t / K);
+            const std::uint32_t k = static_cast<std::uint32_t>(t % K);
+            const std::uint32_t u = q[i];
+            float r = 0.0F;
+            if ((qmask[i] >> k) & 1ULL)
+            {
+                const std::uint64_t idx = static_cast<std::uint64_t>(u) * K + k;
+                r = res[idx];
+                res[idx] = 0.0F;
+                rank[idx] += r;
+            }
+            snap[t] = r;
+        }
+
+        __global__ void seed_rows(std::uint32_t nrows, const std::uint32_t *rows, const unsigned long long *masks,
+                                  const std::uint32_t *lane_src, std::uint32_t K, unsigned long long *mark,
+                                  std::uint32_t *val, std::uint32_t seed_value)
+        {
+            const std::uint32_t t = threadIdx.x;
+            if (t < nrows)
+                mark[rows[t]] = masks[t];
+            if (t < K)
+                val[static_cast<std::uint64_t>(lane_src[t]) * K + t] = seed_value;
+        }
+
+        // Column sums of x = rank + res ([V][K]) into partial[block][K] (double).
+        __global__ __launch_bounds__(kBlock) void pprk_fold(float *rank, const float *res, std::uint64_t V,
+                                                           std::uint32_t K, std::uint32_t KP, double *partial)
+        {
+            __shared__ double s[kBlock];
+            const std::uint32_t rows_per = kBlock / KP;
+            const std::uint32_t k = threadIdx.x % KP;
+            const std::uint32_t r0 = threadIdx.x / KP;
+            double acc = 0.0;
+            if (k < K)
+            {
+                for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * rows_per + r0; v < V;
+                     v += static_cast<std::uint64_t>(gridDim.x) * rows_per)
+                {
+                    const std::uint64_t idx = v * K + k;
+                    const float x = rank[idx] + res[idx];
+                    rank[idx] = x;
+                    acc += x;
+                }
+            }
+            s[threadIdx.x] = acc;
+            __syncthreads();
+            if (threadIdx.x < KP && k < K)
+            {
+                double t = 0.0;
+                for (std::uint32_t r = 0; r < rows_per; ++r)
+                    t += s[r * KP + threadIdx.x];
+                partial[static_cast<std::uint64_t>(blockIdx.x) * 64 + threadIdx.x] = t;
+            }
+        }
+
+        __global__ void pprk_total(double *partial, unsigned nblocks, std::uint32_t K, double *total)
+        {
+            const std::uint32_t k = threadIdx.x;
+            if (k >= K)
+                return;
+            double t = 0.0;
+            for (unsigned b = 0; b < nblocks; ++b)
+                t += partial[static_cast<std::uint64_t>(b) * 64 + k];
+            total[k] = t;
+        }
+
+        __global__ void pprk_normalize(const float *x, std::uint64_t n, std::uint32_t K, const double *total,
+                                       float *out)
+        {
+            std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (; i < n; i += stride)
+            {
+                const double t = total[i % K];
+                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
+            }
+        }
+
+        // Workspace of the batched forms (grown to the largest K used).
+        struct KWorkspace
+        {
+            DeviceBuffer val, rank, snap, mark, qmask, heavy, seeds, partial;
+            std::uint64_t heavy_cap = 0;
+        };
+
+        KWorkspace &kws_of(DeviceGraph &g)
+        {
+            // One batched workspace per graph, kept in a side table keyed by the
+            // graph object (destroyed with the process; graphs are few).
+            static thread_local std::vector<std::pair<const DeviceGraph *, std::unique_ptr<KWorkspace>>> table;
+            for (auto &p : table)
+                if (p.first == &g)
+                    return *p.second;
+            table.emplace_back(&g, std::make_unique<KWorkspace>());
+            return *table.back().second;
+        }
+
+        int pow2_at_least(std::uint32_t K)
+        {
+            int kp = 1;
+            while (kp < static_cast<int>(K))
+                kp <<= 1;
+            return kp < 2 ? 2 : kp;
+        }
+
+        struct SeedPlan
+        {
+            std::vector<std::uint32_t> rows;
+            std::vector<unsigned long long> masks;
+        };
+
+        SeedPlan plan_seeds(const std::uint32_t *src, std::uint32_t K)
+        {
+            SeedPlan p;
+            for (std::uint32_t k = 0; k < K; ++k)
+            {
+                auto it = std::find(p.rows.begin(), p.rows.end(), src[k]);
+                if (it == p.rows.end())
+                {
+                    p.rows.push_back(src[k]);
+                    p.masks.push_back(1ULL << k);
+                }
+                else
+                    p.masks[static_cast<std::size_t>(it - p.rows.begin())] |= 1ULL << k;
+            }
+            return p;
+        }
+
+        void prepare(DeviceGraph &g, KWorkspace &kw, std::uint32_t K, bool ppr, std::uint32_t light_max,
+                     std::uint32_t chunk)
+        {
+            const std::uint64_t VK = g.V * K;
+            kw.val.ensure(VK * 4, "hipMalloc([V][K] state)");
+            if (ppr)
+            {
+                kw.rank.ensure(VK * 4, "hipMalloc([V][K] rank)");
+                kw.snap.ensure(VK * 4, "hipMalloc([V][K] snapshot)");
+            }
+            if (kw.mark.bytes < (g.V + kPad) * 8)
+            {
+                // mark is kept all-zero between calls (compaction clears what it reads)
+                kw.mark.alloc((g.V + kPad) * 8, "hipMalloc(lane marks)");
+                kw.qmask.alloc((g.V + kPad) * 8, "hipMalloc(queue masks)");
+                hip_check(hipMemsetAsync(kw.mark.ptr, 0, (g.V + kPad) * 8, g.stream), "clear marks");
+            }
+            const std::uint64_t cap = g.E / chunk + g.E / (static_cast<std::uint64_t>(light_max) + 1) + 64;
+            if (kw.heavy.bytes < cap * sizeof(HeavyChunkK))
+                kw.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(heavy chunks)");
+            kw.heavy_cap = kw.heavy.bytes / sizeof(HeavyChunkK);
+            kw.seeds.ensure(64 * 4 * 4, "hipMalloc(seeds)");
+            kw.partial.ensure((1024 + 1) * 64 * sizeof(double), "hipMalloc(partials)");
+            ensure_workspace(g, light_max, chunk);
+        }
+
+        void upload_seeds(DeviceGraph &g, KWorkspace &kw, const SeedPlan &p, const std::uint32_t *src,
+                          std::uint32_t K, std::uint32_t *val, std::uint32_t seed_value)
+        {
+            // seeds buffer: rows u32[64] | masks u64[64] | lane sources u32[64]
+            std::vector<unsigned char> blob(64 * 4 + 64 * 8 + 64 * 4, 0);
+            std::memcpy(blob.data(), p.rows.data(), p.rows.size() * 4);
+            std::memcpy(blob.data() + 256, p.masks.data(), p.masks.size() * 8);
+            std::memcpy(blob.data() + 768, src, K * 4);
+            unsigned char *d = kw.seeds.as<unsigned char>();
+            hip_check(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream), "upload seeds");
+            seed_rows<<<1, 64, 0, g.stream>>>(static_cast<std::uint32_t>(p.rows.size()),
+                                              reinterpret_cast<const std::uint32_t *>(d),
+                                              reinterpret_cast<const unsigned long long *>(d + 256),
+                                              reinterpret_cast<const std::uint32_t *>(d + 768), K,
+                                              kw.mark.as<unsigned long long>(), val, seed_value);
+            hip_check(hipGetLastError(), "launch seed_rows");
+            hip_check(hipStreamSynchronize(g.stream), "seed upload");
+        }
+
+        // Rows with a non-zero mark become the next queue; the mark moves into
+        // qmask and is cleared.
+        struct MarkSel
+        {
+            unsigned long long *mark;
+
+            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n,
+                                             unsigned long long (&vals)[4]) const
+            {
+                unsigned long long m4[4];
+                if (n == 4)
+                {
+                    const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(mark + row0);
+                    const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(mark + row0 + 2);
+                    m4[0] = a.x, m4[1] = a.y, m4[2] = b.x, m4[3] = b.y;
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        m4[j] = static_cast<std::uint32_t>(j) < n ? mark[row0 + j] : 0ULL;
+                }
+                std::uint32_t sel = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (m4[j])
+                    {
+                        mark[row0 + j] = 0ULL;
+                        vals[j] = m4[j];
+                        sel |= 1U << j;
+                    }
+                return sel;
+            }
+        };
+
+        Counters compact_marks(DeviceGraph &g, KWorkspace &kw)
+        {
+            Counters *d_ctr = g.ws.ctr.as<Counters>();
+            MarkSel sel{kw.mark.as<unsigned long long>()};
+            compact_rows<unsigned long long, MarkSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+                g.V, sel, g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>(), d_ctr);
+            hip_check(hipGetLastError(), "launch mark compaction");
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
+            hip_check(hipStreamSynchronize(g.stream), "batched round");
+            return *h_ctr;
+        }
+
+        template <class OffT, int KP, class Op>
+        void run_k(DeviceGraph &g, KWorkspace &kw, Op op, std::uint32_t K, const hyt_run_options &o,
+                   hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk, bool ppr)
+        {
+            hipStream_t s = g.stream;
+            Counters *d_ctr = g.ws.ctr.as<Counters>();
+            const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : (ppr ? 100000 : g.V);
+            KArgs a;
+            a.q = g.ws.q.as<std::uint32_t>();
+            a.qmask = kw.qmask.as<unsigned long long>();
+            a.ctr = d_ctr;
+            a.heavy = kw.heavy.as<HeavyChunkK>();
+            a.heavy_cap = kw.heavy_cap;
+            a.light_max = light_max;
+            a.chunk = chunk;
+            a.K = K;
+            a.mark = kw.mark.as<unsigned long long>();
+            hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
+            std::uint32_t F = compact_marks(g, kw).next_count;
+            st.converged = F == 0 ? 1 : 0;
+            for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
+            {
+                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                if (ppr)
+                {
+                    const std::uint64_t n = static_cast<std::uint64_t>(F) * K;
+                    pprk_snapshot<<<static_cast<unsigned>((n + kBlock - 1) / kBlock), kBlock, 0, s>>>(
+                        a.q, F, K, a.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
+                        kw.snap.as<float>());
+                    hip_check(hipGetLastError(), "launch pprk_snapshot");
+                }
+                a.F = F;
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
+                expandk_light<OffT, KP, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                hip_check(hipGetLastError(), "launch expandk_light");
+                expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                hip_check(hipGetLastError(), "launch expandk_heavy");
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
+                const Counters c = compact_marks(g, kw);
+                if (o.time_kernels)
+                {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, g.ev0, g.ev1), "hipEventElapsedTime");
+                    st.expand_ms += ms;
+                    st.expand_launches += 1;
+                }
+                if (c.heavy_overflow)
+                    fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
+                st.frontier_total += F;
+                st.edges_total += c.edges;
+                st.updates_total += c.next_count;
+                st.rounds = round + 1;
+                if (c.overflow)
+                {
+                    st.overflow = 1;
+                    fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
+                }
+                F = c.next_count;
+                if (F == 0)
+                    st.converged = 1;
+            }
+        }
+
+        template <class OffT, class Op>
+        void dispatch_kp(DeviceGraph &g, KWorkspace &kw, Op op, std::uint32_t K, const hyt_run_options &o,
+                         hyt_run_stats &st, std::uint32_t lm, std::uint32_t ch, bool ppr)
+        {
+            switch (pow2_at_least(K))
+            {
+            case 2: run_k<OffT, 2>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            case 4: run_k<OffT, 4>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            case 8: run_k<OffT, 8>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            case 16: run_k<OffT, 16>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            case 32: run_k<OffT, 32>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            default: run_k<OffT, 64>(g, kw, op, K, o, st, lm, ch, ppr); break;
+            }
+        }
+
+        template <class Op>
+        void dispatch_off(DeviceGraph &g, KWorkspace &kw, Op op, std::uint32_t K, const hyt_run_options &o,
+                          hyt_run_stats &st, std::uint32_t lm, std::uint32_t ch, bool ppr)
+        {
+            if (g.off64)
+                dispatch_kp<std::uint64_t>(g, kw, op, K, o, st, lm, ch, ppr);
+            else
+                dispatch_kp<std::uint32_t>(g, kw, op, K, o, st, lm, ch, ppr);
+        }
+
+        void split_params(const hyt_run_options &o, std::uint32_t &lm, std::uint32_t &ch)
+        {
+            lm = o.light_max_degree ? o.light_max_degree : kDefaultLightMaxDegree;
+            ch = o.heavy_chunk ? o.heavy_chunk : kDefaultHeavyChunk;
+            require(lm <= (1U << 20), "light_max_degree must be <= 2^20");
+            require(ch >= 64 && ch <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
+        }
+    } // namespace
+
+    void sssp_batched(DeviceGraph &g, const std::uint32_t *sources, std::uint32_t K, std::uint32_t *dist_out,
+                      const hyt_run_options &o, hyt_run_stats &st)
+    {
+        std::uint32_t lm, ch;
+        split_params(o, lm, ch);
+        KWorkspace &kw = kws_of(g);
+        prepare(g, kw, K, false, lm, ch);
+        const int wm = g.weight_mode();
+        const std::uint32_t inf = (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
+        launch_fill<std::uint32_t>(kw.val.as<std::uint32_t>(), g.V * K, inf, g.stream);
+        const SeedPlan p = plan_seeds(sources, K);
+        upload_seeds(g, kw, p, sources, K, kw.val.as<std::uint32_t>(), 0U);
+        auto run = [&](auto op) {
+            op.col = g.col.as<std::uint32_t>();
+            op.w = g.w.as<std::uint32_t>();
+            op.dist = kw.val.as<std::uint32_t>();
+            op.K = K;
+            op.overflow = &g.ws.ctr.as<Counters>()->overflow;
+            dispatch_off(g, kw, op, K, o, st, lm, ch, false);
+        };
+        switch (wm)
+        {
+        case kWU32: run(SsspKOp<kWU32>{}); break;
+        case kWF32: run(SsspKOp<kWF32>{}); break;
+        case kWUnitU32: run(SsspKOp<kWUnitU32>{}); break;
+        default: run(SsspKOp<kWUnitF32>{}); break;
+        }
+        hip_check(hipMemcpyAsync(dist_out, kw.val.ptr, g.V * K * 4,
+                                 o.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g.stream),
+                  "copy distances");
+        hip_check(hipStreamSynchronize(g.stream), "sssp output");
     }
 
-    void ppr_batched(DeviceGraph &, const std::uint32_t *, std::uint32_t, float, float, float *,
-                     const hyt_run_options &, hyt_run_stats &)
+    void ppr_batched(DeviceGraph &g, const std::uint32_t *seeds, std::uint32_t K, float damping, float eps,
+                     float *out, const hyt_run_options &o, hyt_run_stats &st)
     {
-        fail(HYT_ERR_UNSUPPORTED, "batched PPR (K > 1) is not available in this build");
+        require(damping >= 0.0F && damping < 1.0F, "PageRank damping_factor must be in [0, 1)");
+        require(eps > 0.0F && std::isfinite(eps), "PageRank epsilon must be finite and positive");
+        std::uint32_t lm, ch;
+        split_params(o, lm, ch);
+        KWorkspace &kw = kws_of(g);
+        prepare(g, kw, K, true, lm, ch);
+        const std::uint64_t VK = g.V * K;
+        launch_fill<float>(reinterpret_cast<float *>(kw.val.ptr), VK, 0.0F, g.stream);
+        launch_fill<float>(kw.rank.as<float>(), VK, 0.0F, g.stream);
+        const SeedPlan p = plan_seeds(seeds, K);
+        const float r0 = 1.0F - damping;
+        std::uint32_t r0_bits;
+        std::memcpy(&r0_bits, &r0, 4);
+        upload_seeds(g, kw, p, seeds, K, kw.val.as<std::uint32_t>(), r0_bits);
+        PprKOp op;
+        op.col = g.col.as<std::uint32_t>();
+        op.res = reinterpret_cast<float *>(kw.val.ptr);
+        op.snap = kw.snap.as<float>();
+        op.K = K;
+        op.damping = damping;
+        op.eps = eps;
+        dispatch_off(g, kw, op, K, o, st, lm, ch, true);
+
+        // finalize: x = rank + res, column sums, normalise
+        const std::uint32_t KP = static_cast<std::uint32_t>(pow2_at_least(K));
+        const std::uint64_t rows_per = kBlock / KP;
+        std::uint64_t nb64 = (g.V + rows_per - 1) / rows_per;
+        const unsigned nb = static_cast<unsigned>(nb64 < 1024 ? (nb64 ? nb64 : 1) : 1024);
+        double *partial = kw.partial.as<double>();
+        pprk_fold<<<nb, kBlock, 0, g.stream>>>(kw.rank.as<float>(), op.res, g.V, K, KP, partial);
+        hip_check(hipGetLastError(), "launch pprk_fold");
+        double *total = partial + static_cast<std::uint64_t>(1024) * 64;
+        pprk_total<<<1, 64, 0, g.stream>>>(partial, nb, K, total);
+        hip_check(hipGetLastError(), "launch pprk_total");
+        float *dst = o.out_on_device ? out : kw.snap.as<float>();
+        std::uint64_t b = (VK + kBlock - 1) / kBlock;
+        pprk_normalize<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, g.stream>>>(
+            kw.rank.as<float>(), VK, K, total, dst);
+        hip_check(hipGetLastError(), "launch pprk_normalize");
+        if (!o.out_on_device)
+            hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
+        hip_check(hipStreamSynchronize(g.stream), "ppr output");
     }
 } // namespace hyt

@@ -1,0 +1,86 @@
+// Next-frontier compaction: a dense scan over all rows that selects the rows
+// an expand pass touched, applies per-row side effects (snapshot values,
+// clear marks), and writes them as the next queue.
+//
+// Each thread scans 16 consecutive rows (4 groups of 4, vector loads); the
+// workgroup prefix-sums the per-thread counts with wavefront scans plus an LDS
+// pass, and reserves its output range with one atomic. Rows keep ascending
+// order within a workgroup, so consecutive queue entries are nearby vertices.
+//
+// Sel::select4(row0, n, vals[4]) -> bit mask of the selected rows among
+// [row0, row0+n), n <= 4; vals[j] is the value of row row0+j when selected.
+#pragma once
+
+#include "kern_common.hpp"
+
+namespace hyt
+{
+    constexpr int kCompactGroups = 4;                         // groups of 4 rows per thread
+    constexpr int kCompactRows = 4 * kCompactGroups;          // rows per thread
+    constexpr std::uint64_t kCompactRowsPerBlock = static_cast<std::uint64_t>(kBlock) * kCompactRows;
+
+    inline unsigned compact_blocks(std::uint64_t V)
+    {
+        std::uint64_t b = (V + kCompactRowsPerBlock - 1) / kCompactRowsPerBlock;
+        return static_cast<unsigned>(b == 0 ? 1 : b);
+    }
+
+    template <class Val, class Sel>
+    __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
+                                                           Counters *ctr)
+    {
+        __shared__ std::uint32_t s_wsum[kWavesPerBlock];
+        __shared__ std::uint32_t s_base;
+        const int lane = lane_id();
+        const int wid = static_cast<int>(threadIdx.x) / kWave;
+        const std::uint64_t row0 = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) * kCompactRows;
+
+        // Static indexing only (fully unrolled), so everything stays in VGPRs.
+        Val gv[kCompactGroups][4];
+        std::uint32_t gm[kCompactGroups];
+        std::uint32_t n = 0;
+#pragma unroll
+        for (int g = 0; g < kCompactGroups; ++g)
+        {
+            gm[g] = 0;
+            const std::uint64_t r = row0 + 4 * g;
+            if (r < V)
+            {
+                const std::uint32_t avail = static_cast<std::uint32_t>(V - r < 4 ? V - r : 4);
+                gm[g] = sel.select4(r, avail, gv[g]);
+            }
+            n += static_cast<std::uint32_t>(__popc(gm[g]));
+        }
+
+        const std::uint32_t incl = wave_incl_scan(n);
+        if (lane == kWave - 1)
+            s_wsum[wid] = incl;
+        __syncthreads();
+        std::uint32_t wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w)
+        {
+            if (w < wid)
+                wbase += s_wsum[w];
+            total += s_wsum[w];
+        }
+        if (threadIdx.x == 0)
+            s_base = total ? atomicAdd(&ctr->next_count, total) : 0U;
+        __syncthreads();
+        std::uint32_t pos = s_base + wbase + incl - n;
+#pragma unroll
+        for (int g = 0; g < kCompactGroups; ++g)
+        {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if ((gm[g] >> j) & 1U)
+                {
+                    const std::uint32_t at = pos + static_cast<std::uint32_t>(__popc(gm[g] & ((1U << j) - 1U)));
+                    q[at] = static_cast<std::uint32_t>(row0 + 4 * g + j);
+                    if (qval)
+                        qval[at] = gv[g][j];
+                }
+            pos += static_cast<std::uint32_t>(__popc(gm[g]));
+        }
+    }
+} // namespace hyt

@@ -4,10 +4,11 @@
 //   off  u32[V+1] when E < 2^32, else u64[V+1]
 //   col  u32[E]   destination ids, row-sorted
 //   w    u32[E]   integer weights or float bit patterns (absent if unweighted)
-// Workspace (single-source traversals):
+// Workspace (single-source traversals; every [V] array has 16 spare slots):
 //   dist/res u32[V] (distance or residual bits), rank f32[V] (PageRank),
-//   q[2]  u32[V]  frontier queues (current / next), qd u32[V] snapshot values,
-//   mark  u32[V]  round stamps deduplicating next-frontier appends,
+//   prev  u32[V]  distance at the end of the round a vertex last improved in,
+//   touch u8[V]   set (plain store) by expand when a vertex may have improved,
+//   q     u32[V]  frontier rows, qd u32[V] their snapshot values,
 //   heavy HeavyChunk[cap] chunks of high-degree rows, counters.
 #pragma once
 
@@ -34,20 +35,22 @@ namespace hyt
         std::uint64_t begin;  // first edge index
     };
 
+    // Per-round counters. The first 16 bytes are zeroed at the start of each
+    // round; the two flags are sticky for the whole call.
     struct Counters
     {
-        std::uint32_t next_count;  // appends to the next frontier
-        std::uint32_t heavy_count; // heavy chunks queued this round
-        std::uint32_t overflow;    // sticky: a u32 sum overflowed
-        std::uint32_t heavy_overflow; // sticky: heavy chunk queue overflowed
-        unsigned long long edges;  // edges expanded this round
+        std::uint32_t next_count;     // rows written by the compaction pass
+        std::uint32_t heavy_count;    // heavy chunks queued this round
+        unsigned long long edges;     // edges expanded this round
+        std::uint32_t overflow;       // a u32 sum overflowed
+        std::uint32_t heavy_overflow; // heavy chunk queue overflowed
     };
+    constexpr std::size_t kRoundCounterBytes = 16;
 
     struct Workspace
     {
-        DeviceBuffer dist, rank, q0, q1, qd, mark, heavy, ctr, partial;
+        DeviceBuffer dist, rank, prev, touch, q, qd, heavy, ctr, partial;
         std::uint64_t heavy_cap = 0;
-        std::uint32_t epoch = 0; // last round stamp written into mark
         bool ready = false;
     };
 
@@ -67,9 +70,8 @@ namespace hyt
         ~DeviceGraph();
         std::uint64_t device_bytes() const
         {
-            return off.bytes + col.bytes + w.bytes + ws.dist.bytes + ws.rank.bytes + ws.q0.bytes +
-                   ws.q1.bytes + ws.qd.bytes + ws.mark.bytes + ws.heavy.bytes + ws.ctr.bytes +
-                   ws.partial.bytes;
+            return off.bytes + col.bytes + w.bytes + ws.dist.bytes + ws.rank.bytes + ws.prev.bytes +
+                   ws.touch.bytes + ws.q.bytes + ws.qd.bytes + ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes;
         }
         int weight_mode() const
         {
@@ -86,8 +88,8 @@ namespace hyt
     // Allocates the single-source workspace if needed (sized for the graph).
     void ensure_workspace(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
 
-    // Next epoch stamp for mark[] (clears mark on wrap-around).
-    std::uint32_t next_epoch(DeviceGraph &g);
+    // Spare slots after every [V] workspace array (vector loads past the end).
+    constexpr std::uint64_t kPad = 16;
 } // namespace hyt
 
 struct hyt_graph

@@ -8,14 +8,16 @@
 //     read consecutive column/weight entries of a row (coalesced streaming),
 //     and every lane has work regardless of the degree mix.
 //   * heavy rows (out-degree > light_max): split into `chunk`-edge pieces
-//     appended to a chunk queue; a second kernel gives each piece to one
-//     wavefront, so a hub's edges spread over the whole GPU.
-// Next-frontier appends are wavefront-aggregated (ballot + one atomic).
+//     appended to a chunk queue (one atomic per wavefront); a second kernel
+//     gives each piece to one wavefront, so a hub's edges spread over the GPU.
+// The expand kernels append nothing to the next frontier: updates use
+// non-returning atomics and mark touched vertices with plain stores; a
+// separate compaction pass (compact.hpp) builds the next queue.
 //
 // Op supplies the per-algorithm parts:
 //   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
-//   bool apply(uint32_t v, uint32_t c)            update; true = append v
+//   void apply(uint32_t v, uint32_t c)                 update destination v
 // load returns false to drop the edge (e.g. u32 overflow, flagged by Op).
 #pragma once
 
@@ -25,9 +27,8 @@ namespace hyt
 {
     struct ExpandArgs
     {
-        const std::uint32_t *q;   // current frontier
-        std::uint32_t F;          // its length
-        std::uint32_t *qn;        // next frontier
+        const std::uint32_t *q;   // frontier rows
+        std::uint32_t F;          // their number
         Counters *ctr;
         HeavyChunk *heavy;
         std::uint64_t heavy_cap;
@@ -38,7 +39,7 @@ namespace hyt
     // Processes `kUnroll` x 64 edges per iteration: all loads first, then all
     // updates, so each lane has kUnroll independent memory chains in flight.
     template <class Op, class PosFn>
-    __device__ __forceinline__ void walk_edges(const Op &op, ExpandArgs &a, std::uint32_t total, PosFn pos_of)
+    __device__ __forceinline__ void walk_edges(const Op &op, std::uint32_t total, PosFn pos_of)
     {
         const int lane = lane_id();
         for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
@@ -62,14 +63,35 @@ namespace hyt
             }
 #pragma unroll
             for (int k = 0; k < kUnroll; ++k)
-            {
-                const bool want = ok[k] && op.apply(vv[k], cc[k]);
-                std::uint64_t mask;
-                const std::uint32_t slot = wave_append_slot(want, &a.ctr->next_count, mask);
-                if (want)
-                    a.qn[slot] = vv[k];
-            }
+                if (ok[k])
+                    op.apply(vv[k], cc[k]);
         }
+    }
+
+    // Queues the chunks of this wavefront's heavy rows (nch chunks for a lane's
+    // row, 0 for light rows); one atomic per wavefront.
+    template <class Chunk, class MakeFn>
+    __device__ __forceinline__ void queue_heavy(std::uint32_t nch, Counters *ctr, Chunk *heavy,
+                                                std::uint64_t heavy_cap, MakeFn make)
+    {
+        const int lane = lane_id();
+        const std::uint32_t hincl = wave_incl_scan(nch);
+        const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
+        if (htotal == 0)
+            return;
+        std::uint32_t hbase = 0;
+        if (lane == 0)
+            hbase = atomicAdd(&ctr->heavy_count, htotal);
+        hbase = __shfl(hbase, 0, kWave);
+        if (static_cast<std::uint64_t>(hbase) + htotal > heavy_cap)
+        {
+            if (lane == 0)
+                atomicOr(&ctr->heavy_overflow, 1U);
+            return;
+        }
+        const std::uint32_t slot = hbase + hincl - nch;
+        for (std::uint32_t c = 0; c < nch; ++c)
+            heavy[slot + c] = make(c);
     }
 
     template <class OffT, class Op>
@@ -82,11 +104,10 @@ namespace hyt
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-        const bool valid = i < a.F;
 
         std::uint32_t deg = 0, value = 0, nch = 0;
         OffT begin = 0, full = 0;
-        if (valid)
+        if (i < a.F)
         {
             const std::uint32_t u = a.q[i];
             begin = off[u];
@@ -97,37 +118,15 @@ namespace hyt
             else
                 deg = static_cast<std::uint32_t>(full);
         }
-
-        // Heavy rows: queue their chunks (one atomic per wavefront).
-        const std::uint32_t hincl = wave_incl_scan(nch);
-        const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
-        if (htotal)
-        {
-            std::uint32_t hbase = 0;
-            if (lane == 0)
-                hbase = atomicAdd(&a.ctr->heavy_count, htotal);
-            hbase = __shfl(hbase, 0, kWave);
-            if (static_cast<std::uint64_t>(hbase) + htotal > a.heavy_cap)
-            {
-                if (lane == 0)
-                    atomicOr(&a.ctr->heavy_overflow, 1U);
-            }
-            else if (nch)
-            {
-                std::uint32_t slot = hbase + hincl - nch;
-                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
-                for (std::uint32_t c = 0; c < nch; ++c)
-                {
-                    const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
-                    const std::uint64_t n = (full64 - lo < a.chunk) ? (full64 - lo) : a.chunk;
-                    HeavyChunk h;
-                    h.value = value;
-                    h.count = static_cast<std::uint32_t>(n);
-                    h.begin = static_cast<std::uint64_t>(begin) + lo;
-                    a.heavy[slot + c] = h;
-                }
-            }
-        }
+        queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
+            const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+            const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+            HeavyChunk h;
+            h.value = value;
+            h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+            h.begin = static_cast<std::uint64_t>(begin) + lo;
+            return h;
+        });
 
         const std::uint32_t incl = wave_incl_scan(deg);
         const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
@@ -143,7 +142,7 @@ namespace hyt
         const std::uint32_t *ex = s_ex[wid];
         const OffT *bg = s_begin[wid];
         const std::uint32_t *vl = s_val[wid];
-        walk_edges(op, a, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
+        walk_edges(op, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
             const int j = owner_of(ex, e);
             pos = static_cast<std::uint64_t>(bg[j]) + (e - ex[j]);
             val = vl[j];
@@ -162,7 +161,7 @@ namespace hyt
             const HeavyChunk h = a.heavy[c];
             if (lane == 0)
                 atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
-            walk_edges(op, a, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
+            walk_edges(op, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                 pos = h.begin + e;
                 val = h.value;
             });

@@ -40,32 +40,25 @@ namespace hyt
                     fail(HYT_ERR_INVALID_ARGUMENT, "column index out of range at edge " + std::to_string(e));
         }
 
-        __global__ void fill_u32(std::uint32_t *p, std::uint64_t n, std::uint32_t value)
-        {
-            std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-            std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
-            for (; i < n; i += stride)
-                p[i] = value;
-        }
     } // namespace
 
     void ensure_workspace(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk)
     {
         Workspace &ws = g.ws;
-        const std::uint64_t V = g.V;
+        const std::uint64_t Vp = g.V + kPad;
         const std::uint64_t cap = g.E / chunk + g.E / (static_cast<std::uint64_t>(light_max) + 1) + 64;
         if (!ws.ready)
         {
-            ws.dist.alloc(V * 4, "hipMalloc(dist)");
-            ws.rank.alloc(V * 4, "hipMalloc(rank)");
-            ws.q0.alloc(V * 4, "hipMalloc(queue0)");
-            ws.q1.alloc(V * 4, "hipMalloc(queue1)");
-            ws.qd.alloc(V * 4, "hipMalloc(queue values)");
-            ws.mark.alloc(V * 4, "hipMalloc(mark)");
+            ws.dist.alloc(Vp * 4, "hipMalloc(dist)");
+            ws.rank.alloc(Vp * 4, "hipMalloc(rank)");
+            ws.prev.alloc(Vp * 4, "hipMalloc(prev)");
+            ws.touch.alloc(Vp, "hipMalloc(touch)");
+            ws.q.alloc(Vp * 4, "hipMalloc(queue)");
+            ws.qd.alloc(Vp * 4, "hipMalloc(queue values)");
             ws.ctr.alloc(sizeof(Counters), "hipMalloc(counters)");
             ws.partial.alloc(4096 * sizeof(double), "hipMalloc(partials)");
-            hip_check(hipMemsetAsync(ws.mark.ptr, 0, V * 4, g.stream), "clear mark");
-            ws.epoch = 0;
+            // touch must start (and is kept) all-zero: compaction clears what it reads
+            hip_check(hipMemsetAsync(ws.touch.ptr, 0, Vp, g.stream), "clear touch");
             ws.ready = true;
         }
         if (ws.heavy.bytes < cap * sizeof(HeavyChunk))
@@ -73,16 +66,6 @@ namespace hyt
             ws.heavy.alloc(cap * sizeof(HeavyChunk), "hipMalloc(heavy chunks)");
         }
         ws.heavy_cap = ws.heavy.bytes / sizeof(HeavyChunk);
-    }
-
-    std::uint32_t next_epoch(DeviceGraph &g)
-    {
-        if (g.ws.epoch == 0xFFFFFFF0U)
-        {
-            hip_check(hipMemsetAsync(g.ws.mark.ptr, 0, g.V * 4, g.stream), "clear mark");
-            g.ws.epoch = 0;
-        }
-        return ++g.ws.epoch;
     }
 } // namespace hyt
 

@@ -5,15 +5,23 @@
 // vertex improved in round k-1 offers dist_{k-1}(u) + w(u,v) to each
 // out-neighbour v, v keeps the minimum, and v joins the next frontier when its
 // distance dropped. The original pulls over a transposed CSR and scans every
-// vertex each round; here the frontier pushes over the forward CSR, so a
-// round costs only the frontier's out-edges:
+// edge each round; here the frontier pushes over the forward CSR, so a round
+// reads only the frontier's out-edges:
 //   * dist is u32 (integer weights, or float bit patterns: for values >= +0
 //     the IEEE order equals the unsigned order, so one atomicMin serves both);
-//   * SYNC mode snapshots dist(u) of the frontier before the round (the
-//     Jacobi values), which reproduces the original iteration count;
-//     ASYNC mode reads live distances;
-//   * a round-stamped mark[] deduplicates next-frontier appends with no
-//     per-round clearing pass.
+//   * expand: a plain load of dist[v] filters candidates, survivors issue a
+//     non-returning atomicMin and a plain byte store touch[v] = 1 (no
+//     returning atomics, no shared counters on the edge path);
+//   * compaction: a dense scan of touch[] keeps v when dist[v] < prev[v]
+//     (prev = distance at the end of the round v last improved in), so the
+//     next frontier is exactly the improved set, and records dist[v] as the
+//     snapshot value of the next round;
+//   * SYNC mode expands with the snapshot values (Jacobi), which reproduces
+//     the original iteration count; ASYNC mode reads live distances.
+// Why touch[] cannot miss an improvement: the first atomicMin that lowers
+// dist[v] in a round came from a thread whose filter load passed, and that
+// thread stored touch[v].
+#include "compact.hpp"
 #include "expand.hpp"
 
 #include <chrono>
@@ -66,10 +74,9 @@ namespace hyt
             const std::uint32_t *__restrict__ col;
             const std::uint32_t *__restrict__ w;
             std::uint32_t *dist;
-            std::uint32_t *mark;
+            std::uint8_t *touch;
             const std::uint32_t *qd; // snapshot values (SYNC)
             std::uint32_t *overflow;
-            std::uint32_t epoch;
 
             __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t) const
             {
@@ -83,35 +90,58 @@ namespace hyt
                 atomicOr(overflow, 1U);
                 return false;
             }
-            __device__ bool apply(std::uint32_t v, std::uint32_t c) const
+            __device__ void apply(std::uint32_t v, std::uint32_t c) const
             {
-                // Plain loads may see an older (larger) value; both dist and
-                // mark only move one way within a round, so a stale read only
-                // costs an atomic and never skips a needed one.
-                if (c >= dist[v])
-                    return false;
-                if (c >= atomicMin(&dist[v], c))
-                    return false;
-                if (mark[v] == epoch)
-                    return false;
-                return atomicExch(&mark[v], epoch) != epoch;
+                // A plain load may return an older, larger value; that only
+                // costs an extra atomic, it never skips a needed one.
+                if (c < dist[v])
+                {
+                    atomicMin(&dist[v], c);
+                    touch[v] = 1;
+                }
             }
         };
 
-        __global__ void snapshot_kernel(const std::uint32_t *q, std::uint32_t F, const std::uint32_t *dist,
-                                        std::uint32_t *qd)
+        struct SsspSel
         {
-            const std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-            if (i < F)
-                qd[i] = dist[q[i]];
-        }
+            std::uint8_t *touch;
+            const std::uint32_t *dist;
+            std::uint32_t *prev;
 
-        __global__ void seed_kernel(std::uint32_t *dist, std::uint32_t *q, std::uint32_t src, std::uint32_t zero)
+            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4]) const
+            {
+                // touch has spare bytes past V, so the 4-byte word is in range
+                std::uint32_t *tw = reinterpret_cast<std::uint32_t *>(touch + row0);
+                const std::uint32_t t = *tw;
+                if (t == 0)
+                    return 0;
+                *tw = 0;
+                std::uint32_t m = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    if (static_cast<std::uint32_t>(j) < n && ((t >> (8 * j)) & 0xFFU))
+                    {
+                        const std::uint64_t v = row0 + j;
+                        const std::uint32_t d = dist[v];
+                        if (d < prev[v])
+                        {
+                            prev[v] = d;
+                            vals[j] = d;
+                            m |= 1U << j;
+                        }
+                    }
+                }
+                return m;
+            }
+        };
+
+        __global__ void seed_kernel(std::uint32_t *dist, std::uint8_t *touch, std::uint32_t src, std::uint32_t zero)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
             {
                 dist[src] = zero;
-                q[0] = src;
+                touch[src] = 1;
             }
         }
 
@@ -122,52 +152,59 @@ namespace hyt
             std::uint32_t light_max, chunk;
         };
 
+        // Compaction into the queue; returns the counters after it.
+        Counters compact_and_read(DeviceGraph &g, const SsspSel &sel)
+        {
+            Workspace &ws = g.ws;
+            Counters *d_ctr = ws.ctr.as<Counters>();
+            compact_rows<std::uint32_t, SsspSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr);
+            hip_check(hipGetLastError(), "launch sssp compaction");
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
+            hip_check(hipStreamSynchronize(g.stream), "sssp round");
+            return *h_ctr;
+        }
+
         template <class OffT, int WM, bool SYNC>
         void run_rounds(Launch &L, std::uint32_t src, hyt_run_stats &st)
         {
             DeviceGraph &g = L.g;
             Workspace &ws = g.ws;
             hipStream_t s = g.stream;
-            Counters *h_ctr = g.host_ctr->as<Counters>();
             Counters *d_ctr = ws.ctr.as<Counters>();
 
             launch_fill<std::uint32_t>(ws.dist.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
-            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), ws.q0.as<std::uint32_t>(), src, 0U);
+            launch_fill<std::uint32_t>(ws.prev.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
+            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), ws.touch.as<std::uint8_t>(), src, 0U);
             hip_check(hipGetLastError(), "launch seed");
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
 
-            std::uint32_t *qc = ws.q0.as<std::uint32_t>();
-            std::uint32_t *qn = ws.q1.as<std::uint32_t>();
-            std::uint32_t F = 1;
-            const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
+            SsspSel sel{ws.touch.as<std::uint8_t>(), ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>()};
+            std::uint32_t F = compact_and_read(g, sel).next_count;
 
             SsspOp<WM, SYNC> op;
             op.col = g.col.as<std::uint32_t>();
             op.w = g.w.as<std::uint32_t>();
             op.dist = ws.dist.as<std::uint32_t>();
-            op.mark = ws.mark.as<std::uint32_t>();
+            op.touch = ws.touch.as<std::uint8_t>();
             op.qd = ws.qd.as<std::uint32_t>();
             op.overflow = &d_ctr->overflow;
 
             ExpandArgs a;
+            a.q = ws.q.as<std::uint32_t>();
             a.ctr = d_ctr;
             a.heavy = ws.heavy.as<HeavyChunk>();
             a.heavy_cap = ws.heavy_cap;
             a.light_max = L.light_max;
             a.chunk = L.chunk;
 
+            const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
             st.converged = 0;
-            for (std::uint64_t round = 0; round < max_rounds; ++round)
+            for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
             {
-                op.epoch = next_epoch(g);
-                if (SYNC)
-                {
-                    snapshot_kernel<<<blocks_for(F), kBlock, 0, s>>>(qc, F, op.dist, ws.qd.as<std::uint32_t>());
-                    hip_check(hipGetLastError(), "launch snapshot");
-                }
-                a.q = qc;
                 a.F = F;
-                a.qn = qn;
+                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
                 expand_light<OffT, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
@@ -176,9 +213,7 @@ namespace hyt
                 hip_check(hipGetLastError(), "launch sssp_expand_heavy");
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s),
-                          "read counters");
-                hip_check(hipStreamSynchronize(s), "sssp round");
+                const Counters c = compact_and_read(g, sel);
                 if (L.o.time_kernels)
                 {
                     float ms = 0;
@@ -186,7 +221,6 @@ namespace hyt
                     st.expand_ms += ms;
                     st.expand_launches += 1;
                 }
-                const Counters c = *h_ctr;
                 if (c.heavy_overflow)
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.frontier_total += F;
@@ -198,15 +232,9 @@ namespace hyt
                     st.overflow = 1;
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
-                if (c.next_count == 0)
-                {
-                    st.converged = 1;
-                    break;
-                }
                 F = c.next_count;
-                std::swap(qc, qn);
-                hip_check(hipMemsetAsync(d_ctr, 0, offsetof(Counters, overflow), s), "reset counters");
-                hip_check(hipMemsetAsync(&d_ctr->edges, 0, sizeof(unsigned long long), s), "reset counters");
+                if (F == 0)
+                    st.converged = 1;
             }
         }
 
@@ -240,10 +268,20 @@ namespace hyt
         require(L.light_max <= (1U << 20), "light_max_degree must be <= 2^20");
         require(L.chunk >= 64 && L.chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
         ensure_workspace(g, L.light_max, L.chunk);
-        if (g.off64)
-            dispatch_weights<std::uint64_t>(L, src, st);
-        else
-            dispatch_weights<std::uint32_t>(L, src, st);
+        try
+        {
+            if (g.off64)
+                dispatch_weights<std::uint64_t>(L, src, st);
+            else
+                dispatch_weights<std::uint32_t>(L, src, st);
+        }
+        catch (...)
+        {
+            // a failed round may leave touch[] marks behind: restore all-zero
+            (void)hipMemsetAsync(g.ws.touch.ptr, 0, g.V + kPad, g.stream);
+            (void)hipStreamSynchronize(g.stream);
+            throw;
+        }
         hip_check(hipMemcpyAsync(dist_out, g.ws.dist.ptr, g.V * 4,
                                  o.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g.stream),
                   "copy distances");

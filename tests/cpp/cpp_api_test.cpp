@@ -1,0 +1,182 @@
+// C++ API tests in the original project's style (expect(cond, msg) throwing,
+// exit code 77 when a GPU test finds no device - tests/cuda_algorithm_tests.cu:91-99).
+//   cpp_api_test cpu   host-side API: CSRGraph, contributions
+//   cpp_api_test gpu   drop-in sssp_gpu / pagerank_gpu vs the original's known answers
+#include "algorithms/pagerank.hpp"
+#include "algorithms/sssp.hpp"
+#include "graph/csr_graph.hpp"
+#include "hytgraph_amd.h"
+
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace
+{
+    using hytgraph::graph::CSRGraph;
+    namespace alg = hytgraph::algorithms;
+
+    void expect(bool condition, const std::string &message)
+    {
+        if (!condition)
+            throw std::runtime_error("FAIL: " + message);
+    }
+
+    template <class Ex, class Fn>
+    void expect_throw(Fn fn, const std::string &message)
+    {
+        bool threw = false;
+        try
+        {
+            fn();
+        }
+        catch (const Ex &)
+        {
+            threw = true;
+        }
+        expect(threw, message);
+    }
+
+    // ------------------------------------------------------------- host
+    void test_csr_validation()
+    {
+        CSRGraph g(4, {0, 2, 4, 5, 5}, {1, 2, 2, 3, 3}, {1, 4, 2, 6, 1});
+        expect(g.num_vertices() == 4 && g.num_edges() == 5 && g.has_weights(), "CSR sizes");
+        expect(g.out_degree(1) == 2 && g.neighbor_at(4) == 3 && g.weight_at(1) == 4.0F, "CSR accessors");
+        expect_throw<std::out_of_range>([&] { (void)g.neighbor_range(4); }, "neighbor_range range check");
+        expect_throw<std::out_of_range>([&] { (void)g.neighbor_at(5); }, "neighbor_at range check");
+        expect_throw<std::invalid_argument>([] { CSRGraph(2, {0, 1}, {1}); }, "offset count");
+        expect_throw<std::invalid_argument>([] { CSRGraph(2, {1, 1, 1}, {1}); }, "first offset zero");
+        expect_throw<std::invalid_argument>([] { CSRGraph(2, {0, 2, 1}, {1, 0}); }, "monotone offsets");
+        expect_throw<std::invalid_argument>([] { CSRGraph(2, {0, 1, 1}, {2}); }, "column range");
+        expect_throw<std::invalid_argument>([] { CSRGraph(2, {0, 1, 1}, {1}, {1, 2}); }, "weight count");
+        CSRGraph empty(0, {0}, {});
+        expect(empty.num_vertices() == 0, "empty graph is valid");
+        CSRGraph unweighted(3, {0, 1, 2, 2}, {1, 0});
+        expect_throw<std::logic_error>([&] { (void)unweighted.weight_at(0); }, "weight_at on unweighted");
+    }
+
+    void test_reorder()
+    {
+        // 0->1 (1), 0->2 (2), 1->2 (3); order[new] = old = {2, 0, 1}
+        CSRGraph g(3, {0, 2, 3, 3}, {1, 2, 2}, {1, 2, 3});
+        g.reorder_vertices({2, 0, 1});
+        expect(g.row_offsets() == std::vector<CSRGraph::offset_type>({0, 0, 2, 3}), "reordered offsets");
+        expect(g.column_indices() == std::vector<CSRGraph::vertex_id>({2, 0, 0}), "renamed destinations");
+        expect(g.edge_weights() == std::vector<float>({1, 2, 3}), "weights follow their edges");
+        expect_throw<std::invalid_argument>([&] { g.reorder_vertices({0, 0, 1}); }, "duplicate ids");
+        expect_throw<std::invalid_argument>([&] { g.reorder_vertices({0, 1}); }, "short order");
+    }
+
+    void test_contributions()
+    {
+        // unit_tests.cpp:2922-2962: one step from uniform on 0->1, 0->2, 1->0, 2->0
+        CSRGraph g(3, {0, 2, 3, 4}, {1, 2, 0, 0});
+        auto c = alg::pagerank_contributions(g, {1.0F, 0.0F, 0.0F});
+        // next = 0.05 + 0.85 * in-sum: v0 <- 0, v1,v2 <- 0.5
+        expect(std::fabs(c[0].contribution - 0.95) < 1e-6, "contribution v0");
+        expect(std::fabs(c[1].contribution - 0.475) < 1e-6, "contribution v1");
+        expect(std::fabs(c[2].contribution - 0.475) < 1e-6, "contribution v2");
+        const float inf = std::numeric_limits<float>::infinity();
+        auto s = alg::sssp_contributions({5.0F, inf, 3.0F, inf}, {2.0F, 7.0F, 3.0F, inf});
+        expect(s[0].contribution == 3.0 && s[1].contribution == 1.0 && s[2].contribution == 0.0 &&
+                   s[3].contribution == 0.0,
+               "sssp contributions");
+        expect_throw<std::invalid_argument>([] { (void)alg::sssp_contributions({1.0F}, {}); }, "size mismatch");
+    }
+
+    // -------------------------------------------------------------- gpu
+    void test_sssp_gpu_known_answers()
+    {
+        // algorithm_tests.cpp:100-129 / cuda_algorithm_tests.cu:58-87
+        CSRGraph g(4, {0, 2, 4, 5, 5}, {1, 2, 2, 3, 3}, {1.0F, 4.0F, 2.0F, 6.0F, 1.0F});
+        alg::SSSPOptions o;
+        o.source = 0;
+        const auto r = alg::sssp_gpu(g, o);
+        expect(r.converged, "GPU SSSP converged");
+        expect(r.distances == std::vector<float>({0.0F, 1.0F, 3.0F, 4.0F}), "GPU SSSP distances");
+        // algorithm_tests.cpp:131-157: unweighted chain, unreachable vertex
+        CSRGraph chain(4, {0, 1, 2, 2, 2}, {1, 2});
+        const auto c = alg::sssp_gpu(chain, o);
+        expect(c.distances[0] == 0.0F && c.distances[1] == 1.0F && c.distances[2] == 2.0F &&
+                   std::isinf(c.distances[3]),
+               "GPU SSSP unweighted / unreachable");
+        expect(c.iterations == 3, "GPU SSSP iteration count equals the original's");
+        // algorithm_tests.cpp:159-206
+        expect_throw<std::invalid_argument>([] { (void)alg::sssp_gpu(CSRGraph(2, {0, 1, 1}, {1}, {-1.0F})); },
+                                            "negative weight rejected");
+        alg::SSSPOptions bad;
+        bad.source = 2;
+        expect_throw<std::invalid_argument>([&] { (void)alg::sssp_gpu(CSRGraph(2, {0, 0, 0}, {}), bad); },
+                                            "out-of-range source rejected");
+        auto batch = alg::sssp_gpu_batch(g, {0, 1, 3});
+        expect(batch.distances[0] == r.distances, "batched column 0");
+        expect(batch.distances[1][2] == 2.0F && std::isinf(batch.distances[1][0]), "batched column 1");
+        expect(batch.distances[2][3] == 0.0F, "batched column 2");
+    }
+
+    void test_pagerank_gpu_known_answers()
+    {
+        // cuda_algorithm_tests.cu:23-56 graph
+        CSRGraph g(3, {0, 2, 3, 4}, {1, 2, 2, 0});
+        alg::PageRankOptions o;
+        o.max_iterations = 200;
+        o.tolerance = 1.0e-6F;
+        const auto r = alg::pagerank_gpu(g, o);
+        expect(r.converged, "GPU PageRank converged");
+        // fixed point of this graph (FP64 power iteration to 1e-14)
+        const double x0 = 0.387789712, x1 = 0.214810627, x2 = 0.397399661;
+        expect(std::fabs(r.ranks[0] - x0) <= 1e-5 && std::fabs(r.ranks[1] - x1) <= 1e-5 &&
+                   std::fabs(r.ranks[2] - x2) <= 1e-5,
+               "GPU PageRank matches the fixed point within 1e-5");
+        // algorithm_tests.cpp:22-47: 3-cycle -> 1/3
+        const auto cyc = alg::pagerank_gpu(CSRGraph(3, {0, 1, 2, 3}, {1, 2, 0}), o);
+        for (float v : cyc.ranks)
+            expect(std::fabs(v - 1.0F / 3.0F) <= 1e-5F, "cycle ranks uniform");
+        // algorithm_tests.cpp:49-75: dangling vertex, mass preserved
+        const auto dg = alg::pagerank_gpu(CSRGraph(3, {0, 1, 2, 2}, {1, 0}), o);
+        expect(std::fabs(dg.ranks[0] + dg.ranks[1] + dg.ranks[2] - 1.0F) <= 1e-5F, "mass preserved");
+        // algorithm_tests.cpp:77-98
+        alg::PageRankOptions bad;
+        bad.damping_factor = 1.0F;
+        expect_throw<std::invalid_argument>([&] { (void)alg::pagerank_gpu(CSRGraph(1, {0, 0}, {}), bad); },
+                                            "damping 1 rejected");
+        auto ppr = alg::ppr_gpu(g, {0, 2}, 0.85F, 1e-9F);
+        expect(std::fabs(ppr.ranks[0][0] + ppr.ranks[0][1] + ppr.ranks[0][2] - 1.0F) <= 1e-5F, "ppr mass");
+    }
+} // namespace
+
+int main(int argc, char **argv)
+{
+    const std::string what = argc > 1 ? argv[1] : "cpu";
+    try
+    {
+        if (what == "cpu")
+        {
+            test_csr_validation();
+            test_reorder();
+            test_contributions();
+        }
+        else
+        {
+            int n = 0;
+            if (hyt_device_count(&n) != HYT_OK || n == 0)
+            {
+                std::cout << "GPU API tests skipped: no HIP device available.\n";
+                return 77;
+            }
+            test_sssp_gpu_known_answers();
+            test_pagerank_gpu_known_answers();
+        }
+        std::cout << "C++ API tests (" << what << ") passed.\n";
+        return 0;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << e.what() << '\n';
+        return 1;
+    }
+}

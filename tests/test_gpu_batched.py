@@ -1,0 +1,96 @@
+"""GPU parity: batched [V][K] SSSP and PPR through the C ABI.
+
+Bar: every SSSP column equals the single-source oracle bit-exactly; every PPR
+column is within the stated L1 tolerance of the FP64 oracle of its seed.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+D = 0.85
+
+
+def _say(msg):
+    print(f"[gpu-batched] {msg}", flush=True)
+
+
+@pytest.mark.parametrize("K", [2, 5, 8, 32, 64])
+def test_sssp_k_matches_single_source(gpu, abi, orc, K):
+    hg = abi.HostGraph(scale=13, edge_factor=16, seed=21, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 100 + K, K)
+    src[-1] = src[0]  # a repeated source shares a row
+    _say(f"K={K} sources {src[:4].tolist()}...")
+    d, st = g.sssp(src)
+    assert d.shape == (hg.V, K)
+    assert st["converged"] == 1
+    for k in range(K):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[k]))), k
+
+
+def test_sssp_k_float_and_unweighted(gpu, abi, orc):
+    hg = abi.HostGraph(scale=12, edge_factor=8, seed=5, weight_min=1, weight_max=64)
+    src = np.array([0, 7, 9, 1000], np.uint32)
+    gf = abi.DeviceGraph.from_host(hg, weights_f32=True)
+    d, _ = gf.sssp(src)
+    for k, s in enumerate(src):
+        ref, _, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), int(s))
+        assert np.array_equal(d[:, k], ref.view(np.uint32))
+    gu = abi.DeviceGraph(hg.offsets, hg.columns)
+    d, _ = gu.sssp(src)
+    for k, s in enumerate(src):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, None, int(s)))
+
+
+@pytest.mark.parametrize("light_max,chunk", [(4, 64), (1 << 20, 1 << 16)])
+def test_sssp_k_work_split(gpu, abi, orc, light_max, chunk):
+    hg = abi.HostGraph(scale=13, edge_factor=16, seed=4, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 3, 16)
+    d, _ = g.sssp(src, light_max_degree=light_max, heavy_chunk=chunk)
+    for k in range(16):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[k])))
+
+
+def test_sssp_k_repeated_calls_reuse_state(gpu, abi, orc):
+    """The [V][K] workspace is reused across calls of different K: marks must
+    start clean every time."""
+    hg = abi.HostGraph(scale=11, edge_factor=16, seed=6, weight_min=1, weight_max=64)
+    g = abi.DeviceGraph.from_host(hg)
+    for K in (8, 3, 16, 8):
+        src = abi.sample_sources(hg.offsets, 40 + K, K)
+        d, _ = g.sssp(src)
+        for k in range(K):
+            assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[k])))
+        d1, _ = g.sssp(int(src[0]))
+        assert np.array_equal(d1, d[:, 0])
+
+
+@pytest.mark.parametrize("K", [2, 8, 64])
+def test_ppr_k_matches_fp64(gpu, abi, orc, K):
+    hg = abi.HostGraph(scale=12, edge_factor=16, seed=2, weight_min=0, weight_max=0)
+    g = abi.DeviceGraph.from_host(hg)
+    seeds = abi.sample_sources(hg.offsets, 7 + K, K)
+    eps = 1e-8
+    r, st = g.ppr(seeds, D, eps)
+    assert r.shape == (hg.V, K) and st["converged"] == 1
+    worst_same = worst_exact = 0.0
+    for k in range(K):
+        same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps, seed=int(seeds[k]))
+        exact = orc.pagerank_f64(hg.offsets, hg.columns, D, seed=int(seeds[k]))
+        worst_same = max(worst_same, np.abs(r[:, k] - same).sum())
+        worst_exact = max(worst_exact, np.abs(r[:, k] - exact).sum())
+        assert abs(r[:, k].sum() - 1) < 1e-5
+    _say(f"K={K}: worst L1 vs same-procedure {worst_same:.2e}, vs exact {worst_exact:.2e}")
+    assert worst_same <= 2e-6
+    assert worst_exact <= 1e-4
+
+
+def test_ppr_k_column_equals_single(gpu, abi):
+    hg = abi.HostGraph(scale=12, edge_factor=16, seed=3, weight_min=0, weight_max=0)
+    g = abi.DeviceGraph.from_host(hg)
+    seeds = np.array([5, 900, 5, 17], np.uint32)
+    r, _ = g.ppr(seeds, D, 1e-9)
+    for k, s in enumerate(seeds):
+        one, _ = g.ppr(int(s), D, 1e-9)
+        assert np.abs(r[:, k] - one).sum() <= 2e-6
