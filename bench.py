@@ -11,11 +11,12 @@ value = GTEPS = sum over queries of m_reach / timed seconds, where m_reach is th
 out-degree sum of the reached vertices (Graph500 convention).
 
 roofline: the expand kernels (light + heavy chunk kernels of the relaxation
-round). Algorithmic bytes per round B = 16 F + 12 R + 4 U (F frontier rows:
-4 B queue id + 8 B offsets + 4 B distance; R relaxed edges: 4 B column + 4 B
-weight + 4 B distance atomic; U next-frontier appends: 4 B). achieved =
-sum B / sum of expand-kernel time measured with HIP events on the library's
-stream. peak = 8000 GB/s (MI355X HBM3E).
+round, push or pull). Algorithmic bytes per push round B = 16 F + 12 R (F
+frontier rows: 4 B queue id + 8 B offsets + 4 B distance; R relaxed edges:
+4 B column + 4 B weight + 4 B distance update); per pull round
+B = 12 V + 8 R + 4 A (see sssp_bytes). achieved = sum B / sum of
+relaxation-kernel time measured with HIP events on the library's stream.
+peak = 8000 GB/s (MI355X HBM3E).
 
 cpu_baseline: the original sssp_cpu compiled from source (oracle/_ref), single
 thread, on the same scale-24 graph and first source: runs with max_iterations
@@ -85,6 +86,24 @@ def cpu_baseline(hg, src: int, rounds: int, m_reach: int) -> dict:
     }
 
 
+def sssp_bytes(stats: list, V: int) -> int:
+    """Algorithmic bytes of the relaxation kernels over a set of calls.
+
+    push round: 16 F + 12 R  (queue id 4 + offsets 8 + distance 4 per frontier
+                row; column 4 + weight 4 + distance update 4 per edge)
+    pull round: 12 V + 8 R + 4 A  (offsets 4 + bitmap 4 + distance 4 per
+                destination; column 4 + weight 4 per in-edge scanned; source
+                distance 4 per in-edge whose source is in the frontier)
+    """
+    total = 0
+    for s in stats:
+        push_f = s["frontier_total"] - s["pull_frontier"]
+        push_r = s["edges_total"] - s["pull_edges"]
+        total += 16 * push_f + 12 * push_r
+        total += s["pull_rounds"] * 12 * V + 8 * s["pull_edges"] + 4 * s["pull_active_edges"]
+    return total
+
+
 def hg_scale(hg) -> int:
     return int(round(np.log2(hg.V)))
 
@@ -99,6 +118,9 @@ def main(argv=None) -> int:
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["async", "sync"], default="async")
+    ap.add_argument("--direction", choices=["auto", "push", "pull"], default="auto")
+    ap.add_argument("--pull-percent", type=int, default=0, help="auto: pull when frontier edges >= pct%% of E")
+    ap.add_argument("--delta", type=int, default=0, help="async SSSP delta-stepping bucket width (0 = off)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pagerank", action="store_true")
     args = ap.parse_args(argv)
@@ -124,22 +146,25 @@ def main(argv=None) -> int:
     sources = abi.sample_sources(hg.offsets, args.seed + 1, args.sources)
     deg = np.diff(hg.offsets.astype(np.int64))
     mode = abi.HYT_SSSP_ASYNC if args.mode == "async" else abi.HYT_SSSP_SYNC
+    direction = {"auto": abi.HYT_DIR_AUTO, "push": abi.HYT_DIR_PUSH, "pull": abi.HYT_DIR_PULL}[args.direction]
+    dopt = {"direction": direction, "pull_percent": args.pull_percent}
+    sopt = dict(dopt, delta=args.delta)
 
     out = torch.empty(hg.V, dtype=torch.int32, device="cuda")
     # m_reach per source and the SYNC round counts (untimed)
     m_reach = []
     sync_rounds = []
     for s in sources:
-        d, st = g.sssp(int(s), mode=abi.HYT_SSSP_SYNC)
+        d, st = g.sssp(int(s), mode=abi.HYT_SSSP_SYNC, **dopt)
         m_reach.append(int(deg[d != 0xFFFFFFFF].sum()))
         sync_rounds.append(st["rounds"])
-        d2, _ = g.sssp(int(s), mode=abi.HYT_SSSP_ASYNC)
+        d2, _ = g.sssp(int(s), mode=abi.HYT_SSSP_ASYNC, **sopt)
         assert np.array_equal(d, d2), "SYNC and ASYNC distances differ"
     log(f"sources {sources.tolist()} m_reach {m_reach} sync rounds {sync_rounds}")
 
     def step(stats_acc=None):
         for s in sources:
-            _, st = g.sssp(int(s), mode=mode, out_ptr=out.data_ptr(), time_kernels=1)
+            _, st = g.sssp(int(s), mode=mode, out_ptr=out.data_ptr(), time_kernels=1, **sopt)
             if stats_acc is not None:
                 stats_acc.append(st)
 
@@ -163,7 +188,8 @@ def main(argv=None) -> int:
     exp_ms = sum(s["expand_ms"] for s in acc)
     launches = sum(s["expand_launches"] for s in acc)
     rounds = sum(s["rounds"] for s in acc)
-    alg_bytes = 16 * F + 12 * R + 4 * U
+    pull_rounds = sum(s["pull_rounds"] for s in acc)
+    alg_bytes = sssp_bytes(acc, hg.V)
     achieved = alg_bytes / (exp_ms / 1e3) / 1e9 if exp_ms > 0 else 0.0
 
     result = {
@@ -188,6 +214,8 @@ def main(argv=None) -> int:
             "edges": hg.E,
             "sources_per_step": len(sources),
             "mode": args.mode,
+            "direction": args.direction,
+            "delta": args.delta,
         },
         "roofline": {
             "bound": "hbm",
@@ -196,7 +224,7 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "expand_light + expand_heavy (SSSP)",
+            "kernel": "SSSP relaxation (expand_light+expand_heavy push / pull_light+pull_heavy)",
             "algorithmic_bytes_per_launch": alg_bytes / max(launches, 1),
             "avg_launch_ms": exp_ms / max(launches, 1),
             "launches": launches,
@@ -208,25 +236,37 @@ def main(argv=None) -> int:
             "m_reach_per_query": sum(m_reach) / len(m_reach),
             "expand_ms_per_query": exp_ms / max(len(acc), 1),
             "sync_rounds": sync_rounds,
+            "pull_rounds_per_query": pull_rounds / max(len(acc), 1),
+            "direction": args.direction,
         },
     }
 
     if not args.no_pagerank:
-        t0 = time.perf_counter()
         pr_out = torch.empty(hg.V, dtype=torch.float32, device="cuda")
-        _, pst = g.pagerank(0.85, 1e-7, out_ptr=pr_out.data_ptr(), time_kernels=1)
+        g.pagerank(0.85, 1e-7, out_ptr=pr_out.data_ptr(), **dopt)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, pst = g.pagerank(0.85, 1e-7, out_ptr=pr_out.data_ptr(), time_kernels=1, **dopt)
         torch.cuda.synchronize()
         pr_s = time.perf_counter() - t0
-        P, Rp = pst["frontier_total"], pst["edges_total"]
-        pr_bytes = 28 * P + 12 * Rp
+        push_p = pst["frontier_total"] - pst["pull_frontier"]
+        push_r = pst["edges_total"] - pst["pull_edges"]
+        # push: 28 per pushing vertex (queue 4, offsets 8, residual 8, rank 8),
+        # 12 per pushed edge (column 4, residual atomic 8); pull: 12 per vertex,
+        # 4 per in-edge scanned, 4 per share gathered from a frontier source
+        pr_bytes = (28 * push_p + 12 * push_r + pst["pull_rounds"] * 12 * hg.V + 4 * pst["pull_edges"]
+                    + 4 * pst["pull_active_edges"])
         result["extras"]["pagerank"] = {
             "eps": 1e-7, "damping": 0.85, "seconds": pr_s, "rounds": pst["rounds"],
-            "edges_pushed": Rp, "pushes": P,
-            "edges_pushed_per_s_G": Rp / pr_s / 1e9,
-            "expand_GBs": pr_bytes / (pst["expand_ms"] / 1e3) / 1e9 if pst["expand_ms"] else None,
+            "pull_rounds": pst["pull_rounds"], "pushes": pst["frontier_total"],
+            "edges_visited": pst["edges_total"],
+            "edges_visited_per_s_G": pst["edges_total"] / pr_s / 1e9,
+            "kernel_GBs": pr_bytes / (pst["expand_ms"] / 1e3) / 1e9 if pst["expand_ms"] else None,
+            "kernel_ms": pst["expand_ms"],
             "normalised_residual": pst["residual"],
         }
-        log(f"pagerank: {pr_s:.3f} s, rounds {pst['rounds']}, pushed {Rp}")
+        log(f"pagerank: {pr_s:.3f} s, rounds {pst['rounds']} (pull {pst['pull_rounds']}), "
+            f"edges visited {pst['edges_total']}")
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(hg, int(sources[0]), sync_rounds[0], m_reach[0])

@@ -33,6 +33,9 @@ HYT_GRAPH_WEIGHTS_U32 = 0
 HYT_GRAPH_WEIGHTS_F32 = 1
 HYT_SSSP_SYNC = 0
 HYT_SSSP_ASYNC = 1
+HYT_DIR_AUTO = 0
+HYT_DIR_PUSH = 1
+HYT_DIR_PULL = 2
 
 U32_INF = 0xFFFFFFFF
 F32_INF_BITS = 0x7F800000
@@ -62,6 +65,9 @@ class RunOptions(C.Structure):
         ("time_kernels", C.c_uint32),
         ("light_max_degree", C.c_uint32),
         ("heavy_chunk", C.c_uint32),
+        ("direction", C.c_uint32),
+        ("pull_percent", C.c_uint32),
+        ("delta", C.c_uint32),
         ("reserved", C.c_uint32),
     ]
 
@@ -78,6 +84,10 @@ class RunStats(C.Structure):
         ("expand_launches", C.c_uint64),
         ("wall_ms", C.c_double),
         ("residual", C.c_double),
+        ("pull_rounds", C.c_uint64),
+        ("pull_active_edges", C.c_uint64),
+        ("pull_edges", C.c_uint64),
+        ("pull_frontier", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -212,8 +222,9 @@ def sample_sources(offsets: np.ndarray, seed: int, count: int) -> np.ndarray:
 
 
 def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
-             light_max_degree=0, heavy_chunk=0) -> RunOptions:
-    return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk, 0)
+             light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0) -> RunOptions:
+    return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk,
+                      direction, pull_percent, delta, 0)
 
 
 class DeviceGraph:

@@ -449,10 +449,12 @@ namespace hyt
         // qmask and is cleared.
         struct MarkSel
         {
+            using Aux = NoAux;
             unsigned long long *mark;
 
+            __device__ void publish(const Aux &, Counters *) const {}
             __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n,
-                                             unsigned long long (&vals)[4]) const
+                                             unsigned long long (&vals)[4], Aux &) const
             {
                 unsigned long long m4[4];
                 if (n == 4)
@@ -484,8 +486,8 @@ namespace hyt
         {
             Counters *d_ctr = g.ws.ctr.as<Counters>();
             MarkSel sel{kw.mark.as<unsigned long long>()};
-            compact_rows<unsigned long long, MarkSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
-                g.V, sel, g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>(), d_ctr);
+            compact_rows<unsigned long long, std::uint32_t, MarkSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+                g.V, sel, g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>(), d_ctr, nullptr, nullptr);
             hip_check(hipGetLastError(), "launch mark compaction");
             Counters *h_ctr = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");

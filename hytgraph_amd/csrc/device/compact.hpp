@@ -7,14 +7,25 @@
 // pass, and reserves its output range with one atomic. Rows keep ascending
 // order within a workgroup, so consecutive queue entries are nearby vertices.
 //
-// Sel::select4(row0, n, vals[4]) -> bit mask of the selected rows among
+// Sel::select4(row0, n, vals[4], aux) -> bit mask of the selected rows among
 // [row0, row0+n), n <= 4; vals[j] is the value of row row0+j when selected.
+// Sel::Aux is per-thread side information (e.g. held-back rows) that
+// Sel::publish(aux, ctr) reduces across the wavefront and publishes.
+// Optionally (non-null pointers) the pass also writes the frontier bitmap
+// (16 rows per thread = one u16 word, every word rewritten) and adds the
+// out-degree sum of the selected rows to ctr->frontier_edges, which the host
+// uses to choose a push or a pull round.
 #pragma once
 
 #include "kern_common.hpp"
 
 namespace hyt
 {
+    // Selector side information for selectors that have none.
+    struct NoAux
+    {
+    };
+
     constexpr int kCompactGroups = 4;                         // groups of 4 rows per thread
     constexpr int kCompactRows = 4 * kCompactGroups;          // rows per thread
     constexpr std::uint64_t kCompactRowsPerBlock = static_cast<std::uint64_t>(kBlock) * kCompactRows;
@@ -25,9 +36,9 @@ namespace hyt
         return static_cast<unsigned>(b == 0 ? 1 : b);
     }
 
-    template <class Val, class Sel>
+    template <class Val, class OffT, class Sel>
     __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
-                                                           Counters *ctr)
+                                                           Counters *ctr, const OffT *off, std::uint16_t *active)
     {
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
         __shared__ std::uint32_t s_base;
@@ -39,6 +50,7 @@ namespace hyt
         Val gv[kCompactGroups][4];
         std::uint32_t gm[kCompactGroups];
         std::uint32_t n = 0;
+        typename Sel::Aux aux{};
 #pragma unroll
         for (int g = 0; g < kCompactGroups; ++g)
         {
@@ -47,9 +59,32 @@ namespace hyt
             if (r < V)
             {
                 const std::uint32_t avail = static_cast<std::uint32_t>(V - r < 4 ? V - r : 4);
-                gm[g] = sel.select4(r, avail, gv[g]);
+                gm[g] = sel.select4(r, avail, gv[g], aux);
             }
             n += static_cast<std::uint32_t>(__popc(gm[g]));
+        }
+
+        sel.publish(aux, ctr);
+        const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+        if (active != nullptr && row0 < V)
+            active[tid] = static_cast<std::uint16_t>(gm[0] | (gm[1] << 4) | (gm[2] << 8) | (gm[3] << 12));
+        if (off != nullptr)
+        {
+            unsigned long long dsum = 0;
+#pragma unroll
+            for (int g = 0; g < kCompactGroups; ++g)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((gm[g] >> j) & 1U)
+                    {
+                        const std::uint64_t r = row0 + 4 * g + j;
+                        dsum += static_cast<unsigned long long>(off[r + 1] - off[r]);
+                    }
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+                dsum += __shfl_xor(dsum, d, kWave);
+            if (lane == 0 && dsum)
+                atomicAdd(&ctr->frontier_edges, dsum);
         }
 
         const std::uint32_t incl = wave_incl_scan(n);

@@ -39,17 +39,22 @@ namespace hyt
     // round; the two flags are sticky for the whole call.
     struct Counters
     {
-        std::uint32_t next_count;     // rows written by the compaction pass
-        std::uint32_t heavy_count;    // heavy chunks queued this round
-        unsigned long long edges;     // edges expanded this round
-        std::uint32_t overflow;       // a u32 sum overflowed
-        std::uint32_t heavy_overflow; // heavy chunk queue overflowed
+        std::uint32_t next_count;          // rows written by the compaction pass
+        std::uint32_t heavy_count;         // heavy chunks queued this round
+        unsigned long long edges;          // edges expanded this round
+        unsigned long long frontier_edges; // out-degree sum of the compacted rows
+        std::uint32_t pending;             // improved rows held back (at/above the bucket bound)
+        std::uint32_t pending_inv_max;     // max of ~dist over held-back rows (= ~min dist)
+        std::uint32_t overflow;            // a u32 sum overflowed
+        std::uint32_t heavy_overflow;      // heavy chunk queue overflowed
     };
-    constexpr std::size_t kRoundCounterBytes = 16;
+    constexpr std::size_t kRoundCounterBytes = 32;
 
     struct Workspace
     {
         DeviceBuffer dist, rank, prev, touch, q, qd, heavy, ctr, partial;
+        DeviceBuffer active; // frontier bitmap (bit v = row v selected by the last compaction)
+        DeviceBuffer share;  // PageRank pull: d * r_u / outdeg(u) of the frontier rows, by vertex
         std::uint64_t heavy_cap = 0;
         bool ready = false;
     };
@@ -62,6 +67,10 @@ namespace hyt
         bool weights_f32 = false;
         int device = 0;
         DeviceBuffer off, col, w;
+        // incoming CSR for pull rounds (built on first use)
+        DeviceBuffer in_off, in_col, in_w, in_heavy;
+        std::uint64_t in_heavy_count = 0;
+        std::uint32_t in_heavy_light_max = 0, in_heavy_chunk = 0;
         hipStream_t stream = nullptr;
         Workspace ws;
         std::unique_ptr<PinnedBuffer> host_ctr;
@@ -70,8 +79,9 @@ namespace hyt
         ~DeviceGraph();
         std::uint64_t device_bytes() const
         {
-            return off.bytes + col.bytes + w.bytes + ws.dist.bytes + ws.rank.bytes + ws.prev.bytes +
-                   ws.touch.bytes + ws.q.bytes + ws.qd.bytes + ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes;
+            return off.bytes + col.bytes + w.bytes + in_off.bytes + in_col.bytes + in_w.bytes + in_heavy.bytes +
+                   ws.dist.bytes + ws.rank.bytes + ws.prev.bytes + ws.touch.bytes + ws.q.bytes + ws.qd.bytes +
+                   ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes;
         }
         int weight_mode() const
         {
@@ -84,12 +94,22 @@ namespace hyt
     // Default work-split thresholds (degree binning).
     constexpr std::uint32_t kDefaultLightMaxDegree = 256;
     constexpr std::uint32_t kDefaultHeavyChunk = 1024;
+    // Direction choice: pull when the frontier's out-edges reach this % of E.
+    // Measured on RMAT scale 24 (MI355X): PageRank pull rounds (one gather per
+    // in-edge, no atomics) beat float-atomic pushes from ~25% of E; SSSP push
+    // (one filtered atomic per edge) beats pull (bitmap probe + gather per
+    // in-edge) even at 50%, so SSSP pulls only when asked.
+    constexpr std::uint64_t kDefaultPullPercentPagerank = 25;
+    constexpr std::uint64_t kDefaultPullPercentSssp = 101;
 
     // Allocates the single-source workspace if needed (sized for the graph).
     void ensure_workspace(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
 
     // Spare slots after every [V] workspace array (vector loads past the end).
     constexpr std::uint64_t kPad = 16;
+
+    // Builds the incoming CSR and the static chunk list of its heavy rows.
+    void ensure_transpose(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
 } // namespace hyt
 
 struct hyt_graph

@@ -13,9 +13,13 @@
 // The frontier of a round is exactly {u : r_u > eps}: the compaction pass
 // scans the residuals, moves the selected ones into rank and records them
 // (the snapshot), so which vertices push in a round does not depend on
-// thread timing. Pushes are non-returning float atomics.
+// thread timing. Push rounds use non-returning float atomics; rounds whose
+// frontier out-edges are a large share of E pull instead: each destination
+// sums d * r_u / outdeg(u) over in-edges from frontier sources in LDS and adds
+// the total to its residual with one store.
 #include "compact.hpp"
 #include "expand.hpp"
+#include "pull.hpp"
 
 #include <chrono>
 #include <cmath>
@@ -49,14 +53,22 @@ namespace hyt
             }
         };
 
-        // Selects r_u > eps, moves r_u into rank_u and zeroes it.
+        // Selects r_u > eps, moves r_u into rank_u and zeroes it, and records
+        // the per-edge share d * r_u / outdeg(u) by vertex for pull rounds.
+        template <class OffT>
         struct PrSel
         {
+            using Aux = NoAux;
             float *res;
             float *rank;
+            float *share;
+            const OffT *off;
+            float damping;
             float eps;
 
-            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4]) const
+            __device__ void publish(const Aux &, Counters *) const {}
+            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
+                                             Aux &) const
             {
                 float r[4];
                 if (n == 4)
@@ -71,18 +83,60 @@ namespace hyt
                         r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0.0F;
                 }
                 std::uint32_t m = 0;
+                float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                 {
                     if (r[j] > eps)
                     {
-                        res[row0 + j] = 0.0F;
-                        rank[row0 + j] += r[j];
+                        const std::uint64_t v = row0 + j;
+                        res[v] = 0.0F;
+                        rank[v] += r[j];
                         vals[j] = __float_as_uint(r[j]);
+                        const OffT deg = off[v + 1] - off[v];
+                        sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
                         m |= 1U << j;
                     }
                 }
+                // every row's share is rewritten (0 outside the frontier), so a
+                // pull round needs no frontier test: one gather per in-edge
+                if (n == 4)
+                    *reinterpret_cast<float4 *>(share + row0) = make_float4(sh[0], sh[1], sh[2], sh[3]);
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (static_cast<std::uint32_t>(j) < n)
+                            share[row0 + j] = sh[j];
+                }
                 return m;
+            }
+        };
+
+        struct PrPullOp
+        {
+            const std::uint32_t *__restrict__ in_col;
+            const float *share;
+            float *res;
+
+            __device__ std::uint32_t identity() const { return 0U; }
+            __device__ bool edge(std::uint64_t pos, std::uint32_t &c) const
+            {
+                c = __float_as_uint(share[in_col[pos]]);
+                return c != 0U;
+            }
+            __device__ std::uint32_t fold(std::uint32_t a, std::uint32_t b) const
+            {
+                return __float_as_uint(__uint_as_float(a) + __uint_as_float(b));
+            }
+            __device__ void finish(std::uint32_t v, std::uint32_t acc) const
+            {
+                if (acc != 0U)
+                    res[v] += __uint_as_float(acc);
+            }
+            __device__ void finish_atomic(std::uint32_t v, std::uint32_t acc) const
+            {
+                unsafeAtomicAdd(&res[v], __uint_as_float(acc));
             }
         };
 
@@ -179,12 +233,14 @@ namespace hyt
 
         constexpr unsigned kFoldBlocks = 1024; // partial holds 2*kFoldBlocks+2 doubles
 
-        Counters compact_and_read(DeviceGraph &g, const PrSel &sel)
+        template <class OffT>
+        Counters compact_and_read(DeviceGraph &g, const PrSel<OffT> &sel)
         {
             Workspace &ws = g.ws;
             Counters *d_ctr = ws.ctr.as<Counters>();
-            compact_rows<std::uint32_t, PrSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
-                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr);
+            compact_rows<std::uint32_t, OffT, PrSel<OffT>><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr, g.off.as<OffT>(),
+                static_cast<std::uint16_t *>(nullptr));
             hip_check(hipGetLastError(), "launch pagerank compaction");
             Counters *h_ctr = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
@@ -201,7 +257,8 @@ namespace hyt
             Counters *d_ctr = ws.ctr.as<Counters>();
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : 100000;
 
-            PrSel sel{ws.dist.as<float>(), ws.rank.as<float>(), eps};
+            PrSel<OffT> sel{ws.dist.as<float>(), ws.rank.as<float>(), ws.share.as<float>(), g.off.as<OffT>(),
+                            damping, eps};
             PrOp op;
             op.col = g.col.as<std::uint32_t>();
             op.res = ws.dist.as<float>();
@@ -216,22 +273,48 @@ namespace hyt
             a.light_max = light_max;
             a.chunk = chunk;
 
+            const std::uint32_t dir = o.direction;
+            const std::uint64_t pct = o.pull_percent ? o.pull_percent : kDefaultPullPercentPagerank;
+            if (dir != HYT_DIR_PUSH && g.E > 0 && g.E < (1ULL << 31))
+                ensure_transpose(g, light_max, chunk);
+            const bool can_pull = g.in_col.ptr != nullptr;
+            require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
+            PrPullOp pop{g.in_col.as<std::uint32_t>(), ws.share.as<float>(), ws.dist.as<float>()};
+
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            std::uint32_t F = compact_and_read(g, sel).next_count;
+            Counters c0 = compact_and_read<OffT>(g, sel);
+            std::uint32_t F = c0.next_count;
+            unsigned long long Fe = c0.frontier_edges;
             st.converged = F == 0 ? 1 : 0;
             for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
             {
                 a.F = F;
                 hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                const bool pull = can_pull && (dir == HYT_DIR_PULL ||
+                                               (dir == HYT_DIR_AUTO && Fe * 100ULL >= pct * g.E));
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                expand_light<OffT, PrOp><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                hip_check(hipGetLastError(), "launch pr_expand_light");
-                expand_heavy<PrOp><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-                hip_check(hipGetLastError(), "launch pr_expand_heavy");
+                if (pull)
+                {
+                    pull_light<OffT, PrPullOp><<<blocks_for(g.V), kBlock, 0, s>>>(g.in_off.as<OffT>(), pop, g.V,
+                                                                                   light_max, d_ctr);
+                    hip_check(hipGetLastError(), "launch pr_pull_light");
+                    pull_heavy<PrPullOp><<<kHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
+                                                                      g.in_heavy_count, d_ctr);
+                    hip_check(hipGetLastError(), "launch pr_pull_heavy");
+                    st.pull_rounds += 1;
+                    st.pull_active_edges += Fe;
+                }
+                else
+                {
+                    expand_light<OffT, PrOp><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    hip_check(hipGetLastError(), "launch pr_expand_light");
+                    expand_heavy<PrOp><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    hip_check(hipGetLastError(), "launch pr_expand_heavy");
+                }
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_and_read(g, sel);
+                const Counters c = compact_and_read<OffT>(g, sel);
                 if (o.time_kernels)
                 {
                     float ms = 0;
@@ -243,9 +326,15 @@ namespace hyt
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.frontier_total += F;
                 st.edges_total += c.edges;
+                if (pull)
+                {
+                    st.pull_edges += c.edges;
+                    st.pull_frontier += F;
+                }
                 st.updates_total += c.next_count;
                 st.rounds = round + 1;
                 F = c.next_count;
+                Fe = c.frontier_edges;
                 if (F == 0)
                     st.converged = 1;
             }

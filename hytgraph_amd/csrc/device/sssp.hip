@@ -17,14 +17,20 @@
 //     next frontier is exactly the improved set, and records dist[v] as the
 //     snapshot value of the next round;
 //   * SYNC mode expands with the snapshot values (Jacobi), which reproduces
-//     the original iteration count; ASYNC mode reads live distances.
+//     the original iteration count; ASYNC mode reads live distances;
+//   * direction: a round whose frontier out-edges are a large share of E runs
+//     as a pull over the transpose instead (each destination takes the
+//     minimum over in-edges from frontier sources, prev[] being exactly the
+//     end-of-previous-round distances); same result, no global atomics.
 // Why touch[] cannot miss an improvement: the first atomicMin that lowers
 // dist[v] in a round came from a thread whose filter load passed, and that
 // thread stored touch[v].
 #include "compact.hpp"
 #include "expand.hpp"
+#include "pull.hpp"
 
 #include <chrono>
+#include <cstring>
 #include <string>
 
 namespace hyt
@@ -102,21 +108,49 @@ namespace hyt
             }
         };
 
+        // Touched rows that improved join the frontier when below `bound`
+        // (delta-stepping bucket; 0xFFFFFFFF = no bucketing); improved rows at
+        // or above the bound stay marked and are counted as pending.
         struct SsspSel
         {
+            struct Aux
+            {
+                std::uint32_t pending = 0;
+                std::uint32_t inv_max = 0;
+            };
             std::uint8_t *touch;
             const std::uint32_t *dist;
             std::uint32_t *prev;
+            std::uint32_t bound;
 
-            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4]) const
+            __device__ void publish(const Aux &aux, Counters *ctr) const
+            {
+                if (bound == 0xFFFFFFFFU)
+                    return;
+                std::uint32_t p = aux.pending, m = aux.inv_max;
+#pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1)
+                {
+                    p += __shfl_xor(p, d, kWave);
+                    const std::uint32_t o = __shfl_xor(m, d, kWave);
+                    m = o > m ? o : m;
+                }
+                if (lane_id() == 0 && p)
+                {
+                    atomicAdd(&ctr->pending, p);
+                    atomicMax(&ctr->pending_inv_max, m);
+                }
+            }
+
+            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
+                                             Aux &aux) const
             {
                 // touch has spare bytes past V, so the 4-byte word is in range
                 std::uint32_t *tw = reinterpret_cast<std::uint32_t *>(touch + row0);
                 const std::uint32_t t = *tw;
                 if (t == 0)
                     return 0;
-                *tw = 0;
-                std::uint32_t m = 0;
+                std::uint32_t keep = 0, m = 0;
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
                 {
@@ -126,13 +160,66 @@ namespace hyt
                         const std::uint32_t d = dist[v];
                         if (d < prev[v])
                         {
-                            prev[v] = d;
-                            vals[j] = d;
-                            m |= 1U << j;
+                            if (d < bound)
+                            {
+                                prev[v] = d;
+                                vals[j] = d;
+                                m |= 1U << j;
+                            }
+                            else
+                            {
+                                keep |= 0xFFU << (8 * j);
+                                aux.pending += 1;
+                                aux.inv_max = (~d > aux.inv_max) ? ~d : aux.inv_max;
+                            }
                         }
                     }
                 }
+                if (keep != t)
+                    *tw = keep;
                 return m;
+            }
+        };
+
+        template <int WM, bool SYNC>
+        struct SsspPullOp
+        {
+            const std::uint32_t *__restrict__ in_col;
+            const std::uint32_t *__restrict__ in_w;
+            const std::uint32_t *active;
+            const std::uint32_t *prev;
+            std::uint32_t *dist;
+            std::uint8_t *touch;
+            std::uint32_t *overflow;
+
+            __device__ std::uint32_t identity() const { return 0xFFFFFFFFU; }
+            __device__ bool edge(std::uint64_t pos, std::uint32_t &c) const
+            {
+                const std::uint32_t u = in_col[pos];
+                if (!bit_set(active, u))
+                    return false;
+                const std::uint32_t du = SYNC ? prev[u] : dist[u];
+                if (Weights<WM>::add(du, Weights<WM>::load(in_w, pos), c))
+                    return true;
+                atomicOr(overflow, 1U);
+                return false;
+            }
+            __device__ std::uint32_t fold(std::uint32_t a, std::uint32_t b) const { return a < b ? a : b; }
+            __device__ void finish(std::uint32_t v, std::uint32_t acc) const
+            {
+                if (acc < dist[v])
+                {
+                    dist[v] = acc;
+                    touch[v] = 1;
+                }
+            }
+            __device__ void finish_atomic(std::uint32_t v, std::uint32_t acc) const
+            {
+                if (acc < dist[v])
+                {
+                    atomicMin(&dist[v], acc);
+                    touch[v] = 1;
+                }
             }
         };
 
@@ -152,13 +239,16 @@ namespace hyt
             std::uint32_t light_max, chunk;
         };
 
-        // Compaction into the queue; returns the counters after it.
+        // Compaction into the queue (plus frontier bitmap and out-edge sum);
+        // returns the counters after it.
+        template <class OffT>
         Counters compact_and_read(DeviceGraph &g, const SsspSel &sel)
         {
             Workspace &ws = g.ws;
             Counters *d_ctr = ws.ctr.as<Counters>();
-            compact_rows<std::uint32_t, SsspSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
-                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr);
+            compact_rows<std::uint32_t, OffT, SsspSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr, g.off.as<OffT>(),
+                ws.active.as<std::uint16_t>());
             hip_check(hipGetLastError(), "launch sssp compaction");
             Counters *h_ctr = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
@@ -180,8 +270,46 @@ namespace hyt
             hip_check(hipGetLastError(), "launch seed");
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
 
-            SsspSel sel{ws.touch.as<std::uint8_t>(), ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>()};
-            std::uint32_t F = compact_and_read(g, sel).next_count;
+            // delta-stepping buckets (ASYNC only): admit improved rows below
+            // `bound`; when none is left, move the bound past the nearest
+            // held-back distance
+            const bool bucketed = !SYNC && L.o.delta > 0;
+            auto bound_after = [&](std::uint32_t d) -> std::uint32_t {
+                if (WM == kWF32 || WM == kWUnitF32)
+                {
+                    float f;
+                    std::memcpy(&f, &d, 4);
+                    f += static_cast<float>(L.o.delta);
+                    std::uint32_t b;
+                    std::memcpy(&b, &f, 4);
+                    return b;
+                }
+                const std::uint64_t b = static_cast<std::uint64_t>(d) + L.o.delta;
+                return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
+            };
+            SsspSel sel{ws.touch.as<std::uint8_t>(), ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(),
+                        bucketed ? bound_after(0U) : 0xFFFFFFFFU};
+            Counters c0 = compact_and_read<OffT>(g, sel);
+            std::uint32_t F = c0.next_count;
+            unsigned long long Fe = c0.frontier_edges;
+            std::uint32_t pending = c0.pending;
+            std::uint32_t pending_min = ~c0.pending_inv_max;
+
+            const std::uint32_t dir = L.o.direction;
+            const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
+            const bool may_pull = dir == HYT_DIR_PULL || (dir == HYT_DIR_AUTO && pct <= 100);
+            if (may_pull && g.E > 0 && g.E < (1ULL << 31))
+                ensure_transpose(g, L.light_max, L.chunk);
+            const bool can_pull = g.in_col.ptr != nullptr;
+            require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
+            SsspPullOp<WM, SYNC> pop;
+            pop.in_col = g.in_col.as<std::uint32_t>();
+            pop.in_w = g.in_w.as<std::uint32_t>();
+            pop.active = ws.active.as<std::uint32_t>();
+            pop.prev = ws.prev.as<std::uint32_t>();
+            pop.dist = ws.dist.as<std::uint32_t>();
+            pop.touch = ws.touch.as<std::uint8_t>();
+            pop.overflow = &d_ctr->overflow;
 
             SsspOp<WM, SYNC> op;
             op.col = g.col.as<std::uint32_t>();
@@ -201,19 +329,48 @@ namespace hyt
 
             const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
             st.converged = 0;
-            for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
+            for (std::uint64_t round = 0; round < max_rounds && (F > 0 || pending > 0);)
             {
+                if (F == 0)
+                {
+                    // bucket exhausted: advance to the nearest held-back row
+                    sel.bound = bound_after(pending_min);
+                    hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                    const Counters cb = compact_and_read<OffT>(g, sel);
+                    F = cb.next_count;
+                    Fe = cb.frontier_edges;
+                    pending = cb.pending;
+                    pending_min = ~cb.pending_inv_max;
+                    continue;
+                }
+                ++round;
                 a.F = F;
                 hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                const bool pull = can_pull && (dir == HYT_DIR_PULL ||
+                                               (dir == HYT_DIR_AUTO && Fe * 100ULL >= pct * g.E));
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                expand_light<OffT, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                hip_check(hipGetLastError(), "launch sssp_expand_light");
-                expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-                hip_check(hipGetLastError(), "launch sssp_expand_heavy");
+                if (pull)
+                {
+                    pull_light<OffT, SsspPullOp<WM, SYNC>><<<blocks_for(g.V), kBlock, 0, s>>>(
+                        g.in_off.as<OffT>(), pop, g.V, L.light_max, d_ctr);
+                    hip_check(hipGetLastError(), "launch sssp_pull_light");
+                    pull_heavy<SsspPullOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
+                                                                                  g.in_heavy_count, d_ctr);
+                    hip_check(hipGetLastError(), "launch sssp_pull_heavy");
+                    st.pull_rounds += 1;
+                    st.pull_active_edges += Fe;
+                }
+                else
+                {
+                    expand_light<OffT, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    hip_check(hipGetLastError(), "launch sssp_expand_light");
+                    expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    hip_check(hipGetLastError(), "launch sssp_expand_heavy");
+                }
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_and_read(g, sel);
+                const Counters c = compact_and_read<OffT>(g, sel);
                 if (L.o.time_kernels)
                 {
                     float ms = 0;
@@ -225,16 +382,30 @@ namespace hyt
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.frontier_total += F;
                 st.edges_total += c.edges;
+                if (pull)
+                {
+                    st.pull_edges += c.edges;
+                    st.pull_frontier += F;
+                }
                 st.updates_total += c.next_count;
-                st.rounds = round + 1;
+                st.rounds = round;
                 if (c.overflow)
                 {
                     st.overflow = 1;
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
                 F = c.next_count;
-                if (F == 0)
+                Fe = c.frontier_edges;
+                pending = c.pending;
+                pending_min = ~c.pending_inv_max;
+                if (F == 0 && pending == 0)
                     st.converged = 1;
+            }
+            if (pending > 0)
+            {
+                // stopped by max_rounds with held-back rows: touch[] must be
+                // all-zero for the next call
+                hip_check(hipMemsetAsync(ws.touch.ptr, 0, g.V + kPad, s), "clear touch");
             }
         }
 

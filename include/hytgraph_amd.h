@@ -119,6 +119,12 @@ enum hyt_sssp_mode {
     HYT_SSSP_ASYNC = 1
 };
 
+enum hyt_direction {
+    HYT_DIR_AUTO = 0, /* per round: pull when the frontier's out-edges are a large share of E */
+    HYT_DIR_PUSH = 1, /* always push along out-edges */
+    HYT_DIR_PULL = 2  /* always pull along in-edges (builds the transpose on first use) */
+};
+
 typedef struct hyt_run_options {
     uint32_t mode;           /* hyt_sssp_mode (SSSP only) */
     uint32_t out_on_device;  /* 1: output pointer is device memory */
@@ -126,6 +132,10 @@ typedef struct hyt_run_options {
     uint32_t time_kernels;   /* 1: time the expand kernels with HIP events */
     uint32_t light_max_degree; /* 0: default; vertices above are chunked */
     uint32_t heavy_chunk;      /* 0: default edges per wave chunk */
+    uint32_t direction;        /* hyt_direction (single-source SSSP and PageRank) */
+    uint32_t pull_percent;     /* AUTO: pull when frontier out-edges >= pct% of E; 0 = default */
+    uint32_t delta;            /* ASYNC SSSP: delta-stepping bucket width in weight units
+                                  (0 = plain rounds); SYNC mode ignores it */
     uint32_t reserved;
 } hyt_run_options;
 
@@ -140,6 +150,10 @@ typedef struct hyt_run_stats {
     uint64_t expand_launches; /* number of timed expand launches */
     double wall_ms;           /* host wall time of the call */
     double residual;          /* PageRank: normalised L1 of leftover residual */
+    uint64_t pull_rounds;     /* rounds run as pull over the transpose */
+    uint64_t pull_active_edges; /* pull rounds: in-edges whose source was active */
+    uint64_t pull_edges;      /* pull rounds: in-edges scanned (part of edges_total) */
+    uint64_t pull_frontier;   /* pull rounds: frontier rows (part of frontier_total) */
 } hyt_run_stats;
 
 /*

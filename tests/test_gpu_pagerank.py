@@ -45,10 +45,11 @@ def test_closed_forms(gpu, abi):
 
 
 @pytest.mark.parametrize("scale,eps", [(12, 1e-7), (16, 1e-7)])
-def test_rmat_pagerank_vs_fp64(gpu, abi, orc, scale, eps):
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_rmat_pagerank_vs_fp64(gpu, abi, orc, scale, eps, direction):
     hg = abi.HostGraph(scale=scale, edge_factor=16, seed=3, weight_min=0, weight_max=0)
     g = abi.DeviceGraph.from_host(hg)
-    r, st = g.pagerank(D, eps)
+    r, st = g.pagerank(D, eps, direction=direction)
     assert st["converged"] == 1
     same, rounds, edges = orc.delta_push_f64(hg.offsets, hg.columns, D, eps)
     exact = orc.pagerank_f64(hg.offsets, hg.columns, D)
@@ -65,20 +66,22 @@ def test_rmat_pagerank_vs_fp64(gpu, abi, orc, scale, eps):
 
 
 @pytest.mark.parametrize("light_max,chunk", [(4, 64), (1 << 20, 1 << 16)])
-def test_pagerank_work_split_invariance(gpu, abi, orc, light_max, chunk):
+@pytest.mark.parametrize("direction", [1, 2])
+def test_pagerank_work_split_invariance(gpu, abi, orc, light_max, chunk, direction):
     hg = abi.HostGraph(scale=13, edge_factor=16, seed=8, weight_min=0, weight_max=0)
     g = abi.DeviceGraph.from_host(hg)
-    r, _ = g.pagerank(D, 1e-8, light_max_degree=light_max, heavy_chunk=chunk)
+    r, _ = g.pagerank(D, 1e-8, light_max_degree=light_max, heavy_chunk=chunk, direction=direction)
     exact = orc.pagerank_f64(hg.offsets, hg.columns, D)
     assert np.abs(r - exact).sum() <= 2 * D * 1e-8 / (1 - D) ** 2 + 2e-6
 
 
 @pytest.mark.parametrize("seed_vertex", [0, 100])
-def test_ppr_single_seed(gpu, abi, orc, seed_vertex):
+@pytest.mark.parametrize("direction", [0, 2])
+def test_ppr_single_seed(gpu, abi, orc, seed_vertex, direction):
     hg = abi.HostGraph(scale=13, edge_factor=16, seed=2, weight_min=0, weight_max=0)
     g = abi.DeviceGraph.from_host(hg)
     eps = 1e-9
-    r, st = g.ppr(seed_vertex, D, eps)
+    r, st = g.ppr(seed_vertex, D, eps, direction=direction)
     same, rounds, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps, seed=seed_vertex)
     exact = orc.pagerank_f64(hg.offsets, hg.columns, D, seed=seed_vertex)
     _say(f"ppr seed {seed_vertex}: L1 same {np.abs(r - same).sum():.3e} exact {np.abs(r - exact).sum():.3e}")

@@ -39,39 +39,67 @@ def test_reference_cuda_test_graph(gpu, abi, orc, mode):
         assert st["rounds"] == it
 
 
-def test_golden_sssp_cases(gpu, abi):
+@pytest.mark.parametrize("direction", [1, 2])
+def test_golden_sssp_cases(gpu, abi, direction):
     data = json.loads((GOLDEN / "reference_cases.json").read_text())
     for case in data["sssp"]:
         if case["tolerance"] != 0.0:
-            continue  # order-dependent tolerance rule: covered by the drop-in pull path
+            continue  # order-dependent tolerance rule: not implemented on the GPU path
         _say(case["name"])
         w = None if case["weights"] is None else np.array(case["weights"], np.float32)
         g = abi.DeviceGraph(np.array(case["offsets"], np.uint64), np.array(case["columns"], np.uint32), w,
                             weights_f32=True)
-        d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"])
+        d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"], direction=direction)
         assert d.tolist() == case["distance_bits"], case["name"]
         assert st["rounds"] == case["iterations"] and bool(st["converged"]) == case["converged"], case["name"]
 
 
+DIRECTIONS = [(0, 0), (1, 0), (2, 0), (0, 1), (0, 50)]  # (direction, pull_percent)
+
+
 @pytest.mark.parametrize("seed", [1, 2])
-def test_rmat16_config1_bitexact(gpu, abi, orc, seed):
-    """RMAT scale 16, ef 16, u32 weights in [1,64], source 0 (the CPU-runnable case)."""
+@pytest.mark.parametrize("direction,pct", DIRECTIONS)
+def test_rmat16_config1_bitexact(gpu, abi, orc, seed, direction, pct):
+    """RMAT scale 16, ef 16, u32 weights in [1,64], source 0 (the CPU-runnable
+    case), for push-only, pull-only and mixed rounds."""
     hg = rmat(abi, 16, seed)
     g = abi.DeviceGraph.from_host(hg)
     dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0)
     ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 0)
     assert np.array_equal(orc.f32_to_u32_dist(ref), dj)
     for mode in (0, 1):
-        _say(f"seed {seed} mode {mode}")
-        d, st = g.sssp(0, mode=mode)
+        _say(f"seed {seed} mode {mode} direction {direction}/{pct}")
+        d, st = g.sssp(0, mode=mode, direction=direction, pull_percent=pct)
         assert np.array_equal(d, dj)
         assert st["converged"] == 1
+        if direction == 2:
+            assert st["pull_rounds"] == st["rounds"]
         if mode == 0:
             assert st["rounds"] == it
     # float-weight graph of the same topology: float bit patterns equal the original's
     gf = abi.DeviceGraph.from_host(hg, weights_f32=True)
-    d, st = gf.sssp(0, mode=0)
+    d, st = gf.sssp(0, mode=0, direction=direction, pull_percent=pct)
     assert np.array_equal(d, ref.view(np.uint32)) and st["rounds"] == it
+
+
+@pytest.mark.parametrize("delta", [1, 16, 100, 1000])
+def test_delta_stepping_bitexact(gpu, abi, orc, delta):
+    """ASYNC rounds with delta-stepping buckets: same distances, any width."""
+    hg = rmat(abi, 14, 7, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    for src in (0, 4242):
+        dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src)
+        d, st = g.sssp(src, mode=1, delta=delta)
+        assert np.array_equal(d, dj), (src, delta)
+        assert st["converged"] == 1
+        # a truncated bucketed run leaves the workspace clean for the next call
+        g.sssp(src, mode=1, delta=delta, max_rounds=2)
+        d2, _ = g.sssp(src, mode=1)
+        assert np.array_equal(d2, dj)
+    gf = abi.DeviceGraph.from_host(hg, weights_f32=True)
+    ref, _, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 0)
+    d, _ = gf.sssp(0, mode=1, delta=delta)
+    assert np.array_equal(d, ref.view(np.uint32))
 
 
 def test_fractional_float_weights_bitexact(gpu, abi, orc, rng):
@@ -89,13 +117,14 @@ def test_fractional_float_weights_bitexact(gpu, abi, orc, rng):
 
 
 @pytest.mark.parametrize("max_rounds", [1, 2, 3, 5])
-def test_truncated_rounds_match_original_state(gpu, abi, orc, max_rounds):
-    """SYNC mode is round-for-round Jacobi: the state after k rounds equals the
-    original's state after max_iterations = k."""
+@pytest.mark.parametrize("direction", [1, 2])
+def test_truncated_rounds_match_original_state(gpu, abi, orc, max_rounds, direction):
+    """SYNC mode is round-for-round Jacobi (push or pull): the state after k
+    rounds equals the original's state after max_iterations = k."""
     hg = rmat(abi, 12, 9)
     g = abi.DeviceGraph.from_host(hg, weights_f32=True)
     ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 1, 0.0, max_rounds)
-    d, st = g.sssp(1, mode=0, max_rounds=max_rounds)
+    d, st = g.sssp(1, mode=0, max_rounds=max_rounds, direction=direction)
     assert np.array_equal(d, ref.view(np.uint32))
     assert st["rounds"] == it and bool(st["converged"]) == conv
 
@@ -115,13 +144,14 @@ def test_unweighted_and_er(gpu, abi, orc):
 @pytest.mark.parametrize("light_max,chunk", [(0, 0), (4, 64), (64, 128), (1 << 20, 1 << 16)])
 def test_degree_binning_thresholds(gpu, abi, orc, light_max, chunk):
     """Every work split (light rows only, tiny chunks, huge chunks) gives the
-    same distances; hub rows span many chunks."""
+    same distances, push or pull; hub rows span many chunks."""
     hg = rmat(abi, 14, 6, wmax=255)
     g = abi.DeviceGraph.from_host(hg)
     dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0)
     for mode in (0, 1):
-        d, st = g.sssp(0, mode=mode, light_max_degree=light_max, heavy_chunk=chunk)
-        assert np.array_equal(d, dj)
+        for direction in (1, 2):
+            d, st = g.sssp(0, mode=mode, light_max_degree=light_max, heavy_chunk=chunk, direction=direction)
+            assert np.array_equal(d, dj), (mode, direction)
 
 
 def test_star_hub_and_edge_cases(gpu, abi, orc):
@@ -133,8 +163,9 @@ def test_star_hub_and_edge_cases(gpu, abi, orc):
     g = abi.DeviceGraph(off, col, w.astype(np.uint32))
     dj = orc.sssp_dijkstra(off, col, w.astype(np.uint32), 0)
     ref, it, _ = orc.sssp_jacobi_f32(off, col, w.astype(np.float32), 0)
-    d, st = g.sssp(0, mode=0)
-    assert np.array_equal(d, dj) and st["rounds"] == it
+    for direction in (1, 2):
+        d, st = g.sssp(0, mode=0, direction=direction)
+        assert np.array_equal(d, dj) and st["rounds"] == it
     # source with no out-edges: only itself reached, one quiet round
     d, st = g.sssp(5999)
     assert d[5999] == 0 and np.all(np.delete(d, 5999) == 0xFFFFFFFF) and st["rounds"] == 1
