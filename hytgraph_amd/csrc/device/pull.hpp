@@ -117,11 +117,21 @@ namespace hyt
             if (lane == 0)
                 atomicAdd(&ctr->edges, static_cast<unsigned long long>(h.count));
             std::uint32_t acc = op.identity();
-            for (std::uint32_t e = static_cast<std::uint32_t>(lane); e < h.count; e += kWave)
+            for (std::uint32_t base = 0; base < h.count; base += kWave * kUnroll)
             {
-                std::uint32_t c2;
-                if (op.edge(h.begin + e, c2))
-                    acc = op.fold(acc, c2);
+                std::uint32_t cc[kUnroll];
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const std::uint32_t e = base + static_cast<std::uint32_t>(k * kWave + lane);
+                    cc[k] = op.identity();
+                    std::uint32_t c2;
+                    if (e < h.count && op.edge(h.begin + e, c2))
+                        cc[k] = c2;
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                    acc = op.fold(acc, cc[k]);
             }
 #pragma unroll
             for (int d = kWave / 2; d > 0; d >>= 1)
