@@ -94,6 +94,35 @@ class RunStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class CostParams(C.Structure):
+    _fields_ = [
+        ("alpha", C.c_double), ("beta", C.c_double), ("gamma", C.c_double),
+        ("request_bytes", C.c_uint64), ("requests_per_tlp", C.c_uint64),
+        ("rtt", C.c_double), ("cpu_bytes_per_second", C.c_double),
+        ("entry_bytes", C.c_uint64), ("index_bytes", C.c_uint64), ("alignment_bytes", C.c_uint64),
+    ]
+
+
+class Partition(C.Structure):
+    _fields_ = [("vertex_begin", C.c_uint32), ("vertex_end", C.c_uint32),
+                ("edge_begin", C.c_uint64), ("edge_end", C.c_uint64)]
+
+
+class PartitionMetrics(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("active_vertices", "active_edges", "total_edges", "filter_bytes",
+                                         "compaction_bytes", "zc_requests", "zc_alignment", "zc_total_requests")]
+
+
+class PartitionCosts(C.Structure):
+    _fields_ = [("filter", C.c_double), ("compaction", C.c_double), ("zero_copy", C.c_double),
+                ("zc_rtt", C.c_double), ("engine", C.c_int32), ("pad", C.c_int32)]
+
+
+HYT_ENGINE_FILTER = 0
+HYT_ENGINE_COMPACTION = 1
+HYT_ENGINE_ZERO_COPY = 2
+
+
 class HytError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"[status {status}] {message}")
@@ -137,6 +166,14 @@ SIGNATURES = {
         C.c_int,
         [_P, _P, C.c_uint32, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
     ),
+    "hyt_cost_params_default": (C.c_int, [C.POINTER(CostParams)]),
+    "hyt_partition_graph": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, C.c_uint64, _u64p]),
+    "hyt_partition_metrics_host": (
+        C.c_int, [_P, C.c_uint64, _P, _P, C.c_uint64, C.POINTER(CostParams), _P]),
+    "hyt_cost_evaluate": (C.c_int, [C.POINTER(CostParams), _P, C.c_uint64, _P]),
+    "hyt_select_engine": (
+        C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)]),
+    "hyt_combine_tasks": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, _u64p]),
 }
 
 _lib = None
@@ -309,3 +346,63 @@ class DeviceGraph:
         out = np.empty((self.V, K) if K > 1 else self.V, dtype=np.float32)
         check(load().hyt_ppr(self._h, _ptr(sd), K, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
         return out, st.as_dict()
+
+
+# ------------------------------------------------------------ transfer planning
+def cost_params(**overrides) -> CostParams:
+    p = CostParams()
+    check(load().hyt_cost_params_default(C.byref(p)))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def partition_graph(offsets, target_bytes) -> np.ndarray:
+    """Returns an (n, 4) u64 array of [vertex_begin, vertex_end, edge_begin, edge_end]."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    V = len(off) - 1
+    cap = max(V, 1)
+    out = (Partition * cap)()
+    n = C.c_uint64()
+    check(load().hyt_partition_graph(_ptr(off), V, target_bytes, out, cap, C.byref(n)))
+    return np.array([[p.vertex_begin, p.vertex_end, p.edge_begin, p.edge_end] for p in out[: n.value]],
+                    dtype=np.uint64).reshape(-1, 4)
+
+
+def partition_metrics(offsets, active, parts: np.ndarray, params: CostParams) -> np.ndarray:
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    act = np.ascontiguousarray(active, dtype=np.uint8)
+    n = len(parts)
+    pa = (Partition * max(n, 1))(*[Partition(int(a), int(b), int(c), int(d)) for a, b, c, d in parts])
+    out = (PartitionMetrics * max(n, 1))()
+    check(load().hyt_partition_metrics_host(_ptr(off), len(off) - 1, _ptr(act), pa, n, C.byref(params), out))
+    return np.array([[getattr(m, f) for f, _ in PartitionMetrics._fields_] for m in out[:n]],
+                    dtype=np.uint64).reshape(-1, 8)
+
+
+def cost_evaluate(metrics: np.ndarray, params: CostParams):
+    n = len(metrics)
+    ma = (PartitionMetrics * max(n, 1))(*[PartitionMetrics(*map(int, row)) for row in metrics])
+    out = (PartitionCosts * max(n, 1))()
+    check(load().hyt_cost_evaluate(C.byref(params), ma, n, out))
+    costs = np.array([[c.filter, c.compaction, c.zero_copy, c.zc_rtt] for c in out[:n]]).reshape(-1, 4)
+    engines = np.array([c.engine for c in out[:n]], dtype=np.int32)
+    return costs, engines
+
+
+def select_engine(f, c, z, alpha, beta) -> int:
+    e = C.c_int32()
+    check(load().hyt_select_engine(f, c, z, alpha, beta, C.byref(e)))
+    return e.value
+
+
+def combine_tasks(engines, k=4):
+    """Returns [(engine, [members...]), ...] in task order."""
+    e = np.ascontiguousarray(engines, dtype=np.int32)
+    n = len(e)
+    te = np.zeros(max(n, 1), np.int32)
+    to = np.zeros(n + 1, np.uint64)
+    mem = np.zeros(max(n, 1), np.uint64)
+    nt = C.c_uint64()
+    check(load().hyt_combine_tasks(_ptr(e), n, k, _ptr(te), _ptr(to), _ptr(mem), C.byref(nt)))
+    return [(int(te[t]), mem[int(to[t]):int(to[t + 1])].tolist()) for t in range(nt.value)]

@@ -92,7 +92,7 @@ def build_cpp(debug: bool = False) -> Path:
     out = LIBDIR / CPPLIB
     tmp = LIBDIR / (CPPLIB + ".tmp")
     cmd = [cxx, "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-pthread", *opt,
-           f"-I{ROOT / 'include'}", *map(str, srcs), f"-L{LIBDIR}", "-lhytgraph_amd",
+           f"-I{ROOT / 'include'}", f"-I{CSRC / 'host'}", *map(str, srcs), f"-L{LIBDIR}", "-lhytgraph_amd",
            "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

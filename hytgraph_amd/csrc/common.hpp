@@ -47,6 +47,16 @@ namespace hyt
             set_last_error(std::string(where) + ": " + e.what());
             return e.status;
         }
+        catch (const std::invalid_argument &e)
+        {
+            set_last_error(std::string(where) + ": " + e.what());
+            return HYT_ERR_INVALID_ARGUMENT;
+        }
+        catch (const std::overflow_error &e)
+        {
+            set_last_error(std::string(where) + ": " + e.what());
+            return HYT_ERR_OVERFLOW;
+        }
         catch (const std::bad_alloc &)
         {
             set_last_error(std::string(where) + ": host allocation failed");

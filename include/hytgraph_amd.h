@@ -185,6 +185,71 @@ int hyt_ppr(hyt_graph_t g, const uint32_t *seeds, uint32_t num_seeds,
             float damping, float epsilon, float *out,
             const hyt_run_options *opts, hyt_run_stats *stats);
 
+/* ---------------------------------------------- hybrid transfer planning */
+/*
+ * The HyTM planning steps of the original project as plain functions over
+ * arrays (src/graph/partition.cpp:105-174, src/transfer/hytm_cost_model.cpp:
+ * 233-715, src/scheduling/task_combiner.cpp:67-248). The device transfer
+ * engine runs the same code on counts it gathers on the GPU.
+ */
+enum hyt_engine { HYT_ENGINE_FILTER = 0, HYT_ENGINE_COMPACTION = 1, HYT_ENGINE_ZERO_COPY = 2 };
+
+typedef struct hyt_cost_params {
+    double alpha, beta, gamma;       /* selector thresholds, zero-copy RTT mix */
+    uint64_t request_bytes;          /* m  (bytes per memory request) */
+    uint64_t requests_per_tlp;       /* MR (outstanding requests per TLP) */
+    double rtt;                      /* saturated round trip, seconds or any unit */
+    double cpu_bytes_per_second;     /* host compaction throughput */
+    uint64_t entry_bytes;            /* d1: bytes per transferred edge entry */
+    uint64_t index_bytes;            /* d2: bytes per compacted vertex index entry */
+    uint64_t alignment_bytes;        /* zero-copy alignment boundary */
+} hyt_cost_params;
+
+typedef struct hyt_partition {
+    uint32_t vertex_begin, vertex_end;   /* [begin, end) */
+    uint64_t edge_begin, edge_end;       /* [begin, end) */
+} hyt_partition;
+
+typedef struct hyt_partition_metrics {
+    uint64_t active_vertices, active_edges, total_edges;
+    uint64_t filter_bytes, compaction_bytes;
+    uint64_t zc_requests, zc_alignment, zc_total_requests;
+} hyt_partition_metrics;
+
+typedef struct hyt_partition_costs {
+    double filter, compaction, zero_copy, zc_rtt;
+    int32_t engine;                      /* hyt_engine */
+    int32_t pad;
+} hyt_partition_costs;
+
+/* The original's defaults (alpha .8, beta .4, gamma .625, m 128, MR 256,
+ * rtt 1, cpu 1 B/s, d1 4, d2 8, align 128). */
+int hyt_cost_params_default(hyt_cost_params *out);
+
+/* Greedy contiguous partitioning: close the range before vertex v when its
+ * column bytes (4 B per edge) would exceed target_bytes; a lone oversized
+ * row stays whole. *count receives the number written (<= capacity). */
+int hyt_partition_graph(const uint64_t *row_offsets, uint64_t num_vertices, uint64_t target_bytes,
+                        hyt_partition *out, uint64_t capacity, uint64_t *count);
+
+/* Host metric collection for a per-vertex activity byte array. */
+int hyt_partition_metrics_host(const uint64_t *row_offsets, uint64_t num_vertices, const uint8_t *active,
+                               const hyt_partition *parts, uint64_t n, const hyt_cost_params *params,
+                               hyt_partition_metrics *out);
+
+/* Cost equations (1)-(3) and the strict selector, per partition. */
+int hyt_cost_evaluate(const hyt_cost_params *params, const hyt_partition_metrics *metrics, uint64_t n,
+                      hyt_partition_costs *out);
+int hyt_select_engine(double filter_cost, double compaction_cost, double zero_copy_cost, double alpha,
+                      double beta, int32_t *engine);
+
+/* Task combination: filter runs of <= k consecutive partitions, one task of
+ * all compaction partitions, one of all zero-copy partitions, ordered by first
+ * partition. Task t has engine task_engine[t] and members
+ * members[task_offset[t] .. task_offset[t+1]). Arrays hold n (+1) entries. */
+int hyt_combine_tasks(const int32_t *engines, uint64_t n, uint64_t k, int32_t *task_engine,
+                      uint64_t *task_offset, uint64_t *members, uint64_t *num_tasks);
+
 #ifdef __cplusplus
 }
 #endif

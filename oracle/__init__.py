@@ -66,6 +66,9 @@ def _load_ref():
         lib = C.CDLL(str(REF_LIB))
         lib.ref_sssp_cpu.argtypes = [_u64, _P, _P, _P, C.c_uint32, C.c_float, _u64, _P, _P, _P]
         lib.ref_pagerank_cpu.argtypes = [_u64, _P, _P, C.c_float, C.c_float, _u64, _P, _P, _P, _P]
+        lib.ref_partition.argtypes = [_u64, _P, _P, _u64, _P, _P, _P, _P, _u64, _P]
+        lib.ref_cost_evaluate.argtypes = [_u64, _P, _P, _P, _u64, _P, _P, _P, _P, _P, _u64, _P]
+        lib.ref_combine.argtypes = [_P, _u64, _u64, _P, _P, _P, _P]
         _ref = lib
     return _ref
 
@@ -153,3 +156,47 @@ def f32_to_u32_dist(d: np.ndarray) -> np.ndarray:
     fin = np.isfinite(d)
     out[fin] = d[fin].astype(np.uint32)
     return out
+
+
+# ----------------------------------------------- original planning code (via _ref)
+def ref_partition(off, col, target_bytes):
+    """The original LogicalPartitioner: (n, 4) u64 [vb, ve, eb, ee]."""
+    off, col = _arr(off, np.uint64), _arr(col, np.uint32)
+    V = len(off) - 1
+    cap = max(V, 1)
+    vb, ve = np.zeros(cap, np.uint32), np.zeros(cap, np.uint32)
+    eb, ee = np.zeros(cap, np.uint64), np.zeros(cap, np.uint64)
+    n = C.c_uint64()
+    _check(_load_ref().ref_partition(V, _p(off), _p(col), target_bytes, _p(vb), _p(ve), _p(eb), _p(ee), cap,
+                                     C.byref(n)), "ref_partition")
+    k = n.value
+    return np.stack([vb[:k].astype(np.uint64), ve[:k].astype(np.uint64), eb[:k], ee[:k]], axis=1)
+
+
+def ref_cost_evaluate(off, col, active, target_bytes, alpha, beta, gamma, rtt, cpu, m, mr, d1, d2, align):
+    """The original HyTMCostModel over its own partitions: (metrics (n,8), costs (n,4), engines (n))."""
+    off, col = _arr(off, np.uint64), _arr(col, np.uint32)
+    act = _arr(active, np.uint8)
+    V = len(off) - 1
+    cap = max(V, 1)
+    dp = np.array([alpha, beta, gamma, rtt, cpu], np.float64)
+    up = np.array([m, mr, d1, d2, align], np.uint64)
+    metrics = np.zeros((cap, 8), np.uint64)
+    costs = np.zeros((cap, 4), np.float64)
+    engines = np.zeros(cap, np.int32)
+    n = C.c_uint64()
+    _check(_load_ref().ref_cost_evaluate(V, _p(off), _p(col), _p(act), target_bytes, _p(dp), _p(up), _p(metrics),
+                                         _p(costs), _p(engines), cap, C.byref(n)), "ref_cost_evaluate")
+    k = n.value
+    return metrics[:k], costs[:k], engines[:k]
+
+
+def ref_combine(engines, k=4):
+    e = _arr(engines, np.int32)
+    n = len(e)
+    te = np.zeros(max(n, 1), np.int32)
+    to = np.zeros(n + 1, np.uint64)
+    mem = np.zeros(max(n, 1), np.uint64)
+    nt = C.c_uint64()
+    _check(_load_ref().ref_combine(_p(e), n, k, _p(te), _p(to), _p(mem), C.byref(nt)), "ref_combine")
+    return [(int(te[t]), mem[int(to[t]):int(to[t + 1])].tolist()) for t in range(nt.value)]

@@ -4,10 +4,17 @@
 // oracle restatement and as the cpu_baseline of bench.py ("reference" kind).
 //
 // Wraps: sssp_cpu (src/algorithms/sssp.cpp:109), pagerank_cpu
-// (src/algorithms/pagerank.cpp:89), via CSRGraph (src/graph/csr_graph.cpp:11).
+// (src/algorithms/pagerank.cpp:89), via CSRGraph (src/graph/csr_graph.cpp:11);
+// LogicalPartitioner::partition (src/graph/partition.cpp:105),
+// HyTMCostModel::evaluate_partitions (src/transfer/hytm_cost_model.cpp:617),
+// TaskCombiner::combine (src/scheduling/task_combiner.cpp:67).
 #include "algorithms/pagerank.hpp"
 #include "algorithms/sssp.hpp"
+#include "graph/activity_tracker.hpp"
 #include "graph/csr_graph.hpp"
+#include "graph/partition.hpp"
+#include "scheduling/task_combiner.hpp"
+#include "transfer/hytm_cost_model.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -88,6 +95,119 @@ int ref_pagerank_cpu(std::uint64_t V, const std::uint64_t *off, const std::uint3
         *iterations = r.iterations;
         *residual = r.residual;
         *converged = r.converged ? 1 : 0;
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+// Partitions: out arrays of capacity cap: vb, ve (u32), eb, ee (u64).
+int ref_partition(std::uint64_t V, const std::uint64_t *off, const std::uint32_t *col, std::uint64_t bytes,
+                  std::uint32_t *vb, std::uint32_t *ve, std::uint64_t *eb, std::uint64_t *ee, std::uint64_t cap,
+                  std::uint64_t *count)
+{
+    try
+    {
+        const CSRGraph g = make_graph(V, off, col, nullptr);
+        const auto parts = hytgraph::graph::LogicalPartitioner(bytes).partition(g);
+        *count = parts.size();
+        if (parts.size() > cap)
+            return 7;
+        for (std::size_t i = 0; i < parts.size(); ++i)
+        {
+            vb[i] = parts[i].vertex_begin();
+            ve[i] = parts[i].vertex_end();
+            eb[i] = parts[i].edge_begin();
+            ee[i] = parts[i].edge_end();
+        }
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+// Cost evaluation of the partitions of `bytes` for an activity byte array.
+// params: alpha beta gamma rtt cpu (doubles) ; m MR d1 d2 align (u64).
+// out per partition: 8 u64 metrics + 4 doubles (f c z rtt_zc) + engine (i32).
+int ref_cost_evaluate(std::uint64_t V, const std::uint64_t *off, const std::uint32_t *col, const std::uint8_t *active,
+                      std::uint64_t bytes, const double *dp, const std::uint64_t *up, std::uint64_t *metrics,
+                      double *costs, std::int32_t *engines, std::uint64_t cap, std::uint64_t *count)
+{
+    try
+    {
+        const CSRGraph g = make_graph(V, off, col, nullptr);
+        const auto parts = hytgraph::graph::LogicalPartitioner(bytes).partition(g);
+        hytgraph::graph::ActivityTracker act(V);
+        for (std::uint64_t v = 0; v < V; ++v)
+            if (active[v])
+                act.set_active(static_cast<std::uint32_t>(v));
+        hytgraph::transfer::HyTMCostModelOptions o;
+        o.alpha = dp[0];
+        o.beta = dp[1];
+        o.gamma = dp[2];
+        o.rtt = dp[3];
+        o.cpu_compaction_throughput_bytes_per_second = dp[4];
+        o.request_payload_bytes = up[0];
+        o.max_requests_per_tlp = up[1];
+        o.destination_entry_bytes = up[2];
+        o.vertex_index_bytes = up[3];
+        o.alignment_bytes = up[4];
+        const auto res = hytgraph::transfer::HyTMCostModel(o).evaluate_partitions(g, parts, act);
+        *count = res.size();
+        if (res.size() > cap)
+            return 7;
+        for (std::size_t i = 0; i < res.size(); ++i)
+        {
+            const auto &m = res[i].metrics;
+            std::uint64_t *mo = metrics + 8 * i;
+            mo[0] = m.active_vertices;
+            mo[1] = m.active_edges;
+            mo[2] = m.total_edges;
+            mo[3] = m.filter_transfer_bytes;
+            mo[4] = m.compaction_bytes;
+            mo[5] = m.zero_copy_memory_requests;
+            mo[6] = m.zero_copy_alignment_overhead;
+            mo[7] = m.zero_copy_total_requests;
+            costs[4 * i + 0] = res[i].filter_cost;
+            costs[4 * i + 1] = res[i].compaction_cost;
+            costs[4 * i + 2] = res[i].zero_copy_cost;
+            costs[4 * i + 3] = res[i].zero_copy_rtt;
+            engines[i] = static_cast<std::int32_t>(res[i].selected_engine);
+        }
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+// Task combination; engines 0 filter, 1 compaction, 2 zero-copy.
+int ref_combine(const std::int32_t *engines, std::uint64_t n, std::uint64_t k, std::int32_t *task_engine,
+                std::uint64_t *task_offset, std::uint64_t *members, std::uint64_t *num_tasks)
+{
+    try
+    {
+        std::vector<hytgraph::transfer::TransferEngine> e(n);
+        for (std::uint64_t i = 0; i < n; ++i)
+            e[i] = static_cast<hytgraph::transfer::TransferEngine>(engines[i]);
+        hytgraph::scheduling::TaskCombinerOptions o;
+        o.filter_combine_k = k;
+        const auto plan = hytgraph::scheduling::TaskCombiner(o).combine(e);
+        *num_tasks = plan.tasks.size();
+        std::uint64_t at = 0;
+        for (std::size_t t = 0; t < plan.tasks.size(); ++t)
+        {
+            task_engine[t] = static_cast<std::int32_t>(plan.tasks[t].engine);
+            task_offset[t] = at;
+            for (auto m : plan.tasks[t].partition_indices)
+                members[at++] = m;
+        }
+        task_offset[plan.tasks.size()] = at;
         return 0;
     }
     catch (...)
