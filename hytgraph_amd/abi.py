@@ -118,6 +118,30 @@ class PartitionCosts(C.Structure):
                 ("zc_rtt", C.c_double), ("engine", C.c_int32), ("pad", C.c_int32)]
 
 
+class HybridConfig(C.Structure):
+    _fields_ = [
+        ("device_budget_bytes", C.c_uint64), ("partition_edges", C.c_uint64), ("pack_edges", C.c_uint64),
+        ("filter_combine_k", C.c_uint32), ("force_engine", C.c_int32), ("host_threads", C.c_uint32),
+        ("calibrate", C.c_uint32), ("cache", C.c_uint32), ("reserved", C.c_uint32), ("cost", CostParams),
+    ]
+
+
+class HybridStats(C.Structure):
+    _fields_ = [
+        ("run", RunStats), ("partitions", C.c_uint64), ("parts_cached", C.c_uint64),
+        ("parts_filter", C.c_uint64), ("parts_compaction", C.c_uint64), ("parts_zero_copy", C.c_uint64),
+        ("tasks", C.c_uint64), ("bytes_filter", C.c_uint64), ("bytes_compaction", C.c_uint64),
+        ("bytes_zero_copy", C.c_uint64), ("copy_ms", C.c_double), ("pack_ms", C.c_double), ("plan_ms", C.c_double),
+        ("link_gbs_peak", C.c_double), ("cpu_pack_gbs", C.c_double), ("zero_copy_gbs", C.c_double),
+        ("device_edge_bytes", C.c_uint64), ("device_state_bytes", C.c_uint64),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "run"}
+        d["run"] = self.run.as_dict()
+        return d
+
+
 HYT_ENGINE_FILTER = 0
 HYT_ENGINE_COMPACTION = 1
 HYT_ENGINE_ZERO_COPY = 2
@@ -174,6 +198,13 @@ SIGNATURES = {
     "hyt_select_engine": (
         C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)]),
     "hyt_combine_tasks": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, _u64p]),
+    "hyt_hybrid_config_default": (C.c_int, [C.POINTER(HybridConfig)]),
+    "hyt_hybrid_create": (C.c_int, [_P, C.c_uint32, C.POINTER(HybridConfig), C.POINTER(_P)]),
+    "hyt_hybrid_free": (C.c_int, [_P]),
+    "hyt_hybrid_info": (C.c_int, [_P, _u64p, C.POINTER(CostParams), C.POINTER(HybridStats)]),
+    "hyt_hybrid_sssp": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
+    "hyt_hybrid_pagerank": (
+        C.c_int, [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
 }
 
 _lib = None
@@ -406,3 +437,63 @@ def combine_tasks(engines, k=4):
     nt = C.c_uint64()
     check(load().hyt_combine_tasks(_ptr(e), n, k, _ptr(te), _ptr(to), _ptr(mem), C.byref(nt)))
     return [(int(te[t]), mem[int(to[t]):int(to[t + 1])].tolist()) for t in range(nt.value)]
+
+
+# ------------------------------------------------------------ hybrid engine
+def hybrid_config(**overrides) -> HybridConfig:
+    c = HybridConfig()
+    check(load().hyt_hybrid_config_default(C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class HybridEngine:
+    """Out-of-HBM traversal over a pinned HostGraph (kept alive by this object)."""
+
+    def __init__(self, host: HostGraph, weights_f32=False, **config):
+        self.host = host
+        self.V = host.V
+        cfg = hybrid_config(**config)
+        h = C.c_void_p()
+        check(load().hyt_hybrid_create(host._h, 1 if weights_f32 else 0, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def info(self) -> dict:
+        n, p, st = C.c_uint64(), CostParams(), HybridStats()
+        check(load().hyt_hybrid_info(self._h, C.byref(n), C.byref(p), C.byref(st)))
+        d = st.as_dict()
+        d["params"] = {f: getattr(p, f) for f, _ in CostParams._fields_}
+        return d
+
+    def sssp(self, source, out_ptr=None, **opt):
+        o = _options(mode=HYT_SSSP_ASYNC, out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = HybridStats()
+        if out_ptr is not None:
+            check(load().hyt_hybrid_sssp(self._h, int(source), C.c_void_p(out_ptr), C.byref(o), C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty(self.V, np.uint32)
+        check(load().hyt_hybrid_sssp(self._h, int(source), _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    def pagerank(self, damping=0.85, epsilon=1e-7, out_ptr=None, **opt):
+        o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = HybridStats()
+        if out_ptr is not None:
+            check(load().hyt_hybrid_pagerank(self._h, damping, epsilon, C.c_void_p(out_ptr), C.byref(o),
+                                             C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty(self.V, np.float32)
+        check(load().hyt_hybrid_pagerank(self._h, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(load().hyt_hybrid_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

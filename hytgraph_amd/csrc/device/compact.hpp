@@ -36,9 +36,14 @@ namespace hyt
         return static_cast<unsigned>(b == 0 ? 1 : b);
     }
 
+    // block_counts == nullptr: rows go straight to q at a position reserved
+    // with one atomic per workgroup. Otherwise (ordered mode) workgroup b
+    // writes its rows at q[b * kCompactRowsPerBlock ...] and its count to
+    // block_counts[b]; gather_ordered() then packs them in vertex order.
     template <class Val, class OffT, class Sel>
     __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
-                                                           Counters *ctr, const OffT *off, std::uint16_t *active)
+                                                           Counters *ctr, const OffT *off, std::uint16_t *active,
+                                                           std::uint32_t *block_counts = nullptr)
     {
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
         __shared__ std::uint32_t s_base;
@@ -100,7 +105,15 @@ namespace hyt
             total += s_wsum[w];
         }
         if (threadIdx.x == 0)
-            s_base = total ? atomicAdd(&ctr->next_count, total) : 0U;
+        {
+            if (block_counts != nullptr)
+            {
+                block_counts[blockIdx.x] = total;
+                s_base = static_cast<std::uint32_t>(blockIdx.x * kCompactRowsPerBlock);
+            }
+            else
+                s_base = total ? atomicAdd(&ctr->next_count, total) : 0U;
+        }
         __syncthreads();
         std::uint32_t pos = s_base + wbase + incl - n;
 #pragma unroll
@@ -117,5 +130,27 @@ namespace hyt
                 }
             pos += static_cast<std::uint32_t>(__popc(gm[g]));
         }
+    }
+    // Ordered mode, second pass: workgroup b copies its block_counts[b] rows
+    // from the staging queue to q[block_offsets[b] ...]; the total lands in
+    // ctr->next_count.
+    template <class Val>
+    __global__ __launch_bounds__(kBlock) void gather_ordered(const std::uint32_t *stage_q, const Val *stage_val,
+                                                             const std::uint32_t *block_counts,
+                                                             const std::uint32_t *block_offsets, unsigned nblocks,
+                                                             std::uint32_t *q, Val *qval, Counters *ctr)
+    {
+        const unsigned b = blockIdx.x;
+        const std::uint32_t n = block_counts[b];
+        const std::uint32_t dst = block_offsets[b];
+        const std::uint64_t src = static_cast<std::uint64_t>(b) * kCompactRowsPerBlock;
+        for (std::uint32_t k = threadIdx.x; k < n; k += kBlock)
+        {
+            q[dst + k] = stage_q[src + k];
+            if (qval)
+                qval[dst + k] = stage_val[src + k];
+        }
+        if (b == nblocks - 1 && threadIdx.x == 0)
+            ctr->next_count = dst + n;
     }
 } // namespace hyt

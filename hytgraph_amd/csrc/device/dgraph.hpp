@@ -108,6 +108,10 @@ namespace hyt
     // Spare slots after every [V] workspace array (vector loads past the end).
     constexpr std::uint64_t kPad = 16;
 
+    // x = rank + residual, normalised to sum 1, into out (host or device);
+    // st.residual = unpushed mass / total.
+    void finalize_ranks(DeviceGraph &g, float *out, bool out_on_device, hyt_run_stats &st);
+
     // Builds the incoming CSR and the static chunk list of its heavy rows.
     void ensure_transpose(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
 } // namespace hyt

@@ -19,15 +19,56 @@
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
 //   void apply(uint32_t v, uint32_t c)                 update destination v
 // load returns false to drop the edge (e.g. u32 overflow, flagged by Op).
+//
+// RowSrc says where a frontier row's edges are: GraphRows reads the CSR
+// offsets and clips them to an edge window [lo, hi) (positions relative to
+// lo: the whole graph in HBM, one partition in a device slot, or the mapped
+// host graph for zero-copy); PackedRows reads host-compacted rows. Op's
+// column/weight pointers address the same window.
 #pragma once
 
 #include "kern_common.hpp"
 
 namespace hyt
 {
+    template <class OffT>
+    struct GraphRows
+    {
+        const OffT *off;
+        const std::uint32_t *q;
+        std::uint64_t lo, hi; // edge window
+
+        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint64_t &begin, std::uint64_t &count,
+                            std::uint64_t &full) const
+        {
+            u = q[i];
+            const std::uint64_t b = off[u], e = off[u + 1];
+            full = e - b;
+            const std::uint64_t cb = b > lo ? b : lo;
+            const std::uint64_t ce = e < hi ? e : hi;
+            count = ce > cb ? ce - cb : 0;
+            begin = count ? cb - lo : 0;
+        }
+    };
+
+    struct PackedRows
+    {
+        const std::uint32_t *rows;  // vertex of packed row i
+        const std::uint64_t *index; // packed row i spans [index[i], index[i+1])
+
+        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint64_t &begin, std::uint64_t &count,
+                            std::uint64_t &full) const
+        {
+            u = rows[i];
+            begin = index[i];
+            count = index[i + 1] - begin;
+            full = count;
+        }
+    };
+
     struct ExpandArgs
     {
-        const std::uint32_t *q;   // frontier rows
+        const std::uint32_t *q;   // frontier rows (GraphRows reads them through its own pointer)
         std::uint32_t F;          // their number
         Counters *ctr;
         HeavyChunk *heavy;
@@ -94,11 +135,11 @@ namespace hyt
             heavy[slot + c] = make(c);
     }
 
-    template <class OffT, class Op>
-    __global__ __launch_bounds__(kBlock) void expand_light(const OffT *__restrict__ off, Op op, ExpandArgs a)
+    template <class RowSrc, class Op>
+    __global__ __launch_bounds__(kBlock) void expand_light(RowSrc rs, Op op, ExpandArgs a)
     {
         __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
-        __shared__ OffT s_begin[kWavesPerBlock][kWave];
+        __shared__ std::uint64_t s_begin[kWavesPerBlock][kWave];
         __shared__ std::uint32_t s_val[kWavesPerBlock][kWave];
 
         const int lane = lane_id();
@@ -106,25 +147,24 @@ namespace hyt
         const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 
         std::uint32_t deg = 0, value = 0, nch = 0;
-        OffT begin = 0, full = 0;
+        std::uint64_t begin = 0, count = 0;
         if (i < a.F)
         {
-            const std::uint32_t u = a.q[i];
-            begin = off[u];
-            full = off[u + 1] - begin;
-            value = op.row_value(i, u, static_cast<std::uint64_t>(full));
-            if (full > a.light_max)
-                nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
+            std::uint32_t u;
+            std::uint64_t full;
+            rs.get(i, u, begin, count, full);
+            value = op.row_value(i, u, full);
+            if (count > a.light_max)
+                nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
             else
-                deg = static_cast<std::uint32_t>(full);
+                deg = static_cast<std::uint32_t>(count);
         }
         queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
             const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
-            const std::uint64_t full64 = static_cast<std::uint64_t>(full);
             HeavyChunk h;
             h.value = value;
-            h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
-            h.begin = static_cast<std::uint64_t>(begin) + lo;
+            h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
+            h.begin = begin + lo;
             return h;
         });
 
@@ -140,13 +180,20 @@ namespace hyt
             atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
 
         const std::uint32_t *ex = s_ex[wid];
-        const OffT *bg = s_begin[wid];
+        const std::uint64_t *bg = s_begin[wid];
         const std::uint32_t *vl = s_val[wid];
         walk_edges(op, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
             const int j = owner_of(ex, e);
-            pos = static_cast<std::uint64_t>(bg[j]) + (e - ex[j]);
+            pos = bg[j] + (e - ex[j]);
             val = vl[j];
         });
+    }
+
+    // The whole graph resident in HBM: window [0, 2^64).
+    template <class OffT>
+    __host__ __device__ inline GraphRows<OffT> whole_graph(const OffT *off, const std::uint32_t *q)
+    {
+        return GraphRows<OffT>{off, q, 0, ~0ULL};
     }
 
     template <class Op>

@@ -1,0 +1,240 @@
+// Per-algorithm parts of the traversal kernels (row values, edge loads,
+// destination updates, compaction selectors), shared by the in-HBM drivers
+// and the hybrid transfer engine.
+#pragma once
+
+#include "compact.hpp"
+#include "expand.hpp"
+
+namespace hyt
+{
+    template <int WM>
+    struct Weights;
+
+    template <>
+    struct Weights<kWU32>
+    {
+        static constexpr std::uint32_t kInf = 0xFFFFFFFFU;
+        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+        __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
+        {
+            const std::uint64_t s = static_cast<std::uint64_t>(d) + w;
+            out = static_cast<std::uint32_t>(s);
+            return s < 0xFFFFFFFFULL;
+        }
+    };
+    template <>
+    struct Weights<kWUnitU32> : Weights<kWU32>
+    {
+        __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 1U; }
+    };
+    template <>
+    struct Weights<kWF32>
+    {
+        static constexpr std::uint32_t kInf = 0x7F800000U;
+        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+        __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
+        {
+            out = __float_as_uint(__fadd_rn(__uint_as_float(d), __uint_as_float(w)));
+            return true;
+        }
+    };
+    template <>
+    struct Weights<kWUnitF32> : Weights<kWF32>
+    {
+        __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 0x3F800000U; }
+    };
+
+    template <int WM, bool SYNC>
+    struct SsspOp
+    {
+        const std::uint32_t *__restrict__ col;
+        const std::uint32_t *__restrict__ w;
+        std::uint32_t *dist;
+        std::uint8_t *touch;
+        const std::uint32_t *qd; // snapshot values (SYNC)
+        std::uint32_t *overflow;
+
+        __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t) const
+        {
+            return SYNC ? qd[i] : dist[u];
+        }
+        __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
+        {
+            v = col[pos];
+            if (Weights<WM>::add(du, Weights<WM>::load(w, pos), c))
+                return true;
+            atomicOr(overflow, 1U);
+            return false;
+        }
+        __device__ void apply(std::uint32_t v, std::uint32_t c) const
+        {
+            // A plain load may return an older, larger value; that only
+            // costs an extra atomic, it never skips a needed one.
+            if (c < dist[v])
+            {
+                atomicMin(&dist[v], c);
+                touch[v] = 1;
+            }
+        }
+    };
+
+    // Touched rows that improved join the frontier when below `bound`
+    // (delta-stepping bucket; 0xFFFFFFFF = no bucketing); improved rows at
+    // or above the bound stay marked and are counted as pending.
+    struct SsspSel
+    {
+        struct Aux
+        {
+            std::uint32_t pending = 0;
+            std::uint32_t inv_max = 0;
+        };
+        std::uint8_t *touch;
+        const std::uint32_t *dist;
+        std::uint32_t *prev;
+        std::uint32_t bound;
+
+        __device__ void publish(const Aux &aux, Counters *ctr) const
+        {
+            if (bound == 0xFFFFFFFFU)
+                return;
+            std::uint32_t p = aux.pending, m = aux.inv_max;
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+            {
+                p += __shfl_xor(p, d, kWave);
+                const std::uint32_t o = __shfl_xor(m, d, kWave);
+                m = o > m ? o : m;
+            }
+            if (lane_id() == 0 && p)
+            {
+                atomicAdd(&ctr->pending, p);
+                atomicMax(&ctr->pending_inv_max, m);
+            }
+        }
+
+        __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
+                                         Aux &aux) const
+        {
+            // touch has spare bytes past V, so the 4-byte word is in range
+            std::uint32_t *tw = reinterpret_cast<std::uint32_t *>(touch + row0);
+            const std::uint32_t t = *tw;
+            if (t == 0)
+                return 0;
+            std::uint32_t keep = 0, m = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                if (static_cast<std::uint32_t>(j) < n && ((t >> (8 * j)) & 0xFFU))
+                {
+                    const std::uint64_t v = row0 + j;
+                    const std::uint32_t d = dist[v];
+                    if (d < prev[v])
+                    {
+                        if (d < bound)
+                        {
+                            prev[v] = d;
+                            vals[j] = d;
+                            m |= 1U << j;
+                        }
+                        else
+                        {
+                            keep |= 0xFFU << (8 * j);
+                            aux.pending += 1;
+                            aux.inv_max = (~d > aux.inv_max) ? ~d : aux.inv_max;
+                        }
+                    }
+                }
+            }
+            if (keep != t)
+                *tw = keep;
+            return m;
+        }
+    };
+
+    struct PrOp
+    {
+        const std::uint32_t *__restrict__ col;
+        float *res;
+        const std::uint32_t *qd; // snapshot residuals (bits)
+        float damping;
+
+        __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t, std::uint64_t deg) const
+        {
+            if (deg == 0)
+                return 0U;
+            return __float_as_uint(damping * __uint_as_float(qd[i]) / static_cast<float>(deg));
+        }
+        __device__ bool load(std::uint64_t pos, std::uint32_t share, std::uint32_t &v, std::uint32_t &c) const
+        {
+            v = col[pos];
+            c = share;
+            return true;
+        }
+        __device__ void apply(std::uint32_t v, std::uint32_t c) const
+        {
+            unsafeAtomicAdd(&res[v], __uint_as_float(c));
+        }
+    };
+
+    // Selects r_u > eps, moves r_u into rank_u and zeroes it, and records
+    // the per-edge share d * r_u / outdeg(u) by vertex for pull rounds.
+    template <class OffT>
+    struct PrSel
+    {
+        using Aux = NoAux;
+        float *res;
+        float *rank;
+        float *share;
+        const OffT *off;
+        float damping;
+        float eps;
+
+        __device__ void publish(const Aux &, Counters *) const {}
+        __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
+                                         Aux &) const
+        {
+            float r[4];
+            if (n == 4)
+            {
+                const float4 v4 = *reinterpret_cast<const float4 *>(res + row0);
+                r[0] = v4.x, r[1] = v4.y, r[2] = v4.z, r[3] = v4.w;
+            }
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0.0F;
+            }
+            std::uint32_t m = 0;
+            float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                if (r[j] > eps)
+                {
+                    const std::uint64_t v = row0 + j;
+                    res[v] = 0.0F;
+                    rank[v] += r[j];
+                    vals[j] = __float_as_uint(r[j]);
+                    const OffT deg = off[v + 1] - off[v];
+                    sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
+                    m |= 1U << j;
+                }
+            }
+            // every row's share is rewritten (0 outside the frontier), so a
+            // pull round needs no frontier test: one gather per in-edge
+            if (n == 4)
+                *reinterpret_cast<float4 *>(share + row0) = make_float4(sh[0], sh[1], sh[2], sh[3]);
+            else
+            {
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (static_cast<std::uint32_t>(j) < n)
+                        share[row0 + j] = sh[j];
+            }
+            return m;
+        }
+    };
+
+} // namespace hyt

@@ -17,8 +17,7 @@
 // frontier out-edges are a large share of E pull instead: each destination
 // sums d * r_u / outdeg(u) over in-edges from frontier sources in LDS and adds
 // the total to its residual with one store.
-#include "compact.hpp"
-#include "expand.hpp"
+#include "ops.hpp"
 #include "pull.hpp"
 
 #include <chrono>
@@ -28,91 +27,6 @@ namespace hyt
 {
     namespace
     {
-        struct PrOp
-        {
-            const std::uint32_t *__restrict__ col;
-            float *res;
-            const std::uint32_t *qd; // snapshot residuals (bits)
-            float damping;
-
-            __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t, std::uint64_t deg) const
-            {
-                if (deg == 0)
-                    return 0U;
-                return __float_as_uint(damping * __uint_as_float(qd[i]) / static_cast<float>(deg));
-            }
-            __device__ bool load(std::uint64_t pos, std::uint32_t share, std::uint32_t &v, std::uint32_t &c) const
-            {
-                v = col[pos];
-                c = share;
-                return true;
-            }
-            __device__ void apply(std::uint32_t v, std::uint32_t c) const
-            {
-                unsafeAtomicAdd(&res[v], __uint_as_float(c));
-            }
-        };
-
-        // Selects r_u > eps, moves r_u into rank_u and zeroes it, and records
-        // the per-edge share d * r_u / outdeg(u) by vertex for pull rounds.
-        template <class OffT>
-        struct PrSel
-        {
-            using Aux = NoAux;
-            float *res;
-            float *rank;
-            float *share;
-            const OffT *off;
-            float damping;
-            float eps;
-
-            __device__ void publish(const Aux &, Counters *) const {}
-            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
-                                             Aux &) const
-            {
-                float r[4];
-                if (n == 4)
-                {
-                    const float4 v4 = *reinterpret_cast<const float4 *>(res + row0);
-                    r[0] = v4.x, r[1] = v4.y, r[2] = v4.z, r[3] = v4.w;
-                }
-                else
-                {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0.0F;
-                }
-                std::uint32_t m = 0;
-                float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    if (r[j] > eps)
-                    {
-                        const std::uint64_t v = row0 + j;
-                        res[v] = 0.0F;
-                        rank[v] += r[j];
-                        vals[j] = __float_as_uint(r[j]);
-                        const OffT deg = off[v + 1] - off[v];
-                        sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
-                        m |= 1U << j;
-                    }
-                }
-                // every row's share is rewritten (0 outside the frontier), so a
-                // pull round needs no frontier test: one gather per in-edge
-                if (n == 4)
-                    *reinterpret_cast<float4 *>(share + row0) = make_float4(sh[0], sh[1], sh[2], sh[3]);
-                else
-                {
-#pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        if (static_cast<std::uint32_t>(j) < n)
-                            share[row0 + j] = sh[j];
-                }
-                return m;
-            }
-        };
-
         struct PrPullOp
         {
             const std::uint32_t *__restrict__ in_col;
@@ -307,7 +221,8 @@ namespace hyt
                 }
                 else
                 {
-                    expand_light<OffT, PrOp><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    expand_light<GraphRows<OffT>, PrOp><<<blocks_for(F), kBlock, 0, s>>>(whole_graph(g.off.as<OffT>(), a.q),
+                                                                                     op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_light");
                     expand_heavy<PrOp><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_heavy");
@@ -340,35 +255,36 @@ namespace hyt
             }
         }
 
-        void finalize(DeviceGraph &g, float *out, bool out_on_device, hyt_run_stats &st)
-        {
-            Workspace &ws = g.ws;
-            hipStream_t s = g.stream;
-            double *partial = ws.partial.as<double>();
-            std::uint64_t nb64 = (g.V + kBlock - 1) / kBlock;
-            const unsigned nb = static_cast<unsigned>(nb64 < kFoldBlocks ? (nb64 ? nb64 : 1) : kFoldBlocks);
-            pr_fold<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), ws.dist.as<float>(), g.V, partial);
-            hip_check(hipGetLastError(), "launch pr_fold");
-            pr_total<<<1, kBlock, 0, s>>>(partial, nb);
-            hip_check(hipGetLastError(), "launch pr_total");
-            double totals[2] = {0, 0};
-            hip_check(hipMemcpyAsync(totals, partial + 2 * nb, sizeof(totals), hipMemcpyDeviceToHost, s),
-                      "read totals");
-            float *dst = out_on_device ? out : ws.dist.as<float>();
-            pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), g.V, partial + 2 * nb, dst);
-            hip_check(hipGetLastError(), "launch pr_normalize");
-            if (!out_on_device)
-                hip_check(hipMemcpyAsync(out, dst, g.V * 4, hipMemcpyDeviceToHost, s), "copy ranks");
-            hip_check(hipStreamSynchronize(s), "pagerank output");
-            st.residual = totals[0] > 0 ? totals[1] / totals[0] : 0.0;
-        }
-
         void check_params(float damping, float eps)
         {
             require(damping >= 0.0F && damping < 1.0F, "PageRank damping_factor must be in [0, 1)");
             require(eps > 0.0F && std::isfinite(eps), "PageRank epsilon must be finite and positive");
         }
     } // namespace
+
+    void finalize_ranks(DeviceGraph &g, float *out, bool out_on_device, hyt_run_stats &st)
+    {
+        Workspace &ws = g.ws;
+        hipStream_t s = g.stream;
+        double *partial = ws.partial.as<double>();
+        std::uint64_t nb64 = (g.V + kBlock - 1) / kBlock;
+        const unsigned nb = static_cast<unsigned>(nb64 < kFoldBlocks ? (nb64 ? nb64 : 1) : kFoldBlocks);
+        pr_fold<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), ws.dist.as<float>(), g.V, partial);
+        hip_check(hipGetLastError(), "launch pr_fold");
+        pr_total<<<1, kBlock, 0, s>>>(partial, nb);
+        hip_check(hipGetLastError(), "launch pr_total");
+        double totals[2] = {0, 0};
+        hip_check(hipMemcpyAsync(totals, partial + 2 * nb, sizeof(totals), hipMemcpyDeviceToHost, s),
+                  "read totals");
+        float *dst = out_on_device ? out : ws.dist.as<float>();
+        pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), g.V, partial + 2 * nb, dst);
+        hip_check(hipGetLastError(), "launch pr_normalize");
+        if (!out_on_device)
+            hip_check(hipMemcpyAsync(out, dst, g.V * 4, hipMemcpyDeviceToHost, s), "copy ranks");
+        hip_check(hipStreamSynchronize(s), "pagerank output");
+        st.residual = totals[0] > 0 ? totals[1] / totals[0] : 0.0;
+    }
+
 
     // One delta-push solve; seed == UINT32_MAX means global PageRank.
     void pagerank_single(DeviceGraph &g, std::uint32_t seed, float damping, float eps, float *out,
@@ -400,7 +316,7 @@ namespace hyt
             run_push<std::uint64_t>(g, o, damping, eps, st, light_max, chunk);
         else
             run_push<std::uint32_t>(g, o, damping, eps, st, light_max, chunk);
-        finalize(g, out, o.out_on_device != 0, st);
+        finalize_ranks(g, out, o.out_on_device != 0, st);
     }
 
     void ppr_batched(DeviceGraph &g, const std::uint32_t *seeds, std::uint32_t K, float damping, float eps,

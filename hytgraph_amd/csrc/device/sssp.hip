@@ -25,8 +25,7 @@
 // Why touch[] cannot miss an improvement: the first atomicMin that lowers
 // dist[v] in a round came from a thread whose filter load passed, and that
 // thread stored touch[v].
-#include "compact.hpp"
-#include "expand.hpp"
+#include "ops.hpp"
 #include "pull.hpp"
 
 #include <chrono>
@@ -37,150 +36,6 @@ namespace hyt
 {
     namespace
     {
-        template <int WM>
-        struct Weights;
-
-        template <>
-        struct Weights<kWU32>
-        {
-            static constexpr std::uint32_t kInf = 0xFFFFFFFFU;
-            __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
-            __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
-            {
-                const std::uint64_t s = static_cast<std::uint64_t>(d) + w;
-                out = static_cast<std::uint32_t>(s);
-                return s < 0xFFFFFFFFULL;
-            }
-        };
-        template <>
-        struct Weights<kWUnitU32> : Weights<kWU32>
-        {
-            __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 1U; }
-        };
-        template <>
-        struct Weights<kWF32>
-        {
-            static constexpr std::uint32_t kInf = 0x7F800000U;
-            __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
-            __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
-            {
-                out = __float_as_uint(__fadd_rn(__uint_as_float(d), __uint_as_float(w)));
-                return true;
-            }
-        };
-        template <>
-        struct Weights<kWUnitF32> : Weights<kWF32>
-        {
-            __device__ static std::uint32_t load(const std::uint32_t *, std::uint64_t) { return 0x3F800000U; }
-        };
-
-        template <int WM, bool SYNC>
-        struct SsspOp
-        {
-            const std::uint32_t *__restrict__ col;
-            const std::uint32_t *__restrict__ w;
-            std::uint32_t *dist;
-            std::uint8_t *touch;
-            const std::uint32_t *qd; // snapshot values (SYNC)
-            std::uint32_t *overflow;
-
-            __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t) const
-            {
-                return SYNC ? qd[i] : dist[u];
-            }
-            __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
-            {
-                v = col[pos];
-                if (Weights<WM>::add(du, Weights<WM>::load(w, pos), c))
-                    return true;
-                atomicOr(overflow, 1U);
-                return false;
-            }
-            __device__ void apply(std::uint32_t v, std::uint32_t c) const
-            {
-                // A plain load may return an older, larger value; that only
-                // costs an extra atomic, it never skips a needed one.
-                if (c < dist[v])
-                {
-                    atomicMin(&dist[v], c);
-                    touch[v] = 1;
-                }
-            }
-        };
-
-        // Touched rows that improved join the frontier when below `bound`
-        // (delta-stepping bucket; 0xFFFFFFFF = no bucketing); improved rows at
-        // or above the bound stay marked and are counted as pending.
-        struct SsspSel
-        {
-            struct Aux
-            {
-                std::uint32_t pending = 0;
-                std::uint32_t inv_max = 0;
-            };
-            std::uint8_t *touch;
-            const std::uint32_t *dist;
-            std::uint32_t *prev;
-            std::uint32_t bound;
-
-            __device__ void publish(const Aux &aux, Counters *ctr) const
-            {
-                if (bound == 0xFFFFFFFFU)
-                    return;
-                std::uint32_t p = aux.pending, m = aux.inv_max;
-#pragma unroll
-                for (int d = kWave / 2; d > 0; d >>= 1)
-                {
-                    p += __shfl_xor(p, d, kWave);
-                    const std::uint32_t o = __shfl_xor(m, d, kWave);
-                    m = o > m ? o : m;
-                }
-                if (lane_id() == 0 && p)
-                {
-                    atomicAdd(&ctr->pending, p);
-                    atomicMax(&ctr->pending_inv_max, m);
-                }
-            }
-
-            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
-                                             Aux &aux) const
-            {
-                // touch has spare bytes past V, so the 4-byte word is in range
-                std::uint32_t *tw = reinterpret_cast<std::uint32_t *>(touch + row0);
-                const std::uint32_t t = *tw;
-                if (t == 0)
-                    return 0;
-                std::uint32_t keep = 0, m = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                {
-                    if (static_cast<std::uint32_t>(j) < n && ((t >> (8 * j)) & 0xFFU))
-                    {
-                        const std::uint64_t v = row0 + j;
-                        const std::uint32_t d = dist[v];
-                        if (d < prev[v])
-                        {
-                            if (d < bound)
-                            {
-                                prev[v] = d;
-                                vals[j] = d;
-                                m |= 1U << j;
-                            }
-                            else
-                            {
-                                keep |= 0xFFU << (8 * j);
-                                aux.pending += 1;
-                                aux.inv_max = (~d > aux.inv_max) ? ~d : aux.inv_max;
-                            }
-                        }
-                    }
-                }
-                if (keep != t)
-                    *tw = keep;
-                return m;
-            }
-        };
-
         template <int WM, bool SYNC>
         struct SsspPullOp
         {
@@ -363,7 +218,8 @@ namespace hyt
                 }
                 else
                 {
-                    expand_light<OffT, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
+                        whole_graph(g.off.as<OffT>(), a.q), op, a);
                     hip_check(hipGetLastError(), "launch sssp_expand_light");
                     expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                     hip_check(hipGetLastError(), "launch sssp_expand_heavy");

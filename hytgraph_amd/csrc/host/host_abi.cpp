@@ -78,6 +78,15 @@ struct hyt_host_graph
     std::unique_ptr<hyt::HostGraph> g;
 };
 
+namespace hyt
+{
+    HostGraph *host_graph_of(hyt_host_graph_t h)
+    {
+        require(h != nullptr && h->g, "null host graph");
+        return h->g.get();
+    }
+} // namespace hyt
+
 extern "C" {
 
 const char *hyt_last_error(void) { return hyt::g_last_error.c_str(); }

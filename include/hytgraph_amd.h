@@ -250,6 +250,71 @@ int hyt_select_engine(double filter_cost, double compaction_cost, double zero_co
 int hyt_combine_tasks(const int32_t *engines, uint64_t n, uint64_t k, int32_t *task_engine,
                       uint64_t *task_offset, uint64_t *members, uint64_t *num_tasks);
 
+/* ------------------------------------------------- hybrid transfer engine */
+/*
+ * Out-of-HBM traversal (the HyTGraph hybrid transfer management): the CSR
+ * stays in pinned, device-mapped host memory; vertex state and row offsets
+ * live in HBM; each round the engine gathers per-partition activity on the
+ * GPU, runs the cost model and task combiner, and moves the active
+ * partitions' edges by the chosen engine:
+ *   filter      whole partition copied (hipMemcpyAsync on a copy stream) into
+ *               an HBM slot; slots double as an LRU cache across rounds;
+ *   compaction  host threads pack the active rows' edges into pinned staging,
+ *               one copy, then the kernel walks the packed rows;
+ *   zero-copy   the kernel reads the mapped host arrays directly.
+ * No counterpart in the original runs these movers (its engines only plan).
+ */
+typedef struct hyt_hybrid *hyt_hybrid_t;
+
+typedef struct hyt_hybrid_config {
+    uint64_t device_budget_bytes; /* HBM for edge data: cache slots + packed staging */
+    uint64_t partition_edges;     /* partition target (edges; 2^20 -> 4 MiB of columns) */
+    uint64_t pack_edges;          /* packed-staging capacity in edges (0: default) */
+    uint32_t filter_combine_k;    /* task combiner k (0: 4) */
+    int32_t force_engine;         /* -1: cost model; else hyt_engine for every partition */
+    uint32_t host_threads;        /* compaction packers (0: min(hw, 16)) */
+    uint32_t calibrate;           /* 1: measure host link, packing, zero-copy; set rtt/cpu */
+    uint32_t cache;               /* 1: filtered partitions stay resident (LRU) */
+    uint32_t reserved;
+    hyt_cost_params cost;         /* used as given when calibrate == 0 */
+} hyt_hybrid_config;
+
+typedef struct hyt_hybrid_stats {
+    hyt_run_stats run;
+    uint64_t partitions;          /* logical partitions of the graph */
+    uint64_t parts_cached;        /* partition visits served from the HBM cache */
+    uint64_t parts_filter, parts_compaction, parts_zero_copy; /* visits per engine */
+    uint64_t tasks;               /* executable tasks over the run */
+    uint64_t bytes_filter;        /* host->device bytes copied for filter */
+    uint64_t bytes_compaction;    /* host->device bytes copied for compaction */
+    uint64_t bytes_zero_copy;     /* estimated: 128 B x zero-copy requests */
+    double copy_ms;               /* copy-stream busy time (HIP events) */
+    double pack_ms;               /* host packing time */
+    double plan_ms;               /* activity readback + cost model + combine */
+    double link_gbs_peak;         /* measured pinned H2D bandwidth */
+    double cpu_pack_gbs;          /* measured host packing throughput */
+    double zero_copy_gbs;         /* measured zero-copy kernel read bandwidth */
+    uint64_t device_edge_bytes;   /* HBM used for edge data (<= budget) */
+    uint64_t device_state_bytes;  /* HBM used for offsets and vertex state */
+} hyt_hybrid_stats;
+
+int hyt_hybrid_config_default(hyt_hybrid_config *out);
+
+/* The host graph must be pinned (HYT_HOST_PINNED) and outlive the engine.
+ * weights_f32: weights are float bit patterns. */
+int hyt_hybrid_create(hyt_host_graph_t host, uint32_t weights_f32, const hyt_hybrid_config *config,
+                      hyt_hybrid_t *out);
+int hyt_hybrid_free(hyt_hybrid_t h);
+int hyt_hybrid_info(hyt_hybrid_t h, uint64_t *partitions, hyt_cost_params *params_in_use,
+                    hyt_hybrid_stats *calibration);
+
+/* Single-source SSSP (async rounds) and delta-push PageRank over the hybrid
+ * engine; outputs as hyt_sssp / hyt_pagerank. */
+int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_run_options *opts,
+                    hyt_hybrid_stats *stats);
+int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ranks,
+                        const hyt_run_options *opts, hyt_hybrid_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

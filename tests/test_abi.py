@@ -102,3 +102,12 @@ def test_source_sampling_is_degree_weighted(abi):
     # vertex 0 (largest hub) is drawn about deg(0)/E of the time
     p0 = deg[0] / g.E
     assert abs((s == 0).mean() - p0) < 4 * np.sqrt(p0 / 4000) + 1e-3
+
+
+def test_hybrid_config_defaults_and_host_checks(abi):
+    c = abi.hybrid_config()
+    assert c.device_budget_bytes == 4 << 30 and c.partition_edges == 1 << 20 and c.force_engine == -1
+    assert c.cost.alpha == 0.8 and c.cost.entry_bytes == 8
+    pageable = abi.HostGraph(scale=8, edge_factor=4, seed=1)
+    with pytest.raises(abi.InvalidArgument):
+        abi.HybridEngine(pageable)

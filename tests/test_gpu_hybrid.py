@@ -1,0 +1,101 @@
+"""GPU parity: the hybrid transfer engine (edges in pinned host memory).
+
+Bar: SSSP distances bit-exact against the oracle whatever engine moves the
+edges (cost model, or forced filter / compaction / zero-copy), with caching
+on or off, budgets small enough to force eviction, oversized hub partitions
+and multi-batch packing; PageRank within the stated L1 tolerances of the
+FP64 oracles (same bars as the in-HBM path).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+D = 0.85
+
+
+def _say(msg):
+    print(f"[gpu-hybrid] {msg}", flush=True)
+
+
+def pinned(abi, scale=14, seed=3, wmax=255, kind=0):
+    return abi.HostGraph(kind=kind, scale=scale, edge_factor=16, seed=seed, weight_min=1 if wmax else 0,
+                         weight_max=wmax, memory=abi.HYT_HOST_PINNED)
+
+
+# partition of 2048 edges; budget = packed staging + ~12 slots -> constant eviction
+SMALL = dict(partition_edges=2048, pack_edges=8192, device_budget_bytes=8192 * 20 + 16 + 12 * 2048 * 8,
+             calibrate=0)
+
+
+@pytest.mark.parametrize("engine", [-1, 0, 1, 2])
+@pytest.mark.parametrize("cache", [0, 1])
+def test_hybrid_sssp_bitexact(gpu, abi, orc, engine, cache):
+    hg = pinned(abi)
+    h = abi.HybridEngine(hg, force_engine=engine, cache=cache, **SMALL)
+    for src in abi.sample_sources(hg.offsets, 5, 2):
+        _say(f"engine {engine} cache {cache} src {src}")
+        d, st = h.sssp(int(src))
+        assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src)))
+        assert st["run"]["converged"] == 1
+        if engine == 0:
+            assert st["bytes_filter"] > 0 and st["parts_filter"] > 0
+        if engine == 1:
+            assert st["bytes_compaction"] > 0 and st["parts_compaction"] > 0
+        if engine == 2:
+            assert st["parts_zero_copy"] > 0
+
+
+def test_hybrid_oversized_rows_and_cost_model_mix(gpu, abi, orc):
+    # partitions of 512 edges: RMAT hubs (degree in the thousands) are oversized
+    hg = pinned(abi, scale=15, seed=9)
+    deg = np.diff(hg.offsets.astype(np.int64))
+    assert deg.max() > 4 * 512
+    h = abi.HybridEngine(hg, partition_edges=512, pack_edges=4096, device_budget_bytes=4096 * 20 + 16 + 8 * 512 * 8,
+                         calibrate=0, cost=abi.cost_params(entry_bytes=8, rtt=1e-6, cpu_bytes_per_second=2e9))
+    d, st = h.sssp(0)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
+    _say(f"mix: filter {st['parts_filter']} compaction {st['parts_compaction']} zc {st['parts_zero_copy']} "
+         f"cached {st['parts_cached']}")
+
+
+def test_hybrid_float_weights_and_unweighted(gpu, abi, orc):
+    hg = pinned(abi, scale=13, seed=4, wmax=64)
+    ref, _, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, hg.weights.astype(np.float32), 0)
+    # float bit patterns in the same pinned arrays
+    hg.weights[:] = hg.weights.astype(np.float32).view(np.uint32)
+    h = abi.HybridEngine(hg, weights_f32=True, force_engine=-1, **SMALL)
+    d, _ = h.sssp(0)
+    assert np.array_equal(d, ref.view(np.uint32))
+    hu = pinned(abi, scale=13, seed=6, wmax=0)
+    h = abi.HybridEngine(hu, force_engine=1, **SMALL)
+    d, _ = h.sssp(3)
+    assert np.array_equal(d, orc.sssp_dijkstra(hu.offsets, hu.columns, None, 3))
+
+
+@pytest.mark.parametrize("engine", [-1, 0, 1, 2])
+def test_hybrid_pagerank(gpu, abi, orc, engine):
+    hg = pinned(abi, scale=13, seed=2, wmax=0)
+    h = abi.HybridEngine(hg, force_engine=engine, **SMALL)
+    eps = 1e-7
+    r, st = h.pagerank(D, eps)
+    same, rounds, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps)
+    exact = orc.pagerank_f64(hg.offsets, hg.columns, D)
+    _say(f"engine {engine}: L1 same {np.abs(r - same).sum():.2e} exact {np.abs(r - exact).sum():.2e} "
+         f"rounds {st['run']['rounds']}/{rounds}")
+    assert st["run"]["converged"] == 1
+    assert np.abs(r - same).sum() <= 2e-6
+    assert np.abs(r - exact).sum() <= 2 * D * eps / (1 - D) ** 2 + 2e-6
+
+
+def test_hybrid_reuse_and_errors(gpu, abi, orc):
+    hg = pinned(abi, scale=12, seed=8)
+    h = abi.HybridEngine(hg, **SMALL)
+    for src in (0, 5, 0):
+        d, _ = h.sssp(src)
+        assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src))
+    with pytest.raises(abi.InvalidArgument):
+        h.sssp(hg.V)
+    with pytest.raises(abi.InvalidArgument):
+        abi.HybridEngine(hg, device_budget_bytes=1024, calibrate=0)
+    info = h.info()
+    assert info["partitions"] > 10 and info["device_edge_bytes"] <= SMALL["device_budget_bytes"]
