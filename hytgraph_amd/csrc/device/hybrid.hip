@@ -3,8 +3,15 @@
 // management, executed rather than modelled).
 //
 // Resident in HBM: row offsets, vertex state, the frontier, a partition table,
-// and a budgeted edge area split into partition-sized slots (an LRU cache of
-// whole partitions) plus packed staging for compaction.
+// and a budgeted edge area holding
+//   * a streaming ring: a few buffers of k partitions' edges each; a filter
+//     task's consecutive partitions move with one copy per array;
+//   * the residency cache: units of one partition's columns (or weights);
+//     a partition that streamed in is kept when a unit is free or held by a
+//     partition that is inactive this round (least recently active first),
+//     so a cyclic scan larger than the cache keeps a stable resident set
+//     instead of thrashing;
+//   * packed staging for compaction.
 //
 // A round:
 //   1. ordered compaction -> frontier queue sorted by vertex, so partition p's
@@ -14,11 +21,12 @@
 //   3. host: HyTM cost equations + strict selector (hytm_core.hpp) for every
 //      active, non-resident partition; task combiner (filter runs <= k,
 //      compaction and zero-copy each one task);
-//   4. execute: resident partitions and filter partitions expand from HBM
-//      slots (copies on a copy stream, ordered by events, overlapping the
-//      compute stream); compaction packs the active rows' edges on host
-//      threads into pinned staging and copies them once; zero-copy expands
-//      straight from the mapped host arrays.
+//   4. execute: resident partitions expand from their units; filter tasks
+//      stream through the ring (copies on a copy stream, ordered by events,
+//      overlapping the compute stream) and are promoted into the cache by a
+//      device copy; zero-copy expands straight from the mapped host arrays;
+//      compaction packs the active rows' edges on host threads into pinned
+//      staging (while the GPU works on the rest) and copies them once.
 // The expand kernels are the in-HBM ones with a windowed / packed row source.
 #include "host_graph.hpp"
 #include "ops.hpp"
@@ -131,13 +139,13 @@ namespace hyt
                 sink[0] = acc;
         }
 
-        struct Slot
+        struct Resident
         {
-            std::int64_t part = -1;       // cached partition, -1 = none
-            std::uint64_t tick = 0;       // last use (LRU)
-            hipEvent_t used = nullptr;    // recorded on the compute stream after its last kernel
-            bool used_valid = false;
+            std::int64_t col = -1, w = -1; // units holding the columns / weights, -1 = none
+            std::int64_t pos = -1;         // index in the resident list
         };
+
+        constexpr std::size_t kRingBuffers = 4;
 
         struct EventPair
         {
@@ -159,12 +167,27 @@ namespace hyt
         // edge area
         hyt_hybrid_config cfg{};
         hytm::Params params;
-        std::uint64_t edge_bytes = 8; // bytes per edge on the device (column + weight)
-        std::uint64_t slot_edges = 0;
-        DeviceBuffer slot_mem;
-        std::vector<Slot> slots;
-        std::vector<std::int64_t> part_slot; // partition -> slot or -1
-        std::uint64_t tick = 0;
+        std::uint64_t edge_bytes = 8; // bytes per edge of the host graph (column + weight)
+        // per run: whether weights move (SSSP on a weighted graph) and the
+        // cost-model parameters with the entry size of what moves
+        bool need_w = true;
+        std::uint64_t run_edge_bytes = 8;
+        hytm::Params run_params;
+        // residency cache
+        std::uint64_t unit_edges = 0; // 4-byte entries per unit (= partition_edges)
+        std::uint64_t units = 0;
+        DeviceBuffer unit_mem;
+        std::vector<std::uint32_t> free_units;
+        std::vector<Resident> res;                // per partition
+        std::vector<std::uint64_t> resident;      // partitions holding units
+        std::vector<std::uint64_t> last_active;   // per partition: stamp of the last round it was active
+        std::uint64_t stamp = 0;                  // round stamp (monotone across runs)
+        // streaming ring
+        std::uint64_t ring_edges = 0; // edges per ring buffer
+        DeviceBuffer ring_mem;
+        hipEvent_t ring_used[kRingBuffers] = {};
+        bool ring_valid[kRingBuffers] = {};
+        std::size_t ring_next = 0;
         // packed staging (compaction)
         std::uint64_t pack_cap = 0; // edges
         DeviceBuffer pk_rows, pk_index, pk_col, pk_w, pk_val;
@@ -189,9 +212,9 @@ namespace hyt
                 (void)hipStreamSynchronize(copy);
                 (void)hipStreamDestroy(copy);
             }
-            for (auto &s : slots)
-                if (s.used)
-                    (void)hipEventDestroy(s.used);
+            for (hipEvent_t e : ring_used)
+                if (e)
+                    (void)hipEventDestroy(e);
             for (auto &e : copy_events)
             {
                 (void)hipEventDestroy(e.a);
@@ -209,7 +232,8 @@ namespace hyt
         std::uint64_t state_bytes() const { return dg.device_bytes() + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes + block_offsets.bytes + scan_temp.bytes + stage_q.bytes + stage_val.bytes; }
         std::uint64_t edge_area_bytes() const
         {
-            return slot_mem.bytes + pk_rows.bytes + pk_index.bytes + pk_col.bytes + pk_w.bytes + pk_val.bytes;
+            return unit_mem.bytes + ring_mem.bytes + pk_rows.bytes + pk_index.bytes + pk_col.bytes + pk_w.bytes +
+                   pk_val.bytes;
         }
 
         EventPair &next_copy_events()
@@ -248,33 +272,103 @@ namespace hyt
             hip_check(hipEventRecord(e.b, copy), "hipEventRecord");
         }
 
-        std::uint32_t *slot_col(std::size_t s) const
+        std::uint32_t *unit_ptr(std::int64_t u) const
         {
-            return slot_mem.as<std::uint32_t>() + s * slot_edges * (edge_bytes / 4);
+            return unit_mem.as<std::uint32_t>() + static_cast<std::uint64_t>(u) * unit_edges;
         }
-        std::uint32_t *slot_w(std::size_t s) const { return host->weighted ? slot_col(s) + slot_edges : nullptr; }
-
-        // LRU victim whose last kernel has been ordered before the next copy.
-        std::size_t take_slot()
+        std::uint32_t *ring_col(std::size_t r) const
         {
-            std::size_t best = 0;
-            for (std::size_t s = 1; s < slots.size(); ++s)
-                if (slots[s].tick < slots[best].tick)
-                    best = s;
-            Slot &sl = slots[best];
-            if (sl.part >= 0)
-                part_slot[static_cast<std::size_t>(sl.part)] = -1;
-            sl.part = -1;
-            if (sl.used_valid)
-                hip_check(hipStreamWaitEvent(copy, sl.used, 0), "wait slot");
-            return best;
+            return ring_mem.as<std::uint32_t>() + r * ring_edges * (edge_bytes / 4);
+        }
+        std::uint32_t *ring_w(std::size_t r) const { return need_w ? ring_col(r) + ring_edges : nullptr; }
+
+        // Next ring buffer; the copy stream waits until its last kernel is done.
+        std::size_t ring_take()
+        {
+            const std::size_t r = ring_next;
+            ring_next = (ring_next + 1) % kRingBuffers;
+            if (ring_valid[r])
+                hip_check(hipStreamWaitEvent(copy, ring_used[r], 0), "wait ring buffer");
+            return r;
+        }
+        void ring_release(std::size_t r)
+        {
+            hip_check(hipEventRecord(ring_used[r], dg.stream), "hipEventRecord");
+            ring_valid[r] = true;
         }
 
-        void mark_used(std::size_t s)
+        // Weights move only for runs that read them.
+        void begin_run(bool weights)
         {
-            slots[s].tick = ++tick;
-            hip_check(hipEventRecord(slots[s].used, dg.stream), "hipEventRecord");
-            slots[s].used_valid = true;
+            need_w = weights && host->weighted;
+            run_edge_bytes = need_w ? 8 : 4;
+            run_params = params;
+            if (!need_w)
+                run_params.entry_bytes = 4;
+        }
+
+        bool is_cached(std::uint64_t p) const { return res[p].col >= 0 && (!need_w || res[p].w >= 0); }
+
+        void drop(std::uint64_t p)
+        {
+            Resident &r = res[p];
+            if (r.col >= 0)
+                free_units.push_back(static_cast<std::uint32_t>(r.col));
+            if (r.w >= 0)
+                free_units.push_back(static_cast<std::uint32_t>(r.w));
+            if (r.pos >= 0)
+            {
+                const std::uint64_t last = resident.back();
+                resident[static_cast<std::size_t>(r.pos)] = last;
+                res[last].pos = r.pos;
+                resident.pop_back();
+            }
+            r = Resident{};
+        }
+
+        // Finds units for partition p (this round's stamp protects the active
+        // partitions); false when every resident partition is active now.
+        bool admit(std::uint64_t p)
+        {
+            const std::size_t need = need_w ? 2 : 1;
+            if (res[p].col >= 0)
+                drop(p); // cached without the weights this run needs
+            // a run that reads no weights first takes back resident weight units
+            if (!need_w)
+                for (std::size_t i = 0; i < resident.size() && free_units.size() < need; ++i)
+                {
+                    Resident &o = res[resident[i]];
+                    if (o.w >= 0)
+                    {
+                        free_units.push_back(static_cast<std::uint32_t>(o.w));
+                        o.w = -1;
+                    }
+                }
+            while (free_units.size() < need)
+            {
+                std::int64_t victim = -1;
+                std::uint64_t oldest = stamp;
+                for (std::uint64_t v : resident)
+                    if (last_active[v] < oldest)
+                    {
+                        oldest = last_active[v];
+                        victim = static_cast<std::int64_t>(v);
+                    }
+                if (victim < 0)
+                    return false;
+                drop(static_cast<std::uint64_t>(victim));
+            }
+            Resident &r = res[p];
+            r.col = free_units.back();
+            free_units.pop_back();
+            if (need_w)
+            {
+                r.w = free_units.back();
+                free_units.pop_back();
+            }
+            r.pos = static_cast<std::int64_t>(resident.size());
+            resident.push_back(p);
+            return true;
         }
     };
 } // namespace hyt
@@ -293,7 +387,8 @@ namespace hyt
             hipStream_t s = E.dg.stream;
             // pinned H2D peak: copy up to 256 MiB of the host column array
             const std::uint64_t bytes = std::min<std::uint64_t>(256ULL << 20, E.host->E * 4);
-            if (bytes >= (4ULL << 20) && E.slot_mem.bytes >= bytes)
+            DeviceBuffer &dst = E.unit_mem.bytes >= bytes ? E.unit_mem : E.ring_mem;
+            if (bytes >= (4ULL << 20) && dst.bytes >= bytes)
             {
                 hipEvent_t a, b;
                 hip_check(hipEventCreate(&a), "hipEventCreate");
@@ -301,7 +396,7 @@ namespace hyt
                 for (int rep = 0; rep < 2; ++rep)
                 {
                     hip_check(hipEventRecord(a, s), "hipEventRecord");
-                    hip_check(hipMemcpyAsync(E.slot_mem.ptr, E.host->col, bytes, hipMemcpyHostToDevice, s), "copy");
+                    hip_check(hipMemcpyAsync(dst.ptr, E.host->col, bytes, hipMemcpyHostToDevice, s), "copy");
                     hip_check(hipEventRecord(b, s), "hipEventRecord");
                     hip_check(hipEventSynchronize(b), "sync");
                 }
@@ -425,7 +520,8 @@ namespace hyt
             hip_check(hipMemcpy(E.part_vbegin.ptr, vb.data(), (P + 1) * 4, hipMemcpyHostToDevice), "copy partitions");
             E.part_stats_d.alloc(P * 4 * 8, "hipMalloc(partition stats)");
             E.part_stats_h.assign(P * 4, 0);
-            E.part_slot.assign(P, -1);
+            E.res.assign(P, Resident{});
+            E.last_active.assign(P, 0);
 
             // ordered compaction buffers
             const unsigned nb = compact_blocks(host->V);
@@ -440,21 +536,30 @@ namespace hyt
             E.stage_q.alloc(static_cast<std::uint64_t>(nb) * kCompactRowsPerBlock * 4, "hipMalloc(stage queue)");
             E.stage_val.alloc(static_cast<std::uint64_t>(nb) * kCompactRowsPerBlock * 4, "hipMalloc(stage values)");
 
-            // edge area: packed staging first, the rest in partition slots
+            // edge area: packed staging, the streaming ring (at most half of
+            // what is left), the rest in residency units
             const std::uint64_t budget = cfg.device_budget_bytes;
             E.pack_cap = cfg.pack_edges ? cfg.pack_edges : std::max<std::uint64_t>(4 * cfg.partition_edges, 1ULL << 22);
             // exactly what the packed-staging allocations below take
             const std::uint64_t pack_bytes = (E.pack_cap * 4 + 16) * (host->weighted ? 4 : 3) +
                                              ((E.pack_cap + 1) * 8 + 16);
             require(budget > pack_bytes, "device budget too small for the packed staging area");
-            E.slot_edges = std::max<std::uint64_t>(cfg.partition_edges, 1);
-            const std::uint64_t slot_bytes = E.slot_edges * E.edge_bytes;
-            std::uint64_t nslots = (budget - pack_bytes) / slot_bytes;
-            require(nslots >= 2, "device budget too small for two partition slots");
-            E.slot_mem.alloc(nslots * slot_bytes, "hipMalloc(edge slots)");
-            E.slots.resize(nslots);
-            for (auto &s : E.slots)
-                hip_check(hipEventCreateWithFlags(&s.used, hipEventDisableTiming), "hipEventCreate");
+            E.unit_edges = std::max<std::uint64_t>(cfg.partition_edges, 1);
+            const std::uint64_t avail = budget - pack_bytes;
+            const std::uint64_t ring_unit = kRingBuffers * E.unit_edges * E.edge_bytes; // ring of one partition each
+            require(avail >= ring_unit, "device budget too small for the streaming ring");
+            const std::uint64_t k = cfg.filter_combine_k ? cfg.filter_combine_k : 4;
+            const std::uint64_t ring_k = std::max<std::uint64_t>(1, std::min<std::uint64_t>(k, avail / 2 / ring_unit));
+            E.ring_edges = ring_k * E.unit_edges;
+            E.ring_mem.alloc(ring_k * ring_unit, "hipMalloc(streaming ring)");
+            for (hipEvent_t &ev : E.ring_used)
+                hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+            E.units = (avail - ring_k * ring_unit) / (E.unit_edges * 4);
+            if (E.units > 0)
+                E.unit_mem.alloc(E.units * E.unit_edges * 4, "hipMalloc(residency units)");
+            E.free_units.clear();
+            for (std::uint64_t u = E.units; u-- > 0;)
+                E.free_units.push_back(static_cast<std::uint32_t>(u));
             E.pk_rows.alloc(E.pack_cap * 4 + 16, "hipMalloc(packed rows)");
             E.pk_index.alloc((E.pack_cap + 1) * 8 + 16, "hipMalloc(packed index)");
             E.pk_col.alloc(E.pack_cap * 4 + 16, "hipMalloc(packed columns)");
@@ -516,13 +621,15 @@ namespace hyt
         void plan_round(HybridEngine &E, std::uint32_t F, Plan &plan, hyt_hybrid_stats &st)
         {
             const auto t0 = Clock::now();
+            ++E.stamp;
             DeviceGraph &g = E.dg;
             hipStream_t s = g.stream;
             const std::uint64_t P = E.parts.size();
             hip_check(hipMemsetAsync(E.part_stats_d.ptr, 0, P * 32, s), "clear partition stats");
             part_stats<OffT><<<blocks_for(F), kBlock, 0, s>>>(
                 g.ws.q.as<std::uint32_t>(), F, g.off.as<OffT>(), E.part_vbegin.as<std::uint32_t>(),
-                static_cast<std::uint32_t>(P), E.params.request_bytes, E.params.entry_bytes, E.params.alignment_bytes,
+                static_cast<std::uint32_t>(P), E.run_params.request_bytes, E.run_params.entry_bytes,
+                E.run_params.alignment_bytes,
                 E.part_stats_d.as<unsigned long long>());
             hip_check(hipGetLastError(), "launch part_stats");
             hip_check(hipMemcpyAsync(E.part_stats_h.data(), E.part_stats_d.ptr, P * 32, hipMemcpyDeviceToHost, s),
@@ -544,7 +651,8 @@ namespace hyt
                 if (c[1] == 0)
                     continue; // no frontier edges: nothing to move (original filter_engine active test)
                 plan.active.push_back(p);
-                if (E.part_slot[p] >= 0)
+                E.last_active[p] = E.stamp;
+                if (E.is_cached(p))
                 {
                     plan.cached.push_back(p);
                     continue;
@@ -560,8 +668,8 @@ namespace hyt
                     m.total_edges = E.parts[p].edge_end - E.parts[p].edge_begin;
                     m.zc_requests = c[2];
                     m.zc_alignment = c[3];
-                    hytm::finish_metrics(m, E.params);
-                    engine = hytm::evaluate(m, E.params).engine;
+                    hytm::finish_metrics(m, E.run_params);
+                    engine = hytm::evaluate(m, E.run_params).engine;
                 }
                 engines.push_back(engine);
                 members.push_back(p);
@@ -609,200 +717,218 @@ namespace hyt
             const std::uint32_t *q = g.ws.q.as<std::uint32_t>();
             const std::uint32_t *qd = g.ws.qd.as<std::uint32_t>();
             const OffT *off = g.off.as<OffT>();
-            const bool weighted = E.host->weighted;
+            const bool weighted = E.need_w;
 
-            auto run_slot = [&](std::uint64_t p, std::size_t slot) {
-                const hyt_partition &pt = E.parts[p];
-                GraphRows<OffT> rs{off, q + plan.qlo[p], pt.edge_begin, pt.edge_end};
-                expand(E, rs, make(E.slot_col(slot), E.slot_w(slot), qd + plan.qlo[p]),
-                       static_cast<std::uint32_t>(plan.qn[p]));
-                E.mark_used(slot);
+            // maximal runs [p0, p1] of consecutive partition ids in a member list
+            auto for_runs = [](const std::vector<std::uint64_t> &m, auto fn) {
+                std::size_t i = 0;
+                while (i < m.size())
+                {
+                    std::size_t j = i + 1;
+                    while (j < m.size() && m[j] == m[j - 1] + 1)
+                        ++j;
+                    fn(m[i], m[j - 1]);
+                    i = j;
+                }
             };
-            auto copy_into = [&](std::size_t slot, std::uint64_t e0, std::uint64_t n) {
-                E.copy_h2d(E.slot_col(slot), E.host->col + e0, n * 4);
-                if (weighted)
-                    E.copy_h2d(E.slot_w(slot), E.host->w + e0, n * 4);
-                st.bytes_filter += n * E.edge_bytes;
-                hipEvent_t ready = E.copy_events[E.copy_events_used - 1].b;
-                hip_check(hipStreamWaitEvent(s, ready, 0), "wait copy");
+            // frontier rows of partitions [p0, p1]: one range of the sorted queue
+            auto rows_of = [&](std::uint64_t p0, std::uint64_t p1) {
+                return static_cast<std::uint32_t>(plan.qlo[p1] + plan.qn[p1] - plan.qlo[p0]);
             };
 
-            // 1. resident partitions
+            // 1. compaction tasks read their frontier rows back while the stream is idle
+            std::uint64_t crows = 0;
+            for (const Task &t : plan.tasks)
+                if (t.engine == HYT_ENGINE_COMPACTION)
+                    for (std::uint64_t p : t.members)
+                        crows += plan.qn[p];
+            if (crows)
+            {
+                if (E.h_q_cap < crows)
+                {
+                    if (E.h_q)
+                        (void)hipHostFree(E.h_q);
+                    if (E.h_qd)
+                        (void)hipHostFree(E.h_qd);
+                    E.h_q = E.h_qd = nullptr;
+                    E.h_q_cap = crows + 1024;
+                    hip_check(hipHostMalloc(reinterpret_cast<void **>(&E.h_q), E.h_q_cap * 4), "hipHostMalloc");
+                    hip_check(hipHostMalloc(reinterpret_cast<void **>(&E.h_qd), E.h_q_cap * 4), "hipHostMalloc");
+                }
+                std::uint64_t at = 0;
+                for (const Task &t : plan.tasks)
+                    if (t.engine == HYT_ENGINE_COMPACTION)
+                        for_runs(t.members, [&](std::uint64_t p0, std::uint64_t p1) {
+                            const std::uint64_t n = rows_of(p0, p1);
+                            hip_check(hipMemcpyAsync(E.h_q + at, q + plan.qlo[p0], n * 4, hipMemcpyDeviceToHost, s),
+                                      "read frontier");
+                            hip_check(hipMemcpyAsync(E.h_qd + at, qd + plan.qlo[p0], n * 4, hipMemcpyDeviceToHost, s),
+                                      "read frontier values");
+                            at += n;
+                        });
+                hip_check(hipStreamSynchronize(s), "frontier readback");
+            }
+
+            // 2. resident partitions
             for (std::uint64_t p : plan.cached)
             {
-                run_slot(p, static_cast<std::size_t>(E.part_slot[p]));
+                const hyt_partition &pt = E.parts[p];
+                const Resident &r = E.res[p];
+                GraphRows<OffT> rs{off, q + plan.qlo[p], pt.edge_begin, pt.edge_end};
+                expand(E, rs, make(E.unit_ptr(r.col), weighted ? E.unit_ptr(r.w) : nullptr, qd + plan.qlo[p]),
+                       static_cast<std::uint32_t>(plan.qn[p]));
                 st.parts_cached += 1;
             }
+
+            // 3. filter: each run of consecutive partitions streams through the
+            // ring in windows of ring_edges (one copy per array per window);
+            // whole partitions of a window are then promoted into the cache
             for (const Task &t : plan.tasks)
             {
-                if (t.engine == HYT_ENGINE_FILTER)
-                {
-                    for (std::uint64_t p : t.members)
+                if (t.engine != HYT_ENGINE_FILTER)
+                    continue;
+                for_runs(t.members, [&](std::uint64_t p0, std::uint64_t p1) {
+                    st.parts_filter += p1 - p0 + 1;
+                    const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
+                    std::uint64_t pa = p0;
+                    for (std::uint64_t a = e0; a < e1; a += E.ring_edges)
                     {
-                        const hyt_partition &pt = E.parts[p];
-                        const std::uint64_t n = pt.edge_end - pt.edge_begin;
-                        st.parts_filter += 1;
-                        if (n <= E.slot_edges)
-                        {
-                            const std::size_t slot = E.take_slot();
-                            copy_into(slot, pt.edge_begin, n);
-                            run_slot(p, slot);
-                            if (E.cfg.cache)
-                            {
-                                E.slots[slot].part = static_cast<std::int64_t>(p);
-                                E.part_slot[p] = static_cast<std::int64_t>(slot);
-                            }
-                        }
-                        else
-                        {
-                            // oversized partition (one huge row): stream it in slot-sized windows
-                            for (std::uint64_t e0 = pt.edge_begin; e0 < pt.edge_end; e0 += E.slot_edges)
-                            {
-                                const std::uint64_t m = std::min<std::uint64_t>(E.slot_edges, pt.edge_end - e0);
-                                const std::size_t slot = E.take_slot();
-                                copy_into(slot, e0, m);
-                                GraphRows<OffT> rs{off, q + plan.qlo[p], e0, e0 + m};
-                                expand(E, rs, make(E.slot_col(slot), E.slot_w(slot), qd + plan.qlo[p]),
-                                       static_cast<std::uint32_t>(plan.qn[p]));
-                                E.mark_used(slot);
-                            }
-                        }
-                    }
-                }
-                else if (t.engine == HYT_ENGINE_ZERO_COPY)
-                {
-                    // maximal runs of consecutive partitions form one window
-                    std::size_t i = 0;
-                    while (i < t.members.size())
-                    {
-                        std::size_t j = i + 1;
-                        while (j < t.members.size() && t.members[j] == t.members[j - 1] + 1)
-                            ++j;
-                        const std::uint64_t p0 = t.members[i], p1 = t.members[j - 1];
-                        const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
-                        std::uint64_t rows = 0;
-                        for (std::size_t k = i; k < j; ++k)
-                        {
-                            rows += plan.qn[t.members[k]];
-                            // estimated host-link bytes: m per modelled request
-                            st.bytes_zero_copy += E.part_stats_h[4 * t.members[k] + 2] * E.params.request_bytes;
-                        }
-                        GraphRows<OffT> rs{off, q + plan.qlo[p0], e0, e1};
-                        expand(E, rs, make(E.z_col + e0, weighted ? E.z_w + e0 : nullptr, qd + plan.qlo[p0]),
-                               static_cast<std::uint32_t>(rows));
-                        st.parts_zero_copy += j - i;
-                        i = j;
-                    }
-                }
-                else
-                {
-                    // compaction: read back the members' frontier rows, pack on host threads
-                    std::uint64_t rows = 0;
-                    for (std::uint64_t p : t.members)
-                        rows += plan.qn[p];
-                    if (E.h_q_cap < rows)
-                    {
-                        if (E.h_q)
-                            (void)hipHostFree(E.h_q);
-                        if (E.h_qd)
-                            (void)hipHostFree(E.h_qd);
-                        E.h_q_cap = rows + 1024;
-                        hip_check(hipHostMalloc(reinterpret_cast<void **>(&E.h_q), E.h_q_cap * 4), "hipHostMalloc");
-                        hip_check(hipHostMalloc(reinterpret_cast<void **>(&E.h_qd), E.h_q_cap * 4), "hipHostMalloc");
-                    }
-                    std::uint64_t at = 0;
-                    for (std::uint64_t p : t.members)
-                    {
-                        hip_check(hipMemcpyAsync(E.h_q + at, q + plan.qlo[p], plan.qn[p] * 4, hipMemcpyDeviceToHost, s),
-                                  "read frontier");
-                        hip_check(hipMemcpyAsync(E.h_qd + at, qd + plan.qlo[p], plan.qn[p] * 4,
-                                                 hipMemcpyDeviceToHost, s),
-                                  "read frontier values");
-                        at += plan.qn[p];
-                    }
-                    hip_check(hipStreamSynchronize(s), "frontier readback");
-                    st.parts_compaction += t.members.size();
-                    // batches that fit the packed staging
-                    std::uint64_t r0 = 0;
-                    while (r0 < rows)
-                    {
-                        const auto tp = Clock::now();
-                        // the previous batch's kernels must be done with the staging
-                        hip_check(hipEventSynchronize(E.ev_pack), "wait packed staging");
-                        {
-                            // a row larger than the packed staging is read zero-copy
-                            const std::uint32_t u = E.h_q[r0];
-                            const std::uint64_t d = E.host->off[u + 1] - E.host->off[u];
-                            if (d > E.pack_cap)
-                            {
-                                E.h_rows[0] = u;
-                                E.h_val[0] = E.h_qd[r0];
-                                E.copy_h2d(E.pk_rows.ptr, E.h_rows, 4);
-                                E.copy_h2d(E.pk_val.ptr, E.h_val, 4);
-                                hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0),
-                                          "wait row copy");
-                                GraphRows<OffT> rs{off, E.pk_rows.as<std::uint32_t>(), 0, ~0ULL};
-                                expand(E, rs, make(E.z_col, weighted ? E.z_w : nullptr, E.pk_val.as<std::uint32_t>()),
-                                       1U);
-                                hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
-                                st.bytes_zero_copy += d * E.edge_bytes;
-                                ++r0;
-                                continue;
-                            }
-                        }
-                        std::uint64_t r1 = r0, edges = 0;
-                        while (r1 < rows)
-                        {
-                            const std::uint32_t u = E.h_q[r1];
-                            const std::uint64_t d = E.host->off[u + 1] - E.host->off[u];
-                            if (r1 > r0 && (d > E.pack_cap || edges + d > E.pack_cap || r1 - r0 >= E.pack_cap))
-                                break;
-                            edges += d;
-                            ++r1;
-                        }
-                        const std::uint64_t nr = r1 - r0;
-                        // prefix of degrees, then parallel copy of the rows' edges
-                        E.h_index[0] = 0;
-                        for (std::uint64_t k = 0; k < nr; ++k)
-                        {
-                            const std::uint32_t u = E.h_q[r0 + k];
-                            E.h_rows[k] = u;
-                            E.h_val[k] = E.h_qd[r0 + k];
-                            E.h_index[k + 1] = E.h_index[k] + (E.host->off[u + 1] - E.host->off[u]);
-                        }
-                        std::vector<std::thread> pool;
-                        const int T = std::max(1, std::min<int>(E.threads, static_cast<int>(nr / 64 + 1)));
-                        for (int ti = 0; ti < T; ++ti)
-                            pool.emplace_back([&, ti] {
-                                const std::uint64_t lo = nr * ti / T, hi = nr * (ti + 1) / T;
-                                for (std::uint64_t k = lo; k < hi; ++k)
-                                {
-                                    const std::uint32_t u = E.h_rows[k];
-                                    const std::uint64_t b = E.host->off[u], n = E.host->off[u + 1] - b;
-                                    std::memcpy(E.h_col + E.h_index[k], E.host->col + b, n * 4);
-                                    if (weighted)
-                                        std::memcpy(E.h_w + E.h_index[k], E.host->w + b, n * 4);
-                                }
-                            });
-                        for (auto &th : pool)
-                            th.join();
-                        st.pack_ms += ms_since(tp);
-                        E.copy_h2d(E.pk_rows.ptr, E.h_rows, nr * 4);
-                        E.copy_h2d(E.pk_val.ptr, E.h_val, nr * 4);
-                        E.copy_h2d(E.pk_index.ptr, E.h_index, (nr + 1) * 8);
-                        E.copy_h2d(E.pk_col.ptr, E.h_col, edges * 4);
+                        const std::uint64_t b = std::min<std::uint64_t>(e1, a + E.ring_edges);
+                        while (E.parts[pa].edge_end <= a)
+                            ++pa;
+                        std::uint64_t pb = pa;
+                        while (pb < p1 && E.parts[pb + 1].edge_begin < b)
+                            ++pb;
+                        const std::size_t r = E.ring_take();
+                        E.copy_h2d(E.ring_col(r), E.host->col + a, (b - a) * 4);
                         if (weighted)
-                            E.copy_h2d(E.pk_w.ptr, E.h_w, edges * 4);
-                        st.bytes_compaction += nr * 4 + (nr + 1) * 8 + edges * E.edge_bytes;
-                        hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait pack copy");
-                        PackedRows rs{E.pk_rows.as<std::uint32_t>(), E.pk_index.as<std::uint64_t>()};
-                        expand(E, rs, make(E.pk_col.as<std::uint32_t>(), weighted ? E.pk_w.as<std::uint32_t>() : nullptr,
-                                           E.pk_val.as<std::uint32_t>()),
-                               static_cast<std::uint32_t>(nr));
+                            E.copy_h2d(E.ring_w(r), E.host->w + a, (b - a) * 4);
+                        st.bytes_filter += (b - a) * E.run_edge_bytes;
+                        hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait copy");
+                        const std::uint64_t qa = plan.qlo[pa];
+                        GraphRows<OffT> rs{off, q + qa, a, b};
+                        expand(E, rs, make(E.ring_col(r), E.ring_w(r), qd + qa), rows_of(pa, pb));
+                        if (E.cfg.cache)
+                            for (std::uint64_t pp = pa; pp <= pb; ++pp)
+                            {
+                                const hyt_partition &pt = E.parts[pp];
+                                const std::uint64_t n = pt.edge_end - pt.edge_begin;
+                                if (pt.edge_begin < a || pt.edge_end > b || n > E.unit_edges)
+                                    continue;
+                                if (!E.admit(pp))
+                                    break; // every resident partition is active this round
+                                const Resident &rr = E.res[pp];
+                                hip_check(hipMemcpyAsync(E.unit_ptr(rr.col), E.ring_col(r) + (pt.edge_begin - a), n * 4,
+                                                         hipMemcpyDeviceToDevice, s),
+                                          "promote columns");
+                                if (weighted)
+                                    hip_check(hipMemcpyAsync(E.unit_ptr(rr.w), E.ring_w(r) + (pt.edge_begin - a), n * 4,
+                                                             hipMemcpyDeviceToDevice, s),
+                                              "promote weights");
+                            }
+                        E.ring_release(r);
+                    }
+                });
+            }
+
+            // 4. zero-copy: one window per run of consecutive partitions
+            for (const Task &t : plan.tasks)
+            {
+                if (t.engine != HYT_ENGINE_ZERO_COPY)
+                    continue;
+                for_runs(t.members, [&](std::uint64_t p0, std::uint64_t p1) {
+                    const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
+                    for (std::uint64_t p = p0; p <= p1; ++p) // estimated link bytes: m per modelled request
+                        st.bytes_zero_copy += E.part_stats_h[4 * p + 2] * E.run_params.request_bytes;
+                    GraphRows<OffT> rs{off, q + plan.qlo[p0], e0, e1};
+                    expand(E, rs, make(E.z_col + e0, weighted ? E.z_w + e0 : nullptr, qd + plan.qlo[p0]),
+                           rows_of(p0, p1));
+                    st.parts_zero_copy += p1 - p0 + 1;
+                });
+            }
+
+            // 5. compaction: pack the rows read back in step 1 on host threads,
+            // in batches that fit the packed staging (the GPU works on the
+            // expansions queued above meanwhile)
+            std::uint64_t r0 = 0;
+            for (const Task &t : plan.tasks)
+                if (t.engine == HYT_ENGINE_COMPACTION)
+                    st.parts_compaction += t.members.size();
+            while (r0 < crows)
+            {
+                const auto tp = Clock::now();
+                // the previous batch's kernels must be done with the staging
+                hip_check(hipEventSynchronize(E.ev_pack), "wait packed staging");
+                {
+                    // a row larger than the packed staging is read zero-copy
+                    const std::uint32_t u = E.h_q[r0];
+                    const std::uint64_t d = E.host->off[u + 1] - E.host->off[u];
+                    if (d > E.pack_cap)
+                    {
+                        E.h_rows[0] = u;
+                        E.h_val[0] = E.h_qd[r0];
+                        E.copy_h2d(E.pk_rows.ptr, E.h_rows, 4);
+                        E.copy_h2d(E.pk_val.ptr, E.h_val, 4);
+                        hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait row copy");
+                        GraphRows<OffT> rs{off, E.pk_rows.as<std::uint32_t>(), 0, ~0ULL};
+                        expand(E, rs, make(E.z_col, weighted ? E.z_w : nullptr, E.pk_val.as<std::uint32_t>()), 1U);
                         hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
-                        r0 = r1;
+                        st.bytes_zero_copy += d * E.run_edge_bytes;
+                        ++r0;
+                        continue;
                     }
                 }
+                std::uint64_t r1 = r0, edges = 0;
+                while (r1 < crows)
+                {
+                    const std::uint32_t u = E.h_q[r1];
+                    const std::uint64_t d = E.host->off[u + 1] - E.host->off[u];
+                    if (r1 > r0 && (d > E.pack_cap || edges + d > E.pack_cap || r1 - r0 >= E.pack_cap))
+                        break;
+                    edges += d;
+                    ++r1;
+                }
+                const std::uint64_t nr = r1 - r0;
+                // prefix of degrees, then parallel copy of the rows' edges
+                E.h_index[0] = 0;
+                for (std::uint64_t k = 0; k < nr; ++k)
+                {
+                    const std::uint32_t u = E.h_q[r0 + k];
+                    E.h_rows[k] = u;
+                    E.h_val[k] = E.h_qd[r0 + k];
+                    E.h_index[k + 1] = E.h_index[k] + (E.host->off[u + 1] - E.host->off[u]);
+                }
+                std::vector<std::thread> pool;
+                const int T = std::max(1, std::min<int>(E.threads, static_cast<int>(nr / 64 + 1)));
+                for (int ti = 0; ti < T; ++ti)
+                    pool.emplace_back([&, ti] {
+                        const std::uint64_t lo = nr * ti / T, hi = nr * (ti + 1) / T;
+                        for (std::uint64_t k = lo; k < hi; ++k)
+                        {
+                            const std::uint32_t u = E.h_rows[k];
+                            const std::uint64_t b = E.host->off[u], n = E.host->off[u + 1] - b;
+                            std::memcpy(E.h_col + E.h_index[k], E.host->col + b, n * 4);
+                            if (weighted)
+                                std::memcpy(E.h_w + E.h_index[k], E.host->w + b, n * 4);
+                        }
+                    });
+                for (auto &th : pool)
+                    th.join();
+                st.pack_ms += ms_since(tp);
+                E.copy_h2d(E.pk_rows.ptr, E.h_rows, nr * 4);
+                E.copy_h2d(E.pk_val.ptr, E.h_val, nr * 4);
+                E.copy_h2d(E.pk_index.ptr, E.h_index, (nr + 1) * 8);
+                E.copy_h2d(E.pk_col.ptr, E.h_col, edges * 4);
+                if (weighted)
+                    E.copy_h2d(E.pk_w.ptr, E.h_w, edges * 4);
+                st.bytes_compaction += nr * 4 + (nr + 1) * 8 + edges * E.run_edge_bytes;
+                hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait pack copy");
+                PackedRows rs{E.pk_rows.as<std::uint32_t>(), E.pk_index.as<std::uint64_t>()};
+                expand(E, rs, make(E.pk_col.as<std::uint32_t>(), weighted ? E.pk_w.as<std::uint32_t>() : nullptr,
+                                   E.pk_val.as<std::uint32_t>()),
+                       static_cast<std::uint32_t>(nr));
+                hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
+                r0 = r1;
             }
         }
 
@@ -1034,6 +1160,7 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
         const int wm = E.dg.weight_mode();
+        E.begin_run(true);
         auto run = [&](auto off_tag) {
             using OffT = decltype(off_tag);
             switch (wm)
@@ -1086,6 +1213,7 @@ int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ran
         st.device_state_bytes = E.state_bytes();
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
+        E.begin_run(false);
         if (E.dg.off64)
             hybrid_pagerank_typed<std::uint64_t>(E, damping, epsilon, ranks, o, st);
         else

@@ -22,7 +22,8 @@ def pinned(abi, scale=14, seed=3, wmax=255, kind=0):
                          weight_max=wmax, memory=abi.HYT_HOST_PINNED)
 
 
-# partition of 2048 edges; budget = packed staging + ~12 slots -> constant eviction
+# partitions of 2048 edges; budget = packed staging + streaming ring + a few
+# cached partitions -> constant eviction
 SMALL = dict(partition_edges=2048, pack_edges=8192, device_budget_bytes=8192 * 20 + 16 + 12 * 2048 * 8,
              calibrate=0)
 
@@ -50,7 +51,9 @@ def test_hybrid_oversized_rows_and_cost_model_mix(gpu, abi, orc):
     hg = pinned(abi, scale=15, seed=9)
     deg = np.diff(hg.offsets.astype(np.int64))
     assert deg.max() > 4 * 512
-    h = abi.HybridEngine(hg, partition_edges=512, pack_edges=4096, device_budget_bytes=4096 * 20 + 16 + 8 * 512 * 8,
+    # budget: packed staging + a streaming ring of 4 partitions + 8 cached ones
+    h = abi.HybridEngine(hg, partition_edges=512, pack_edges=4096,
+                         device_budget_bytes=4096 * 20 + 16 + 4 * 512 * 8 + 8 * 512 * 8,
                          calibrate=0, cost=abi.cost_params(entry_bytes=8, rtt=1e-6, cpu_bytes_per_second=2e9))
     d, st = h.sssp(0)
     assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
@@ -99,3 +102,25 @@ def test_hybrid_reuse_and_errors(gpu, abi, orc):
         abi.HybridEngine(hg, device_budget_bytes=1024, calibrate=0)
     info = h.info()
     assert info["partitions"] > 10 and info["device_edge_bytes"] <= SMALL["device_budget_bytes"]
+
+
+def test_hybrid_weights_move_only_when_read(gpu, abi, orc):
+    # PageRank moves columns only; SSSP afterwards must not reuse a
+    # column-only cached slot (its distances stay bit-exact)
+    hw = pinned(abi, scale=13, seed=2, wmax=255)
+    hu = pinned(abi, scale=13, seed=2, wmax=0)
+    assert np.array_equal(hw.columns, hu.columns)
+    kw = dict(SMALL, force_engine=0, cache=1, device_budget_bytes=8192 * 20 + 16 + 400 * 2048 * 8)
+    ew, eu = abi.HybridEngine(hw, **kw), abi.HybridEngine(hu, **kw)
+    rw, sw = ew.pagerank(D, 1e-6)
+    ru, su = eu.pagerank(D, 1e-6)
+    _say(f"pagerank filter bytes weighted {sw['bytes_filter']} unweighted {su['bytes_filter']}")
+    assert sw["bytes_filter"] == su["bytes_filter"] > 0
+    assert np.abs(rw - ru).sum() <= 1e-6
+    for src in (0, 7):
+        d, st = ew.sssp(src)
+        assert np.array_equal(d, orc.sssp_dijkstra(hw.offsets, hw.columns, hw.weights, src))
+    # the second SSSP finds its partitions cached with weights
+    assert st["parts_cached"] > 0
+    r2, _ = ew.pagerank(D, 1e-6)
+    assert np.abs(r2 - rw).sum() <= 1e-6
