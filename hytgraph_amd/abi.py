@@ -146,6 +146,37 @@ HYT_ENGINE_FILTER = 0
 HYT_ENGINE_COMPACTION = 1
 HYT_ENGINE_ZERO_COPY = 2
 
+HYT_QUERY_SSSP = 0
+HYT_QUERY_PPR = 1
+
+
+class Request(C.Structure):
+    _fields_ = [("graph", C.c_uint32), ("kind", C.c_uint32), ("source", C.c_uint32), ("reserved", C.c_uint32),
+                ("arrival_s", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("done_s", C.c_double), ("session", C.c_uint32), ("lane", C.c_uint32),
+                ("rounds", C.c_uint32), ("reserved", C.c_uint32), ("reached", C.c_uint64),
+                ("checksum", C.c_uint64)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_lanes", C.c_uint32), ("rounds_per_step", C.c_uint32),
+                ("damping", C.c_float), ("epsilon", C.c_float), ("reserved", C.c_uint32 * 4)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("busy_s", C.c_double), ("steps", C.c_uint64), ("rounds", C.c_uint64),
+                ("edges", C.c_uint64), ("admitted", C.c_uint64), ("graph_bytes", C.c_uint64),
+                ("state_bytes", C.c_uint64), ("sessions", C.c_uint32), ("lanes", C.c_uint32 * 16),
+                ("reserved", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("lanes", "reserved")}
+        d["lanes"] = list(self.lanes)[: self.sessions]
+        return d
+
 
 class HytError(RuntimeError):
     def __init__(self, status: int, message: str):
@@ -205,6 +236,18 @@ SIGNATURES = {
     "hyt_hybrid_sssp": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
     "hyt_hybrid_pagerank": (
         C.c_int, [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
+    "hyt_lanes_state_bytes": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u64p]),
+    "hyt_lanes_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(_P)]),
+    "hyt_lanes_free": (C.c_int, [_P]),
+    "hyt_lanes_admit": (C.c_int, [_P, C.c_uint32, C.c_uint32]),
+    "hyt_lanes_step": (C.c_int, [C.POINTER(_P), C.c_uint32, C.c_uint32, _u64p, _u64p]),
+    "hyt_lanes_read": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _u64p, _u64p, _u32p]),
+    "hyt_lanes_release": (C.c_int, [_P, C.c_uint32]),
+    "hyt_lanes_info": (C.c_int, [_P, _u32p, _u64p, _u64p, _u32p, _u64p]),
+    "hyt_serve_config_default": (C.c_int, [C.POINTER(ServeConfig)]),
+    "hyt_serve_trace": (
+        C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint64,
+                  C.POINTER(_P), C.POINTER(Completion), C.POINTER(ServeStats)]),
 }
 
 _lib = None
@@ -497,3 +540,100 @@ class HybridEngine:
             self.free()
         except Exception:
             pass
+
+
+# ------------------------------------------------------------ continuous batching
+class LaneSession:
+    """K query lanes of one kind over a DeviceGraph (kept alive by this object)."""
+
+    def __init__(self, graph: DeviceGraph, kind=HYT_QUERY_SSSP, lanes=8, damping=0.85, epsilon=1e-6):
+        self.graph = graph
+        self.kind = kind
+        self.V = graph.V
+        h = C.c_void_p()
+        check(load().hyt_lanes_create(graph._h, kind, lanes, damping, epsilon, C.byref(h)))
+        self._h = h
+
+    def admit(self, lane: int, source: int) -> None:
+        check(load().hyt_lanes_admit(self._h, lane, int(source)))
+
+    @staticmethod
+    def step(sessions, rounds=1):
+        """Runs up to `rounds` rounds on every session; returns (finished, running) lane masks."""
+        n = len(sessions)
+        arr = (C.c_void_p * max(n, 1))(*[s._h for s in sessions])
+        fin = (C.c_uint64 * max(n, 1))()
+        run = (C.c_uint64 * max(n, 1))()
+        check(load().hyt_lanes_step(arr, n, rounds, fin, run))
+        return list(fin)[:n], list(run)[:n]
+
+    def read(self, lane: int, out_ptr=None):
+        """Returns (result u32[V] / f32[V] or None with out_ptr, reached, checksum, rounds)."""
+        reached, checksum, rounds = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        if out_ptr is not None:
+            check(load().hyt_lanes_read(self._h, lane, C.c_void_p(out_ptr), 1, C.byref(reached),
+                                        C.byref(checksum), C.byref(rounds)))
+            return None, reached.value, checksum.value, rounds.value
+        out = np.empty(self.V, dtype=np.float32 if self.kind == HYT_QUERY_PPR else np.uint32)
+        check(load().hyt_lanes_read(self._h, lane, _ptr(out), 0, C.byref(reached), C.byref(checksum),
+                                    C.byref(rounds)))
+        return out, reached.value, checksum.value, rounds.value
+
+    def release(self, lane: int) -> None:
+        check(load().hyt_lanes_release(self._h, lane))
+
+    def info(self) -> dict:
+        k, occ, run, F, b = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint64()
+        check(load().hyt_lanes_info(self._h, C.byref(k), C.byref(occ), C.byref(run), C.byref(F), C.byref(b)))
+        return {"lanes": k.value, "occupied": occ.value, "running": run.value, "frontier": F.value,
+                "state_bytes": b.value}
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(load().hyt_lanes_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def lanes_state_bytes(V: int, E: int, max_degree: int, kind: int, lanes: int) -> int:
+    b = C.c_uint64()
+    check(load().hyt_lanes_state_bytes(V, E, max_degree, kind, lanes, C.byref(b)))
+    return b.value
+
+
+def serve_config(**overrides) -> ServeConfig:
+    c = ServeConfig()
+    check(load().hyt_serve_config_default(C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+def serve_trace(graphs, requests, keep_results=False, **config):
+    """Replays requests = [(arrival_s, graph_index, kind, source), ...] (sorted by
+    arrival). Returns (completions as a structured numpy array, stats dict,
+    results list or None)."""
+    n = len(requests)
+    req = (Request * max(n, 1))()
+    for i, (t, gi, kind, src) in enumerate(requests):
+        req[i] = Request(int(gi), int(kind), int(src), 0, float(t))
+    garr = (C.c_void_p * len(graphs))(*[g._h for g in graphs])
+    comp = (Completion * max(n, 1))()
+    st = ServeStats()
+    cfg = serve_config(**config)
+    results = None
+    res_arr = None
+    if keep_results:
+        results = [np.empty(graphs[r[1]].V, dtype=np.float32 if r[2] == HYT_QUERY_PPR else np.uint32)
+                   for r in requests]
+        res_arr = (C.c_void_p * max(n, 1))(*[a.ctypes.data for a in results])
+    check(load().hyt_serve_trace(garr, len(graphs), C.byref(cfg), req, n, res_arr, comp, C.byref(st)))
+    dt = np.dtype([("admit_s", "f8"), ("done_s", "f8"), ("session", "u4"), ("lane", "u4"), ("rounds", "u4"),
+                   ("reserved", "u4"), ("reached", "u8"), ("checksum", "u8")])
+    arr = np.frombuffer(bytes(comp), dtype=dt, count=n).copy() if n else np.zeros(0, dt)
+    return arr, st.as_dict(), results

@@ -62,6 +62,7 @@ namespace hyt
     struct DeviceGraph
     {
         std::uint64_t V = 0, E = 0;
+        std::uint64_t max_degree = 0;
         bool off64 = false;
         bool weighted = false;
         bool weights_f32 = false;

@@ -111,6 +111,8 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
         g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters));
 
         const std::uint64_t V = num_vertices, E = num_edges;
+        for (std::uint64_t v = 0; v < V; ++v)
+            g.max_degree = std::max<std::uint64_t>(g.max_degree, row_offsets[v + 1] - row_offsets[v]);
         if (g.off64)
         {
             g.off.alloc((V + 1) * 8, "hipMalloc(row offsets)");
@@ -133,7 +135,8 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
             if (E)
                 hip_check(hipMemcpy(g.w.ptr, weights, E * 4, hipMemcpyHostToDevice), "copy weights");
         }
-        ensure_workspace(g, kDefaultLightMaxDegree, kDefaultHeavyChunk);
+        // the traversal workspace is allocated by the first single-source or
+        // batched call (lane sessions bring their own)
         hip_check(hipStreamSynchronize(g.stream), "upload synchronize");
         *out = h.release();
     });

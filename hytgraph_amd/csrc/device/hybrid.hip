@@ -482,6 +482,8 @@ namespace hyt
             g.device = require_device();
             g.V = host->V;
             g.E = host->E;
+            for (std::uint64_t v = 0; v < host->V; ++v)
+                g.max_degree = std::max<std::uint64_t>(g.max_degree, host->off[v + 1] - host->off[v]);
             g.off64 = host->E >= 0xFFFFFFFFULL;
             g.weighted = host->weighted;
             g.weights_f32 = f32;

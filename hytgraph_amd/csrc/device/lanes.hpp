@@ -1,0 +1,359 @@
+// K-lane frontier kernels shared by the batched [V][K] queries and the
+// continuous-batching lane sessions.
+//
+// State is row-major [V][K]: the K query values of a vertex are contiguous,
+// so one edge (u, v) is handled by a group of KP = next_pow2(K) lanes that
+// read and update K consecutive words of row u and row v - coalesced, and
+// the column/weight of the edge are read once for all K queries. A wavefront
+// holds G = 64 / KP such groups, i.e. walks G edges per step.
+//
+// Frontier: a queue of rows plus, per row, the u64 mask of queries for which
+// the row is active. During a round, the lanes that improve v OR their bits
+// into mark[v] (non-returning atomic, one per edge group); the compaction pass
+// scans mark[], moves every non-zero mask into the next queue and clears it.
+// A queue may hold one vertex more than once with disjoint masks (a query
+// admitted into a running batch appends its source row).
+#pragma once
+
+#include "compact.hpp"
+#include "expand.hpp"
+
+namespace hyt
+{
+    namespace lanes
+    {
+        namespace // each translation unit gets its own kernels
+        {
+        struct HeavyChunkK
+        {
+            std::uint32_t row;   // SSSP: vertex id; PPR: queue index
+            std::uint32_t count; // edges in the chunk
+            std::uint64_t begin; // first edge
+            std::uint64_t mask;  // active lanes of the row
+            std::uint32_t scale; // PPR: damping / outdeg (float bits)
+            std::uint32_t pad;
+        };
+
+        struct KArgs
+        {
+            const std::uint32_t *q;
+            const unsigned long long *qmask;
+            std::uint32_t F;
+            Counters *ctr;
+            HeavyChunkK *heavy;
+            std::uint64_t heavy_cap;
+            std::uint32_t light_max;
+            std::uint32_t chunk;
+            std::uint32_t K;
+            unsigned long long *mark;
+        };
+
+        template <int KP>
+        __device__ __forceinline__ std::uint64_t group_bits(std::uint64_t ballot, int group)
+        {
+            if constexpr (KP == 64)
+                return ballot;
+            else
+                return (ballot >> (group * KP)) & ((1ULL << KP) - 1ULL);
+        }
+
+        // Walks `total` edges, G per step, kUnroll steps in flight.
+        template <int KP, class Op, class PosFn>
+        __device__ __forceinline__ void walk_edges_k(const Op &op, KArgs &a, std::uint32_t total, PosFn pos_of)
+        {
+            constexpr int G = kWave / KP;
+            const int lane = lane_id();
+            const int group = lane / KP;
+            const int kl = lane % KP;
+            for (std::uint32_t base = 0; base < total; base += G * kUnroll)
+            {
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll];
+                std::uint64_t m[kUnroll];
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const std::uint32_t e = base + static_cast<std::uint32_t>(k * G + group);
+                    m[k] = 0;
+                    vv[k] = ww[k] = row[k] = scale[k] = 0;
+                    if (e < total)
+                    {
+                        std::uint64_t pos;
+                        pos_of(e, pos, row[k], m[k], scale[k]);
+                        op.load(pos, vv[k], ww[k]);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const bool act = static_cast<std::uint32_t>(kl) < a.K && ((m[k] >> kl) & 1ULL);
+                    const bool imp = act && op.update(row[k], scale[k], vv[k], ww[k], kl);
+                    const std::uint64_t gb = group_bits<KP>(__ballot(imp), group);
+                    if (kl == 0 && gb != 0)
+                        atomicOr(&a.mark[vv[k]], static_cast<unsigned long long>(gb));
+                }
+            }
+        }
+
+        template <class OffT, int KP, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_light(const OffT *__restrict__ off, Op op, KArgs a)
+        {
+            __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
+            __shared__ OffT s_begin[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
+            __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
+
+            const int lane = lane_id();
+            const int wid = static_cast<int>(threadIdx.x) / kWave;
+            const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+            std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
+            std::uint64_t mask = 0;
+            OffT begin = 0, full = 0;
+            if (i < a.F)
+            {
+                const std::uint32_t u = a.q[i];
+                begin = off[u];
+                full = off[u + 1] - begin;
+                mask = a.qmask[i];
+                row = Op::kRowIsQueueIndex ? i : u;
+                scale = op.row_scale(static_cast<std::uint64_t>(full));
+                if (full > a.light_max)
+                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
+                else
+                    deg = static_cast<std::uint32_t>(full);
+            }
+            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                HeavyChunkK h;
+                h.row = row;
+                h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+                h.begin = static_cast<std::uint64_t>(begin) + lo;
+                h.mask = mask;
+                h.scale = scale;
+                h.pad = 0;
+                return h;
+            });
+            const std::uint32_t incl = wave_incl_scan(deg);
+            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+            s_ex[wid][lane] = incl - deg;
+            s_begin[wid][lane] = begin;
+            s_row[wid][lane] = row;
+            s_scale[wid][lane] = scale;
+            s_mask[wid][lane] = mask;
+            __syncthreads();
+            if (total == 0)
+                return;
+            if (lane == 0)
+                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
+            const std::uint32_t *ex = s_ex[wid];
+            walk_edges_k<KP>(op, a, total,
+                             [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
+                                 std::uint32_t &sc) {
+                                 const int j = owner_of(ex, e);
+                                 pos = static_cast<std::uint64_t>(s_begin[wid][j]) + (e - ex[j]);
+                                 r = s_row[wid][j];
+                                 m = s_mask[wid][j];
+                                 sc = s_scale[wid][j];
+                             });
+        }
+
+        template <int KP, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_heavy(Op op, KArgs a)
+        {
+            const int lane = lane_id();
+            const std::uint32_t nh = a.ctr->heavy_count;
+            const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
+            const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+            for (std::uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kWave; c < n; c += waves)
+            {
+                const HeavyChunkK h = a.heavy[c];
+                if (lane == 0)
+                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                walk_edges_k<KP>(op, a, h.count,
+                                 [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
+                                     std::uint32_t &sc) {
+                                     pos = h.begin + e;
+                                     r = h.row;
+                                     m = h.mask;
+                                     sc = h.scale;
+                                 });
+            }
+        }
+
+        // ---------------------------------------------------------- SSSP-K
+        template <int WM>
+        struct SsspKOp
+        {
+            static constexpr bool kRowIsQueueIndex = false;
+            const std::uint32_t *__restrict__ col;
+            const std::uint32_t *__restrict__ w;
+            std::uint32_t *dist;
+            std::uint32_t K;
+            std::uint32_t *overflow;
+
+            __device__ std::uint32_t row_scale(std::uint64_t) const { return 0U; }
+            __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
+            {
+                v = col[pos];
+                if constexpr (WM == kWU32 || WM == kWF32)
+                    wt = w[pos];
+                else if constexpr (WM == kWUnitU32)
+                    wt = 1U;
+                else
+                    wt = 0x3F800000U;
+            }
+            __device__ bool update(std::uint32_t u, std::uint32_t, std::uint32_t v, std::uint32_t wt, int kl) const
+            {
+                const std::uint32_t du = dist[static_cast<std::uint64_t>(u) * K + kl];
+                std::uint32_t c;
+                if constexpr (WM == kWU32 || WM == kWUnitU32)
+                {
+                    const std::uint64_t s = static_cast<std::uint64_t>(du) + wt;
+                    if (s >= 0xFFFFFFFFULL)
+                    {
+                        atomicOr(overflow, 1U);
+                        return false;
+                    }
+                    c = static_cast<std::uint32_t>(s);
+                }
+                else
+                {
+                    c = __float_as_uint(__fadd_rn(__uint_as_float(du), __uint_as_float(wt)));
+                }
+                std::uint32_t *p = dist + static_cast<std::uint64_t>(v) * K + kl;
+                if (c >= *p)
+                    return false;
+                atomicMin(p, c);
+                return true;
+            }
+        };
+
+        // ----------------------------------------------------------- PPR-K
+        struct PprKOp
+        {
+            static constexpr bool kRowIsQueueIndex = true;
+            const std::uint32_t *__restrict__ col;
+            float *res;
+            const float *snap; // [F][K] snapshot residuals
+            std::uint32_t K;
+            float damping;
+            float eps;
+
+            __device__ std::uint32_t row_scale(std::uint64_t deg) const
+            {
+                return deg ? __float_as_uint(damping / static_cast<float>(deg)) : 0U;
+            }
+            __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
+            {
+                v = col[pos];
+                wt = 0;
+            }
+            __device__ bool update(std::uint32_t i, std::uint32_t scale, std::uint32_t v, std::uint32_t, int kl) const
+            {
+                const float add = __uint_as_float(scale) * snap[static_cast<std::uint64_t>(i) * K + kl];
+                if (!(add > 0.0F))
+                    return false;
+                const float old = unsafeAtomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl], add);
+                return old <= eps && old + add > eps;
+            }
+        };
+
+        // Moves the active residuals of the frontier rows into rank.
+        __global__ void pprk_snapshot(const std::uint32_t *q, std::uint32_t F, std::uint32_t K,
+                                      const unsigned long long *qmask, float *res, float *rank, float *snap)
+        {
+            const std::uint64_t n = static_cast<std::uint64_t>(F) * K;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+                 t += stride)
+            {
+                const std::uint32_t i = static_cast<std::uint32_t>(t / K);
+                const std::uint32_t k = static_cast<std::uint32_t>(t % K);
+                const std::uint32_t u = q[i];
+                float r = 0.0F;
+                if ((qmask[i] >> k) & 1ULL)
+                {
+                    const std::uint64_t idx = static_cast<std::uint64_t>(u) * K + k;
+                    r = res[idx];
+                    res[idx] = 0.0F;
+                    rank[idx] += r;
+                }
+                snap[t] = r;
+            }
+        }
+
+        inline unsigned snapshot_blocks(std::uint64_t n)
+        {
+            const std::uint64_t b = (n + kBlock - 1) / kBlock;
+            return static_cast<unsigned>(b == 0 ? 1 : (b < 65536 ? b : 65536));
+        }
+
+        int pow2_at_least(std::uint32_t K)
+        {
+            int kp = 1;
+            while (kp < static_cast<int>(K))
+                kp <<= 1;
+            return kp < 2 ? 2 : kp;
+        }
+
+        // Rows with a non-zero mark become the next queue; the mark moves into
+        // qmask and is cleared.
+        struct MarkSel
+        {
+            struct Aux
+            {
+                unsigned long long lanes = 0; // OR of the selected masks
+            };
+            unsigned long long *mark;
+            unsigned long long *lanes_out; // optional: OR of every selected mask
+
+            __device__ void publish(const Aux &aux, Counters *) const
+            {
+                if (lanes_out == nullptr)
+                    return;
+                std::uint32_t lo = static_cast<std::uint32_t>(aux.lanes);
+                std::uint32_t hi = static_cast<std::uint32_t>(aux.lanes >> 32);
+#pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1)
+                {
+                    lo |= __shfl_xor(lo, d, kWave);
+                    hi |= __shfl_xor(hi, d, kWave);
+                }
+                const unsigned long long all = (static_cast<unsigned long long>(hi) << 32) | lo;
+                if (lane_id() == 0 && all != 0ULL)
+                    atomicOr(lanes_out, all);
+            }
+            __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n,
+                                             unsigned long long (&vals)[4], Aux &aux) const
+            {
+                unsigned long long m4[4];
+                if (n == 4)
+                {
+                    const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(mark + row0);
+                    const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(mark + row0 + 2);
+                    m4[0] = a.x, m4[1] = a.y, m4[2] = b.x, m4[3] = b.y;
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        m4[j] = static_cast<std::uint32_t>(j) < n ? mark[row0 + j] : 0ULL;
+                }
+                std::uint32_t sel = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (m4[j])
+                    {
+                        mark[row0 + j] = 0ULL;
+                        vals[j] = m4[j];
+                        aux.lanes |= m4[j];
+                        sel |= 1U << j;
+                    }
+                return sel;
+            }
+        };
+
+        } // namespace
+    } // namespace lanes
+} // namespace hyt

@@ -1,0 +1,815 @@
+// Continuous batching: K-lane query sessions and the request-serving loop.
+//
+// A lane session holds K query slots ("lanes") of one kind (SSSP or PPR) over
+// one HBM-resident graph, with its own [V][K] state, frontier, counters and
+// stream. Rounds run for all occupied lanes at once (the K-lane kernels of
+// lanes.hpp: each edge is read once for every lane whose row is active). A
+// query is admitted into a free lane between rounds: its column of the state
+// is reset, its source row is appended to the current queue with its lane
+// bit, and it starts in the next round next to queries that are already
+// running. A lane finishes when the compaction pass finds no frontier row
+// carrying its bit; its result stays in the state until the lane is reused.
+//
+// Sessions of different graphs and kinds run on their own streams, so one
+// step launches a round on every busy session before waiting for any of
+// them: their rounds overlap on the device.
+//
+// The serving loop replays a request trace (arrival times, graph, kind,
+// source) against the wall clock: arrivals queue per session, are admitted
+// as lanes free up, lanes per session are sized from the device budget
+// (admission by memory), and each request's admit and completion times are
+// recorded.
+#include "lanes.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <vector>
+
+namespace hyt
+{
+    namespace
+    {
+        using namespace lanes;
+
+        // a[v*K + k] = value for every v.
+        __global__ void lane_fill(std::uint32_t *a, std::uint64_t V, std::uint32_t K, std::uint32_t k,
+                                  std::uint32_t value)
+        {
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+                a[v * K + k] = value;
+        }
+
+        // Seeds lane k at source s and appends (s, bit k) to the queue at F.
+        __global__ void lane_seed(std::uint32_t *val, std::uint32_t K, std::uint32_t k, std::uint32_t s,
+                                  std::uint32_t seed, std::uint32_t *q, unsigned long long *qmask, std::uint32_t F)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+            {
+                val[static_cast<std::uint64_t>(s) * K + k] = seed;
+                q[F] = s;
+                qmask[F] = 1ULL << k;
+            }
+        }
+
+        // out[v] = val[v][k] (SSSP) or rank[v][k] + res[v][k] (PPR), with
+        // per-block partial sums of x in double (PPR normalisation).
+        __global__ __launch_bounds__(kBlock) void lane_gather(const std::uint32_t *val, const float *rank,
+                                                              std::uint64_t V, std::uint32_t K, std::uint32_t k,
+                                                              std::uint32_t *out, double *partial)
+        {
+            __shared__ double s[kBlock];
+            double acc = 0.0;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+            {
+                const std::uint64_t idx = v * K + k;
+                if (rank != nullptr)
+                {
+                    const float x = rank[idx] + __uint_as_float(val[idx]);
+                    out[v] = __float_as_uint(x);
+                    acc += static_cast<double>(x);
+                }
+                else
+                    out[v] = val[idx];
+            }
+            if (partial == nullptr)
+                return;
+            s[threadIdx.x] = acc;
+            __syncthreads();
+            for (int d = kBlock / 2; d > 0; d >>= 1)
+            {
+                if (static_cast<int>(threadIdx.x) < d)
+                    s[threadIdx.x] += s[threadIdx.x + d];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+                partial[blockIdx.x] = s[0];
+        }
+
+        __global__ void lane_total(const double *partial, unsigned n, double *total)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+            {
+                double t = 0.0;
+                for (unsigned b = 0; b < n; ++b)
+                    t += partial[b];
+                total[0] = t;
+            }
+        }
+
+        __global__ void lane_normalize(std::uint32_t *x, std::uint64_t V, const double *total)
+        {
+            const double t = total[0];
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+                x[v] = __float_as_uint(
+                    static_cast<float>(t > 0.0 ? static_cast<double>(__uint_as_float(x[v])) / t : 0.0));
+        }
+
+        // Entries != skip and the u64 sum of their bit patterns (result checksum).
+        __global__ __launch_bounds__(kBlock) void lane_checksum(const std::uint32_t *x, std::uint64_t V,
+                                                                std::uint32_t skip, unsigned long long *acc)
+        {
+            unsigned long long n = 0, sum = 0;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+                if (x[v] != skip)
+                {
+                    n += 1;
+                    sum += x[v];
+                }
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+            {
+                n += __shfl_xor(n, d, kWave);
+                sum += __shfl_xor(sum, d, kWave);
+            }
+            if (lane_id() == 0 && n)
+            {
+                atomicAdd(&acc[0], n);
+                atomicAdd(&acc[1], sum);
+            }
+        }
+
+        constexpr unsigned kGatherBlocks = 1024;
+
+        unsigned grid_for(std::uint64_t n)
+        {
+            const std::uint64_t b = (n + kBlock - 1) / kBlock;
+            return static_cast<unsigned>(b == 0 ? 1 : (b < kGatherBlocks ? b : kGatherBlocks));
+        }
+    } // namespace
+
+    struct LaneSession
+    {
+        DeviceGraph *g = nullptr; // borrowed
+        int kind = HYT_QUERY_SSSP;
+        std::uint32_t K = 0, KP = 0;
+        float damping = 0.85F, eps = 1e-6F;
+        int wm = kWU32;
+        std::uint32_t inf_bits = 0xFFFFFFFFU, seed_bits = 0;
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        DeviceBuffer val, rank, snap, mark, qmask, q, heavy, ctr, lanes, colbuf, partial, acc;
+        std::uint64_t heavy_cap = 0;
+        std::unique_ptr<PinnedBuffer> host; // Counters | lanes u64 | checksum u64[2]
+        std::uint32_t F = 0;
+        std::uint32_t appended = 0; // rows appended by admissions since the last compaction
+        std::uint64_t running = 0; // lanes holding a query that has not finished
+        std::uint64_t occupied = 0; // lanes not free (running, or finished and unread)
+        bool launched = false;
+        std::uint32_t lane_rounds[64] = {};
+        std::uint64_t rounds = 0, edges = 0;
+
+        ~LaneSession()
+        {
+            if (stream)
+            {
+                (void)hipStreamSynchronize(stream);
+                (void)hipStreamDestroy(stream);
+            }
+            if (done)
+                (void)hipEventDestroy(done);
+        }
+
+        // A queue holds each vertex once after a compaction, plus up to 64
+        // rows appended by admissions (a vertex may then appear twice).
+        static constexpr std::uint64_t kAppend = 64;
+
+        // Heavy chunks: every row of degree > light_max once, plus kAppend
+        // appended rows of at most max_degree each.
+        static std::uint64_t heavy_chunks(std::uint64_t E, std::uint64_t max_degree)
+        {
+            return E / kDefaultHeavyChunk + E / (kDefaultLightMaxDegree + 1ULL) + 64 +
+                   kAppend * (max_degree / kDefaultHeavyChunk + 1);
+        }
+        static std::uint64_t fixed_bytes(std::uint64_t V, std::uint64_t E, std::uint64_t max_degree)
+        {
+            const std::uint64_t Vp = V + kPad;
+            return Vp * 8 + (Vp + kAppend) * 8 + (Vp + kAppend) * 4 + heavy_chunks(E, max_degree) * sizeof(HeavyChunkK) +
+                   Vp * 4 + (kGatherBlocks + 1) * sizeof(double) + sizeof(Counters) + 8 + 16;
+        }
+        // val [V][K] (+ rank [V][K] and snapshot [V+kAppend][K] for PPR)
+        static std::uint64_t lane_bytes(std::uint64_t V, int kind)
+        {
+            const std::uint64_t Vp = V + kPad;
+            return kind == HYT_QUERY_PPR ? Vp * 4 * 2 + (Vp + kAppend) * 4 : Vp * 4;
+        }
+    };
+
+    namespace
+    {
+        void create_session(LaneSession &S, DeviceGraph *g, int kind, std::uint32_t K, float damping, float eps)
+        {
+            require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
+            require(K >= 1 && K <= 64, "lanes must be in [1, 64]");
+            require(g->V <= 0xFFFFFFF0ULL, "vertex count too large for lane sessions");
+            if (kind == HYT_QUERY_PPR)
+            {
+                require(damping >= 0.0F && damping < 1.0F, "PageRank damping_factor must be in [0, 1)");
+                require(eps > 0.0F && std::isfinite(eps), "PageRank epsilon must be finite and positive");
+            }
+            S.g = g;
+            S.kind = kind;
+            S.K = K;
+            S.KP = static_cast<std::uint32_t>(pow2_at_least(K));
+            S.damping = damping;
+            S.eps = eps;
+            S.wm = g->weight_mode();
+            if (kind == HYT_QUERY_SSSP)
+            {
+                S.inf_bits = (S.wm == kWF32 || S.wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
+                S.seed_bits = 0U;
+            }
+            else
+            {
+                S.inf_bits = 0U; // residuals start at 0
+                const float r0 = 1.0F - damping;
+                std::memcpy(&S.seed_bits, &r0, 4);
+            }
+            hip_check(hipSetDevice(g->device), "hipSetDevice");
+            hip_check(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&S.done, hipEventDisableTiming), "hipEventCreate");
+            const std::uint64_t V = g->V, Vp = V + kPad;
+            const std::uint64_t QA = LaneSession::kAppend;
+            S.val.alloc(Vp * K * 4, "hipMalloc(lane state)");
+            if (kind == HYT_QUERY_PPR)
+            {
+                S.rank.alloc(Vp * K * 4, "hipMalloc(lane ranks)");
+                S.snap.alloc((Vp + QA) * K * 4, "hipMalloc(lane snapshot)"); // [F][K], F <= V + QA
+            }
+            S.mark.alloc(Vp * 8, "hipMalloc(lane marks)");
+            S.qmask.alloc((Vp + QA) * 8, "hipMalloc(lane queue masks)");
+            S.q.alloc((Vp + QA) * 4, "hipMalloc(lane queue)");
+            const std::uint64_t cap = LaneSession::heavy_chunks(g->E, g->max_degree);
+            S.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(lane heavy chunks)");
+            S.heavy_cap = cap;
+            S.ctr.alloc(sizeof(Counters), "hipMalloc(lane counters)");
+            S.lanes.alloc(8, "hipMalloc(lane mask)");
+            S.colbuf.alloc(Vp * 4, "hipMalloc(lane output)");
+            S.partial.alloc((kGatherBlocks + 1) * sizeof(double), "hipMalloc(lane partials)");
+            S.acc.alloc(16, "hipMalloc(lane checksum)");
+            S.host = std::make_unique<PinnedBuffer>(sizeof(Counters) + 8 + 16);
+            hip_check(hipMemsetAsync(S.mark.ptr, 0, Vp * 8, S.stream), "clear marks");
+            hip_check(hipMemsetAsync(S.ctr.ptr, 0, sizeof(Counters), S.stream), "clear counters");
+            // every lane starts free with a defined state
+            launch_fill<std::uint32_t>(S.val.as<std::uint32_t>(), Vp * K, S.inf_bits, S.stream);
+            if (kind == HYT_QUERY_PPR)
+                launch_fill<float>(S.rank.as<float>(), Vp * K, 0.0F, S.stream);
+            hip_check(hipStreamSynchronize(S.stream), "lane session init");
+        }
+
+        void admit(LaneSession &S, std::uint32_t lane, std::uint32_t source)
+        {
+            require(lane < S.K, "lane out of range");
+            require(((S.occupied >> lane) & 1ULL) == 0, "lane is occupied");
+            require(source < S.g->V, "source vertex is out of range");
+            require(!S.launched, "a round is in flight");
+            require(S.appended < LaneSession::kAppend, "too many admissions since the last round");
+            const std::uint64_t V = S.g->V;
+            lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.val.as<std::uint32_t>(), V, S.K, lane, S.inf_bits);
+            hip_check(hipGetLastError(), "launch lane_fill");
+            if (S.kind == HYT_QUERY_PPR)
+            {
+                lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.rank.as<std::uint32_t>(), V, S.K, lane, 0U);
+                hip_check(hipGetLastError(), "launch lane_fill");
+            }
+            lane_seed<<<1, 64, 0, S.stream>>>(S.val.as<std::uint32_t>(), S.K, lane, source, S.seed_bits,
+                                              S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>(), S.F);
+            hip_check(hipGetLastError(), "launch lane_seed");
+            S.F += 1;
+            S.appended += 1;
+            S.running |= 1ULL << lane;
+            S.occupied |= 1ULL << lane;
+            S.lane_rounds[lane] = 0;
+        }
+
+        template <class OffT, int KP, class Op>
+        void launch_round_t(LaneSession &S, const Op &op)
+        {
+            DeviceGraph &g = *S.g;
+            hipStream_t s = S.stream;
+            Counters *d_ctr = S.ctr.as<Counters>();
+            hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset lane counters");
+            hip_check(hipMemsetAsync(S.lanes.ptr, 0, 8, s), "reset lane mask");
+            KArgs a;
+            a.q = S.q.as<std::uint32_t>();
+            a.qmask = S.qmask.as<unsigned long long>();
+            a.F = S.F;
+            a.ctr = d_ctr;
+            a.heavy = S.heavy.as<HeavyChunkK>();
+            a.heavy_cap = S.heavy_cap;
+            a.light_max = kDefaultLightMaxDegree;
+            a.chunk = kDefaultHeavyChunk;
+            a.K = S.K;
+            a.mark = S.mark.as<unsigned long long>();
+            if (S.kind == HYT_QUERY_PPR)
+            {
+                const std::uint64_t n = static_cast<std::uint64_t>(S.F) * S.K;
+                pprk_snapshot<<<snapshot_blocks(n), kBlock, 0, s>>>(
+                    a.q, S.F, S.K, a.qmask, reinterpret_cast<float *>(S.val.ptr), S.rank.as<float>(),
+                    S.snap.as<float>());
+                hip_check(hipGetLastError(), "launch pprk_snapshot");
+            }
+            expandk_light<OffT, KP, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+            hip_check(hipGetLastError(), "launch expandk_light");
+            expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+            hip_check(hipGetLastError(), "launch expandk_heavy");
+            MarkSel sel{S.mark.as<unsigned long long>(), S.lanes.as<unsigned long long>()};
+            compact_rows<unsigned long long, std::uint32_t, MarkSel><<<compact_blocks(g.V), kBlock, 0, s>>>(
+                g.V, sel, S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>(), d_ctr, nullptr, nullptr);
+            hip_check(hipGetLastError(), "launch lane compaction");
+            unsigned char *h = S.host->as<unsigned char>();
+            hip_check(hipMemcpyAsync(h, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read lane counters");
+            hip_check(hipMemcpyAsync(h + sizeof(Counters), S.lanes.ptr, 8, hipMemcpyDeviceToHost, s),
+                      "read lane mask");
+            hip_check(hipEventRecord(S.done, s), "hipEventRecord");
+        }
+
+        template <class OffT, int KP>
+        void launch_kind(LaneSession &S)
+        {
+            DeviceGraph &g = *S.g;
+            if (S.kind == HYT_QUERY_PPR)
+            {
+                PprKOp op;
+                op.col = g.col.as<std::uint32_t>();
+                op.res = reinterpret_cast<float *>(S.val.ptr);
+                op.snap = S.snap.as<float>();
+                op.K = S.K;
+                op.damping = S.damping;
+                op.eps = S.eps;
+                launch_round_t<OffT, KP>(S, op);
+                return;
+            }
+            auto run = [&](auto op) {
+                op.col = g.col.as<std::uint32_t>();
+                op.w = g.w.as<std::uint32_t>();
+                op.dist = S.val.as<std::uint32_t>();
+                op.K = S.K;
+                op.overflow = &S.ctr.as<Counters>()->overflow;
+                launch_round_t<OffT, KP>(S, op);
+            };
+            switch (S.wm)
+            {
+            case kWU32: run(SsspKOp<kWU32>{}); break;
+            case kWF32: run(SsspKOp<kWF32>{}); break;
+            case kWUnitU32: run(SsspKOp<kWUnitU32>{}); break;
+            default: run(SsspKOp<kWUnitF32>{}); break;
+            }
+        }
+
+        template <class OffT>
+        void launch_kp(LaneSession &S)
+        {
+            switch (S.KP)
+            {
+            case 2: launch_kind<OffT, 2>(S); break;
+            case 4: launch_kind<OffT, 4>(S); break;
+            case 8: launch_kind<OffT, 8>(S); break;
+            case 16: launch_kind<OffT, 16>(S); break;
+            case 32: launch_kind<OffT, 32>(S); break;
+            default: launch_kind<OffT, 64>(S); break;
+            }
+        }
+
+        void launch_round(LaneSession &S)
+        {
+            S.launched = false;
+            if (S.F == 0)
+                return;
+            hip_check(hipSetDevice(S.g->device), "hipSetDevice");
+            if (S.g->off64)
+                launch_kp<std::uint64_t>(S);
+            else
+                launch_kp<std::uint32_t>(S);
+            S.launched = true;
+        }
+
+        // Waits for the round, updates the frontier size and the lane masks;
+        // returns the lanes that finished in it.
+        std::uint64_t finish_round(LaneSession &S)
+        {
+            if (!S.launched)
+                return 0;
+            S.launched = false;
+            hip_check(hipEventSynchronize(S.done), "lane round");
+            const unsigned char *h = S.host->as<unsigned char>();
+            Counters c;
+            std::memcpy(&c, h, sizeof(Counters));
+            std::uint64_t active;
+            std::memcpy(&active, h + sizeof(Counters), 8);
+            if (c.heavy_overflow)
+                fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
+            if (c.overflow)
+                fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
+            for (std::uint32_t k = 0; k < S.K; ++k)
+                if ((S.running >> k) & 1ULL)
+                    S.lane_rounds[k] += 1;
+            S.rounds += 1;
+            S.edges += c.edges;
+            S.F = c.next_count;
+            S.appended = 0;
+            const std::uint64_t finished = S.running & ~active;
+            S.running &= active;
+            return finished;
+        }
+
+        // Result of a finished lane into S.colbuf (SSSP distances, or PPR
+        // ranks normalised to sum 1); optional checksum (entries != skip,
+        // u64 sum of their bits) and copy to out.
+        void read_lane(LaneSession &S, std::uint32_t lane, void *out, bool out_on_device, std::uint64_t *reached,
+                       std::uint64_t *checksum)
+        {
+            require(lane < S.K, "lane out of range");
+            require(((S.occupied >> lane) & 1ULL) != 0, "lane holds no query");
+            require(((S.running >> lane) & 1ULL) == 0, "lane is still running");
+            require(!S.launched, "a round is in flight");
+            const std::uint64_t V = S.g->V;
+            hipStream_t s = S.stream;
+            const bool ppr = S.kind == HYT_QUERY_PPR;
+            double *partial = S.partial.as<double>();
+            const unsigned nb = grid_for(V);
+            lane_gather<<<nb, kBlock, 0, s>>>(S.val.as<std::uint32_t>(), ppr ? S.rank.as<float>() : nullptr, V, S.K,
+                                              lane, S.colbuf.as<std::uint32_t>(), ppr ? partial : nullptr);
+            hip_check(hipGetLastError(), "launch lane_gather");
+            if (ppr)
+            {
+                lane_total<<<1, 64, 0, s>>>(partial, nb, partial + kGatherBlocks);
+                hip_check(hipGetLastError(), "launch lane_total");
+                lane_normalize<<<nb, kBlock, 0, s>>>(S.colbuf.as<std::uint32_t>(), V, partial + kGatherBlocks);
+                hip_check(hipGetLastError(), "launch lane_normalize");
+            }
+            std::uint64_t *hacc = reinterpret_cast<std::uint64_t *>(S.host->as<unsigned char>() + sizeof(Counters) + 8);
+            if (reached || checksum)
+            {
+                hip_check(hipMemsetAsync(S.acc.ptr, 0, 16, s), "clear checksum");
+                lane_checksum<<<nb, kBlock, 0, s>>>(S.colbuf.as<std::uint32_t>(), V, ppr ? 0U : S.inf_bits,
+                                                    S.acc.as<unsigned long long>());
+                hip_check(hipGetLastError(), "launch lane_checksum");
+                hip_check(hipMemcpyAsync(hacc, S.acc.ptr, 16, hipMemcpyDeviceToHost, s), "read checksum");
+            }
+            if (out)
+                hip_check(hipMemcpyAsync(out, S.colbuf.ptr, V * 4,
+                                         out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s),
+                          "copy lane result");
+            hip_check(hipStreamSynchronize(s), "lane read");
+            if (reached)
+                *reached = hacc[0];
+            if (checksum)
+                *checksum = hacc[1];
+        }
+
+        void step(LaneSession *const *ss, std::uint32_t n, std::uint32_t rounds, std::uint64_t *finished)
+        {
+            for (std::uint32_t i = 0; i < n; ++i)
+                finished[i] = 0;
+            for (std::uint32_t r = 0; r < rounds; ++r)
+            {
+                bool any = false;
+                for (std::uint32_t i = 0; i < n; ++i)
+                {
+                    launch_round(*ss[i]);
+                    any = any || ss[i]->launched;
+                }
+                if (!any)
+                    break;
+                for (std::uint32_t i = 0; i < n; ++i)
+                    finished[i] |= finish_round(*ss[i]);
+            }
+        }
+    } // namespace
+} // namespace hyt
+
+struct hyt_lanes
+{
+    hyt::LaneSession s;
+};
+
+namespace hyt
+{
+    namespace
+    {
+        using Clock = std::chrono::steady_clock;
+
+        double secs(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+        // Lanes per session from the budget: every session gets one lane,
+        // then lanes are added round-robin (cheapest lane first) while the
+        // state fits and demand is left.
+        std::vector<std::uint32_t> plan_lanes(const std::vector<std::uint64_t> &fixed,
+                                              const std::vector<std::uint64_t> &per_lane,
+                                              const std::vector<std::uint64_t> &demand, std::uint64_t avail,
+                                              std::uint32_t max_lanes)
+        {
+            const std::size_t n = fixed.size();
+            std::vector<std::uint32_t> K(n, 1);
+            std::uint64_t used = 0;
+            for (std::size_t i = 0; i < n; ++i)
+                used += fixed[i] + per_lane[i];
+            if (used > avail)
+                fail(HYT_ERR_OUT_OF_MEMORY, "device budget too small for one lane per session (" +
+                                                std::to_string(used) + " bytes needed, " + std::to_string(avail) +
+                                                " available)");
+            std::vector<std::size_t> order(n);
+            for (std::size_t i = 0; i < n; ++i)
+                order[i] = i;
+            std::sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) { return per_lane[a] < per_lane[b]; });
+            bool grew = true;
+            while (grew)
+            {
+                grew = false;
+                for (std::size_t i : order)
+                {
+                    if (K[i] >= max_lanes || K[i] >= demand[i] || used + per_lane[i] > avail)
+                        continue;
+                    K[i] += 1;
+                    used += per_lane[i];
+                    grew = true;
+                }
+            }
+            return K;
+        }
+
+        void serve_trace(const hyt_graph_t *graphs, std::uint32_t ngraphs, const hyt_serve_config &cfg,
+                         const hyt_request *reqs, std::uint64_t n, void *const *results, hyt_completion *out,
+                         hyt_serve_stats &st)
+        {
+            require(graphs != nullptr && ngraphs > 0, "no graphs");
+            require(n == 0 || (reqs != nullptr && out != nullptr), "null requests or completions");
+            require(cfg.max_lanes >= 1 && cfg.max_lanes <= 64, "max_lanes must be in [1, 64]");
+            for (std::uint64_t i = 0; i < n; ++i)
+            {
+                require(reqs[i].graph < ngraphs, "request graph index out of range");
+                require(reqs[i].kind == HYT_QUERY_SSSP || reqs[i].kind == HYT_QUERY_PPR, "request kind");
+                require(graphs[reqs[i].graph] != nullptr, "null graph");
+                require(reqs[i].source < graphs[reqs[i].graph]->g.V, "request source out of range");
+                require(std::isfinite(reqs[i].arrival_s) && reqs[i].arrival_s >= 0.0, "arrival time");
+                require(i == 0 || reqs[i].arrival_s >= reqs[i - 1].arrival_s, "requests must be sorted by arrival");
+            }
+            // sessions: one per (graph, kind) that appears
+            std::vector<int> sid(static_cast<std::size_t>(ngraphs) * 2, -1);
+            std::vector<std::uint64_t> fixed, per_lane, demand;
+            std::vector<std::pair<std::uint32_t, int>> skey;
+            for (std::uint64_t i = 0; i < n; ++i)
+            {
+                const std::size_t key = static_cast<std::size_t>(reqs[i].graph) * 2 + reqs[i].kind;
+                if (sid[key] < 0)
+                {
+                    sid[key] = static_cast<int>(skey.size());
+                    skey.emplace_back(reqs[i].graph, static_cast<int>(reqs[i].kind));
+                    const DeviceGraph &g = graphs[reqs[i].graph]->g;
+                    fixed.push_back(LaneSession::fixed_bytes(g.V, g.E, g.max_degree));
+                    per_lane.push_back(LaneSession::lane_bytes(g.V, static_cast<int>(reqs[i].kind)));
+                    demand.push_back(0);
+                }
+                demand[static_cast<std::size_t>(sid[key])] += 1;
+            }
+            std::uint64_t graph_bytes = 0;
+            for (std::uint32_t i = 0; i < ngraphs; ++i)
+                if (graphs[i])
+                    graph_bytes += graphs[i]->g.device_bytes();
+            require(cfg.device_budget_bytes > graph_bytes, "device budget smaller than the resident graphs");
+            st = hyt_serve_stats{};
+            st.graph_bytes = graph_bytes;
+            const std::size_t S = skey.size();
+            std::vector<std::unique_ptr<LaneSession>> ses;
+            if (S > 0)
+            {
+                const std::vector<std::uint32_t> K =
+                    plan_lanes(fixed, per_lane, demand, cfg.device_budget_bytes - graph_bytes, cfg.max_lanes);
+                for (std::size_t i = 0; i < S; ++i)
+                {
+                    ses.push_back(std::make_unique<LaneSession>());
+                    create_session(*ses.back(), &graphs[skey[i].first]->g, skey[i].second, K[i], cfg.damping,
+                                   cfg.epsilon);
+                    st.state_bytes += fixed[i] + K[i] * per_lane[i];
+                    if (i < 16)
+                        st.lanes[i] = K[i];
+                }
+            }
+            st.sessions = static_cast<std::uint32_t>(S);
+            const std::uint32_t rps = cfg.rounds_per_step ? cfg.rounds_per_step : 1;
+
+            std::vector<std::deque<std::uint64_t>> waiting(S);
+            std::vector<std::vector<std::int64_t>> lane_req(S);
+            for (std::size_t i = 0; i < S; ++i)
+                lane_req[i].assign(ses[i]->K, -1);
+            std::vector<LaneSession *> busy;
+            std::vector<std::size_t> busy_id;
+            std::vector<std::uint64_t> fin;
+            std::uint64_t next = 0, completed = 0;
+            const auto t0 = Clock::now();
+            while (completed < n)
+            {
+                double now = secs(t0);
+                while (next < n && reqs[next].arrival_s <= now)
+                {
+                    const std::size_t key = static_cast<std::size_t>(reqs[next].graph) * 2 + reqs[next].kind;
+                    waiting[static_cast<std::size_t>(sid[key])].push_back(next);
+                    ++next;
+                }
+                busy.clear();
+                busy_id.clear();
+                for (std::size_t i = 0; i < S; ++i)
+                {
+                    LaneSession &L = *ses[i];
+                    while (!waiting[i].empty() && L.occupied != (L.K == 64 ? ~0ULL : ((1ULL << L.K) - 1ULL)))
+                    {
+                        std::uint32_t lane = 0;
+                        while ((L.occupied >> lane) & 1ULL)
+                            ++lane;
+                        const std::uint64_t r = waiting[i].front();
+                        waiting[i].pop_front();
+                        admit(L, lane, reqs[r].source);
+                        lane_req[i][lane] = static_cast<std::int64_t>(r);
+                        out[r] = hyt_completion{};
+                        out[r].admit_s = now;
+                        out[r].session = static_cast<std::uint32_t>(i);
+                        out[r].lane = lane;
+                        st.admitted += 1;
+                    }
+                    if (L.running)
+                    {
+                        busy.push_back(&L);
+                        busy_id.push_back(i);
+                    }
+                }
+                if (busy.empty())
+                {
+                    if (next < n)
+                    {
+                        const double wait = reqs[next].arrival_s - secs(t0);
+                        if (wait > 0)
+                            std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 2e-4)));
+                    }
+                    continue;
+                }
+                const auto ts = Clock::now();
+                fin.assign(busy.size(), 0);
+                step(busy.data(), static_cast<std::uint32_t>(busy.size()), rps, fin.data());
+                st.busy_s += secs(ts);
+                st.steps += 1;
+                for (std::size_t b = 0; b < busy.size(); ++b)
+                {
+                    LaneSession &L = *busy[b];
+                    const std::size_t i = busy_id[b];
+                    for (std::uint32_t lane = 0; lane < L.K; ++lane)
+                    {
+                        if (((fin[b] >> lane) & 1ULL) == 0)
+                            continue;
+                        const std::uint64_t r = static_cast<std::uint64_t>(lane_req[i][lane]);
+                        read_lane(L, lane, results ? results[r] : nullptr, false, &out[r].reached, &out[r].checksum);
+                        out[r].rounds = L.lane_rounds[lane];
+                        out[r].done_s = secs(t0);
+                        L.occupied &= ~(1ULL << lane);
+                        lane_req[i][lane] = -1;
+                        ++completed;
+                    }
+                }
+            }
+            st.wall_s = secs(t0);
+            for (auto &L : ses)
+            {
+                st.rounds += L->rounds;
+                st.edges += L->edges;
+            }
+        }
+    } // namespace
+} // namespace hyt
+
+extern "C" {
+
+int hyt_lanes_state_bytes(uint64_t num_vertices, uint64_t num_edges, uint64_t max_degree, uint32_t kind,
+                          uint32_t lanes, uint64_t *bytes)
+{
+    return hyt::guarded("hyt_lanes_state_bytes", [&] {
+        hyt::require(bytes != nullptr, "null output");
+        hyt::require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
+        *bytes = hyt::LaneSession::fixed_bytes(num_vertices, num_edges, max_degree) +
+                 static_cast<std::uint64_t>(lanes) * hyt::LaneSession::lane_bytes(num_vertices, static_cast<int>(kind));
+    });
+}
+
+int hyt_lanes_create(hyt_graph_t g, uint32_t kind, uint32_t lanes, float damping, float epsilon, hyt_lanes_t *out)
+{
+    return hyt::guarded("hyt_lanes_create", [&] {
+        hyt::require(g != nullptr && out != nullptr, "null pointer");
+        *out = nullptr;
+        auto h = std::make_unique<hyt_lanes>();
+        hyt::create_session(h->s, &g->g, static_cast<int>(kind), lanes, damping, epsilon);
+        *out = h.release();
+    });
+}
+
+int hyt_lanes_free(hyt_lanes_t s)
+{
+    return hyt::guarded("hyt_lanes_free", [&] { delete s; });
+}
+
+int hyt_lanes_admit(hyt_lanes_t s, uint32_t lane, uint32_t source)
+{
+    return hyt::guarded("hyt_lanes_admit", [&] {
+        hyt::require(s != nullptr, "null session");
+        hyt::admit(s->s, lane, source);
+    });
+}
+
+int hyt_lanes_step(hyt_lanes_t *sessions, uint32_t count, uint32_t rounds, uint64_t *finished, uint64_t *running)
+{
+    return hyt::guarded("hyt_lanes_step", [&] {
+        hyt::require(count == 0 || (sessions != nullptr && finished != nullptr), "null pointer");
+        std::vector<hyt::LaneSession *> ss(count);
+        for (uint32_t i = 0; i < count; ++i)
+        {
+            hyt::require(sessions[i] != nullptr, "null session");
+            ss[i] = &sessions[i]->s;
+            for (uint32_t j = 0; j < i; ++j)
+                hyt::require(ss[j] != ss[i], "a session appears twice");
+        }
+        hyt::step(ss.data(), count, rounds, finished);
+        if (running)
+            for (uint32_t i = 0; i < count; ++i)
+                running[i] = ss[i]->running;
+    });
+}
+
+int hyt_lanes_read(hyt_lanes_t s, uint32_t lane, void *out, uint32_t out_on_device, uint64_t *reached,
+                   uint64_t *checksum, uint32_t *rounds)
+{
+    return hyt::guarded("hyt_lanes_read", [&] {
+        hyt::require(s != nullptr, "null session");
+        hyt::read_lane(s->s, lane, out, out_on_device != 0, reached, checksum);
+        if (rounds)
+            *rounds = lane < s->s.K ? s->s.lane_rounds[lane] : 0;
+    });
+}
+
+int hyt_lanes_release(hyt_lanes_t s, uint32_t lane)
+{
+    return hyt::guarded("hyt_lanes_release", [&] {
+        hyt::require(s != nullptr, "null session");
+        hyt::require(lane < s->s.K, "lane out of range");
+        hyt::require(((s->s.running >> lane) & 1ULL) == 0, "lane is still running");
+        s->s.occupied &= ~(1ULL << lane);
+    });
+}
+
+int hyt_lanes_info(hyt_lanes_t s, uint32_t *lanes, uint64_t *occupied, uint64_t *running, uint32_t *frontier,
+                   uint64_t *state_bytes)
+{
+    return hyt::guarded("hyt_lanes_info", [&] {
+        hyt::require(s != nullptr, "null session");
+        const hyt::LaneSession &L = s->s;
+        if (lanes)
+            *lanes = L.K;
+        if (occupied)
+            *occupied = L.occupied;
+        if (running)
+            *running = L.running;
+        if (frontier)
+            *frontier = L.F;
+        if (state_bytes)
+            *state_bytes = L.val.bytes + L.rank.bytes + L.snap.bytes + L.mark.bytes + L.qmask.bytes + L.q.bytes +
+                           L.heavy.bytes + L.ctr.bytes + L.lanes.bytes + L.colbuf.bytes + L.partial.bytes +
+                           L.acc.bytes;
+    });
+}
+
+int hyt_serve_config_default(hyt_serve_config *out)
+{
+    return hyt::guarded("hyt_serve_config_default", [&] {
+        hyt::require(out != nullptr, "null output");
+        hyt_serve_config c{};
+        c.device_budget_bytes = 64ULL << 30;
+        c.max_lanes = 64;
+        c.rounds_per_step = 1;
+        c.damping = 0.85F;
+        c.epsilon = 1e-6F;
+        *out = c;
+    });
+}
+
+int hyt_serve_trace(const hyt_graph_t *graphs, uint32_t num_graphs, const hyt_serve_config *config,
+                    const hyt_request *requests, uint64_t num_requests, void *const *results,
+                    hyt_completion *completions, hyt_serve_stats *stats)
+{
+    return hyt::guarded("hyt_serve_trace", [&] {
+        hyt::require(config != nullptr, "null config");
+        hyt_serve_stats st{};
+        hyt::serve_trace(graphs, num_graphs, *config, requests, num_requests, results, completions, st);
+        if (stats)
+            *stats = st;
+    });
+}
+
+} // extern "C"

@@ -315,6 +315,97 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
 int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ranks,
                         const hyt_run_options *opts, hyt_hybrid_stats *stats);
 
+/* ------------------------------------------- continuous batching (serving) */
+/*
+ * No counterpart in the original project (which answers one query per call):
+ * these serve many SSSP / personalized-PageRank queries on one GPU by running
+ * them as lanes of shared [V][K] rounds and admitting new queries between
+ * rounds, for request loops that batch queries as they arrive.
+ *
+ * A lane session holds K <= 64 query slots of one kind over one HBM-resident
+ * graph (which must outlive it), with its own state and stream. Admit a
+ * source into a free lane, step (each step runs rounds for every occupied
+ * lane of every given session; sessions overlap on the device), and read a
+ * lane once it has finished. Results equal the single-query forms: SSSP
+ * distances bit-exact, PPR within the delta-push tolerance, normalised to 1.
+ * A finished lane keeps its result until read and released (or re-admitted
+ * after hyt_lanes_release).
+ */
+enum hyt_query_kind { HYT_QUERY_SSSP = 0, HYT_QUERY_PPR = 1 };
+
+typedef struct hyt_lanes *hyt_lanes_t;
+
+/* Device bytes of a session of `lanes` lanes (admission planning);
+ * max_degree = largest out-degree of the graph. */
+int hyt_lanes_state_bytes(uint64_t num_vertices, uint64_t num_edges, uint64_t max_degree, uint32_t kind,
+                          uint32_t lanes, uint64_t *bytes);
+/* damping / epsilon are used by PPR sessions only. */
+int hyt_lanes_create(hyt_graph_t g, uint32_t kind, uint32_t lanes, float damping, float epsilon,
+                     hyt_lanes_t *out);
+int hyt_lanes_free(hyt_lanes_t s);
+/* The lane must be free; the query starts with the next round. At most 64
+ * admissions between two rounds. */
+int hyt_lanes_admit(hyt_lanes_t s, uint32_t lane, uint32_t source);
+/* Runs up to `rounds` rounds on every session (stops early when all are
+ * idle); finished[i] = lanes of session i that finished during the call,
+ * running[i] (optional) = lanes still running. */
+int hyt_lanes_step(hyt_lanes_t *sessions, uint32_t count, uint32_t rounds, uint64_t *finished,
+                   uint64_t *running);
+/* Result of a finished lane: u32[V] distances (SSSP) or float[V] ranks (PPR)
+ * into out (host or device, optional); reached = entries that are not
+ * unreachable (SSSP) / not zero (PPR), checksum = u64 sum of their bit
+ * patterns; rounds = rounds the query ran. All outputs optional. */
+int hyt_lanes_read(hyt_lanes_t s, uint32_t lane, void *out, uint32_t out_on_device, uint64_t *reached,
+                   uint64_t *checksum, uint32_t *rounds);
+/* Frees a finished lane. */
+int hyt_lanes_release(hyt_lanes_t s, uint32_t lane);
+int hyt_lanes_info(hyt_lanes_t s, uint32_t *lanes, uint64_t *occupied, uint64_t *running, uint32_t *frontier,
+                   uint64_t *state_bytes);
+
+typedef struct hyt_request {
+    uint32_t graph;     /* index into the graph array */
+    uint32_t kind;      /* hyt_query_kind */
+    uint32_t source;
+    uint32_t reserved;
+    double arrival_s;   /* seconds after the start of the trace (non-decreasing) */
+} hyt_request;
+
+typedef struct hyt_completion {
+    double admit_s, done_s; /* seconds after the start of the trace */
+    uint32_t session, lane, rounds, reserved;
+    uint64_t reached, checksum; /* as hyt_lanes_read */
+} hyt_completion;
+
+typedef struct hyt_serve_config {
+    uint64_t device_budget_bytes; /* resident graphs + every session's state */
+    uint32_t max_lanes;           /* per session, <= 64 */
+    uint32_t rounds_per_step;     /* rounds between admission checks (0 = 1) */
+    float damping, epsilon;       /* PPR */
+    uint32_t reserved[4];
+} hyt_serve_config;
+
+typedef struct hyt_serve_stats {
+    double wall_s, busy_s;        /* trace wall time; time inside steps */
+    uint64_t steps, rounds, edges, admitted;
+    uint64_t graph_bytes, state_bytes;
+    uint32_t sessions;
+    uint32_t lanes[16];           /* lanes of the first 16 sessions (one per graph and kind, in order of first request) */
+    uint32_t reserved;
+} hyt_serve_stats;
+
+int hyt_serve_config_default(hyt_serve_config *out); /* 64 GiB, 64 lanes, 1 round, 0.85, 1e-6 */
+
+/*
+ * Replays a request trace against the wall clock: arrivals queue per (graph,
+ * kind) session, sessions get lanes from the device budget, queries are
+ * admitted into free lanes between steps and read as they finish.
+ * results: optional per-request host buffers of V words (NULL entries or a
+ * NULL array skip the copy). completions: one per request.
+ */
+int hyt_serve_trace(const hyt_graph_t *graphs, uint32_t num_graphs, const hyt_serve_config *config,
+                    const hyt_request *requests, uint64_t num_requests, void *const *results,
+                    hyt_completion *completions, hyt_serve_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

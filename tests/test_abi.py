@@ -111,3 +111,35 @@ def test_hybrid_config_defaults_and_host_checks(abi):
     pageable = abi.HostGraph(scale=8, edge_factor=4, seed=1)
     with pytest.raises(abi.InvalidArgument):
         abi.HybridEngine(pageable)
+
+
+def test_struct_layouts_match_header(abi, tmp_path):
+    # every ctypes mirror has the size and field offsets of its C struct
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    structs = {
+        "hyt_gen_params": abi.GenParams, "hyt_run_options": abi.RunOptions, "hyt_run_stats": abi.RunStats,
+        "hyt_cost_params": abi.CostParams, "hyt_partition": abi.Partition,
+        "hyt_partition_metrics": abi.PartitionMetrics, "hyt_partition_costs": abi.PartitionCosts,
+        "hyt_hybrid_config": abi.HybridConfig, "hyt_hybrid_stats": abi.HybridStats,
+        "hyt_request": abi.Request, "hyt_completion": abi.Completion, "hyt_serve_config": abi.ServeConfig,
+        "hyt_serve_stats": abi.ServeStats,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hytgraph_amd.h"', "int main(void){"]
+    expect = []
+    for cname, py in structs.items():
+        lines.append(f'printf("%zu\\n", sizeof({cname}));')
+        expect.append(C.sizeof(py))
+        for f, _ in py._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({cname}, {f}));')
+            expect.append(getattr(py, f).offset)
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == expect
