@@ -251,9 +251,221 @@ def bench_hybrid(args) -> dict:
     }
 
 
+def bench_serve(args) -> dict:
+    """Served form: several HBM-resident RMAT graphs, a 70/30 mix of SSSP and
+    PPR queries on degree-weighted sources, continuous batching into K-lane
+    sessions sized from the device budget. Capacity = queries/s of a burst
+    (every request queued at t=0); latency = p50/p99 of a Poisson trace at a
+    fixed offered rate; baseline = the same queries one at a time through the
+    single-query entry points."""
+    import torch
+    from hytgraph_amd import abi
+
+    scales = [int(x) for x in args.serve_scales.split(",")]
+    graphs, hosts, pools, degs = [], [], [], []
+    for i, sc in enumerate(scales):
+        t0 = time.perf_counter()
+        hg = abi.HostGraph(scale=sc, edge_factor=args.edge_factor, seed=args.seed + i, weight_min=1,
+                           weight_max=255)
+        g = abi.DeviceGraph.from_host(hg)
+        graphs.append(g)
+        hosts.append(hg)
+        pools.append(abi.sample_sources(hg.offsets, args.seed + 100 + i, 4096))
+        degs.append(np.diff(hg.offsets.astype(np.int64)))
+        log(f"graph {i}: RMAT scale {sc} V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(args.seed)
+
+    def trace(n, qps):
+        arr = np.zeros(n) if qps is None else np.cumsum(rng.exponential(1.0 / qps, n))
+        out = []
+        for t in arr:
+            gi = int(rng.integers(0, len(graphs)))
+            kind = abi.HYT_QUERY_SSSP if rng.random() < 0.7 else abi.HYT_QUERY_PPR
+            out.append((float(t), gi, kind, int(pools[gi][int(rng.integers(0, len(pools[gi])))])))
+        return out
+
+    eps = args.ppr_eps
+    budget = args.budget_gb << 30
+    burst = trace(args.serve_requests, None)
+
+    # one at a time: the first requests of the burst through the single-query paths
+    nb = min(args.serve_baseline, len(burst))
+    dev_out = torch.empty(max(h.V for h in hosts), dtype=torch.int32, device="cuda")
+    alone_sum = {}
+    for i, (_, gi, kind, src) in enumerate(burst[:min(nb, 16)]):  # warm-up + checksums of SSSP results
+        if kind == abi.HYT_QUERY_SSSP:
+            d, _ = graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC)
+            fin = d != 0xFFFFFFFF
+            alone_sum[i] = (int(fin.sum()), int(d[fin].astype(np.uint64).sum()))
+        else:
+            graphs[gi].ppr(src, 0.85, eps, out_ptr=dev_out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _, gi, kind, src in burst[:nb]:
+        if kind == abi.HYT_QUERY_SSSP:
+            graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC, out_ptr=dev_out.data_ptr())
+        else:
+            graphs[gi].ppr(src, 0.85, eps, out_ptr=dev_out.data_ptr())
+    torch.cuda.synchronize()
+    base_s = time.perf_counter() - t0
+    base_qps = nb / base_s
+    log(f"one at a time: {nb} queries in {base_s:.2f} s = {base_qps:.1f} queries/s")
+
+    # capacity: the whole burst queued at t=0
+    comp, st, _ = abi.serve_trace(graphs, burst, device_budget_bytes=budget, epsilon=eps)
+    cap_s = float(comp["done_s"].max())
+    cap_qps = len(burst) / cap_s
+    for i, (reached, checksum) in alone_sum.items():
+        if (int(comp["reached"][i]), int(comp["checksum"][i])) != (reached, checksum):
+            raise SystemExit(f"served SSSP result differs from the one-at-a-time result (request {i})")
+    occ = st["lane_rows"] / max(st["frontier_rows"], 1)
+    log(f"burst: {len(burst)} queries in {cap_s:.2f} s = {cap_qps:.1f} queries/s "
+        f"({cap_qps / base_qps:.2f}x one at a time); lanes {st['lanes']}, state {st['state_bytes'] / 2**30:.1f} GiB; "
+        f"queries per frontier row {occ:.2f}; {len(alone_sum)} SSSP checksums equal to one-at-a-time")
+
+    # latency: Poisson arrivals at a fixed offered rate
+    qps = args.qps if args.qps > 0 else 0.7 * cap_qps
+    pois = trace(args.serve_requests, qps)
+    comp2, st2, _ = abi.serve_trace(graphs, pois, device_budget_bytes=budget, epsilon=eps)
+    arr = np.array([r[0] for r in pois])
+    lat = comp2["done_s"] - arr
+    kinds = np.array([r[2] for r in pois])
+    span = float(comp2["done_s"].max() - arr.min())
+    served_qps = len(pois) / span
+    pct = lambda a, q: float(np.percentile(a, q)) * 1e3 if len(a) else None
+    log(f"poisson {qps:.1f} queries/s offered: served {served_qps:.1f} queries/s, latency p50 "
+        f"{pct(lat, 50):.1f} ms p99 {pct(lat, 99):.1f} ms; queries per frontier row "
+        f"{st2['lane_rows'] / max(st2['frontier_rows'], 1):.2f}")
+    per_graph = []
+    for gi in range(len(graphs)):
+        m = np.array([r[1] == gi for r in pois])
+        per_graph.append({"scale": scales[gi], "vertices": hosts[gi].V, "edges": hosts[gi].E,
+                          "queries": int(m.sum()), "p50_ms": pct(lat[m], 50), "p99_ms": pct(lat[m], 99)})
+    return {
+        "metric": METRIC,
+        "value": round(cap_qps, 3),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": len(burst),
+        "warmup": 0,
+        "ms_per_step": round(cap_s * 1e3 / len(burst), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (SSSP) / f32 (PPR)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"served mixed SSSP (70%) / PPR (30%, eps {eps:g}, d 0.85) queries on degree-weighted "
+                        f"sources over RMAT scales {scales} ef{args.edge_factor} (u32 weights [1,255]), all in HBM, "
+                        f"{args.budget_gb} GiB device budget, continuous batching into K-lane sessions",
+            "scales": scales,
+            "budget_gib": args.budget_gb,
+            "requests": len(burst),
+        },
+        "baseline_one_at_a_time_qps": base_qps,
+        "speedup_vs_one_at_a_time": cap_qps / base_qps,
+        "burst": {"queries_per_s": cap_qps, "seconds": cap_s, "lanes": st["lanes"],
+                  "state_gib": st["state_bytes"] / 2**30, "graph_gib": st["graph_bytes"] / 2**30,
+                  "session_rounds": st["rounds"], "edges": st["edges"],
+                  "queries_per_frontier_row": occ,
+                  "sssp_checksums_equal_alone": len(alone_sum)},
+        "poisson": {"offered_qps": qps, "served_qps": served_qps, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
+                    "sssp_p50_ms": pct(lat[kinds == 0], 50), "sssp_p99_ms": pct(lat[kinds == 0], 99),
+                    "ppr_p50_ms": pct(lat[kinds == 1], 50), "ppr_p99_ms": pct(lat[kinds == 1], 99),
+                    "per_graph": per_graph},
+    }
+
+
+def bench_batched(args) -> dict:
+    """Batched multi-source form in HBM: K SSSP sources / K PPR seeds per call
+    with [V][K] state, against the same queries one at a time."""
+    import torch
+    from hytgraph_amd import abi
+
+    t0 = time.perf_counter()
+    hg = abi.HostGraph(scale=args.scale, edge_factor=args.edge_factor, seed=args.seed, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    log(f"RMAT scale {args.scale}: V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s")
+    eps = args.ppr_eps
+    ks = [int(x) for x in args.ks.split(",")]
+    srcs = abi.sample_sources(hg.offsets, args.seed + 1, max(ks))
+    one = torch.empty(hg.V, dtype=torch.int32, device="cuda")
+    # one at a time (first 8 sources)
+    n1 = min(8, len(srcs))
+    g.sssp(int(srcs[0]), mode=abi.HYT_SSSP_ASYNC, out_ptr=one.data_ptr())
+    g.ppr(int(srcs[0]), 0.85, eps, out_ptr=one.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in srcs[:n1]:
+        g.sssp(int(s), mode=abi.HYT_SSSP_ASYNC, out_ptr=one.data_ptr())
+    torch.cuda.synchronize()
+    sssp1 = n1 / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for s in srcs[:n1]:
+        g.ppr(int(s), 0.85, eps, out_ptr=one.data_ptr())
+    torch.cuda.synchronize()
+    ppr1 = n1 / (time.perf_counter() - t0)
+    log(f"one at a time: SSSP {sssp1:.2f} queries/s, PPR {ppr1:.2f} queries/s")
+    rows = []
+    out = torch.empty(hg.V * max(ks), dtype=torch.int32, device="cuda")
+    for K in ks:
+        src = srcs[:K]
+        g.sssp(src, out_ptr=out.data_ptr())  # warm-up (workspace)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, st = g.sssp(src, out_ptr=out.data_ptr(), time_kernels=1)
+        torch.cuda.synchronize()
+        ts = time.perf_counter() - t0
+        g.ppr(src, 0.85, eps, out_ptr=out.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, pst = g.ppr(src, 0.85, eps, out_ptr=out.data_ptr(), time_kernels=1)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - t0
+        rows.append({"K": K, "sssp_s": ts, "sssp_qps": K / ts, "sssp_speedup": K / ts / sssp1,
+                     "sssp_rounds": st["rounds"], "sssp_row_edges": st["edges_total"],
+                     "sssp_kernel_ms": st["expand_ms"],
+                     "ppr_s": tp, "ppr_qps": K / tp, "ppr_speedup": K / tp / ppr1, "ppr_rounds": pst["rounds"],
+                     "ppr_row_edges": pst["edges_total"], "ppr_kernel_ms": pst["expand_ms"]})
+        log(f"K={K}: SSSP {ts * 1e3:.1f} ms ({K / ts:.1f} q/s, {K / ts / sssp1:.2f}x), rounds {st['rounds']}; "
+            f"PPR {tp * 1e3:.1f} ms ({K / tp:.1f} q/s, {K / tp / ppr1:.2f}x), rounds {pst['rounds']}")
+    best = max(rows, key=lambda r: r["sssp_qps"])
+    return {
+        "metric": METRIC, "value": round(best["sssp_qps"], 3), "unit": "queries/s", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": round(best["sssp_s"] * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u32 (SSSP) / f32 (PPR)", "data": "synthetic",
+        "config": {"workload": f"batched [V][K] SSSP + PPR (eps {eps:g}) in HBM, RMAT scale-{args.scale} "
+                               f"ef{args.edge_factor}, u32 weights [1,255], degree-weighted sources, K in {ks}",
+                   "scale": args.scale, "ks": ks},
+        "one_at_a_time": {"sssp_qps": sssp1, "ppr_qps": ppr1},
+        "batched": rows,
+    }
+
+
+def bench_copy(args) -> dict:
+    """Reference kernel for HBM traffic counters: a plain device copy of a
+    known size (reads N bytes, writes N bytes), so counter readings of the
+    traversal kernels can be scaled to bytes."""
+    import torch
+
+    n = args.copy_mib << 20
+    a = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    return {"workload": "device copy", "bytes_read": n, "bytes_written": n, "copies": reps + 1,
+            "GBps": 2 * n / sec / 1e9}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["plain", "hybrid"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "hybrid", "batched", "serve", "copy"], default="plain")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -267,10 +479,17 @@ def main(argv=None) -> int:
     ap.add_argument("--delta", type=int, default=0, help="async SSSP delta-stepping bucket width (0 = off)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pagerank", action="store_true")
-    ap.add_argument("--budget-gb", type=int, default=4, help="hybrid: HBM edge budget in GiB")
+    ap.add_argument("--budget-gb", type=int, default=None, help="HBM budget in GiB (hybrid edges: 4, serve: 64)")
     ap.add_argument("--part-log2", type=int, default=20, help="hybrid: log2 edges per partition")
     ap.add_argument("--pr-eps", type=float, default=1e-7)
     ap.add_argument("--no-check", action="store_true", help="hybrid: skip the in-HBM bit-exact check")
+    ap.add_argument("--serve-scales", default="22,24,26", help="serve: RMAT scales of the graphs")
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--serve-baseline", type=int, default=60, help="serve: queries timed one at a time")
+    ap.add_argument("--qps", type=float, default=0.0, help="serve: offered rate (0 = 0.7 x burst capacity)")
+    ap.add_argument("--ppr-eps", type=float, default=1e-6)
+    ap.add_argument("--ks", default="8,32,64", help="batched: lanes per call")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="copy: MiB per copy")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -288,6 +507,17 @@ def main(argv=None) -> int:
         args.scale = 27 if hybrid else 24
     if args.sources is None:
         args.sources = 3 if hybrid else 8
+    if args.budget_gb is None:
+        args.budget_gb = 64 if args.workload == "serve" else 4
+    if args.workload == "serve":
+        print(json.dumps(bench_serve(args)), flush=True)
+        return 0
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args)), flush=True)
+        return 0
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args)), flush=True)
+        return 0
     if hybrid:
         print(json.dumps(bench_hybrid(args)), flush=True)
         return 0
@@ -348,6 +578,13 @@ def main(argv=None) -> int:
     alg_bytes = sssp_bytes(acc, hg.V)
     achieved = alg_bytes / (exp_ms / 1e3) / 1e9 if exp_ms > 0 else 0.0
 
+    # HBM traffic per relaxation round from the rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (scaled by a plain copy's readings)
+    traffic = None
+    tfile = ROOT / "profiles" / "sssp_rmat24_traffic.json"
+    if tfile.exists() and args.scale == 24 and args.edge_factor == 16 and args.mode == "async" and pull_rounds == 0:
+        traffic = json.loads(tfile.read_text())["traffic_bytes_per_round"]
+
     result = {
         "metric": METRIC,
         "value": round(gteps, 4),
@@ -379,7 +616,10 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_GBs": round(traffic / (exp_ms / max(launches, 1) / 1e3) / 1e9, 1) if traffic else None,
+            "traffic_note": "bytes moved between L2 and memory per round (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE "
+                            "x 1.0, factors from a 1 GiB device copy); traffic_GBs = traffic / avg round time",
             "kernel": "SSSP relaxation (expand_light+expand_heavy push / pull_light+pull_heavy)",
             "algorithmic_bytes_per_launch": alg_bytes / max(launches, 1),
             "avg_launch_ms": exp_ms / max(launches, 1),

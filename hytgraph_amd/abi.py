@@ -162,14 +162,15 @@ class Completion(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_lanes", C.c_uint32), ("rounds_per_step", C.c_uint32),
+    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_lanes", C.c_uint32), ("reserved0", C.c_uint32),
                 ("damping", C.c_float), ("epsilon", C.c_float), ("reserved", C.c_uint32 * 4)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("busy_s", C.c_double), ("steps", C.c_uint64), ("rounds", C.c_uint64),
                 ("edges", C.c_uint64), ("admitted", C.c_uint64), ("graph_bytes", C.c_uint64),
-                ("state_bytes", C.c_uint64), ("sessions", C.c_uint32), ("lanes", C.c_uint32 * 16),
+                ("state_bytes", C.c_uint64), ("frontier_rows", C.c_uint64), ("lane_rows", C.c_uint64),
+                ("sessions", C.c_uint32), ("lanes", C.c_uint32 * 16),
                 ("reserved", C.c_uint32)]
 
     def as_dict(self) -> dict:

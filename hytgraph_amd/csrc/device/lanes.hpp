@@ -304,25 +304,33 @@ namespace hyt
             struct Aux
             {
                 unsigned long long lanes = 0; // OR of the selected masks
+                std::uint32_t pairs = 0;      // sum of their popcounts
             };
             unsigned long long *mark;
-            unsigned long long *lanes_out; // optional: OR of every selected mask
+            // optional: OR of every selected mask; with it, the number of
+            // (row, lane) pairs selected goes to ctr->frontier_edges
+            unsigned long long *lanes_out;
 
-            __device__ void publish(const Aux &aux, Counters *) const
+            __device__ void publish(const Aux &aux, Counters *ctr) const
             {
                 if (lanes_out == nullptr)
                     return;
                 std::uint32_t lo = static_cast<std::uint32_t>(aux.lanes);
                 std::uint32_t hi = static_cast<std::uint32_t>(aux.lanes >> 32);
+                std::uint32_t pairs = aux.pairs;
 #pragma unroll
                 for (int d = kWave / 2; d > 0; d >>= 1)
                 {
                     lo |= __shfl_xor(lo, d, kWave);
                     hi |= __shfl_xor(hi, d, kWave);
+                    pairs += __shfl_xor(pairs, d, kWave);
                 }
                 const unsigned long long all = (static_cast<unsigned long long>(hi) << 32) | lo;
                 if (lane_id() == 0 && all != 0ULL)
+                {
                     atomicOr(lanes_out, all);
+                    atomicAdd(&ctr->frontier_edges, static_cast<unsigned long long>(pairs));
+                }
             }
             __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n,
                                              unsigned long long (&vals)[4], Aux &aux) const
@@ -348,6 +356,7 @@ namespace hyt
                         mark[row0 + j] = 0ULL;
                         vals[j] = m4[j];
                         aux.lanes |= m4[j];
+                        aux.pairs += static_cast<std::uint32_t>(__popcll(m4[j]));
                         sel |= 1U << j;
                     }
                 return sel;

@@ -15,10 +15,13 @@
 // them: their rounds overlap on the device.
 //
 // The serving loop replays a request trace (arrival times, graph, kind,
-// source) against the wall clock: arrivals queue per session, are admitted
-// as lanes free up, lanes per session are sized from the device budget
-// (admission by memory), and each request's admit and completion times are
-// recorded.
+// source) against the wall clock: arrivals queue per session, lanes per
+// session are sized from the device budget (admission by memory), and the
+// loop polls every session: when its round has completed, finished lanes are
+// read and freed, queued queries are admitted into free lanes, and the next
+// round is launched - each session advances at its own pace, so a small
+// graph's queries are not held back by a large graph's rounds. Each request's
+// admit and completion times are recorded.
 #include "lanes.hpp"
 
 #include <algorithm>
@@ -169,6 +172,8 @@ namespace hyt
         bool launched = false;
         std::uint32_t lane_rounds[64] = {};
         std::uint64_t rounds = 0, edges = 0;
+        std::uint64_t frontier_rows = 0, lane_rows = 0; // lane occupancy of the rounds run
+        std::uint64_t pairs = 0;                         // (row, lane) pairs of the current queue
 
         ~LaneSession()
         {
@@ -287,6 +292,7 @@ namespace hyt
                                               S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>(), S.F);
             hip_check(hipGetLastError(), "launch lane_seed");
             S.F += 1;
+            S.pairs += 1;
             S.appended += 1;
             S.running |= 1ULL << lane;
             S.occupied |= 1ULL << lane;
@@ -417,11 +423,24 @@ namespace hyt
                     S.lane_rounds[k] += 1;
             S.rounds += 1;
             S.edges += c.edges;
+            S.frontier_rows += S.F;
+            S.lane_rows += S.pairs;
+            S.pairs = c.frontier_edges; // (row, lane) pairs of the next frontier
             S.F = c.next_count;
             S.appended = 0;
             const std::uint64_t finished = S.running & ~active;
             S.running &= active;
             return finished;
+        }
+
+        // True when the session's round in flight has completed.
+        bool round_ready(LaneSession &S)
+        {
+            const hipError_t e = hipEventQuery(S.done);
+            if (e == hipErrorNotReady)
+                return false;
+            hip_check(e, "lane round");
+            return true;
         }
 
         // Result of a finished lane into S.colbuf (SSSP distances, or PPR
@@ -598,32 +617,54 @@ namespace hyt
                 }
             }
             st.sessions = static_cast<std::uint32_t>(S);
-            const std::uint32_t rps = cfg.rounds_per_step ? cfg.rounds_per_step : 1;
 
             std::vector<std::deque<std::uint64_t>> waiting(S);
             std::vector<std::vector<std::int64_t>> lane_req(S);
             for (std::size_t i = 0; i < S; ++i)
                 lane_req[i].assign(ses[i]->K, -1);
-            std::vector<LaneSession *> busy;
-            std::vector<std::size_t> busy_id;
-            std::vector<std::uint64_t> fin;
             std::uint64_t next = 0, completed = 0;
             const auto t0 = Clock::now();
+            double busy_from = -1.0; // start of the current stretch with a round in flight
             while (completed < n)
             {
-                double now = secs(t0);
+                const double now = secs(t0);
                 while (next < n && reqs[next].arrival_s <= now)
                 {
                     const std::size_t key = static_cast<std::size_t>(reqs[next].graph) * 2 + reqs[next].kind;
                     waiting[static_cast<std::size_t>(sid[key])].push_back(next);
                     ++next;
                 }
-                busy.clear();
-                busy_id.clear();
+                bool in_flight = false, progress = false;
                 for (std::size_t i = 0; i < S; ++i)
                 {
                     LaneSession &L = *ses[i];
-                    while (!waiting[i].empty() && L.occupied != (L.K == 64 ? ~0ULL : ((1ULL << L.K) - 1ULL)))
+                    if (L.launched)
+                    {
+                        if (!round_ready(L))
+                        {
+                            in_flight = true;
+                            continue;
+                        }
+                        const std::uint64_t fin = finish_round(L);
+                        st.steps += 1;
+                        progress = true;
+                        for (std::uint32_t lane = 0; lane < L.K; ++lane)
+                        {
+                            if (((fin >> lane) & 1ULL) == 0)
+                                continue;
+                            const std::uint64_t r = static_cast<std::uint64_t>(lane_req[i][lane]);
+                            read_lane(L, lane, results ? results[r] : nullptr, false, &out[r].reached,
+                                      &out[r].checksum);
+                            out[r].rounds = L.lane_rounds[lane];
+                            out[r].done_s = secs(t0);
+                            L.occupied &= ~(1ULL << lane);
+                            lane_req[i][lane] = -1;
+                            ++completed;
+                        }
+                    }
+                    // admission into free lanes, then the session's next round
+                    const std::uint64_t all = L.K == 64 ? ~0ULL : ((1ULL << L.K) - 1ULL);
+                    while (!waiting[i].empty() && L.occupied != all && L.appended < LaneSession::kAppend)
                     {
                         std::uint32_t lane = 0;
                         while ((L.occupied >> lane) & 1ULL)
@@ -633,18 +674,26 @@ namespace hyt
                         admit(L, lane, reqs[r].source);
                         lane_req[i][lane] = static_cast<std::int64_t>(r);
                         out[r] = hyt_completion{};
-                        out[r].admit_s = now;
+                        out[r].admit_s = secs(t0);
                         out[r].session = static_cast<std::uint32_t>(i);
                         out[r].lane = lane;
                         st.admitted += 1;
+                        progress = true;
                     }
                     if (L.running)
                     {
-                        busy.push_back(&L);
-                        busy_id.push_back(i);
+                        launch_round(L);
+                        in_flight = in_flight || L.launched;
                     }
                 }
-                if (busy.empty())
+                if (in_flight && busy_from < 0)
+                    busy_from = now;
+                if (!in_flight && busy_from >= 0)
+                {
+                    st.busy_s += secs(t0) - busy_from;
+                    busy_from = -1.0;
+                }
+                if (!in_flight)
                 {
                     if (next < n)
                     {
@@ -652,36 +701,19 @@ namespace hyt
                         if (wait > 0)
                             std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 2e-4)));
                     }
-                    continue;
                 }
-                const auto ts = Clock::now();
-                fin.assign(busy.size(), 0);
-                step(busy.data(), static_cast<std::uint32_t>(busy.size()), rps, fin.data());
-                st.busy_s += secs(ts);
-                st.steps += 1;
-                for (std::size_t b = 0; b < busy.size(); ++b)
-                {
-                    LaneSession &L = *busy[b];
-                    const std::size_t i = busy_id[b];
-                    for (std::uint32_t lane = 0; lane < L.K; ++lane)
-                    {
-                        if (((fin[b] >> lane) & 1ULL) == 0)
-                            continue;
-                        const std::uint64_t r = static_cast<std::uint64_t>(lane_req[i][lane]);
-                        read_lane(L, lane, results ? results[r] : nullptr, false, &out[r].reached, &out[r].checksum);
-                        out[r].rounds = L.lane_rounds[lane];
-                        out[r].done_s = secs(t0);
-                        L.occupied &= ~(1ULL << lane);
-                        lane_req[i][lane] = -1;
-                        ++completed;
-                    }
-                }
+                else if (!progress)
+                    std::this_thread::yield();
             }
+            if (busy_from >= 0)
+                st.busy_s += secs(t0) - busy_from;
             st.wall_s = secs(t0);
             for (auto &L : ses)
             {
                 st.rounds += L->rounds;
                 st.edges += L->edges;
+                st.frontier_rows += L->frontier_rows;
+                st.lane_rows += L->lane_rows;
             }
         }
     } // namespace
@@ -792,7 +824,6 @@ int hyt_serve_config_default(hyt_serve_config *out)
         hyt_serve_config c{};
         c.device_budget_bytes = 64ULL << 30;
         c.max_lanes = 64;
-        c.rounds_per_step = 1;
         c.damping = 0.85F;
         c.epsilon = 1e-6F;
         *out = c;

@@ -379,26 +379,28 @@ typedef struct hyt_completion {
 typedef struct hyt_serve_config {
     uint64_t device_budget_bytes; /* resident graphs + every session's state */
     uint32_t max_lanes;           /* per session, <= 64 */
-    uint32_t rounds_per_step;     /* rounds between admission checks (0 = 1) */
+    uint32_t reserved0;
     float damping, epsilon;       /* PPR */
     uint32_t reserved[4];
 } hyt_serve_config;
 
 typedef struct hyt_serve_stats {
-    double wall_s, busy_s;        /* trace wall time; time inside steps */
-    uint64_t steps, rounds, edges, admitted;
+    double wall_s, busy_s;        /* trace wall time; time with a round in flight */
+    uint64_t steps, rounds, edges, admitted; /* steps = session rounds completed by the loop */
     uint64_t graph_bytes, state_bytes;
+    uint64_t frontier_rows, lane_rows; /* frontier rows expanded / (row, query) pairs among them */
     uint32_t sessions;
     uint32_t lanes[16];           /* lanes of the first 16 sessions (one per graph and kind, in order of first request) */
     uint32_t reserved;
 } hyt_serve_stats;
 
-int hyt_serve_config_default(hyt_serve_config *out); /* 64 GiB, 64 lanes, 1 round, 0.85, 1e-6 */
+int hyt_serve_config_default(hyt_serve_config *out); /* 64 GiB, 64 lanes, 0.85, 1e-6 */
 
 /*
  * Replays a request trace against the wall clock: arrivals queue per (graph,
- * kind) session, sessions get lanes from the device budget, queries are
- * admitted into free lanes between steps and read as they finish.
+ * kind) session, sessions get lanes from the device budget, and each session
+ * runs its own rounds: between two of them its finished queries are read and
+ * queued ones are admitted into the free lanes.
  * results: optional per-request host buffers of V words (NULL entries or a
  * NULL array skip the copy). completions: one per request.
  */
