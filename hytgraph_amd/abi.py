@@ -68,7 +68,7 @@ class RunOptions(C.Structure):
         ("direction", C.c_uint32),
         ("pull_percent", C.c_uint32),
         ("delta", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("lane_form", C.c_uint32),
     ]
 
 
@@ -162,7 +162,7 @@ class Completion(C.Structure):
 
 
 class ServeConfig(C.Structure):
-    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_lanes", C.c_uint32), ("reserved0", C.c_uint32),
+    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_lanes", C.c_uint32), ("lane_form", C.c_uint32),
                 ("damping", C.c_float), ("epsilon", C.c_float), ("reserved", C.c_uint32 * 4)]
 
 
@@ -170,6 +170,7 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("busy_s", C.c_double), ("steps", C.c_uint64), ("rounds", C.c_uint64),
                 ("edges", C.c_uint64), ("admitted", C.c_uint64), ("graph_bytes", C.c_uint64),
                 ("state_bytes", C.c_uint64), ("frontier_rows", C.c_uint64), ("lane_rows", C.c_uint64),
+                ("item_rounds", C.c_uint64),
                 ("sessions", C.c_uint32), ("lanes", C.c_uint32 * 16),
                 ("reserved", C.c_uint32)]
 
@@ -244,6 +245,7 @@ SIGNATURES = {
     "hyt_lanes_step": (C.c_int, [C.POINTER(_P), C.c_uint32, C.c_uint32, _u64p, _u64p]),
     "hyt_lanes_read": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _u64p, _u64p, _u32p]),
     "hyt_lanes_release": (C.c_int, [_P, C.c_uint32]),
+    "hyt_lanes_set_form": (C.c_int, [_P, C.c_uint32]),
     "hyt_lanes_info": (C.c_int, [_P, _u32p, _u64p, _u64p, _u32p, _u64p]),
     "hyt_serve_config_default": (C.c_int, [C.POINTER(ServeConfig)]),
     "hyt_serve_trace": (
@@ -333,10 +335,16 @@ def sample_sources(offsets: np.ndarray, seed: int, count: int) -> np.ndarray:
     return out
 
 
+HYT_LANES_AUTO = 0
+HYT_LANES_GROUP = 1
+HYT_LANES_ITEMS = 2
+
+
 def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
-             light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0) -> RunOptions:
+             light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0,
+             lane_form=HYT_LANES_AUTO) -> RunOptions:
     return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk,
-                      direction, pull_percent, delta, 0)
+                      direction, pull_percent, delta, lane_form)
 
 
 class DeviceGraph:
@@ -582,6 +590,9 @@ class LaneSession:
 
     def release(self, lane: int) -> None:
         check(load().hyt_lanes_release(self._h, lane))
+
+    def set_form(self, form: int) -> None:
+        check(load().hyt_lanes_set_form(self._h, form))
 
     def info(self) -> dict:
         k, occ, run, F, b = C.c_uint32(), C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint64()

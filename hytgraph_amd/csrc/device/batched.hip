@@ -88,7 +88,7 @@ namespace hyt
         // Workspace of the batched forms (grown to the largest K used).
         struct KWorkspace
         {
-            DeviceBuffer val, rank, snap, mark, qmask, heavy, seeds, partial;
+            DeviceBuffer val, rank, snap, mark, qmask, heavy, seeds, partial, lanes;
             std::uint64_t heavy_cap = 0;
         };
 
@@ -150,6 +150,7 @@ namespace hyt
                 kw.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(heavy chunks)");
             kw.heavy_cap = kw.heavy.bytes / sizeof(HeavyChunkK);
             kw.seeds.ensure(64 * 4 * 4, "hipMalloc(seeds)");
+            kw.lanes.ensure(8, "hipMalloc(lane mask)");
             kw.partial.ensure((1024 + 1) * 64 * sizeof(double), "hipMalloc(partials)");
             ensure_workspace(g, light_max, chunk);
         }
@@ -176,7 +177,7 @@ namespace hyt
         Counters compact_marks(DeviceGraph &g, KWorkspace &kw)
         {
             Counters *d_ctr = g.ws.ctr.as<Counters>();
-            MarkSel sel{kw.mark.as<unsigned long long>(), nullptr};
+            MarkSel sel{kw.mark.as<unsigned long long>(), kw.lanes.as<unsigned long long>()};
             compact_rows<unsigned long long, std::uint32_t, MarkSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
                 g.V, sel, g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>(), d_ctr, nullptr, nullptr);
             hip_check(hipGetLastError(), "launch mark compaction");
@@ -204,7 +205,9 @@ namespace hyt
             a.K = K;
             a.mark = kw.mark.as<unsigned long long>();
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            std::uint32_t F = compact_marks(g, kw).next_count;
+            const Counters c0 = compact_marks(g, kw);
+            std::uint32_t F = c0.next_count;
+            std::uint64_t pairs = c0.frontier_edges; // (row, query) pairs of the queue
             st.converged = F == 0 ? 1 : 0;
             for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
             {
@@ -220,10 +223,20 @@ namespace hyt
                 a.F = F;
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                expandk_light<OffT, KP, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                hip_check(hipGetLastError(), "launch expandk_light");
-                expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-                hip_check(hipGetLastError(), "launch expandk_heavy");
+                if (use_items(o.lane_form, pairs, F, KP))
+                {
+                    expandk_light_items<OffT, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    hip_check(hipGetLastError(), "launch expandk_light_items");
+                    expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    hip_check(hipGetLastError(), "launch expandk_heavy_items");
+                }
+                else
+                {
+                    expandk_light<OffT, KP, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                    hip_check(hipGetLastError(), "launch expandk_light");
+                    expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    hip_check(hipGetLastError(), "launch expandk_heavy");
+                }
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
                 const Counters c = compact_marks(g, kw);
@@ -246,6 +259,7 @@ namespace hyt
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
                 F = c.next_count;
+                pairs = c.frontier_edges;
                 if (F == 0)
                     st.converged = 1;
             }
@@ -282,6 +296,7 @@ namespace hyt
             ch = o.heavy_chunk ? o.heavy_chunk : kDefaultHeavyChunk;
             require(lm <= (1U << 20), "light_max_degree must be <= 2^20");
             require(ch >= 64 && ch <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
+            require(o.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");
         }
     } // namespace
 

@@ -181,6 +181,186 @@ namespace hyt
             }
         }
 
+        // ------------------------------------------------- sparse lane masks
+        // When few queries are active per frontier row, a group of KP lanes
+        // per edge leaves most lanes idle. The item form gives every lane one
+        // (edge, active query) pair instead: a row with c active queries and
+        // d edges contributes d*c items, consecutive lanes take the c queries
+        // of one edge (same column/weight, neighbouring [V][K] words), and the
+        // wavefront stays full whatever the mask density.
+
+        // Position of the r-th set bit (0-based) of m; m has more than r set bits.
+        __device__ __forceinline__ std::uint32_t nth_set_bit(std::uint64_t m, std::uint32_t r)
+        {
+            std::uint32_t k = 0;
+#pragma unroll
+            for (int w = 32; w > 0; w >>= 1)
+            {
+                const std::uint64_t low = m & ((1ULL << w) - 1ULL);
+                const std::uint32_t c = static_cast<std::uint32_t>(__popcll(low));
+                if (r >= c)
+                {
+                    r -= c;
+                    m >>= w;
+                    k += static_cast<std::uint32_t>(w);
+                }
+                else
+                    m = low;
+            }
+            return k;
+        }
+
+        // Walks `total` items, kUnroll per lane in flight; item_of(e, pos, row,
+        // scale, k) names the edge position, row, row scale and query of item e.
+        // An improved (v, k) sets bit k of mark[v] unless it is already set.
+        template <class Op, class ItemFn>
+        __device__ __forceinline__ void walk_items_k(const Op &op, KArgs &a, std::uint32_t total, ItemFn item_of)
+        {
+            const int lane = lane_id();
+            for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
+            {
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], kk[kUnroll];
+                bool ok[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                {
+                    const std::uint32_t e = base + static_cast<std::uint32_t>(u * kWave + lane);
+                    ok[u] = e < total;
+                    vv[u] = ww[u] = row[u] = scale[u] = kk[u] = 0;
+                    if (ok[u])
+                    {
+                        std::uint64_t pos;
+                        item_of(e, pos, row[u], scale[u], kk[u]);
+                        op.load(pos, vv[u], ww[u]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                    if (ok[u] && op.update(row[u], scale[u], vv[u], ww[u], static_cast<int>(kk[u])))
+                    {
+                        unsigned long long *mp = a.mark + vv[u];
+                        const unsigned long long bit = 1ULL << kk[u];
+                        if ((*mp & bit) == 0ULL)
+                            atomicOr(mp, bit);
+                    }
+            }
+        }
+
+        template <class OffT, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_light_items(const OffT *__restrict__ off, Op op, KArgs a)
+        {
+            __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
+            __shared__ OffT s_begin[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
+            __shared__ std::uint32_t s_c[kWavesPerBlock][kWave];
+            __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
+
+            const int lane = lane_id();
+            const int wid = static_cast<int>(threadIdx.x) / kWave;
+            const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+            std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0;
+            std::uint64_t mask = 0;
+            OffT begin = 0, full = 0;
+            if (i < a.F)
+            {
+                const std::uint32_t u = a.q[i];
+                begin = off[u];
+                full = off[u + 1] - begin;
+                mask = a.qmask[i];
+                c = static_cast<std::uint32_t>(__popcll(mask));
+                row = Op::kRowIsQueueIndex ? i : u;
+                scale = op.row_scale(static_cast<std::uint64_t>(full));
+                if (full > a.light_max)
+                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
+                else
+                {
+                    deg = static_cast<std::uint32_t>(full);
+                    items = deg * c;
+                }
+            }
+            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t ch) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(ch) * a.chunk;
+                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                HeavyChunkK h;
+                h.row = row;
+                h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+                h.begin = static_cast<std::uint64_t>(begin) + lo;
+                h.mask = mask;
+                h.scale = scale;
+                h.pad = 0;
+                return h;
+            });
+            const std::uint32_t incl = wave_incl_scan(items);
+            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+            std::uint32_t dsum = deg;
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+                dsum += __shfl_xor(dsum, d, kWave);
+            s_ex[wid][lane] = incl - items;
+            s_begin[wid][lane] = begin;
+            s_row[wid][lane] = row;
+            s_scale[wid][lane] = scale;
+            s_c[wid][lane] = c;
+            s_mask[wid][lane] = mask;
+            __syncthreads();
+            if (total == 0)
+                return;
+            if (lane == 0)
+                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(dsum));
+            const std::uint32_t *ex = s_ex[wid];
+            walk_items_k(op, a, total,
+                         [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint32_t &sc,
+                             std::uint32_t &k) {
+                             const int j = owner_of(ex, e);
+                             const std::uint32_t local = e - ex[j];
+                             const std::uint32_t cj = s_c[wid][j];
+                             const std::uint32_t edge = local / cj;
+                             k = nth_set_bit(s_mask[wid][j], local - edge * cj);
+                             pos = static_cast<std::uint64_t>(s_begin[wid][j]) + edge;
+                             r = s_row[wid][j];
+                             sc = s_scale[wid][j];
+                         });
+        }
+
+        template <class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_heavy_items(Op op, KArgs a)
+        {
+            const int lane = lane_id();
+            const std::uint32_t nh = a.ctr->heavy_count;
+            const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
+            const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+            for (std::uint32_t ci = (blockIdx.x * kBlock + threadIdx.x) / kWave; ci < n; ci += waves)
+            {
+                const HeavyChunkK h = a.heavy[ci];
+                if (lane == 0)
+                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                const std::uint32_t c = static_cast<std::uint32_t>(__popcll(h.mask));
+                if (c == 0)
+                    continue;
+                walk_items_k(op, a, h.count * c,
+                             [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint32_t &sc,
+                                 std::uint32_t &k) {
+                                 const std::uint32_t edge = e / c;
+                                 k = nth_set_bit(h.mask, e - edge * c);
+                                 pos = h.begin + edge;
+                                 r = h.row;
+                                 sc = h.scale;
+                             });
+            }
+        }
+
+        // Item form when asked, or (auto) when the queue's (row, query) pairs
+        // fill under half of the group form's lane slots.
+        inline bool use_items(std::uint32_t form, std::uint64_t pairs, std::uint64_t F, std::uint32_t KP)
+        {
+            if (form == HYT_LANES_GROUP)
+                return false;
+            if (form == HYT_LANES_ITEMS)
+                return true;
+            return pairs * 2 < F * static_cast<std::uint64_t>(KP);
+        }
+
         // ---------------------------------------------------------- SSSP-K
         template <int WM>
         struct SsspKOp

@@ -174,6 +174,8 @@ namespace hyt
         std::uint64_t rounds = 0, edges = 0;
         std::uint64_t frontier_rows = 0, lane_rows = 0; // lane occupancy of the rounds run
         std::uint64_t pairs = 0;                         // (row, lane) pairs of the current queue
+        std::uint64_t item_rounds = 0;                   // rounds run in the item form
+        std::uint32_t form = HYT_LANES_AUTO;
 
         ~LaneSession()
         {
@@ -326,10 +328,21 @@ namespace hyt
                     S.snap.as<float>());
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
-            expandk_light<OffT, KP, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-            hip_check(hipGetLastError(), "launch expandk_light");
-            expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-            hip_check(hipGetLastError(), "launch expandk_heavy");
+            if (use_items(S.form, S.pairs, S.F, S.KP))
+            {
+                expandk_light_items<OffT, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                hip_check(hipGetLastError(), "launch expandk_light_items");
+                expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                hip_check(hipGetLastError(), "launch expandk_heavy_items");
+                S.item_rounds += 1;
+            }
+            else
+            {
+                expandk_light<OffT, KP, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                hip_check(hipGetLastError(), "launch expandk_light");
+                expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                hip_check(hipGetLastError(), "launch expandk_heavy");
+            }
             MarkSel sel{S.mark.as<unsigned long long>(), S.lanes.as<unsigned long long>()};
             compact_rows<unsigned long long, std::uint32_t, MarkSel><<<compact_blocks(g.V), kBlock, 0, s>>>(
                 g.V, sel, S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>(), d_ctr, nullptr, nullptr);
@@ -566,6 +579,7 @@ namespace hyt
             require(graphs != nullptr && ngraphs > 0, "no graphs");
             require(n == 0 || (reqs != nullptr && out != nullptr), "null requests or completions");
             require(cfg.max_lanes >= 1 && cfg.max_lanes <= 64, "max_lanes must be in [1, 64]");
+            require(cfg.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");
             for (std::uint64_t i = 0; i < n; ++i)
             {
                 require(reqs[i].graph < ngraphs, "request graph index out of range");
@@ -611,6 +625,7 @@ namespace hyt
                     ses.push_back(std::make_unique<LaneSession>());
                     create_session(*ses.back(), &graphs[skey[i].first]->g, skey[i].second, K[i], cfg.damping,
                                    cfg.epsilon);
+                    ses.back()->form = cfg.lane_form;
                     st.state_bytes += fixed[i] + K[i] * per_lane[i];
                     if (i < 16)
                         st.lanes[i] = K[i];
@@ -714,6 +729,7 @@ namespace hyt
                 st.edges += L->edges;
                 st.frontier_rows += L->frontier_rows;
                 st.lane_rows += L->lane_rows;
+                st.item_rounds += L->item_rounds;
             }
         }
     } // namespace
@@ -793,6 +809,15 @@ int hyt_lanes_release(hyt_lanes_t s, uint32_t lane)
         hyt::require(lane < s->s.K, "lane out of range");
         hyt::require(((s->s.running >> lane) & 1ULL) == 0, "lane is still running");
         s->s.occupied &= ~(1ULL << lane);
+    });
+}
+
+int hyt_lanes_set_form(hyt_lanes_t s, uint32_t form)
+{
+    return hyt::guarded("hyt_lanes_set_form", [&] {
+        hyt::require(s != nullptr, "null session");
+        hyt::require(form <= HYT_LANES_ITEMS, "lane form must be 0, 1 or 2");
+        s->s.form = form;
     });
 }
 

@@ -136,8 +136,16 @@ typedef struct hyt_run_options {
     uint32_t pull_percent;     /* AUTO: pull when frontier out-edges >= pct% of E; 0 = default */
     uint32_t delta;            /* ASYNC SSSP: delta-stepping bucket width in weight units
                                   (0 = plain rounds); SYNC mode ignores it */
-    uint32_t reserved;
+    uint32_t lane_form;        /* batched (K >= 2): hyt_lane_form */
 } hyt_run_options;
+
+/* Work split of the K-lane rounds (batched queries and lane sessions). */
+enum hyt_lane_form {
+    HYT_LANES_AUTO = 0,  /* per round: items when the queue's (row, query) pairs
+                            fill under half of the group form's lanes */
+    HYT_LANES_GROUP = 1, /* next_pow2(K) lanes per edge, one per query */
+    HYT_LANES_ITEMS = 2  /* one lane per (edge, active query) pair */
+};
 
 typedef struct hyt_run_stats {
     uint64_t rounds;          /* iterations in the original's sense */
@@ -359,6 +367,8 @@ int hyt_lanes_read(hyt_lanes_t s, uint32_t lane, void *out, uint32_t out_on_devi
                    uint64_t *checksum, uint32_t *rounds);
 /* Frees a finished lane. */
 int hyt_lanes_release(hyt_lanes_t s, uint32_t lane);
+/* hyt_lane_form of the session's rounds (default HYT_LANES_AUTO). */
+int hyt_lanes_set_form(hyt_lanes_t s, uint32_t form);
 int hyt_lanes_info(hyt_lanes_t s, uint32_t *lanes, uint64_t *occupied, uint64_t *running, uint32_t *frontier,
                    uint64_t *state_bytes);
 
@@ -379,7 +389,7 @@ typedef struct hyt_completion {
 typedef struct hyt_serve_config {
     uint64_t device_budget_bytes; /* resident graphs + every session's state */
     uint32_t max_lanes;           /* per session, <= 64 */
-    uint32_t reserved0;
+    uint32_t lane_form;           /* hyt_lane_form of every session */
     float damping, epsilon;       /* PPR */
     uint32_t reserved[4];
 } hyt_serve_config;
@@ -389,6 +399,7 @@ typedef struct hyt_serve_stats {
     uint64_t steps, rounds, edges, admitted; /* steps = session rounds completed by the loop */
     uint64_t graph_bytes, state_bytes;
     uint64_t frontier_rows, lane_rows; /* frontier rows expanded / (row, query) pairs among them */
+    uint64_t item_rounds;         /* rounds run with one lane per (edge, query) pair (sparse masks) */
     uint32_t sessions;
     uint32_t lanes[16];           /* lanes of the first 16 sessions (one per graph and kind, in order of first request) */
     uint32_t reserved;

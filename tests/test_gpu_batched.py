@@ -94,3 +94,25 @@ def test_ppr_k_column_equals_single(gpu, abi):
     for k, s in enumerate(seeds):
         one, _ = g.ppr(int(s), D, 1e-9)
         assert np.abs(r[:, k] - one).sum() <= 2e-6
+
+
+@pytest.mark.parametrize("form", [1, 2])
+@pytest.mark.parametrize("K", [3, 8, 64])
+def test_k_lane_forms_match_oracle(gpu, abi, orc, K, form):
+    # group form (next_pow2(K) lanes per edge) and item form (one lane per
+    # (edge, query) pair) give the same answers; small light_max/chunk so
+    # both the light and the chunked heavy kernels of each form run
+    hg = abi.HostGraph(scale=12, edge_factor=16, seed=31, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 50 + K, K)
+    src[-1] = src[0]
+    for lm, ch in ((256, 1024), (4, 64)):
+        d, _ = g.sssp(src, lane_form=form, light_max_degree=lm, heavy_chunk=ch)
+        for k in range(K):
+            assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[k]))), k
+    hu = abi.HostGraph(scale=12, edge_factor=16, seed=32, weight_min=0, weight_max=0)
+    gu = abi.DeviceGraph.from_host(hu)
+    r, st = gu.ppr(src, D, 1e-8, lane_form=form, light_max_degree=4, heavy_chunk=64)
+    for k in range(K):
+        same, _, _ = orc.delta_push_f64(hu.offsets, hu.columns, D, 1e-8, seed=int(src[k]))
+        assert np.abs(r[:, k] - same).sum() <= 2e-6, k

@@ -42,9 +42,11 @@ def _drain(abi, sessions, owners, pending, results, rounds=1):
             return
 
 
-def test_sssp_lanes_staggered_admission(gpu, abi, orc):
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_sssp_lanes_staggered_admission(gpu, abi, orc, form):
     hg, g = _graph(abi, 12, 11)
     S = abi.LaneSession(g, abi.HYT_QUERY_SSSP, lanes=4)
+    S.set_form(form)
     srcs = [int(s) for s in abi.sample_sources(hg.offsets, 3, 11)]
     owners = [[None] * 4]
     pending = [srcs[4:]]
@@ -74,10 +76,12 @@ def test_sssp_lanes_staggered_admission(gpu, abi, orc):
     assert S.info()["occupied"] == 0
 
 
-def test_ppr_lanes_staggered_admission(gpu, abi, orc):
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_ppr_lanes_staggered_admission(gpu, abi, orc, form):
     hg, g = _graph(abi, 12, 2, wmax=0)
     eps = 1e-8
     S = abi.LaneSession(g, abi.HYT_QUERY_PPR, lanes=3, damping=D, epsilon=eps)
+    S.set_form(form)
     seeds = [int(s) for s in abi.sample_sources(hg.offsets, 21, 7)] + [5, 5]
     owners = [[None] * 3]
     pending = [seeds[2:]]
@@ -145,6 +149,7 @@ def test_serve_trace_mixed_results_match_alone(gpu, abi, orc):
     comp, st, res = abi.serve_trace([g for _, g in graphs], reqs, keep_results=True,
                                     device_budget_bytes=gbytes + lane_room + 4096, epsilon=1e-8)
     _say(f"serve: {st}")
+    assert 0 < st["item_rounds"] <= st["rounds"]
     assert st["sessions"] == 4 and all(1 <= k <= 3 for k in st["lanes"])
     assert st["admitted"] == n
     for i, (t, gi, kind, src) in enumerate(reqs):
