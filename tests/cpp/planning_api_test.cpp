@@ -1,0 +1,295 @@
+// Host planning API in the original project's test style: the known answers
+// of its tests/unit_tests.cpp for ExpTM-Filter (918-1121), ExpTM-Compaction
+// (1289-1563), ImpTM zero-copy (1737-2056) and the contribution scheduler
+// (2751-2887), plus seeded random graphs checked against direct per-vertex
+// recomputation (partition walking, per-partition totals, coverage errors).
+#include "graph/activity_tracker.hpp"
+#include "graph/csr_graph.hpp"
+#include "graph/partition.hpp"
+#include "scheduling/contribution_scheduler.hpp"
+#include "transfer/compaction_engine.hpp"
+#include "transfer/filter_engine.hpp"
+#include "transfer/zero_copy_engine.hpp"
+
+#include <cmath>
+#include <cstdint>
+#include <iostream>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace
+{
+    using hytgraph::graph::ActivityTracker;
+    using hytgraph::graph::CSRGraph;
+    using hytgraph::graph::LogicalPartition;
+    using hytgraph::graph::LogicalPartitioner;
+    namespace tr = hytgraph::transfer;
+    namespace sc = hytgraph::scheduling;
+    using Ids = std::vector<CSRGraph::vertex_id>;
+    using Offs = std::vector<CSRGraph::offset_type>;
+
+    void expect(bool condition, const std::string &message)
+    {
+        if (!condition)
+            throw std::runtime_error("FAIL: " + message);
+    }
+
+    template <class Ex, class Fn>
+    void expect_throw(Fn fn, const std::string &message)
+    {
+        bool threw = false;
+        try
+        {
+            fn();
+        }
+        catch (const Ex &)
+        {
+            threw = true;
+        }
+        expect(threw, message);
+    }
+
+    void test_filter_known_answers()
+    {
+        CSRGraph g(4, {0, 2, 3, 4, 5}, {1, 2, 2, 0, 0});
+        const auto parts = LogicalPartitioner(2U * sizeof(CSRGraph::vertex_id)).partition(g);
+        ActivityTracker a(g.num_vertices());
+        a.set_active(0);
+        const auto plan = tr::ExpTMFilter(true).plan(g, parts, a);
+        expect(plan.partitions.size() == parts.size(), "filter: every partition has a decision");
+        expect(plan.active_partition_count() >= 1U, "filter: active partition");
+        expect(plan.transferred_partition_count() == plan.active_partition_count(), "filter: moves active ones");
+        expect(plan.transferred_edge_count() > 0U && plan.transferred_edge_count() < g.num_edges(),
+               "filter: skips inactive partitions");
+        std::size_t bytes = 0;
+        CSRGraph::offset_type edges = 0;
+        for (std::size_t i = 0; i < parts.size(); ++i)
+            if (plan.partitions[i].transferred)
+            {
+                edges += parts[i].edge_count();
+                bytes += parts[i].edge_data_bytes();
+                expect(plan.partitions[i].transferred_edges == parts[i].edge_count(), "filter: whole partition");
+            }
+        expect(plan.transferred_edge_count() == edges && plan.transferred_byte_count() == bytes, "filter totals");
+        const auto all = tr::ExpTMFilter(false).plan(g, parts, a);
+        expect(all.transferred_partition_count() == parts.size() && all.transferred_edge_count() == g.num_edges(),
+               "disabled filter moves everything");
+        ActivityTracker wrong(3);
+        expect_throw<std::invalid_argument>([&] { (void)tr::ExpTMFilter().plan(g, parts, wrong); },
+                                            "filter: activity size checked");
+        std::vector<LogicalPartition> gap(parts.begin() + 1, parts.end());
+        expect_throw<std::invalid_argument>([&] { (void)tr::ExpTMFilter().plan(g, gap, a); },
+                                            "filter: coverage checked");
+    }
+
+    void test_compaction_known_answers()
+    {
+        CSRGraph g(4, {0, 2, 3, 5, 6}, {1, 2, 2, 0, 3, 1});
+        const auto parts = LogicalPartitioner(16U * sizeof(CSRGraph::vertex_id)).partition(g);
+        expect(parts.size() == 1U, "compaction: one partition");
+        ActivityTracker a(g.num_vertices());
+        a.set_active(0);
+        a.set_active(2);
+        const auto r = tr::ExpTMCompaction(true).compact(g, parts, a);
+        expect(r.compacted_partition_count() == 1U && r.active_vertex_count() == 2U && r.active_edge_count() == 4U,
+               "compaction counts");
+        const auto &p = r.partitions.front();
+        expect(p.active_vertices == Ids({0, 2}), "compaction: active vertices ascending");
+        expect(p.neighbors == Ids({1, 2, 0, 3}), "compaction: neighbours in source order");
+        expect(p.neighbor_index == Offs({0, 2, 4}), "compaction: prefix index");
+        expect(tr::ExpTMCompaction::expanded_neighbors(p) == Ids({1, 2, 0, 3}), "compaction: expansion");
+        expect(r.total_bytes() == 4 * 4 + 3 * 8, "compaction: payload bytes");
+        // zero-degree active vertex
+        CSRGraph z(3, {0, 1, 1, 2}, {1, 0});
+        ActivityTracker az(3);
+        az.set_active(1);
+        const auto rz = tr::ExpTMCompaction(true).compact(z, LogicalPartitioner(64).partition(z), az);
+        expect(rz.compacted_partition_count() == 1U && rz.partitions[0].active_vertices == Ids({1}) &&
+                   rz.partitions[0].neighbors.empty() && rz.partitions[0].neighbor_index == Offs({0, 0}),
+               "compaction: zero-degree vertex keeps an empty range");
+        // all active = the original CSR
+        ActivityTracker aa(4);
+        for (CSRGraph::vertex_id v = 0; v < 4; ++v)
+            aa.set_active(v);
+        const auto ra = tr::ExpTMCompaction(true).compact(g, parts, aa);
+        expect(ra.partitions[0].neighbors == g.column_indices() && ra.partitions[0].neighbor_index == g.row_offsets(),
+               "compaction: all active reproduces the CSR");
+        expect(tr::ExpTMCompaction(false).compact(g, parts, aa).partitions.empty(), "disabled compaction is empty");
+        tr::CompactedPartition bad = p;
+        bad.neighbor_index = {0, 3, 2};
+        expect_throw<std::invalid_argument>([&] { (void)tr::ExpTMCompaction::expanded_neighbors(bad); },
+                                            "expansion: monotone index");
+        bad.neighbor_index = {0, 2};
+        expect_throw<std::invalid_argument>([&] { (void)tr::ExpTMCompaction::expanded_neighbors(bad); },
+                                            "expansion: index size");
+    }
+
+    void test_zero_copy_known_answers()
+    {
+        CSRGraph g(4, {0, 2, 3, 4, 4}, {1, 2, 2, 0});
+        const auto parts = LogicalPartitioner(2U * sizeof(CSRGraph::vertex_id)).partition(g);
+        ActivityTracker a(4);
+        a.set_active_vertices({0, 2});
+        const auto r = tr::ImpTMZeroCopy().prepare(g, parts, a);
+        expect(r.mode == tr::ZeroCopyMode::Modeled && !r.host_memory_mapped, "zero-copy: modeled");
+        expect(r.active_vertex_count == 2 && r.active_edge_count == 3, "zero-copy: active counts");
+        expect(r.memory_request_count == 2 && r.alignment_overhead_count == 1 && r.modeled_tlp_count == 1,
+               "zero-copy: 2 requests, 1 alignment, 1 TLP");
+        std::size_t seen = 0;
+        for (const auto &p : r.partitions)
+        {
+            expect(p.active_vertices.size() == p.vertex_metrics.size() && p.modeled_tlp_count == 0,
+                   "zero-copy: per-partition metrics");
+            seen += p.active_vertices.size();
+        }
+        expect(seen == 2, "zero-copy: each active vertex once");
+        CSRGraph wide(2, {0, 40, 40}, Ids(40, 1));
+        ActivityTracker aw(2);
+        aw.set_active(0);
+        tr::ZeroCopyOptions o;
+        o.request_payload_bytes = 16U;
+        o.max_requests_per_tlp = 2U;
+        const auto rw = tr::ImpTMZeroCopy(o).prepare(wide, LogicalPartitioner(1024U).partition(wide), aw);
+        expect(rw.memory_request_count == 10 && rw.modeled_tlp_count == 5, "zero-copy: m=16 -> 10 requests, 5 TLPs");
+        CSRGraph zd(3, {0, 1, 1, 1}, {1});
+        ActivityTracker az(3);
+        az.set_active(1);
+        const auto rz = tr::ImpTMZeroCopy().prepare(zd, LogicalPartitioner(1024U).partition(zd), az);
+        expect(rz.active_vertex_count == 1 && rz.active_edge_count == 0 && rz.memory_request_count == 0 &&
+                   rz.alignment_overhead_count == 0 && rz.modeled_tlp_count == 0,
+               "zero-copy: zero-degree vertex costs nothing");
+        tr::ZeroCopyOptions zero;
+        zero.request_payload_bytes = 0;
+        expect_throw<std::invalid_argument>([&] { (void)tr::ImpTMZeroCopy(zero).prepare(g, parts, a); },
+                                            "zero-copy: options checked");
+    }
+
+    void test_scheduler_known_answers()
+    {
+        using P = sc::ContributionPriority;
+        sc::ContributionScheduler s;
+        auto plan = s.schedule({{0, 0.10}, {1, 0.90}, {2, 0.30}, {3, 0.70}});
+        expect(plan.ordered_item_indices == std::vector<std::size_t>({1, 3, 2, 0}), "scheduler: descending");
+        expect(plan.size() == 4 && plan.metrics.input_item_count == 4 && plan.metrics.scheduled_item_count == 4,
+               "scheduler: counts");
+        expect(std::fabs(plan.metrics.scheduling_overhead - 2.0) < 1e-12, "scheduler: log2 overhead");
+        plan = s.schedule({{3, 0.5}, {1, 0.5}, {2, 0.9}, {0, 0.5}});
+        expect(plan.ordered_item_indices == std::vector<std::size_t>({2, 0, 1, 3}), "scheduler: ties by index");
+        sc::ContributionSchedulerOptions eo;
+        eo.zero_contribution_epsilon = 1.0e-6;
+        plan = sc::ContributionScheduler(eo).schedule({{0, 0.0}, {1, 1e-8}, {2, 0.25}, {3, -1e-8}});
+        expect(plan.metrics.zero_contribution_count == 3 &&
+                   plan.ordered_item_indices == std::vector<std::size_t>({2, 1, 0, 3}),
+               "scheduler: zero epsilon");
+        plan = s.schedule({{0, 0.10}, {1, 0.90}, {2, 0.30}});
+        expect(plan.metrics.reordered_item_count == 3 && plan.reordered(), "scheduler: reordered count");
+        plan = sc::ContributionScheduler::synchronous_reference({{5, 0.1}, {2, 0.9}, {8, 0.3}});
+        expect(plan.ordered_item_indices == std::vector<std::size_t>({5, 2, 8}), "scheduler: synchronous order");
+        plan = s.schedule({{4, 1.0}, {4, 2.0}, {1, 0.0}});
+        expect(plan.metrics.redundant_work_count == 1, "scheduler: duplicate items");
+        expect_throw<std::invalid_argument>([&] { (void)s.schedule({P{0, NAN}}); }, "scheduler: non-finite");
+        expect_throw<std::invalid_argument>([] { (void)sc::make_contribution_priorities({1, 2}, {0.5}); },
+                                            "priorities: sizes");
+        sc::ContributionSchedulerOptions nd;
+        nd.deterministic_tie_break = false;
+        plan = sc::ContributionScheduler(nd).schedule({{3, 0.5}, {1, 0.5}, {0, 0.5}});
+        expect(plan.ordered_item_indices == std::vector<std::size_t>({3, 1, 0}), "scheduler: stable without tie break");
+        const auto m = sc::ContributionScheduler::measure_work_metrics({{1, 0.0}, {1, 1.0}}, {{1, true}, {2, false}});
+        expect(m.zero_contribution_count == 1 && m.redundant_work_count == 1 && m.stale_work_count == 1,
+               "scheduler: work metrics");
+    }
+
+    // Seeded random graphs: every planner against a direct recomputation.
+    void test_random_graphs()
+    {
+        std::mt19937_64 rng(12345);
+        for (int trial = 0; trial < 40; ++trial)
+        {
+            const std::uint32_t V = 1 + static_cast<std::uint32_t>(rng() % 300);
+            Offs off(V + 1, 0);
+            Ids col;
+            for (std::uint32_t v = 0; v < V; ++v)
+            {
+                const std::uint32_t d = (rng() % 7 == 0) ? static_cast<std::uint32_t>(rng() % 90)
+                                                         : static_cast<std::uint32_t>(rng() % 6);
+                for (std::uint32_t k = 0; k < d; ++k)
+                    col.push_back(static_cast<CSRGraph::vertex_id>(rng() % V));
+                off[v + 1] = col.size();
+            }
+            CSRGraph g(V, off, col);
+            const auto parts = LogicalPartitioner(4U * (1 + rng() % 64)).partition(g);
+            ActivityTracker a(V);
+            for (std::uint32_t v = 0; v < V; ++v)
+                if (rng() % 3 == 0)
+                    a.set_active(v);
+            tr::ZeroCopyOptions zo;
+            zo.request_payload_bytes = 4U * (1 + rng() % 40);
+            zo.max_requests_per_tlp = 1 + rng() % 8;
+            zo.alignment_bytes = 4U * (1 + rng() % 40);
+            const auto fp = tr::ExpTMFilter().plan(g, parts, a);
+            const auto cp = tr::ExpTMCompaction().compact(g, parts, a);
+            const auto zp = tr::ImpTMZeroCopy(zo).prepare(g, parts, a);
+            std::size_t ci = 0;
+            std::uint64_t all_req = 0;
+            for (std::size_t i = 0; i < parts.size(); ++i)
+            {
+                const auto &p = parts[i];
+                Ids act, nb;
+                Offs idx{0};
+                std::uint64_t aedges = 0, req = 0, al = 0;
+                for (auto v = p.vertex_begin(); v < p.vertex_end(); ++v)
+                {
+                    if (!a.is_active(v))
+                        continue;
+                    act.push_back(v);
+                    const std::uint64_t d = off[v + 1] - off[v];
+                    for (auto e = off[v]; e < off[v + 1]; ++e)
+                        nb.push_back(col[e]);
+                    idx.push_back(nb.size());
+                    aedges += d;
+                    req += (d * 4 + zo.request_payload_bytes - 1) / zo.request_payload_bytes;
+                    al += (d != 0 && (off[v] * 4) % zo.alignment_bytes != 0) ? 1 : 0;
+                }
+                expect(fp.partitions[i].active == (aedges != 0) && fp.partitions[i].active_edges == aedges,
+                       "random: filter activity");
+                if (!act.empty())
+                {
+                    expect(ci < cp.partitions.size() && cp.partitions[ci].partition_index == i, "random: compacted id");
+                    expect(cp.partitions[ci].active_vertices == act && cp.partitions[ci].neighbors == nb &&
+                               cp.partitions[ci].neighbor_index == idx,
+                           "random: compacted payload");
+                    ++ci;
+                }
+                expect(zp.partitions[i].active_vertices == act && zp.partitions[i].active_edge_count == aedges &&
+                           zp.partitions[i].memory_request_count == req &&
+                           zp.partitions[i].alignment_overhead_count == al,
+                       "random: zero-copy partition metrics");
+                all_req += req;
+            }
+            expect(ci == cp.partitions.size(), "random: no extra compacted partitions");
+            expect(zp.modeled_tlp_count == (all_req + zo.max_requests_per_tlp - 1) / zo.max_requests_per_tlp,
+                   "random: modeled TLPs");
+        }
+    }
+} // namespace
+
+int main()
+{
+    try
+    {
+        test_filter_known_answers();
+        test_compaction_known_answers();
+        test_zero_copy_known_answers();
+        test_scheduler_known_answers();
+        test_random_graphs();
+        std::cout << "planning API tests passed.\n";
+        return 0;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << e.what() << '\n';
+        return 1;
+    }
+}

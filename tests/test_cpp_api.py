@@ -28,3 +28,10 @@ def test_cpp_gpu_api_skips_without_device(abi):
 def test_cpp_gpu_api(gpu):
     r = _run("gpu")
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_cpp_planning_api():
+    exe = EXE.parent / "planning_api_test"
+    assert exe.exists(), f"{exe} missing: run python -m hytgraph_amd.build"
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
