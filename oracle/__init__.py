@@ -33,10 +33,12 @@ _ref = None
 
 
 def build(with_ref: bool = True) -> None:
-    """Compiles liborc.so, and _ref/libhyt_ref.so when the original tree is present."""
+    """Compiles liborc.so, and (when the original tree is present) _ref/libhyt_ref.so
+    and the original's SEP header tests built against this repository's headers."""
     subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
     if with_ref and REFERENCE_TREE.exists():
         subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF={REFERENCE_TREE}"], check=True)
+        subprocess.run(["make", "-s", "-C", str(HERE), "sep", f"REF={REFERENCE_TREE}"], check=True)
 
 
 def _load_orc():

@@ -6,9 +6,13 @@
 #include "algorithms/sssp.hpp"
 #include "graph/csr_graph.hpp"
 #include "hytgraph_amd.h"
+#include "sep/sep_execution_factory.hpp"
+#include "sep/sep_execution_variant_registry.hpp"
+#include "sep/sep_gpu_drivers.hpp"
 
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <iostream>
 #include <limits>
 #include <stdexcept>
@@ -88,6 +92,59 @@ namespace
         expect_throw<std::invalid_argument>([] { (void)alg::sssp_contributions({1.0F}, {}); }, "size mismatch");
     }
 
+    // A FIFO frontier for the data-driven SEP variants.
+    class QueueFrontier final : public hytgraph::sep::SEPFrontier
+    {
+    public:
+        void clear() noexcept override { q_.clear(); }
+        void push(node_id_type n) override { q_.push_back(n); }
+        bool pop(node_id_type &n) override
+        {
+            if (q_.empty())
+                return false;
+            n = q_.front();
+            q_.pop_front();
+            return true;
+        }
+        std::size_t size() const noexcept override { return q_.size(); }
+        bool empty() const noexcept override { return q_.empty(); }
+
+    private:
+        std::deque<node_id_type> q_;
+    };
+
+    void test_sep_host()
+    {
+        namespace sep = hytgraph::sep;
+        sep::SEPSSSPApplication app(0);
+        float d = 10.0F;
+        auto r = app.accumulate_buffer(0, 1, 2.0F, d, 3.0F);
+        expect(r.changed && r.activate && d == 5.0F, "SEP SSSP accumulate lowers the buffer");
+        r = app.accumulate_buffer(0, 1, 2.0F, d, 4.0F);
+        expect(!r.changed && d == 5.0F, "SEP SSSP accumulate keeps a smaller buffer");
+        expect(app.init_value(0) == 0.0F && std::isinf(app.init_value(1)) && app.requires_edge_weights(),
+               "SEP SSSP init");
+        sep::SEPPageRankApplication pr(0.85F, 1e-3F);
+        float res = 0.0F;
+        expect(pr.accumulate_buffer(0, 1, res, 2e-3F).activate && !pr.accumulate_buffer(0, 1, res, 1e-3F).activate,
+               "SEP PageRank activates on crossing epsilon");
+        sep::SEPSSSPGpuDriver drv;
+        expect(drv.execute_step().status == sep::SEPExecutionStatus::NOT_INITIALIZED, "driver not initialised");
+        drv.initialize(app, sep::SEPExecutionConfig{});
+        expect(drv.execute_step().status == sep::SEPExecutionStatus::INVALID_CONTEXT, "driver without context");
+        CSRGraph g(2, {0, 1, 1}, {1}, {1.0F});
+        sep::SEPExecutionContext no_frontier(const_cast<CSRGraph *>(&g));
+        drv.set_context(no_frontier);
+        expect(drv.execute_step().status == sep::SEPExecutionStatus::INVALID_CONTEXT,
+               "data-driven variant needs a frontier");
+        sep::SEPPageRankApplication bad(1.0F, 1e-6F);
+        expect_throw<std::invalid_argument>([&] { sep::SEPPageRankGpuDriver().initialize(bad, {}); },
+                                            "PageRank driver rejects damping 1");
+        auto deferred = sep::SEPExecutionFactory<float, float, float>::create("ASYNC_PUSH_DD");
+        expect(deferred->execute_step().status == sep::SEPExecutionStatus::INVALID_CONFIGURATION,
+               "generic factory drivers stay deferred");
+    }
+
     // -------------------------------------------------------------- gpu
     void test_sssp_gpu_known_answers()
     {
@@ -147,6 +204,42 @@ namespace
         auto ppr = alg::ppr_gpu(g, {0, 2}, 0.85F, 1e-9F);
         expect(std::fabs(ppr.ranks[0][0] + ppr.ranks[0][1] + ppr.ranks[0][2] - 1.0F) <= 1e-5F, "ppr mass");
     }
+    void test_sep_gpu()
+    {
+        namespace sep = hytgraph::sep;
+        CSRGraph g(4, {0, 2, 4, 5, 5}, {1, 2, 2, 3, 3}, {1.0F, 4.0F, 2.0F, 6.0F, 1.0F});
+        CSRGraph cyc(3, {0, 1, 2, 3}, {1, 2, 0});
+        QueueFrontier frontier;
+        for (const auto &v : sep::SEPExecutionVariantRegistry::all())
+        {
+            sep::SEPSSSPApplication app(0);
+            sep::SEPSSSPGpuDriver drv;
+            drv.initialize(app, sep::SEPExecutionConfig(v));
+            sep::SEPExecutionContext ctx(const_cast<CSRGraph *>(&g), nullptr, nullptr, nullptr, &frontier);
+            drv.set_context(ctx);
+            const auto r = drv.execute_step();
+            expect(r.reached_convergence() && drv.converged(), "SEP SSSP converges: " + v.to_string());
+            expect(drv.values() == std::vector<float>({0.0F, 1.0F, 3.0F, 4.0F}), "SEP SSSP distances: " + v.to_string());
+            expect(drv.execute_step().status == sep::SEPExecutionStatus::CONVERGED, "SEP SSSP stays converged");
+            sep::SEPPageRankApplication pr(0.85F, 1e-9F);
+            sep::SEPPageRankGpuDriver pd;
+            pd.initialize(pr, sep::SEPExecutionConfig(v));
+            sep::SEPExecutionContext pctx(const_cast<CSRGraph *>(&cyc), nullptr, nullptr, nullptr, &frontier);
+            pd.set_context(pctx);
+            expect(pd.execute_step().succeeded(), "SEP PageRank runs: " + v.to_string());
+            for (float x : pd.values())
+                expect(std::fabs(x - 1.0F / 3.0F) <= 1e-5F, "SEP PageRank cycle ranks: " + v.to_string());
+        }
+        // SYNC push rounds equal the original's iteration count
+        sep::SEPSSSPApplication app(0);
+        sep::SEPSSSPGpuDriver drv;
+        drv.initialize(app, sep::SEPExecutionConfig::from_string("SYNC_PUSH_DD"));
+        CSRGraph chain(4, {0, 1, 2, 2, 2}, {1, 2});
+        sep::SEPExecutionContext ctx(const_cast<CSRGraph *>(&chain), nullptr, nullptr, nullptr, &frontier);
+        drv.set_context(ctx);
+        (void)drv.execute_step();
+        expect(drv.rounds() == 3, "SEP SYNC_PUSH_DD rounds equal sssp_cpu iterations");
+    }
 } // namespace
 
 int main(int argc, char **argv)
@@ -159,6 +252,7 @@ int main(int argc, char **argv)
             test_csr_validation();
             test_reorder();
             test_contributions();
+            test_sep_host();
         }
         else
         {
@@ -170,6 +264,7 @@ int main(int argc, char **argv)
             }
             test_sssp_gpu_known_answers();
             test_pagerank_gpu_known_answers();
+            test_sep_gpu();
         }
         std::cout << "C++ API tests (" << what << ") passed.\n";
         return 0;
