@@ -309,7 +309,17 @@ def bench_serve(args) -> dict:
     torch.cuda.synchronize()
     base_s = time.perf_counter() - t0
     base_qps = nb / base_s
-    log(f"one at a time: {nb} queries in {base_s:.2f} s = {base_qps:.1f} queries/s")
+    # plain SSSP one at a time over the same graphs and sources (the
+    # one-at-a-time baseline the served rate is compared with)
+    sssp_only = [r for r in burst if r[2] == abi.HYT_QUERY_SSSP][:nb]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _, gi, _, src in sssp_only:
+        graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC, out_ptr=dev_out.data_ptr())
+    torch.cuda.synchronize()
+    sssp1_qps = len(sssp_only) / (time.perf_counter() - t0)
+    log(f"one at a time: {nb} mixed queries in {base_s:.2f} s = {base_qps:.1f} queries/s; "
+        f"plain SSSP only {sssp1_qps:.1f} queries/s")
 
     # capacity: the whole burst queued at t=0
     comp, st, _ = abi.serve_trace(graphs, burst, device_budget_bytes=budget, epsilon=eps)
@@ -320,7 +330,7 @@ def bench_serve(args) -> dict:
             raise SystemExit(f"served SSSP result differs from the one-at-a-time result (request {i})")
     occ = st["lane_rows"] / max(st["frontier_rows"], 1)
     log(f"burst: {len(burst)} queries in {cap_s:.2f} s = {cap_qps:.1f} queries/s "
-        f"({cap_qps / base_qps:.2f}x one at a time); lanes {st['lanes']}, state {st['state_bytes'] / 2**30:.1f} GiB; "
+        f"({cap_qps / base_qps:.2f}x the mix one at a time, {cap_qps / sssp1_qps:.2f}x plain SSSP one at a time); lanes {st['lanes']}, state {st['state_bytes'] / 2**30:.1f} GiB; "
         f"queries per frontier row {occ:.2f}; {len(alone_sum)} SSSP checksums equal to one-at-a-time")
 
     # latency: Poisson arrivals at a fixed offered rate
@@ -364,6 +374,8 @@ def bench_serve(args) -> dict:
         },
         "baseline_one_at_a_time_qps": base_qps,
         "speedup_vs_one_at_a_time": cap_qps / base_qps,
+        "baseline_plain_sssp_one_at_a_time_qps": sssp1_qps,
+        "speedup_vs_plain_sssp_one_at_a_time": cap_qps / sssp1_qps,
         "burst": {"queries_per_s": cap_qps, "seconds": cap_s, "lanes": st["lanes"],
                   "state_gib": st["state_bytes"] / 2**30, "graph_gib": st["graph_bytes"] / 2**30,
                   "session_rounds": st["rounds"], "edges": st["edges"],
@@ -479,6 +491,7 @@ def main(argv=None) -> int:
     ap.add_argument("--delta", type=int, default=0, help="async SSSP delta-stepping bucket width (0 = off)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pagerank", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="plain: autotune the degree-bin thresholds first")
     ap.add_argument("--budget-gb", type=int, default=None, help="HBM budget in GiB (hybrid edges: 4, serve: 64)")
     ap.add_argument("--part-log2", type=int, default=20, help="hybrid: log2 edges per partition")
     ap.add_argument("--pr-eps", type=float, default=1e-7)
@@ -536,6 +549,14 @@ def main(argv=None) -> int:
     dopt = {"direction": direction, "pull_percent": args.pull_percent}
     sopt = dict(dopt, delta=args.delta)
 
+    tuned = None
+    if args.tune:
+        # untimed: pick the degree-bin thresholds on these sources
+        tuned = abi.tune(g, sources, tune_lanes=False)
+        sopt.update(light_max_degree=tuned["light_max_degree"], heavy_chunk=tuned["heavy_chunk"])
+        log(f"tuned split: light_max {tuned['light_max_degree']} chunk {tuned['heavy_chunk']} "
+            f"({tuned['sssp_ms']:.2f} ms/query vs {tuned['default_sssp_ms']:.2f} ms default)")
+
     out = torch.empty(hg.V, dtype=torch.int32, device="cuda")
     # m_reach per source and the SYNC round counts (untimed)
     m_reach = []
@@ -582,7 +603,8 @@ def main(argv=None) -> int:
     # WRITE_SIZE runs of this workload (scaled by a plain copy's readings)
     traffic = None
     tfile = ROOT / "profiles" / "sssp_rmat24_traffic.json"
-    if tfile.exists() and args.scale == 24 and args.edge_factor == 16 and args.mode == "async" and pull_rounds == 0:
+    if (tfile.exists() and args.scale == 24 and args.edge_factor == 16 and args.mode == "async" and pull_rounds == 0
+            and not args.tune and args.delta == 0):
         traffic = json.loads(tfile.read_text())["traffic_bytes_per_round"]
 
     result = {
@@ -609,6 +631,8 @@ def main(argv=None) -> int:
             "mode": args.mode,
             "direction": args.direction,
             "delta": args.delta,
+            "tuned_split": {"light_max_degree": tuned["light_max_degree"],
+                            "heavy_chunk": tuned["heavy_chunk"]} if tuned else None,
         },
         "roofline": {
             "bound": "hbm",

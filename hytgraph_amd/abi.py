@@ -180,6 +180,20 @@ class ServeStats(C.Structure):
         return d
 
 
+class TuneOptions(C.Structure):
+    _fields_ = [("tune_split", C.c_uint32), ("tune_lanes", C.c_uint32), ("max_lanes", C.c_uint32),
+                ("reps", C.c_uint32)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("light_max_degree", C.c_uint32), ("heavy_chunk", C.c_uint32), ("lanes", C.c_uint32),
+                ("candidates", C.c_uint32), ("sssp_ms", C.c_double), ("default_sssp_ms", C.c_double),
+                ("lanes_qps", C.c_double), ("partition_edges", C.c_uint64), ("hybrid_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class HytError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"[status {status}] {message}")
@@ -248,6 +262,9 @@ SIGNATURES = {
     "hyt_lanes_set_form": (C.c_int, [_P, C.c_uint32]),
     "hyt_lanes_info": (C.c_int, [_P, _u32p, _u64p, _u64p, _u32p, _u64p]),
     "hyt_serve_config_default": (C.c_int, [C.POINTER(ServeConfig)]),
+    "hyt_tune": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(TuneOptions), C.POINTER(TuneResult)]),
+    "hyt_tune_hybrid": (C.c_int, [_P, C.c_uint32, C.POINTER(HybridConfig), _P, C.c_uint32, _P, C.c_uint32,
+                                  C.POINTER(TuneResult)]),
     "hyt_serve_trace": (
         C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint64,
                   C.POINTER(_P), C.POINTER(Completion), C.POINTER(ServeStats)]),
@@ -649,3 +666,22 @@ def serve_trace(graphs, requests, keep_results=False, **config):
                    ("reserved", "u4"), ("reached", "u8"), ("checksum", "u8")])
     arr = np.frombuffer(bytes(comp), dtype=dt, count=n).copy() if n else np.zeros(0, dt)
     return arr, st.as_dict(), results
+
+
+# ------------------------------------------------------------------ autotuner
+def tune(graph: DeviceGraph, sources, tune_split=True, tune_lanes=True, max_lanes=64, reps=2) -> dict:
+    src = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+    o = TuneOptions(1 if tune_split else 0, 1 if tune_lanes else 0, max_lanes, reps)
+    r = TuneResult()
+    check(load().hyt_tune(graph._h, _ptr(src), len(src), C.byref(o), C.byref(r)))
+    return r.as_dict()
+
+
+def tune_hybrid(host: HostGraph, candidates, sources, weights_f32=False, **config) -> dict:
+    cand = np.ascontiguousarray(np.atleast_1d(candidates), dtype=np.uint64)
+    src = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+    cfg = hybrid_config(**config)
+    r = TuneResult()
+    check(load().hyt_tune_hybrid(host._h, 1 if weights_f32 else 0, C.byref(cfg), _ptr(cand), len(cand), _ptr(src),
+                                 len(src), C.byref(r)))
+    return r.as_dict()

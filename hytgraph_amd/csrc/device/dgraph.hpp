@@ -92,9 +92,23 @@ namespace hyt
         }
     };
 
-    // Default work-split thresholds (degree binning).
-    constexpr std::uint32_t kDefaultLightMaxDegree = 256;
-    constexpr std::uint32_t kDefaultHeavyChunk = 1024;
+    // Default work-split thresholds (degree binning) of single-source SSSP
+    // push rounds: rows up to light_max edges are walked by the light kernel,
+    // longer rows in heavy_chunk-edge pieces. Chosen by the autotuner on RMAT
+    // scales 24 and 26 (MI355X): 2048 / 4096 ran 25.0 -> 18.2 ms (scale 24)
+    // and 90.7 -> 68.9 ms (scale 26, 2048 / 1024) per query against 256 /
+    // 1024; scales 20-22 are within 3% of each other.
+    constexpr std::uint32_t kDefaultLightMaxDegree = 2048;
+    constexpr std::uint32_t kDefaultHeavyChunk = 4096;
+    // PageRank (push and pull rounds), measured the same way on RMAT scale 24
+    // (eps 1e-7): 512 / 2048 took 0.716 s against 0.829 s (256 / 1024) and
+    // 0.878 s (2048 / 4096). The K-lane kernels keep 256 / 1024: a K-lane
+    // light row carries K queries per edge, so long light rows would leave
+    // one wavefront with most of a round's work.
+    constexpr std::uint32_t kPagerankLightMaxDegree = 512;
+    constexpr std::uint32_t kPagerankHeavyChunk = 2048;
+    constexpr std::uint32_t kLaneLightMaxDegree = 256;
+    constexpr std::uint32_t kLaneHeavyChunk = 1024;
     // Direction choice: pull when the frontier's out-edges reach this % of E.
     // Measured on RMAT scale 24 (MI355X): PageRank pull rounds (one gather per
     // in-edge, no atomics) beat float-atomic pushes from ~25% of E; SSSP push

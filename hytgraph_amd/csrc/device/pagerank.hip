@@ -291,8 +291,8 @@ namespace hyt
                          const hyt_run_options &o, hyt_run_stats &st)
     {
         check_params(damping, eps);
-        const std::uint32_t light_max = o.light_max_degree ? o.light_max_degree : kDefaultLightMaxDegree;
-        const std::uint32_t chunk = o.heavy_chunk ? o.heavy_chunk : kDefaultHeavyChunk;
+        const std::uint32_t light_max = o.light_max_degree ? o.light_max_degree : kPagerankLightMaxDegree;
+        const std::uint32_t chunk = o.heavy_chunk ? o.heavy_chunk : kPagerankHeavyChunk;
         require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
         require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
         ensure_workspace(g, light_max, chunk);

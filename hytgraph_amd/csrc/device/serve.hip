@@ -196,8 +196,8 @@ namespace hyt
         // appended rows of at most max_degree each.
         static std::uint64_t heavy_chunks(std::uint64_t E, std::uint64_t max_degree)
         {
-            return E / kDefaultHeavyChunk + E / (kDefaultLightMaxDegree + 1ULL) + 64 +
-                   kAppend * (max_degree / kDefaultHeavyChunk + 1);
+            return E / kLaneHeavyChunk + E / (kLaneLightMaxDegree + 1ULL) + 64 +
+                   kAppend * (max_degree / kLaneHeavyChunk + 1);
         }
         static std::uint64_t fixed_bytes(std::uint64_t V, std::uint64_t E, std::uint64_t max_degree)
         {
@@ -316,8 +316,8 @@ namespace hyt
             a.ctr = d_ctr;
             a.heavy = S.heavy.as<HeavyChunkK>();
             a.heavy_cap = S.heavy_cap;
-            a.light_max = kDefaultLightMaxDegree;
-            a.chunk = kDefaultHeavyChunk;
+            a.light_max = kLaneLightMaxDegree;
+            a.chunk = kLaneHeavyChunk;
             a.K = S.K;
             a.mark = S.mark.as<unsigned long long>();
             if (S.kind == HYT_QUERY_PPR)

@@ -419,6 +419,45 @@ int hyt_serve_trace(const hyt_graph_t *graphs, uint32_t num_graphs, const hyt_se
                     const hyt_request *requests, uint64_t num_requests, void *const *results,
                     hyt_completion *completions, hyt_serve_stats *stats);
 
+/* ------------------------------------------------------------- autotuner */
+/*
+ * No counterpart in the original project: measures candidate settings on
+ * the device (each candidate timed on the caller's sources, best of `reps`)
+ * and keeps the fastest.
+ *   degree-bin thresholds: light_max_degree in {64..2048} then heavy_chunk
+ *     in {256..8192} for single-source SSSP (coordinate descent from the
+ *     defaults);
+ *   lanes: K of batched SSSP with the highest queries/s (K cycles through
+ *     the sources when there are fewer than K);
+ *   partition size (hyt_tune_hybrid): partition_edges of the hybrid engine
+ *     with the shortest SSSP wall time under the given configuration.
+ * Results never change answers, only speed.
+ */
+typedef struct hyt_tune_options {
+    uint32_t tune_split;   /* 1: tune light_max_degree / heavy_chunk */
+    uint32_t tune_lanes;   /* 1: tune K (2..max_lanes) */
+    uint32_t max_lanes;    /* 0 = 64 */
+    uint32_t reps;         /* timed repetitions per candidate (0 = 2) */
+} hyt_tune_options;
+
+typedef struct hyt_tune_result {
+    uint32_t light_max_degree, heavy_chunk; /* fastest single-source split */
+    uint32_t lanes;                         /* K with the highest batched queries/s */
+    uint32_t candidates;                    /* candidates measured */
+    double sssp_ms;                         /* best mean single-source time per query */
+    double default_sssp_ms;                 /* the same with the default split */
+    double lanes_qps;                       /* batched queries/s at the chosen K */
+    uint64_t partition_edges;               /* hyt_tune_hybrid only */
+    double hybrid_ms;                       /* hyt_tune_hybrid: best SSSP wall time per query */
+} hyt_tune_result;
+
+int hyt_tune(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources, const hyt_tune_options *opts,
+             hyt_tune_result *out);
+/* candidates: partition sizes in edges (n >= 1); config: every other setting. */
+int hyt_tune_hybrid(hyt_host_graph_t host, uint32_t weights_f32, const hyt_hybrid_config *config,
+                    const uint64_t *candidates, uint32_t num_candidates, const uint32_t *sources,
+                    uint32_t num_sources, hyt_tune_result *out);
+
 #ifdef __cplusplus
 }
 #endif

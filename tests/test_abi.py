@@ -126,7 +126,8 @@ def test_struct_layouts_match_header(abi, tmp_path):
         "hyt_partition_metrics": abi.PartitionMetrics, "hyt_partition_costs": abi.PartitionCosts,
         "hyt_hybrid_config": abi.HybridConfig, "hyt_hybrid_stats": abi.HybridStats,
         "hyt_request": abi.Request, "hyt_completion": abi.Completion, "hyt_serve_config": abi.ServeConfig,
-        "hyt_serve_stats": abi.ServeStats,
+        "hyt_serve_stats": abi.ServeStats, "hyt_tune_options": abi.TuneOptions,
+        "hyt_tune_result": abi.TuneResult,
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hytgraph_amd.h"', "int main(void){"]
     expect = []
