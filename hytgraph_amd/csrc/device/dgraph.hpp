@@ -109,6 +109,13 @@ namespace hyt
     constexpr std::uint32_t kPagerankHeavyChunk = 2048;
     constexpr std::uint32_t kLaneLightMaxDegree = 256;
     constexpr std::uint32_t kLaneHeavyChunk = 1024;
+    // Hybrid engine expansions read windows of partitions, many of them over
+    // the host link (zero-copy): short light rows and small chunks keep many
+    // wavefronts with reads in flight. RMAT scale 27, 4 GiB budget, one SSSP
+    // query: 64/256 1.00 s, 128/512 1.01 s, 256/1024 1.06 s, 512/2048 1.42 s,
+    // 2048/4096 2.1-2.6 s.
+    constexpr std::uint32_t kHybridLightMaxDegree = 128;
+    constexpr std::uint32_t kHybridHeavyChunk = 512;
     // Direction choice: pull when the frontier's out-edges reach this % of E.
     // Measured on RMAT scale 24 (MI355X): PageRank pull rounds (one gather per
     // in-edge, no atomics) beat float-atomic pushes from ~25% of E; SSSP push

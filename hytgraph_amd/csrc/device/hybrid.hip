@@ -202,6 +202,8 @@ namespace hyt
         int threads = 1;
         // calibration
         double link_gbs = 0, pack_gbs = 0, zc_gbs = 0;
+        // degree-bin split of the current run's expansions
+        std::uint32_t light_max = kHybridLightMaxDegree, chunk = kHybridHeavyChunk;
 
         ~HybridEngine()
         {
@@ -305,6 +307,15 @@ namespace hyt
             run_params = params;
             if (!need_w)
                 run_params.entry_bytes = 4;
+        }
+
+        void set_split(const hyt_run_options &o)
+        {
+            light_max = o.light_max_degree ? o.light_max_degree : kHybridLightMaxDegree;
+            chunk = o.heavy_chunk ? o.heavy_chunk : kHybridHeavyChunk;
+            require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
+            require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
+            ensure_workspace(dg, light_max, chunk);
         }
 
         bool is_cached(std::uint64_t p) const { return res[p].col >= 0 && (!need_w || res[p].w >= 0); }
@@ -505,7 +516,7 @@ namespace hyt
                 upload_offsets<std::uint64_t>(E);
             else
                 upload_offsets<std::uint32_t>(E);
-            ensure_workspace(g, kDefaultLightMaxDegree, kDefaultHeavyChunk);
+            ensure_workspace(g, kHybridLightMaxDegree, kHybridHeavyChunk);
 
             // partitions (target bytes = 4 B per edge, as the original counts)
             E.parts = partition_graph(host->off, host->V, cfg.partition_edges * 4);
@@ -702,8 +713,8 @@ namespace hyt
             a.ctr = d_ctr;
             a.heavy = g.ws.heavy.as<HeavyChunk>();
             a.heavy_cap = g.ws.heavy_cap;
-            a.light_max = kDefaultLightMaxDegree;
-            a.chunk = kDefaultHeavyChunk;
+            a.light_max = E.light_max;
+            a.chunk = E.chunk;
             expand_light<RowSrc, Op><<<blocks_for(F), kBlock, 0, s>>>(rs, op, a);
             hip_check(hipGetLastError(), "launch hybrid expand_light");
             expand_heavy<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
@@ -1163,6 +1174,7 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
         const auto t0 = std::chrono::steady_clock::now();
         const int wm = E.dg.weight_mode();
         E.begin_run(true);
+        E.set_split(o);
         auto run = [&](auto off_tag) {
             using OffT = decltype(off_tag);
             switch (wm)
@@ -1216,6 +1228,7 @@ int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ran
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
         E.begin_run(false);
+        E.set_split(o);
         if (E.dg.off64)
             hybrid_pagerank_typed<std::uint64_t>(E, damping, epsilon, ranks, o, st);
         else
