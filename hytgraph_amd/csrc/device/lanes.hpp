@@ -48,6 +48,57 @@ namespace hyt
             unsigned long long *mark;
         };
 
+        // Work split of item-form rows: a row's work is edges x c (its active
+        // queries). It is light when the work is at most 8 x light_max (2048
+        // items by default), so one wavefront's 64 light rows stay within a few
+        // chunks of work whatever the mask density; heavier rows are cut into
+        // chunks of max(16, 4 x chunk / c) edges (4096 items by default). The
+        // group form keeps the edge split (light <= light_max edges, chunks of
+        // `chunk` edges): its rows carry KP lanes each, and finer chunks there
+        // measured slower (K = 64 SSSP on RMAT 24: 199 ms -> 1.39 s).
+        __device__ __forceinline__ bool light_work(std::uint64_t deg, std::uint32_t w, const KArgs &a)
+        {
+            return deg * w <= 8ULL * a.light_max;
+        }
+        __device__ __forceinline__ std::uint32_t chunk_edges(std::uint32_t w, const KArgs &a)
+        {
+            const std::uint32_t ce = (4U * a.chunk) / (w ? w : 1U);
+            return ce < 16U ? 16U : ce;
+        }
+
+        // Queues this wavefront's heavy-row chunks (nch per lane) with one
+        // atomic. When the queue cannot take them all, the reserved slots that
+        // exist are written as empty chunks and false is returned: the rows
+        // are then walked by this wavefront itself (slower, never wrong).
+        template <class MakeFn>
+        __device__ __forceinline__ bool queue_chunks_k(std::uint32_t nch, const KArgs &a, MakeFn make)
+        {
+            const int lane = lane_id();
+            const std::uint32_t hincl = wave_incl_scan(nch);
+            const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
+            if (htotal == 0)
+                return true;
+            std::uint32_t hbase = 0;
+            if (lane == 0)
+                hbase = atomicAdd(&a.ctr->heavy_count, htotal);
+            hbase = __shfl(hbase, 0, kWave);
+            const std::uint64_t first = static_cast<std::uint64_t>(hbase) + hincl - nch;
+            if (static_cast<std::uint64_t>(hbase) + htotal <= a.heavy_cap)
+            {
+                for (std::uint32_t c = 0; c < nch; ++c)
+                    a.heavy[first + c] = make(c);
+                return true;
+            }
+            for (std::uint32_t c = 0; c < nch; ++c)
+                if (first + c < a.heavy_cap)
+                {
+                    HeavyChunkK h{};
+                    h.count = 0;
+                    a.heavy[first + c] = h;
+                }
+            return false;
+        }
+
         template <int KP>
         __device__ __forceinline__ std::uint64_t group_bits(std::uint64_t ballot, int group)
         {
@@ -106,6 +157,7 @@ namespace hyt
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
             const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+            const std::uint32_t ce = a.chunk;
             std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
             std::uint64_t mask = 0;
             OffT begin = 0, full = 0;
@@ -117,23 +169,32 @@ namespace hyt
                 mask = a.qmask[i];
                 row = Op::kRowIsQueueIndex ? i : u;
                 scale = op.row_scale(static_cast<std::uint64_t>(full));
-                if (full > a.light_max)
-                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
-                else
+                if (full <= a.light_max)
                     deg = static_cast<std::uint32_t>(full);
+                else
+                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
             }
-            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
-                const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+            const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t c) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(c) * ce;
                 const std::uint64_t full64 = static_cast<std::uint64_t>(full);
                 HeavyChunkK h;
                 h.row = row;
-                h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+                h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
                 h.begin = static_cast<std::uint64_t>(begin) + lo;
                 h.mask = mask;
                 h.scale = scale;
                 h.pad = 0;
                 return h;
             });
+            if (!queued && nch)
+            {
+                // chunk queue full: walk the row here (rows of 2^24 edges or
+                // more cannot be - 64 of them must fit a u32 - and are reported)
+                if (static_cast<std::uint64_t>(full) >= (1ULL << 24))
+                    atomicOr(&a.ctr->heavy_overflow, 1U);
+                else
+                    deg = static_cast<std::uint32_t>(full);
+            }
             const std::uint32_t incl = wave_incl_scan(deg);
             const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
             s_ex[wid][lane] = incl - deg;
@@ -188,6 +249,16 @@ namespace hyt
         // d edges contributes d*c items, consecutive lanes take the c queries
         // of one edge (same column/weight, neighbouring [V][K] words), and the
         // wavefront stays full whatever the mask density.
+
+        // Chunk-queue capacity of K-lane rounds: every group-form chunk (rows
+        // above light_max edges, `chunk`-edge pieces) at the given split, plus
+        // V/16 + E/256 for item-form chunks; a full queue only moves rows back
+        // into the light walk.
+        inline std::uint64_t lane_chunk_capacity(std::uint64_t V, std::uint64_t E, std::uint32_t light_max,
+                                                 std::uint32_t chunk)
+        {
+            return E / chunk + E / (static_cast<std::uint64_t>(light_max) + 1) + E / 256 + V / 16 + 64;
+        }
 
         // Position of the r-th set bit (0-based) of m; m has more than r set bits.
         __device__ __forceinline__ std::uint32_t nth_set_bit(std::uint64_t m, std::uint32_t r)
@@ -259,7 +330,7 @@ namespace hyt
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
             const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-            std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0;
+            std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0, ce = 16;
             std::uint64_t mask = 0;
             OffT begin = 0, full = 0;
             if (i < a.F)
@@ -271,26 +342,40 @@ namespace hyt
                 c = static_cast<std::uint32_t>(__popcll(mask));
                 row = Op::kRowIsQueueIndex ? i : u;
                 scale = op.row_scale(static_cast<std::uint64_t>(full));
-                if (full > a.light_max)
-                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + a.chunk - 1) / a.chunk);
-                else
+                ce = chunk_edges(c, a);
+                if (light_work(static_cast<std::uint64_t>(full), c, a))
                 {
                     deg = static_cast<std::uint32_t>(full);
                     items = deg * c;
                 }
+                else
+                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
             }
-            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t ch) {
-                const std::uint64_t lo = static_cast<std::uint64_t>(ch) * a.chunk;
+            const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t ch) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(ch) * ce;
                 const std::uint64_t full64 = static_cast<std::uint64_t>(full);
                 HeavyChunkK h;
                 h.row = row;
-                h.count = static_cast<std::uint32_t>((full64 - lo < a.chunk) ? (full64 - lo) : a.chunk);
+                h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
                 h.begin = static_cast<std::uint64_t>(begin) + lo;
                 h.mask = mask;
                 h.scale = scale;
                 h.pad = 0;
                 return h;
             });
+            if (!queued && nch)
+            {
+                // chunk queue full: walk the row here (work of 2^24 items or
+                // more cannot be - 64 rows' work must fit a u32 - and is reported)
+                const std::uint64_t w = static_cast<std::uint64_t>(full) * c;
+                if (w >= (1ULL << 24))
+                    atomicOr(&a.ctr->heavy_overflow, 1U);
+                else
+                {
+                    deg = static_cast<std::uint32_t>(full);
+                    items = static_cast<std::uint32_t>(w);
+                }
+            }
             const std::uint32_t incl = wave_incl_scan(items);
             const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
             std::uint32_t dsum = deg;

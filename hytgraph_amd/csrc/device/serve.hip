@@ -192,17 +192,15 @@ namespace hyt
         // rows appended by admissions (a vertex may then appear twice).
         static constexpr std::uint64_t kAppend = 64;
 
-        // Heavy chunks: every row of degree > light_max once, plus kAppend
-        // appended rows of at most max_degree each.
-        static std::uint64_t heavy_chunks(std::uint64_t E, std::uint64_t max_degree)
+        // Heavy-chunk queue (a full queue only slows a round down).
+        static std::uint64_t heavy_chunks(std::uint64_t V, std::uint64_t E)
         {
-            return E / kLaneHeavyChunk + E / (kLaneLightMaxDegree + 1ULL) + 64 +
-                   kAppend * (max_degree / kLaneHeavyChunk + 1);
+            return lanes::lane_chunk_capacity(V, E, kLaneLightMaxDegree, kLaneHeavyChunk);
         }
-        static std::uint64_t fixed_bytes(std::uint64_t V, std::uint64_t E, std::uint64_t max_degree)
+        static std::uint64_t fixed_bytes(std::uint64_t V, std::uint64_t E, std::uint64_t /*max_degree*/)
         {
             const std::uint64_t Vp = V + kPad;
-            return Vp * 8 + (Vp + kAppend) * 8 + (Vp + kAppend) * 4 + heavy_chunks(E, max_degree) * sizeof(HeavyChunkK) +
+            return Vp * 8 + (Vp + kAppend) * 8 + (Vp + kAppend) * 4 + heavy_chunks(V, E) * sizeof(HeavyChunkK) +
                    Vp * 4 + (kGatherBlocks + 1) * sizeof(double) + sizeof(Counters) + 8 + 16;
         }
         // val [V][K] (+ rank [V][K] and snapshot [V+kAppend][K] for PPR)
@@ -257,7 +255,7 @@ namespace hyt
             S.mark.alloc(Vp * 8, "hipMalloc(lane marks)");
             S.qmask.alloc((Vp + QA) * 8, "hipMalloc(lane queue masks)");
             S.q.alloc((Vp + QA) * 4, "hipMalloc(lane queue)");
-            const std::uint64_t cap = LaneSession::heavy_chunks(g->E, g->max_degree);
+            const std::uint64_t cap = LaneSession::heavy_chunks(g->V, g->E);
             S.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(lane heavy chunks)");
             S.heavy_cap = cap;
             S.ctr.alloc(sizeof(Counters), "hipMalloc(lane counters)");

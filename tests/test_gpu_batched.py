@@ -116,3 +116,24 @@ def test_k_lane_forms_match_oracle(gpu, abi, orc, K, form):
     for k in range(K):
         same, _, _ = orc.delta_push_f64(hu.offsets, hu.columns, D, 1e-8, seed=int(src[k]))
         assert np.abs(r[:, k] - same).sum() <= 2e-6, k
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_k_lane_full_chunk_queue_falls_back(gpu, abi, orc, form):
+    # a dense graph (1024 vertices, ~0.4 M edges) with tiny light/chunk
+    # thresholds needs far more heavy chunks than the queue holds (about
+    # E/256 + V/16): the rows that do not fit are walked by their own
+    # wavefront, and the answers stay exact
+    hg = abi.HostGraph(kind=abi.HYT_GEN_ER, scale=10, edge_factor=512, seed=17, weight_min=1, weight_max=255)
+    assert hg.E > 300_000
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 9, 64)
+    d, _ = g.sssp(src, lane_form=form, light_max_degree=4, heavy_chunk=64)
+    for k in range(0, 64, 7):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[k]))), k
+    hu = abi.HostGraph(kind=abi.HYT_GEN_ER, scale=10, edge_factor=512, seed=18, weight_min=0, weight_max=0)
+    gu = abi.DeviceGraph.from_host(hu)
+    r, _ = gu.ppr(src, D, 1e-8, lane_form=form, light_max_degree=4, heavy_chunk=64)
+    for k in range(0, 64, 9):
+        same, _, _ = orc.delta_push_f64(hu.offsets, hu.columns, D, 1e-8, seed=int(src[k]))
+        assert np.abs(r[:, k] - same).sum() <= 2e-6, k
