@@ -321,6 +321,10 @@ def bench_serve(args) -> dict:
     log(f"one at a time: {nb} mixed queries in {base_s:.2f} s = {base_qps:.1f} queries/s; "
         f"plain SSSP only {sssp1_qps:.1f} queries/s")
 
+    # the baselines' workspaces and transposes are not part of serving
+    for g in graphs:
+        g.trim()
+
     # capacity: the whole burst queued at t=0
     comp, st, _ = abi.serve_trace(graphs, burst, device_budget_bytes=budget, epsilon=eps)
     cap_s = float(comp["done_s"].max())

@@ -224,6 +224,7 @@ SIGNATURES = {
         [C.c_uint64, C.c_uint64, _P, _P, _P, C.c_uint32, C.POINTER(_P)],
     ),
     "hyt_graph_free": (C.c_int, [_P]),
+    "hyt_graph_trim": (C.c_int, [_P]),
     "hyt_graph_info": (C.c_int, [_P, _u64p, _u64p, _u64p]),
     "hyt_sssp": (
         C.c_int,
@@ -395,6 +396,10 @@ class DeviceGraph:
         V, E, B = C.c_uint64(), C.c_uint64(), C.c_uint64()
         check(load().hyt_graph_info(self._h, C.byref(V), C.byref(E), C.byref(B)))
         return {"V": V.value, "E": E.value, "device_bytes": B.value}
+
+    def trim(self):
+        """Frees the traversal workspaces and the transpose (rebuilt on demand)."""
+        check(load().hyt_graph_trim(self._h))
 
     def free(self):
         if getattr(self, "_h", None):

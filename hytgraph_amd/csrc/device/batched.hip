@@ -92,16 +92,13 @@ namespace hyt
             std::uint64_t heavy_cap = 0;
         };
 
+        // The graph owns its batched workspace (released with it or by
+        // hyt_graph_trim).
         KWorkspace &kws_of(DeviceGraph &g)
         {
-            // One batched workspace per graph, kept in a side table keyed by the
-            // graph object (destroyed with the process; graphs are few).
-            static thread_local std::vector<std::pair<const DeviceGraph *, std::unique_ptr<KWorkspace>>> table;
-            for (auto &p : table)
-                if (p.first == &g)
-                    return *p.second;
-            table.emplace_back(&g, std::make_unique<KWorkspace>());
-            return *table.back().second;
+            if (!g.kws)
+                g.kws = std::make_shared<KWorkspace>();
+            return *static_cast<KWorkspace *>(g.kws.get());
         }
 
 

@@ -74,6 +74,7 @@ namespace hyt
         std::uint32_t in_heavy_light_max = 0, in_heavy_chunk = 0;
         hipStream_t stream = nullptr;
         Workspace ws;
+        std::shared_ptr<void> kws; // workspace of the batched [V][K] calls (owned, type-erased)
         std::unique_ptr<PinnedBuffer> host_ctr;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
 

@@ -142,6 +142,24 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
     });
 }
 
+int hyt_graph_trim(hyt_graph_t g)
+{
+    return hyt::guarded("hyt_graph_trim", [&] {
+        hyt::require(g != nullptr, "null graph");
+        hyt::DeviceGraph &d = g->g;
+        hyt::hip_check(hipStreamSynchronize(d.stream), "hyt_graph_trim");
+        d.ws = hyt::Workspace{};
+        d.kws.reset();
+        d.in_off.release();
+        d.in_col.release();
+        d.in_w.release();
+        d.in_heavy.release();
+        d.in_heavy_count = 0;
+        d.in_heavy_light_max = 0;
+        d.in_heavy_chunk = 0;
+    });
+}
+
 int hyt_graph_free(hyt_graph_t g)
 {
     return hyt::guarded("hyt_graph_free", [&] { delete g; });

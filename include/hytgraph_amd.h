@@ -104,6 +104,9 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges,
                      const uint32_t *weights, uint32_t flags,
                      hyt_graph_t *out);
 int hyt_graph_free(hyt_graph_t g);
+/* Releases the graph's traversal workspaces and its transpose (they are
+ * rebuilt by the next call that needs them); the CSR stays. */
+int hyt_graph_trim(hyt_graph_t g);
 int hyt_graph_info(hyt_graph_t g, uint64_t *num_vertices, uint64_t *num_edges,
                    uint64_t *device_bytes);
 

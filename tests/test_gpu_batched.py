@@ -137,3 +137,21 @@ def test_k_lane_full_chunk_queue_falls_back(gpu, abi, orc, form):
     for k in range(0, 64, 9):
         same, _, _ = orc.delta_push_f64(hu.offsets, hu.columns, D, 1e-8, seed=int(src[k]))
         assert np.abs(r[:, k] - same).sum() <= 2e-6, k
+
+
+def test_trim_releases_and_rebuilds_workspaces(gpu, abi, orc):
+    hg = abi.HostGraph(scale=12, edge_factor=16, seed=23, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 5, 8)
+    g.sssp(src)                                   # batched workspace
+    g.sssp(int(src[0]), direction=abi.HYT_DIR_PULL)  # transpose
+    before = g.info()["device_bytes"]
+    g.trim()
+    after = g.info()["device_bytes"]
+    assert after < before
+    d1, _ = g.sssp(int(src[1]), direction=abi.HYT_DIR_PULL)
+    dk, _ = g.sssp(src)
+    for k, s in enumerate(src):
+        ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(s))
+        assert np.array_equal(dk[:, k], ref)
+    assert np.array_equal(d1, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[1])))
