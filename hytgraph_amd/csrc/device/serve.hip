@@ -16,7 +16,8 @@
 //
 // The serving loop replays a request trace (arrival times, graph, kind,
 // source) against the wall clock: arrivals queue per session, lanes per
-// session are sized from the device budget (admission by memory), and the
+// session are sized from the device budget (admission by memory: each lane
+// goes to the session with the most demand per lane that still fits), and the
 // loop polls every session: when its round has completed, finished lanes are
 // read and freed, queued queries are admitted into free lanes, and the next
 // round is launched - each session advances at its own pace, so a small
@@ -534,8 +535,10 @@ namespace hyt
         double secs(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
         // Lanes per session from the budget: every session gets one lane,
-        // then lanes are added round-robin (cheapest lane first) while the
-        // state fits and demand is left.
+        // then each further lane goes to the session with the most queued
+        // demand per lane that still fits (demand = its requests in the
+        // trace), until nothing fits or every session has min(demand,
+        // max_lanes) lanes.
         std::vector<std::uint32_t> plan_lanes(const std::vector<std::uint64_t> &fixed,
                                               const std::vector<std::uint64_t> &per_lane,
                                               const std::vector<std::uint64_t> &demand, std::uint64_t avail,
@@ -550,22 +553,25 @@ namespace hyt
                 fail(HYT_ERR_OUT_OF_MEMORY, "device budget too small for one lane per session (" +
                                                 std::to_string(used) + " bytes needed, " + std::to_string(avail) +
                                                 " available)");
-            std::vector<std::size_t> order(n);
-            for (std::size_t i = 0; i < n; ++i)
-                order[i] = i;
-            std::sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) { return per_lane[a] < per_lane[b]; });
-            bool grew = true;
-            while (grew)
+            while (true)
             {
-                grew = false;
-                for (std::size_t i : order)
+                std::size_t best = n;
+                double best_ratio = 0.0;
+                for (std::size_t i = 0; i < n; ++i)
                 {
                     if (K[i] >= max_lanes || K[i] >= demand[i] || used + per_lane[i] > avail)
                         continue;
-                    K[i] += 1;
-                    used += per_lane[i];
-                    grew = true;
+                    const double ratio = static_cast<double>(demand[i]) / K[i];
+                    if (best == n || ratio > best_ratio)
+                    {
+                        best = i;
+                        best_ratio = ratio;
+                    }
                 }
+                if (best == n)
+                    break;
+                K[best] += 1;
+                used += per_lane[best];
             }
             return K;
         }

@@ -150,7 +150,10 @@ def test_serve_trace_mixed_results_match_alone(gpu, abi, orc):
                                     device_budget_bytes=gbytes + lane_room + 4096, epsilon=1e-8)
     _say(f"serve: {st}")
     assert 0 < st["item_rounds"] <= st["rounds"]
-    assert st["sessions"] == 4 and all(1 <= k <= 3 for k in st["lanes"])
+    # every session got a lane and the lanes fit the budget (admission by memory)
+    budget = gbytes + lane_room + 4096
+    assert st["sessions"] == 4 and all(k >= 1 for k in st["lanes"])
+    assert st["graph_bytes"] + st["state_bytes"] <= budget
     assert st["admitted"] == n
     for i, (t, gi, kind, src) in enumerate(reqs):
         assert t - 1e-3 <= comp["admit_s"][i] <= comp["done_s"][i]
