@@ -88,7 +88,8 @@ namespace hyt
         // Workspace of the batched forms (grown to the largest K used).
         struct KWorkspace
         {
-            DeviceBuffer val, rank, snap, mark, qmask, heavy, seeds, partial, lanes;
+            DeviceBuffer val, rank, snap, mark, qmask, heavy, seeds, partial, lanes, round;
+            std::unique_ptr<PinnedBuffer> host_round;
             std::uint64_t heavy_cap = 0;
         };
 
@@ -148,6 +149,9 @@ namespace hyt
             kw.heavy_cap = kw.heavy.bytes / sizeof(HeavyChunkK);
             kw.seeds.ensure(64 * 4 * 4, "hipMalloc(seeds)");
             kw.lanes.ensure(8, "hipMalloc(lane mask)");
+            kw.round.ensure(sizeof(RoundState), "hipMalloc(round state)");
+            if (!kw.host_round)
+                kw.host_round = std::make_unique<PinnedBuffer>(sizeof(RoundState));
             kw.partial.ensure((1024 + 1) * 64 * sizeof(double), "hipMalloc(partials)");
             ensure_workspace(g, light_max, chunk);
         }
@@ -184,6 +188,56 @@ namespace hyt
             return *h_ctr;
         }
 
+        // Rounds per captured HIP graph in device-driven mode.
+        constexpr int kRoundsPerGraph = 4;
+
+        // Starts a device-driven round: the queue the last compaction wrote
+        // becomes this round's (or the round limit ends the run), the form is
+        // chosen, and the round counters are cleared (a kernel, not a memset,
+        // so the sequence can be captured).
+        __global__ void round_begin(Counters *ctr, RoundState *rs, std::uint32_t KP, std::uint32_t forced,
+                                    std::uint64_t max_rounds)
+        {
+            if (threadIdx.x != 0 || blockIdx.x != 0)
+                return;
+            std::uint32_t F = ctr->next_count;
+            const std::uint64_t pairs = ctr->frontier_edges;
+            if (F != 0 && rs->rounds >= max_rounds)
+            {
+                rs->truncated = 1;
+                F = 0;
+            }
+            rs->F = F;
+            rs->pairs = pairs;
+            rs->form = forced == HYT_LANES_AUTO
+                           ? (pairs * 2 < static_cast<std::uint64_t>(F) * KP ? HYT_LANES_ITEMS : HYT_LANES_GROUP)
+                           : forced;
+            if (F != 0)
+            {
+                rs->rounds += 1;
+                rs->frontier_total += F;
+            }
+            else
+                rs->done = 1;
+            ctr->next_count = 0;
+            ctr->heavy_count = 0;
+            ctr->edges = 0;
+            ctr->frontier_edges = 0;
+        }
+
+        __global__ void round_end(const Counters *ctr, RoundState *rs)
+        {
+            if (threadIdx.x != 0 || blockIdx.x != 0 || rs->F == 0)
+                return;
+            rs->edges_total += ctr->edges;
+            rs->updates_total += ctr->next_count;
+        }
+
+        // Device-driven rounds: kRoundsPerGraph rounds (round_begin, PPR
+        // snapshot, both forms' light and heavy kernels - the form not chosen
+        // exits at once -, the mark compaction, round_end) are captured into
+        // one HIP graph, which is replayed until a round finds the queue
+        // empty; the host reads the round state once per replay.
         template <class OffT, int KP, class Op>
         void run_k(DeviceGraph &g, KWorkspace &kw, Op op, std::uint32_t K, const hyt_run_options &o,
                    hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk, bool ppr)
@@ -191,9 +245,12 @@ namespace hyt
             hipStream_t s = g.stream;
             Counters *d_ctr = g.ws.ctr.as<Counters>();
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : (ppr ? 100000 : g.V);
+            RoundState *rs = kw.round.as<RoundState>();
             KArgs a;
+            a.rs = rs;
             a.q = g.ws.q.as<std::uint32_t>();
             a.qmask = kw.qmask.as<unsigned long long>();
+            a.F = 0;
             a.ctr = d_ctr;
             a.heavy = kw.heavy.as<HeavyChunkK>();
             a.heavy_cap = kw.heavy_cap;
@@ -202,41 +259,67 @@ namespace hyt
             a.K = K;
             a.mark = kw.mark.as<unsigned long long>();
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            const Counters c0 = compact_marks(g, kw);
-            std::uint32_t F = c0.next_count;
-            std::uint64_t pairs = c0.frontier_edges; // (row, query) pairs of the queue
-            st.converged = F == 0 ? 1 : 0;
-            for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
+            hip_check(hipMemsetAsync(rs, 0, sizeof(RoundState), s), "reset round state");
+            const Counters c0 = compact_marks(g, kw); // seeds -> first queue (eager)
+            st.converged = c0.next_count == 0 ? 1 : 0;
+            if (c0.next_count == 0)
+                return;
+
+            // one 256-row tile per workgroup at the largest queue (V rows):
+            // workgroups past this round's rows exit at once, and no
+            // workgroup waits at a barrier for another tile's slow wavefront
+            const unsigned light_grid = blocks_for(g.V);
+            const unsigned nb = compact_blocks(g.V);
+            MarkSel sel{kw.mark.as<unsigned long long>(), kw.lanes.as<unsigned long long>()};
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin round capture");
+            for (int r = 0; r < kRoundsPerGraph; ++r)
             {
-                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                round_begin<<<1, 64, 0, s>>>(d_ctr, rs, KP, o.lane_form, max_rounds);
                 if (ppr)
+                    pprk_snapshot<<<snapshot_blocks(g.V * static_cast<std::uint64_t>(K)), kBlock, 0, s>>>(
+                        a.q, 0, K, a.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
+                        kw.snap.as<float>(), rs);
+                expandk_light<OffT, KP, Op><<<light_grid, kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                expandk_light_items<OffT, Op><<<light_grid, kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                compact_rows<unsigned long long, std::uint32_t, MarkSel><<<nb, kBlock, 0, s>>>(
+                    g.V, sel, g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>(), d_ctr, nullptr, nullptr);
+                round_end<<<1, 64, 0, s>>>(d_ctr, rs);
+            }
+            const hipError_t launch_err = hipGetLastError();
+            const hipError_t cap_err = hipStreamEndCapture(s, &graph);
+            hip_check(launch_err, "launch captured round");
+            hip_check(cap_err, "end round capture");
+            struct GraphGuard
+            {
+                hipGraph_t g;
+                hipGraphExec_t e;
+                ~GraphGuard()
                 {
-                    const std::uint64_t n = static_cast<std::uint64_t>(F) * K;
-                    pprk_snapshot<<<snapshot_blocks(n), kBlock, 0, s>>>(
-                        a.q, F, K, a.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
-                        kw.snap.as<float>());
-                    hip_check(hipGetLastError(), "launch pprk_snapshot");
+                    if (e)
+                        (void)hipGraphExecDestroy(e);
+                    if (g)
+                        (void)hipGraphDestroy(g);
                 }
-                a.F = F;
+            } guard{graph, nullptr};
+            hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate round graph");
+            guard.e = exec;
+
+            RoundState *h_rs = reinterpret_cast<RoundState *>(kw.host_round->ptr);
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            while (true)
+            {
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                if (use_items(o.lane_form, pairs, F, KP))
-                {
-                    expandk_light_items<OffT, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                    hip_check(hipGetLastError(), "launch expandk_light_items");
-                    expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-                    hip_check(hipGetLastError(), "launch expandk_heavy_items");
-                }
-                else
-                {
-                    expandk_light<OffT, KP, Op><<<blocks_for(F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                    hip_check(hipGetLastError(), "launch expandk_light");
-                    expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
-                    hip_check(hipGetLastError(), "launch expandk_heavy");
-                }
+                hip_check(hipGraphLaunch(exec, s), "launch round graph");
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_marks(g, kw);
+                hip_check(hipMemcpyAsync(h_rs, rs, sizeof(RoundState), hipMemcpyDeviceToHost, s), "read round state");
+                hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read counters");
+                hip_check(hipStreamSynchronize(s), "round graph");
                 if (o.time_kernels)
                 {
                     float ms = 0;
@@ -244,22 +327,21 @@ namespace hyt
                     st.expand_ms += ms;
                     st.expand_launches += 1;
                 }
-                if (c.heavy_overflow)
+                if (h_ctr->heavy_overflow)
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
-                st.frontier_total += F;
-                st.edges_total += c.edges;
-                st.updates_total += c.next_count;
-                st.rounds = round + 1;
-                if (c.overflow)
+                if (h_ctr->overflow)
                 {
                     st.overflow = 1;
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
-                F = c.next_count;
-                pairs = c.frontier_edges;
-                if (F == 0)
-                    st.converged = 1;
+                if (h_rs->done)
+                    break;
             }
+            st.rounds = h_rs->rounds;
+            st.frontier_total = h_rs->frontier_total;
+            st.edges_total = h_rs->edges_total;
+            st.updates_total = h_rs->updates_total;
+            st.converged = h_rs->truncated ? 0 : 1;
         }
 
         template <class OffT, class Op>

@@ -34,8 +34,22 @@ namespace hyt
             std::uint32_t pad;
         };
 
+        // Round state of device-driven rounds (rounds captured into a HIP
+        // graph and replayed without host round trips): the queue size and
+        // the form of the current round are read here by every kernel.
+        struct RoundState
+        {
+            std::uint32_t F;     // rows of the current queue
+            std::uint32_t form;  // HYT_LANES_GROUP or HYT_LANES_ITEMS
+            std::uint64_t pairs; // (row, query) pairs of the current queue
+            std::uint64_t rounds, frontier_total, edges_total, updates_total;
+            std::uint32_t done;      // a round found the queue empty (or the round limit)
+            std::uint32_t truncated; // stopped by the round limit with work left
+        };
+
         struct KArgs
         {
+            const RoundState *rs; // non-null: F and the form come from the device
             const std::uint32_t *q;
             const unsigned long long *qmask;
             std::uint32_t F;
@@ -145,6 +159,19 @@ namespace hyt
             }
         }
 
+        // Queue size of this round; false when a device-driven round runs
+        // the other form (the kernel then has nothing to do).
+        __device__ __forceinline__ bool round_rows(const KArgs &a, std::uint32_t form, std::uint32_t &F)
+        {
+            if (a.rs == nullptr)
+            {
+                F = a.F;
+                return true;
+            }
+            F = a.rs->F;
+            return a.rs->form == form;
+        }
+
         template <class OffT, int KP, class Op>
         __global__ __launch_bounds__(kBlock) void expandk_light(const OffT *__restrict__ off, Op op, KArgs a)
         {
@@ -154,74 +181,86 @@ namespace hyt
             __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
 
+            std::uint32_t F;
+            if (!round_rows(a, HYT_LANES_GROUP, F))
+                return;
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
-            const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
             const std::uint32_t ce = a.chunk;
-            std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
-            std::uint64_t mask = 0;
-            OffT begin = 0, full = 0;
-            if (i < a.F)
+            // a block-uniform loop over 256-row tiles (one tile per block when
+            // the grid is sized from F)
+            const std::uint32_t tiles = (F + kBlock - 1) / kBlock;
+            for (std::uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
             {
-                const std::uint32_t u = a.q[i];
-                begin = off[u];
-                full = off[u + 1] - begin;
-                mask = a.qmask[i];
-                row = Op::kRowIsQueueIndex ? i : u;
-                scale = op.row_scale(static_cast<std::uint64_t>(full));
-                if (full <= a.light_max)
-                    deg = static_cast<std::uint32_t>(full);
-                else
-                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
+                const std::uint32_t i = tile * kBlock + threadIdx.x;
+                std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
+                std::uint64_t mask = 0;
+                OffT begin = 0, full = 0;
+                if (i < F)
+                {
+                    const std::uint32_t u = a.q[i];
+                    begin = off[u];
+                    full = off[u + 1] - begin;
+                    mask = a.qmask[i];
+                    row = Op::kRowIsQueueIndex ? i : u;
+                    scale = op.row_scale(static_cast<std::uint64_t>(full));
+                    if (full <= a.light_max)
+                        deg = static_cast<std::uint32_t>(full);
+                    else
+                        nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
+                }
+                const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t c) {
+                    const std::uint64_t lo = static_cast<std::uint64_t>(c) * ce;
+                    const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                    HeavyChunkK h;
+                    h.row = row;
+                    h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
+                    h.begin = static_cast<std::uint64_t>(begin) + lo;
+                    h.mask = mask;
+                    h.scale = scale;
+                    h.pad = 0;
+                    return h;
+                });
+                if (!queued && nch)
+                {
+                    // chunk queue full: walk the row here (rows of 2^24 edges or
+                    // more cannot be - 64 of them must fit a u32 - and are reported)
+                    if (static_cast<std::uint64_t>(full) >= (1ULL << 24))
+                        atomicOr(&a.ctr->heavy_overflow, 1U);
+                    else
+                        deg = static_cast<std::uint32_t>(full);
+                }
+                const std::uint32_t incl = wave_incl_scan(deg);
+                const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+                s_ex[wid][lane] = incl - deg;
+                s_begin[wid][lane] = begin;
+                s_row[wid][lane] = row;
+                s_scale[wid][lane] = scale;
+                s_mask[wid][lane] = mask;
+                __syncthreads();
+                if (total == 0)
+                    continue; // every wavefront still meets the next tile's barrier
+                if (lane == 0)
+                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
+                const std::uint32_t *ex = s_ex[wid];
+                walk_edges_k<KP>(op, a, total,
+                                 [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
+                                     std::uint32_t &sc) {
+                                     const int j = owner_of(ex, e);
+                                     pos = static_cast<std::uint64_t>(s_begin[wid][j]) + (e - ex[j]);
+                                     r = s_row[wid][j];
+                                     m = s_mask[wid][j];
+                                     sc = s_scale[wid][j];
+                                 });
             }
-            const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t c) {
-                const std::uint64_t lo = static_cast<std::uint64_t>(c) * ce;
-                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
-                HeavyChunkK h;
-                h.row = row;
-                h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
-                h.begin = static_cast<std::uint64_t>(begin) + lo;
-                h.mask = mask;
-                h.scale = scale;
-                h.pad = 0;
-                return h;
-            });
-            if (!queued && nch)
-            {
-                // chunk queue full: walk the row here (rows of 2^24 edges or
-                // more cannot be - 64 of them must fit a u32 - and are reported)
-                if (static_cast<std::uint64_t>(full) >= (1ULL << 24))
-                    atomicOr(&a.ctr->heavy_overflow, 1U);
-                else
-                    deg = static_cast<std::uint32_t>(full);
-            }
-            const std::uint32_t incl = wave_incl_scan(deg);
-            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
-            s_ex[wid][lane] = incl - deg;
-            s_begin[wid][lane] = begin;
-            s_row[wid][lane] = row;
-            s_scale[wid][lane] = scale;
-            s_mask[wid][lane] = mask;
-            __syncthreads();
-            if (total == 0)
-                return;
-            if (lane == 0)
-                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
-            const std::uint32_t *ex = s_ex[wid];
-            walk_edges_k<KP>(op, a, total,
-                             [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
-                                 std::uint32_t &sc) {
-                                 const int j = owner_of(ex, e);
-                                 pos = static_cast<std::uint64_t>(s_begin[wid][j]) + (e - ex[j]);
-                                 r = s_row[wid][j];
-                                 m = s_mask[wid][j];
-                                 sc = s_scale[wid][j];
-                             });
         }
 
         template <int KP, class Op>
         __global__ __launch_bounds__(kBlock) void expandk_heavy(Op op, KArgs a)
         {
+            std::uint32_t F;
+            if (!round_rows(a, HYT_LANES_GROUP, F))
+                return;
             const int lane = lane_id();
             const std::uint32_t nh = a.ctr->heavy_count;
             const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
@@ -327,90 +366,100 @@ namespace hyt
             __shared__ std::uint32_t s_c[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
 
+            std::uint32_t F;
+            if (!round_rows(a, HYT_LANES_ITEMS, F))
+                return;
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
-            const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-            std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0, ce = 16;
-            std::uint64_t mask = 0;
-            OffT begin = 0, full = 0;
-            if (i < a.F)
+            const std::uint32_t tiles = (F + kBlock - 1) / kBlock;
+            for (std::uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
             {
-                const std::uint32_t u = a.q[i];
-                begin = off[u];
-                full = off[u + 1] - begin;
-                mask = a.qmask[i];
-                c = static_cast<std::uint32_t>(__popcll(mask));
-                row = Op::kRowIsQueueIndex ? i : u;
-                scale = op.row_scale(static_cast<std::uint64_t>(full));
-                ce = chunk_edges(c, a);
-                if (light_work(static_cast<std::uint64_t>(full), c, a))
+                const std::uint32_t i = tile * kBlock + threadIdx.x;
+                std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0, ce = 16;
+                std::uint64_t mask = 0;
+                OffT begin = 0, full = 0;
+                if (i < F)
                 {
-                    deg = static_cast<std::uint32_t>(full);
-                    items = deg * c;
+                    const std::uint32_t u = a.q[i];
+                    begin = off[u];
+                    full = off[u + 1] - begin;
+                    mask = a.qmask[i];
+                    c = static_cast<std::uint32_t>(__popcll(mask));
+                    row = Op::kRowIsQueueIndex ? i : u;
+                    scale = op.row_scale(static_cast<std::uint64_t>(full));
+                    ce = chunk_edges(c, a);
+                    if (light_work(static_cast<std::uint64_t>(full), c, a))
+                    {
+                        deg = static_cast<std::uint32_t>(full);
+                        items = deg * c;
+                    }
+                    else
+                        nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
                 }
-                else
-                    nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
-            }
-            const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t ch) {
-                const std::uint64_t lo = static_cast<std::uint64_t>(ch) * ce;
-                const std::uint64_t full64 = static_cast<std::uint64_t>(full);
-                HeavyChunkK h;
-                h.row = row;
-                h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
-                h.begin = static_cast<std::uint64_t>(begin) + lo;
-                h.mask = mask;
-                h.scale = scale;
-                h.pad = 0;
-                return h;
-            });
-            if (!queued && nch)
-            {
-                // chunk queue full: walk the row here (work of 2^24 items or
-                // more cannot be - 64 rows' work must fit a u32 - and is reported)
-                const std::uint64_t w = static_cast<std::uint64_t>(full) * c;
-                if (w >= (1ULL << 24))
-                    atomicOr(&a.ctr->heavy_overflow, 1U);
-                else
+                const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t ch) {
+                    const std::uint64_t lo = static_cast<std::uint64_t>(ch) * ce;
+                    const std::uint64_t full64 = static_cast<std::uint64_t>(full);
+                    HeavyChunkK h;
+                    h.row = row;
+                    h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
+                    h.begin = static_cast<std::uint64_t>(begin) + lo;
+                    h.mask = mask;
+                    h.scale = scale;
+                    h.pad = 0;
+                    return h;
+                });
+                if (!queued && nch)
                 {
-                    deg = static_cast<std::uint32_t>(full);
-                    items = static_cast<std::uint32_t>(w);
+                    // chunk queue full: walk the row here (work of 2^24 items or
+                    // more cannot be - 64 rows' work must fit a u32 - and is reported)
+                    const std::uint64_t w = static_cast<std::uint64_t>(full) * c;
+                    if (w >= (1ULL << 24))
+                        atomicOr(&a.ctr->heavy_overflow, 1U);
+                    else
+                    {
+                        deg = static_cast<std::uint32_t>(full);
+                        items = static_cast<std::uint32_t>(w);
+                    }
                 }
+                const std::uint32_t incl = wave_incl_scan(items);
+                const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+                std::uint32_t dsum = deg;
+    #pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1)
+                    dsum += __shfl_xor(dsum, d, kWave);
+                s_ex[wid][lane] = incl - items;
+                s_begin[wid][lane] = begin;
+                s_row[wid][lane] = row;
+                s_scale[wid][lane] = scale;
+                s_c[wid][lane] = c;
+                s_mask[wid][lane] = mask;
+                __syncthreads();
+                if (total == 0)
+                    continue; // every wavefront still meets the next tile's barrier
+                if (lane == 0)
+                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(dsum));
+                const std::uint32_t *ex = s_ex[wid];
+                walk_items_k(op, a, total,
+                             [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint32_t &sc,
+                                 std::uint32_t &k) {
+                                 const int j = owner_of(ex, e);
+                                 const std::uint32_t local = e - ex[j];
+                                 const std::uint32_t cj = s_c[wid][j];
+                                 const std::uint32_t edge = local / cj;
+                                 k = nth_set_bit(s_mask[wid][j], local - edge * cj);
+                                 pos = static_cast<std::uint64_t>(s_begin[wid][j]) + edge;
+                                 r = s_row[wid][j];
+                                 sc = s_scale[wid][j];
+                             });
             }
-            const std::uint32_t incl = wave_incl_scan(items);
-            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
-            std::uint32_t dsum = deg;
-#pragma unroll
-            for (int d = kWave / 2; d > 0; d >>= 1)
-                dsum += __shfl_xor(dsum, d, kWave);
-            s_ex[wid][lane] = incl - items;
-            s_begin[wid][lane] = begin;
-            s_row[wid][lane] = row;
-            s_scale[wid][lane] = scale;
-            s_c[wid][lane] = c;
-            s_mask[wid][lane] = mask;
-            __syncthreads();
-            if (total == 0)
-                return;
-            if (lane == 0)
-                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(dsum));
-            const std::uint32_t *ex = s_ex[wid];
-            walk_items_k(op, a, total,
-                         [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint32_t &sc,
-                             std::uint32_t &k) {
-                             const int j = owner_of(ex, e);
-                             const std::uint32_t local = e - ex[j];
-                             const std::uint32_t cj = s_c[wid][j];
-                             const std::uint32_t edge = local / cj;
-                             k = nth_set_bit(s_mask[wid][j], local - edge * cj);
-                             pos = static_cast<std::uint64_t>(s_begin[wid][j]) + edge;
-                             r = s_row[wid][j];
-                             sc = s_scale[wid][j];
-                         });
         }
 
         template <class Op>
         __global__ __launch_bounds__(kBlock) void expandk_heavy_items(Op op, KArgs a)
         {
+            std::uint32_t F;
+            if (!round_rows(a, HYT_LANES_ITEMS, F))
+                return;
             const int lane = lane_id();
             const std::uint32_t nh = a.ctr->heavy_count;
             const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
@@ -526,9 +575,10 @@ namespace hyt
 
         // Moves the active residuals of the frontier rows into rank.
         __global__ void pprk_snapshot(const std::uint32_t *q, std::uint32_t F, std::uint32_t K,
-                                      const unsigned long long *qmask, float *res, float *rank, float *snap)
+                                      const unsigned long long *qmask, float *res, float *rank, float *snap,
+                                      const RoundState *rs = nullptr)
         {
-            const std::uint64_t n = static_cast<std::uint64_t>(F) * K;
+            const std::uint64_t n = static_cast<std::uint64_t>(rs ? rs->F : F) * K;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
                  t += stride)

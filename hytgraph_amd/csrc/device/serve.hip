@@ -309,6 +309,7 @@ namespace hyt
             hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset lane counters");
             hip_check(hipMemsetAsync(S.lanes.ptr, 0, 8, s), "reset lane mask");
             KArgs a;
+            a.rs = nullptr;
             a.q = S.q.as<std::uint32_t>();
             a.qmask = S.qmask.as<unsigned long long>();
             a.F = S.F;

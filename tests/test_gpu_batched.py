@@ -155,3 +155,19 @@ def test_trim_releases_and_rebuilds_workspaces(gpu, abi, orc):
         ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(s))
         assert np.array_equal(dk[:, k], ref)
     assert np.array_equal(d1, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src[1])))
+
+
+def test_k_lane_round_limit_and_next_call(gpu, abi, orc):
+    # device-driven rounds stop at max_rounds (not converged), and the next
+    # call starts from clean marks
+    hg = abi.HostGraph(scale=12, edge_factor=16, seed=29, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    src = abi.sample_sources(hg.offsets, 6, 8)
+    _, st = g.sssp(src, max_rounds=2)
+    assert st["rounds"] == 2 and st["converged"] == 0
+    d, st = g.sssp(src)
+    assert st["converged"] == 1 and st["rounds"] > 2
+    for k, s in enumerate(src):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(s)))
+    _, pst = g.ppr(src, D, 1e-6, max_rounds=3)
+    assert pst["rounds"] == 3 and pst["converged"] == 0
