@@ -253,6 +253,9 @@ SIGNATURES = {
     "hyt_hybrid_sssp": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
     "hyt_hybrid_pagerank": (
         C.c_int, [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
+    "hyt_hybrid_sssp_k": (C.c_int, [_P, _P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
+    "hyt_hybrid_ppr_k": (
+        C.c_int, [_P, _P, C.c_uint32, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(HybridStats)]),
     "hyt_lanes_state_bytes": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _u64p]),
     "hyt_lanes_create": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(_P)]),
     "hyt_lanes_free": (C.c_int, [_P]),
@@ -559,6 +562,33 @@ class HybridEngine:
             return None, st.as_dict()
         out = np.empty(self.V, np.float32)
         check(load().hyt_hybrid_pagerank(self._h, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    def sssp_k(self, sources, out_ptr=None, **opt):
+        """K sources in one run (hyt_hybrid_sssp_k): u32[V][K] distances."""
+        src = np.ascontiguousarray(np.atleast_1d(sources), dtype=np.uint32)
+        K = len(src)
+        o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = HybridStats()
+        if out_ptr is not None:
+            check(load().hyt_hybrid_sssp_k(self._h, _ptr(src), K, C.c_void_p(out_ptr), C.byref(o), C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty((self.V, K), np.uint32)
+        check(load().hyt_hybrid_sssp_k(self._h, _ptr(src), K, _ptr(out), C.byref(o), C.byref(st)))
+        return out, st.as_dict()
+
+    def ppr_k(self, seeds, damping=0.85, epsilon=1e-6, out_ptr=None, **opt):
+        """K personalized-PageRank seeds in one run (hyt_hybrid_ppr_k): f32[V][K]."""
+        sd = np.ascontiguousarray(np.atleast_1d(seeds), dtype=np.uint32)
+        K = len(sd)
+        o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
+        st = HybridStats()
+        if out_ptr is not None:
+            check(load().hyt_hybrid_ppr_k(self._h, _ptr(sd), K, damping, epsilon, C.c_void_p(out_ptr), C.byref(o),
+                                          C.byref(st)))
+            return None, st.as_dict()
+        out = np.empty((self.V, K), np.float32)
+        check(load().hyt_hybrid_ppr_k(self._h, _ptr(sd), K, damping, epsilon, _ptr(out), C.byref(o), C.byref(st)))
         return out, st.as_dict()
 
     def free(self):

@@ -20,71 +20,6 @@ namespace hyt
     {
         using namespace lanes;
 
-        __global__ void seed_rows(std::uint32_t nrows, const std::uint32_t *rows, const unsigned long long *masks,
-                                  const std::uint32_t *lane_src, std::uint32_t K, unsigned long long *mark,
-                                  std::uint32_t *val, std::uint32_t seed_value)
-        {
-            const std::uint32_t t = threadIdx.x;
-            if (t < nrows)
-                mark[rows[t]] = masks[t];
-            if (t < K)
-                val[static_cast<std::uint64_t>(lane_src[t]) * K + t] = seed_value;
-        }
-
-        // Column sums of x = rank + res ([V][K]) into partial[block][K] (double).
-        __global__ __launch_bounds__(kBlock) void pprk_fold(float *rank, const float *res, std::uint64_t V,
-                                                           std::uint32_t K, std::uint32_t KP, double *partial)
-        {
-            __shared__ double s[kBlock];
-            const std::uint32_t rows_per = kBlock / KP;
-            const std::uint32_t k = threadIdx.x % KP;
-            const std::uint32_t r0 = threadIdx.x / KP;
-            double acc = 0.0;
-            if (k < K)
-            {
-                for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * rows_per + r0; v < V;
-                     v += static_cast<std::uint64_t>(gridDim.x) * rows_per)
-                {
-                    const std::uint64_t idx = v * K + k;
-                    const float x = rank[idx] + res[idx];
-                    rank[idx] = x;
-                    acc += x;
-                }
-            }
-            s[threadIdx.x] = acc;
-            __syncthreads();
-            if (threadIdx.x < KP && k < K)
-            {
-                double t = 0.0;
-                for (std::uint32_t r = 0; r < rows_per; ++r)
-                    t += s[r * KP + threadIdx.x];
-                partial[static_cast<std::uint64_t>(blockIdx.x) * 64 + threadIdx.x] = t;
-            }
-        }
-
-        __global__ void pprk_total(double *partial, unsigned nblocks, std::uint32_t K, double *total)
-        {
-            const std::uint32_t k = threadIdx.x;
-            if (k >= K)
-                return;
-            double t = 0.0;
-            for (unsigned b = 0; b < nblocks; ++b)
-                t += partial[static_cast<std::uint64_t>(b) * 64 + k];
-            total[k] = t;
-        }
-
-        __global__ void pprk_normalize(const float *x, std::uint64_t n, std::uint32_t K, const double *total,
-                                       float *out)
-        {
-            std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
-            for (; i < n; i += stride)
-            {
-                const double t = total[i % K];
-                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
-            }
-        }
-
         // Workspace of the batched forms (grown to the largest K used).
         struct KWorkspace
         {
@@ -102,29 +37,6 @@ namespace hyt
             return *static_cast<KWorkspace *>(g.kws.get());
         }
 
-
-        struct SeedPlan
-        {
-            std::vector<std::uint32_t> rows;
-            std::vector<unsigned long long> masks;
-        };
-
-        SeedPlan plan_seeds(const std::uint32_t *src, std::uint32_t K)
-        {
-            SeedPlan p;
-            for (std::uint32_t k = 0; k < K; ++k)
-            {
-                auto it = std::find(p.rows.begin(), p.rows.end(), src[k]);
-                if (it == p.rows.end())
-                {
-                    p.rows.push_back(src[k]);
-                    p.masks.push_back(1ULL << k);
-                }
-                else
-                    p.masks[static_cast<std::size_t>(it - p.rows.begin())] |= 1ULL << k;
-            }
-            return p;
-        }
 
         void prepare(DeviceGraph &g, KWorkspace &kw, std::uint32_t K, bool ppr, std::uint32_t light_max,
                      std::uint32_t chunk)
@@ -154,25 +66,6 @@ namespace hyt
                 kw.host_round = std::make_unique<PinnedBuffer>(sizeof(RoundState));
             kw.partial.ensure((1024 + 1) * 64 * sizeof(double), "hipMalloc(partials)");
             ensure_workspace(g, light_max, chunk);
-        }
-
-        void upload_seeds(DeviceGraph &g, KWorkspace &kw, const SeedPlan &p, const std::uint32_t *src,
-                          std::uint32_t K, std::uint32_t *val, std::uint32_t seed_value)
-        {
-            // seeds buffer: rows u32[64] | masks u64[64] | lane sources u32[64]
-            std::vector<unsigned char> blob(64 * 4 + 64 * 8 + 64 * 4, 0);
-            std::memcpy(blob.data(), p.rows.data(), p.rows.size() * 4);
-            std::memcpy(blob.data() + 256, p.masks.data(), p.masks.size() * 8);
-            std::memcpy(blob.data() + 768, src, K * 4);
-            unsigned char *d = kw.seeds.as<unsigned char>();
-            hip_check(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, g.stream), "upload seeds");
-            seed_rows<<<1, 64, 0, g.stream>>>(static_cast<std::uint32_t>(p.rows.size()),
-                                              reinterpret_cast<const std::uint32_t *>(d),
-                                              reinterpret_cast<const unsigned long long *>(d + 256),
-                                              reinterpret_cast<const std::uint32_t *>(d + 768), K,
-                                              kw.mark.as<unsigned long long>(), val, seed_value);
-            hip_check(hipGetLastError(), "launch seed_rows");
-            hip_check(hipStreamSynchronize(g.stream), "seed upload");
         }
 
         Counters compact_marks(DeviceGraph &g, KWorkspace &kw)
@@ -248,8 +141,6 @@ namespace hyt
             RoundState *rs = kw.round.as<RoundState>();
             KArgs a;
             a.rs = rs;
-            a.q = g.ws.q.as<std::uint32_t>();
-            a.qmask = kw.qmask.as<unsigned long long>();
             a.F = 0;
             a.ctr = d_ctr;
             a.heavy = kw.heavy.as<HeavyChunkK>();
@@ -269,6 +160,7 @@ namespace hyt
             // workgroups past this round's rows exit at once, and no
             // workgroup waits at a barrier for another tile's slow wavefront
             const unsigned light_grid = blocks_for(g.V);
+            const QueueRowsK<OffT> rows{g.off.as<OffT>(), g.ws.q.as<std::uint32_t>(), kw.qmask.as<unsigned long long>()};
             const unsigned nb = compact_blocks(g.V);
             MarkSel sel{kw.mark.as<unsigned long long>(), kw.lanes.as<unsigned long long>()};
             hipGraph_t graph = nullptr;
@@ -279,10 +171,10 @@ namespace hyt
                 round_begin<<<1, 64, 0, s>>>(d_ctr, rs, KP, o.lane_form, max_rounds);
                 if (ppr)
                     pprk_snapshot<<<snapshot_blocks(g.V * static_cast<std::uint64_t>(K)), kBlock, 0, s>>>(
-                        a.q, 0, K, a.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
+                        rows.q, 0, K, rows.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
                         kw.snap.as<float>(), rs);
-                expandk_light<OffT, KP, Op><<<light_grid, kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
-                expandk_light_items<OffT, Op><<<light_grid, kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                expandk_light<QueueRowsK<OffT>, KP, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
+                expandk_light_items<QueueRowsK<OffT>, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
                 expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                 expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                 compact_rows<unsigned long long, std::uint32_t, MarkSel><<<nb, kBlock, 0, s>>>(
@@ -390,7 +282,7 @@ namespace hyt
         const std::uint32_t inf = (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
         launch_fill<std::uint32_t>(kw.val.as<std::uint32_t>(), g.V * K, inf, g.stream);
         const SeedPlan p = plan_seeds(sources, K);
-        upload_seeds(g, kw, p, sources, K, kw.val.as<std::uint32_t>(), 0U);
+        upload_seeds(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, sources, K, kw.val.as<std::uint32_t>(), 0U);
         auto run = [&](auto op) {
             op.col = g.col.as<std::uint32_t>();
             op.w = g.w.as<std::uint32_t>();
@@ -428,7 +320,7 @@ namespace hyt
         const float r0 = 1.0F - damping;
         std::uint32_t r0_bits;
         std::memcpy(&r0_bits, &r0, 4);
-        upload_seeds(g, kw, p, seeds, K, kw.val.as<std::uint32_t>(), r0_bits);
+        upload_seeds(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, seeds, K, kw.val.as<std::uint32_t>(), r0_bits);
         PprKOp op;
         op.col = g.col.as<std::uint32_t>();
         op.res = reinterpret_cast<float *>(kw.val.ptr);
@@ -438,22 +330,8 @@ namespace hyt
         op.eps = eps;
         dispatch_off(g, kw, op, K, o, st, lm, ch, true);
 
-        // finalize: x = rank + res, column sums, normalise
-        const std::uint32_t KP = static_cast<std::uint32_t>(pow2_at_least(K));
-        const std::uint64_t rows_per = kBlock / KP;
-        std::uint64_t nb64 = (g.V + rows_per - 1) / rows_per;
-        const unsigned nb = static_cast<unsigned>(nb64 < 1024 ? (nb64 ? nb64 : 1) : 1024);
-        double *partial = kw.partial.as<double>();
-        pprk_fold<<<nb, kBlock, 0, g.stream>>>(kw.rank.as<float>(), op.res, g.V, K, KP, partial);
-        hip_check(hipGetLastError(), "launch pprk_fold");
-        double *total = partial + static_cast<std::uint64_t>(1024) * 64;
-        pprk_total<<<1, 64, 0, g.stream>>>(partial, nb, K, total);
-        hip_check(hipGetLastError(), "launch pprk_total");
         float *dst = o.out_on_device ? out : kw.snap.as<float>();
-        std::uint64_t b = (VK + kBlock - 1) / kBlock;
-        pprk_normalize<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, g.stream>>>(
-            kw.rank.as<float>(), VK, K, total, dst);
-        hip_check(hipGetLastError(), "launch pprk_normalize");
+        pprk_finish(g.stream, kw.rank.as<float>(), op.res, g.V, K, kw.partial.as<double>(), dst);
         if (!o.out_on_device)
             hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
         hip_check(hipStreamSynchronize(g.stream), "ppr output");

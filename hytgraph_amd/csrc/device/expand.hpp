@@ -22,9 +22,11 @@
 //
 // RowSrc says where a frontier row's edges are: GraphRows reads the CSR
 // offsets and clips them to an edge window [lo, hi) (positions relative to
-// lo: the whole graph in HBM, one partition in a device slot, or the mapped
-// host graph for zero-copy); PackedRows reads host-compacted rows. Op's
-// column/weight pointers address the same window.
+// lo: the whole graph in HBM, a window of the streaming ring, or the mapped
+// host graph for zero-copy); PackedRows reads host-compacted rows;
+// CachedRows reads the rows of many cached partitions through a segment
+// table. Op's column/weight pointers address the same window. get() also
+// names the row's value index (the row's index in the frontier arrays).
 #pragma once
 
 #include "kern_common.hpp"
@@ -38,9 +40,10 @@ namespace hyt
         const std::uint32_t *q;
         std::uint64_t lo, hi; // edge window
 
-        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint64_t &begin, std::uint64_t &count,
-                            std::uint64_t &full) const
+        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &vi, std::uint64_t &begin,
+                            std::uint64_t &count, std::uint64_t &full) const
         {
+            vi = i;
             u = q[i];
             const std::uint64_t b = off[u], e = off[u + 1];
             full = e - b;
@@ -56,13 +59,59 @@ namespace hyt
         const std::uint32_t *rows;  // vertex of packed row i
         const std::uint64_t *index; // packed row i spans [index[i], index[i+1])
 
-        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint64_t &begin, std::uint64_t &count,
-                            std::uint64_t &full) const
+        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &vi, std::uint64_t &begin,
+                            std::uint64_t &count, std::uint64_t &full) const
         {
+            vi = i;
             u = rows[i];
             begin = index[i];
             count = index[i + 1] - begin;
             full = count;
+        }
+    };
+
+    // Resident partitions of one round expanded by one launch. Segment s
+    // covers queue rows [qlo, qlo + n) - whole rows of one partition, held in
+    // a cache unit so that edge b of the partition sits at unit_pos + (b -
+    // edge_begin) of the unit area; rows_before is the exclusive prefix of n.
+    struct CacheSeg
+    {
+        std::uint32_t qlo, rows_before;
+        std::uint64_t edge_begin, unit_pos;
+    };
+
+    // Segment of launch row i: the last s with rows_before <= i.
+    __device__ inline const CacheSeg &seg_of(const CacheSeg *seg, std::uint32_t nseg, std::uint32_t i)
+    {
+        std::uint32_t lo = 0, hi = nseg;
+        while (hi - lo > 1)
+        {
+            const std::uint32_t mid = (lo + hi) >> 1;
+            if (seg[mid].rows_before <= i)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        return seg[lo];
+    }
+
+    template <class OffT>
+    struct CachedRows
+    {
+        const OffT *off;
+        const std::uint32_t *q;
+        const CacheSeg *seg;
+        std::uint32_t nseg;
+
+        __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &vi, std::uint64_t &begin,
+                            std::uint64_t &count, std::uint64_t &full) const
+        {
+            const CacheSeg &sg = seg_of(seg, nseg, i);
+            vi = sg.qlo + (i - sg.rows_before);
+            u = q[vi];
+            const std::uint64_t b = off[u];
+            count = full = off[u + 1] - b;
+            begin = sg.unit_pos + (b - sg.edge_begin);
         }
     };
 
@@ -150,10 +199,10 @@ namespace hyt
         std::uint64_t begin = 0, count = 0;
         if (i < a.F)
         {
-            std::uint32_t u;
+            std::uint32_t u, vi;
             std::uint64_t full;
-            rs.get(i, u, begin, count, full);
-            value = op.row_value(i, u, full);
+            rs.get(i, u, vi, begin, count, full);
+            value = op.row_value(vi, u, full);
             if (count > a.light_max)
                 nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
             else

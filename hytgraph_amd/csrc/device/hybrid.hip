@@ -29,6 +29,7 @@
 //      staging (while the GPU works on the rest) and copies them once.
 // The expand kernels are the in-HBM ones with a windowed / packed row source.
 #include "host_graph.hpp"
+#include "lanes.hpp"
 #include "ops.hpp"
 #include "transfer_plan.hpp"
 
@@ -39,6 +40,7 @@
 #include <chrono>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace hyt
@@ -142,7 +144,6 @@ namespace hyt
         struct Resident
         {
             std::int64_t col = -1, w = -1; // units holding the columns / weights, -1 = none
-            std::int64_t pos = -1;         // index in the resident list
         };
 
         constexpr std::size_t kRingBuffers = 4;
@@ -150,6 +151,19 @@ namespace hyt
         struct EventPair
         {
             hipEvent_t a = nullptr, b = nullptr;
+        };
+
+        // [V][K] state of K-lane runs (allocated on first use, grown to the
+        // largest K), outside the edge budget like the single-query state.
+        struct LaneState
+        {
+            DeviceBuffer val, rank, snap, mark, qmask, stage_mask, heavy, seeds, lanes, partial;
+            std::uint64_t heavy_cap = 0;
+            std::uint64_t bytes() const
+            {
+                return val.bytes + rank.bytes + snap.bytes + mark.bytes + qmask.bytes + stage_mask.bytes +
+                       heavy.bytes + seeds.bytes + lanes.bytes + partial.bytes;
+            }
         };
     } // namespace
 
@@ -175,12 +189,17 @@ namespace hyt
         hytm::Params run_params;
         // residency cache
         std::uint64_t unit_edges = 0; // 4-byte entries per unit (= partition_edges)
-        std::uint64_t units = 0;
+        // Unit layout: a partition read with its weights holds column unit j
+        // < pairs and weight unit j + pairs, so one launch addresses every
+        // resident partition (weights at a fixed offset from the columns);
+        // runs that read no weights place columns in any unit.
+        std::uint64_t units = 0, pairs = 0;
         DeviceBuffer unit_mem;
-        std::vector<std::uint32_t> free_units;
+        std::vector<std::int64_t> owner;          // per unit: partition holding it, -1 = free
         std::vector<Resident> res;                // per partition
-        std::vector<std::uint64_t> resident;      // partitions holding units
         std::vector<std::uint64_t> last_active;   // per partition: stamp of the last round it was active
+        std::vector<CacheSeg> seg_h;              // this round's resident segments (host copy)
+        DeviceBuffer seg_d;
         std::uint64_t stamp = 0;                  // round stamp (monotone across runs)
         // streaming ring
         std::uint64_t ring_edges = 0; // edges per ring buffer
@@ -204,6 +223,7 @@ namespace hyt
         double link_gbs = 0, pack_gbs = 0, zc_gbs = 0;
         // degree-bin split of the current run's expansions
         std::uint32_t light_max = kHybridLightMaxDegree, chunk = kHybridHeavyChunk;
+        LaneState lk;
 
         ~HybridEngine()
         {
@@ -231,7 +251,12 @@ namespace hyt
                     (void)hipHostFree(p);
         }
 
-        std::uint64_t state_bytes() const { return dg.device_bytes() + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes + block_offsets.bytes + scan_temp.bytes + stage_q.bytes + stage_val.bytes; }
+        std::uint64_t state_bytes() const
+        {
+            return dg.device_bytes() + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes +
+                   block_offsets.bytes + scan_temp.bytes + stage_q.bytes + stage_val.bytes + seg_d.bytes +
+                   lk.bytes();
+        }
         std::uint64_t edge_area_bytes() const
         {
             return unit_mem.bytes + ring_mem.bytes + pk_rows.bytes + pk_index.bytes + pk_col.bytes + pk_w.bytes +
@@ -309,10 +334,11 @@ namespace hyt
                 run_params.entry_bytes = 4;
         }
 
-        void set_split(const hyt_run_options &o)
+        void set_split(const hyt_run_options &o, std::uint32_t light_default = kHybridLightMaxDegree,
+                       std::uint32_t chunk_default = kHybridHeavyChunk)
         {
-            light_max = o.light_max_degree ? o.light_max_degree : kHybridLightMaxDegree;
-            chunk = o.heavy_chunk ? o.heavy_chunk : kHybridHeavyChunk;
+            light_max = o.light_max_degree ? o.light_max_degree : light_default;
+            chunk = o.heavy_chunk ? o.heavy_chunk : chunk_default;
             require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
             require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
             ensure_workspace(dg, light_max, chunk);
@@ -324,61 +350,95 @@ namespace hyt
         {
             Resident &r = res[p];
             if (r.col >= 0)
-                free_units.push_back(static_cast<std::uint32_t>(r.col));
+                owner[static_cast<std::size_t>(r.col)] = -1;
             if (r.w >= 0)
-                free_units.push_back(static_cast<std::uint32_t>(r.w));
-            if (r.pos >= 0)
-            {
-                const std::uint64_t last = resident.back();
-                resident[static_cast<std::size_t>(r.pos)] = last;
-                res[last].pos = r.pos;
-                resident.pop_back();
-            }
+                owner[static_cast<std::size_t>(r.w)] = -1;
             r = Resident{};
         }
 
-        // Finds units for partition p (this round's stamp protects the active
-        // partitions); false when every resident partition is active now.
+        // Finds units for partition p; partitions active this round (this
+        // round's stamp) keep theirs. False when nothing can be freed.
         bool admit(std::uint64_t p)
         {
-            const std::size_t need = need_w ? 2 : 1;
             if (res[p].col >= 0)
                 drop(p); // cached without the weights this run needs
-            // a run that reads no weights first takes back resident weight units
-            if (!need_w)
-                for (std::size_t i = 0; i < resident.size() && free_units.size() < need; ++i)
-                {
-                    Resident &o = res[resident[i]];
-                    if (o.w >= 0)
-                    {
-                        free_units.push_back(static_cast<std::uint32_t>(o.w));
-                        o.w = -1;
-                    }
-                }
-            while (free_units.size() < need)
-            {
-                std::int64_t victim = -1;
-                std::uint64_t oldest = stamp;
-                for (std::uint64_t v : resident)
-                    if (last_active[v] < oldest)
-                    {
-                        oldest = last_active[v];
-                        victim = static_cast<std::int64_t>(v);
-                    }
-                if (victim < 0)
-                    return false;
-                drop(static_cast<std::uint64_t>(victim));
-            }
             Resident &r = res[p];
-            r.col = free_units.back();
-            free_units.pop_back();
             if (need_w)
             {
-                r.w = free_units.back();
-                free_units.pop_back();
+                // the pair whose holders were active least recently (free first)
+                std::int64_t best = -1;
+                std::uint64_t best_key = ~0ULL;
+                for (std::uint64_t j = 0; j < pairs && best_key != 0; ++j)
+                {
+                    std::uint64_t key = 0;
+                    bool ok = true;
+                    for (const std::int64_t o : {owner[j], owner[j + pairs]})
+                    {
+                        if (o < 0)
+                            continue;
+                        if (last_active[static_cast<std::size_t>(o)] >= stamp)
+                        {
+                            ok = false;
+                            break;
+                        }
+                        key = std::max<std::uint64_t>(key, last_active[static_cast<std::size_t>(o)] + 1);
+                    }
+                    if (ok && key < best_key)
+                    {
+                        best_key = key;
+                        best = static_cast<std::int64_t>(j);
+                    }
+                }
+                if (best < 0)
+                    return false;
+                const std::size_t j = static_cast<std::size_t>(best);
+                for (const std::size_t k : {j, j + pairs})
+                    if (owner[k] >= 0)
+                        drop(static_cast<std::uint64_t>(owner[k]));
+                r.col = best;
+                r.w = best + static_cast<std::int64_t>(pairs);
+                owner[j] = owner[j + pairs] = static_cast<std::int64_t>(p);
+                return true;
             }
-            r.pos = static_cast<std::int64_t>(resident.size());
-            resident.push_back(p);
+            // columns only: a free unit, else a resident's weight unit (it keeps
+            // its columns), else the column unit of the least recently active
+            std::int64_t best = -1;
+            std::uint64_t best_key = ~0ULL;
+            for (std::uint64_t k = 0; k < units && best_key != 0; ++k)
+            {
+                const std::int64_t o = owner[k];
+                std::uint64_t key;
+                if (o < 0)
+                    key = 0;
+                else if (res[static_cast<std::size_t>(o)].w == static_cast<std::int64_t>(k))
+                    key = 1;
+                else if (last_active[static_cast<std::size_t>(o)] < stamp)
+                    key = 2 + last_active[static_cast<std::size_t>(o)];
+                else
+                    continue;
+                if (key < best_key)
+                {
+                    best_key = key;
+                    best = static_cast<std::int64_t>(k);
+                }
+            }
+            if (best < 0)
+                return false;
+            const std::size_t k = static_cast<std::size_t>(best);
+            const std::int64_t o = owner[k];
+            if (o >= 0)
+            {
+                Resident &ro = res[static_cast<std::size_t>(o)];
+                if (ro.w == best)
+                {
+                    ro.w = -1;
+                    owner[k] = -1;
+                }
+                else
+                    drop(static_cast<std::uint64_t>(o));
+            }
+            r.col = best;
+            owner[k] = static_cast<std::int64_t>(p);
             return true;
         }
     };
@@ -568,11 +628,12 @@ namespace hyt
             for (hipEvent_t &ev : E.ring_used)
                 hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
             E.units = (avail - ring_k * ring_unit) / (E.unit_edges * 4);
+            E.pairs = host->weighted ? E.units / 2 : 0;
             if (E.units > 0)
                 E.unit_mem.alloc(E.units * E.unit_edges * 4, "hipMalloc(residency units)");
-            E.free_units.clear();
-            for (std::uint64_t u = E.units; u-- > 0;)
-                E.free_units.push_back(static_cast<std::uint32_t>(u));
+            E.owner.assign(E.units, -1);
+            E.seg_h.resize(P);
+            E.seg_d.alloc(P * sizeof(CacheSeg), "hipMalloc(cache segments)");
             E.pk_rows.alloc(E.pack_cap * 4 + 16, "hipMalloc(packed rows)");
             E.pk_index.alloc((E.pack_cap + 1) * 8 + 16, "hipMalloc(packed index)");
             E.pk_col.alloc(E.pack_cap * 4 + 16, "hipMalloc(packed columns)");
@@ -721,15 +782,63 @@ namespace hyt
             hip_check(hipGetLastError(), "launch hybrid expand_heavy");
         }
 
-        // Op factory: Make(col_base, w_base, qd_base) -> Op
+        // Expander of single-query rounds. A round hands row sets to an
+        // expander X:
+        //   X.window(qa, F, lo, hi, col, w): queue rows [qa, qa + F), edges
+        //       clipped to [lo, hi), col/w address edge lo;
+        //   X.list(rows, vals, F, lo, hi, col, w): F rows of a device list with
+        //       their row values;
+        //   X.packed(rows, index, vals, F, col, w): rows packed on the host;
+        //   X.cached(seg, nseg, F, col, w): the rows of the resident
+        //       partitions (CacheSeg table; col/w address the unit area).
+        // X::kIndexValues: the row values of listed and packed rows are queue
+        // indices (K-lane rounds) instead of the rows' frontier values.
         template <class OffT, class Make>
-        void execute_round(HybridEngine &E, const Plan &plan, Make make, hyt_hybrid_stats &st)
+        struct SingleExpander
+        {
+            static constexpr bool kIndexValues = false;
+            HybridEngine &E;
+            Make make; // Make(col_base, w_base, value_base) -> Op
+            const OffT *off;
+            const std::uint32_t *q, *qd;
+
+            void window(std::uint64_t qa, std::uint32_t F, std::uint64_t lo, std::uint64_t hi,
+                        const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                expand(E, GraphRows<OffT>{off, q + qa, lo, hi}, make(col, w, qd + qa), F);
+            }
+            void list(const std::uint32_t *rows, const std::uint32_t *vals, std::uint32_t F, std::uint64_t lo,
+                      std::uint64_t hi, const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                expand(E, GraphRows<OffT>{off, rows, lo, hi}, make(col, w, vals), F);
+            }
+            void packed(const std::uint32_t *rows, const std::uint64_t *index, const std::uint32_t *vals,
+                        std::uint32_t F, const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                expand(E, PackedRows{rows, index}, make(col, w, vals), F);
+            }
+            void cached(const CacheSeg *seg, std::uint32_t nseg, std::uint32_t F, const std::uint32_t *col,
+                        const std::uint32_t *w) const
+            {
+                expand(E, CachedRows<OffT>{off, q, seg, nseg}, make(col, w, qd), F);
+            }
+        };
+
+        template <class OffT, class Make>
+        SingleExpander<OffT, Make> single_expander(HybridEngine &E, Make make)
+        {
+            return SingleExpander<OffT, Make>{E, make, E.dg.off.template as<OffT>(),
+                                              E.dg.ws.q.template as<std::uint32_t>(),
+                                              E.dg.ws.qd.template as<std::uint32_t>()};
+        }
+
+        template <class OffT, class X>
+        void execute_round(HybridEngine &E, const Plan &plan, const X &x, hyt_hybrid_stats &st)
         {
             DeviceGraph &g = E.dg;
             hipStream_t s = g.stream;
             const std::uint32_t *q = g.ws.q.as<std::uint32_t>();
             const std::uint32_t *qd = g.ws.qd.as<std::uint32_t>();
-            const OffT *off = g.off.as<OffT>();
             const bool weighted = E.need_w;
 
             // maximal runs [p0, p1] of consecutive partition ids in a member list
@@ -775,22 +884,40 @@ namespace hyt
                             const std::uint64_t n = rows_of(p0, p1);
                             hip_check(hipMemcpyAsync(E.h_q + at, q + plan.qlo[p0], n * 4, hipMemcpyDeviceToHost, s),
                                       "read frontier");
-                            hip_check(hipMemcpyAsync(E.h_qd + at, qd + plan.qlo[p0], n * 4, hipMemcpyDeviceToHost, s),
-                                      "read frontier values");
+                            if constexpr (X::kIndexValues)
+                                for (std::uint64_t j = 0; j < n; ++j)
+                                    E.h_qd[at + j] = static_cast<std::uint32_t>(plan.qlo[p0] + j);
+                            else
+                                hip_check(hipMemcpyAsync(E.h_qd + at, qd + plan.qlo[p0], n * 4,
+                                                         hipMemcpyDeviceToHost, s),
+                                          "read frontier values");
                             at += n;
                         });
                 hip_check(hipStreamSynchronize(s), "frontier readback");
             }
 
-            // 2. resident partitions
-            for (std::uint64_t p : plan.cached)
+            // 2. resident partitions: one launch over all of them (segment table)
+            if (!plan.cached.empty())
             {
-                const hyt_partition &pt = E.parts[p];
-                const Resident &r = E.res[p];
-                GraphRows<OffT> rs{off, q + plan.qlo[p], pt.edge_begin, pt.edge_end};
-                expand(E, rs, make(E.unit_ptr(r.col), weighted ? E.unit_ptr(r.w) : nullptr, qd + plan.qlo[p]),
-                       static_cast<std::uint32_t>(plan.qn[p]));
-                st.parts_cached += 1;
+                std::uint64_t rows = 0;
+                for (std::size_t k = 0; k < plan.cached.size(); ++k)
+                {
+                    const std::uint64_t p = plan.cached[k];
+                    CacheSeg &sg = E.seg_h[k];
+                    sg.qlo = static_cast<std::uint32_t>(plan.qlo[p]);
+                    sg.rows_before = static_cast<std::uint32_t>(rows);
+                    sg.edge_begin = E.parts[p].edge_begin;
+                    sg.unit_pos = static_cast<std::uint64_t>(E.res[p].col) * E.unit_edges;
+                    rows += plan.qn[p];
+                }
+                // seg_h is rewritten only after the round's final synchronisation
+                hip_check(hipMemcpyAsync(E.seg_d.ptr, E.seg_h.data(), plan.cached.size() * sizeof(CacheSeg),
+                                         hipMemcpyHostToDevice, s),
+                          "upload cache segments");
+                x.cached(E.seg_d.as<CacheSeg>(), static_cast<std::uint32_t>(plan.cached.size()),
+                         static_cast<std::uint32_t>(rows), E.unit_mem.as<std::uint32_t>(),
+                         weighted ? E.unit_ptr(static_cast<std::int64_t>(E.pairs)) : nullptr);
+                st.parts_cached += plan.cached.size();
             }
 
             // 3. filter: each run of consecutive partitions streams through the
@@ -818,9 +945,7 @@ namespace hyt
                             E.copy_h2d(E.ring_w(r), E.host->w + a, (b - a) * 4);
                         st.bytes_filter += (b - a) * E.run_edge_bytes;
                         hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait copy");
-                        const std::uint64_t qa = plan.qlo[pa];
-                        GraphRows<OffT> rs{off, q + qa, a, b};
-                        expand(E, rs, make(E.ring_col(r), E.ring_w(r), qd + qa), rows_of(pa, pb));
+                        x.window(plan.qlo[pa], rows_of(pa, pb), a, b, E.ring_col(r), E.ring_w(r));
                         if (E.cfg.cache)
                             for (std::uint64_t pp = pa; pp <= pb; ++pp)
                             {
@@ -853,9 +978,8 @@ namespace hyt
                     const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
                     for (std::uint64_t p = p0; p <= p1; ++p) // estimated link bytes: m per modelled request
                         st.bytes_zero_copy += E.part_stats_h[4 * p + 2] * E.run_params.request_bytes;
-                    GraphRows<OffT> rs{off, q + plan.qlo[p0], e0, e1};
-                    expand(E, rs, make(E.z_col + e0, weighted ? E.z_w + e0 : nullptr, qd + plan.qlo[p0]),
-                           rows_of(p0, p1));
+                    x.window(plan.qlo[p0], rows_of(p0, p1), e0, e1, E.z_col + e0,
+                             weighted ? E.z_w + e0 : nullptr);
                     st.parts_zero_copy += p1 - p0 + 1;
                 });
             }
@@ -883,8 +1007,8 @@ namespace hyt
                         E.copy_h2d(E.pk_rows.ptr, E.h_rows, 4);
                         E.copy_h2d(E.pk_val.ptr, E.h_val, 4);
                         hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait row copy");
-                        GraphRows<OffT> rs{off, E.pk_rows.as<std::uint32_t>(), 0, ~0ULL};
-                        expand(E, rs, make(E.z_col, weighted ? E.z_w : nullptr, E.pk_val.as<std::uint32_t>()), 1U);
+                        x.list(E.pk_rows.as<std::uint32_t>(), E.pk_val.as<std::uint32_t>(), 1U, 0, ~0ULL, E.z_col,
+                               weighted ? E.z_w : nullptr);
                         hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
                         st.bytes_zero_copy += d * E.run_edge_bytes;
                         ++r0;
@@ -936,10 +1060,9 @@ namespace hyt
                     E.copy_h2d(E.pk_w.ptr, E.h_w, edges * 4);
                 st.bytes_compaction += nr * 4 + (nr + 1) * 8 + edges * E.run_edge_bytes;
                 hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait pack copy");
-                PackedRows rs{E.pk_rows.as<std::uint32_t>(), E.pk_index.as<std::uint64_t>()};
-                expand(E, rs, make(E.pk_col.as<std::uint32_t>(), weighted ? E.pk_w.as<std::uint32_t>() : nullptr,
-                                   E.pk_val.as<std::uint32_t>()),
-                       static_cast<std::uint32_t>(nr));
+                x.packed(E.pk_rows.as<std::uint32_t>(), E.pk_index.as<std::uint64_t>(), E.pk_val.as<std::uint32_t>(),
+                         static_cast<std::uint32_t>(nr), E.pk_col.as<std::uint32_t>(),
+                         weighted ? E.pk_w.as<std::uint32_t>() : nullptr);
                 hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
                 r0 = r1;
             }
@@ -987,7 +1110,7 @@ namespace hyt
                 hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                execute_round<OffT>(E, plan, make, st);
+                execute_round<OffT>(E, plan, single_expander<OffT>(E, make), st);
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
                 Counters cc;
@@ -1053,7 +1176,7 @@ namespace hyt
                 hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                execute_round<OffT>(E, plan, make, st);
+                execute_round<OffT>(E, plan, single_expander<OffT>(E, make), st);
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
                 Counters cc;
@@ -1078,7 +1201,359 @@ namespace hyt
             }
             finalize_ranks(g, out, o.out_on_device != 0, st.run);
         }
+
+        // ------------------------------------------------------ K-lane runs
+        // K queries per run with [V][K] state (the kernels of lanes.hpp): the
+        // frontier is the sorted queue of rows with their query masks, so the
+        // planner sees each row once whatever its mask, and every partition
+        // that moves (or is read zero-copy) is walked once for all K queries.
+
+        // Expander of K-lane rounds: row values are queue indices, from which
+        // the kernels take the row's mask and PPR snapshot row.
+        template <class OffT, int KP, class Op>
+        struct LaneExpander
+        {
+            static constexpr bool kIndexValues = true;
+            HybridEngine &E;
+            Op op;
+            lanes::KArgs a;
+            bool items; // item form this round
+
+            static void set_edges(Op &o, const std::uint32_t *col, const std::uint32_t *w)
+            {
+                o.col = col;
+                if constexpr (!std::is_same_v<Op, lanes::PprKOp>)
+                    o.w = w;
+                else
+                    (void)w;
+            }
+            template <class RowSrc>
+            void run(const RowSrc &rows, const std::uint32_t *col, const std::uint32_t *w, std::uint32_t F) const
+            {
+                if (F == 0)
+                    return;
+                hipStream_t s = E.dg.stream;
+                Op o = op;
+                set_edges(o, col, w);
+                lanes::KArgs k = a;
+                k.F = F;
+                hip_check(hipMemsetAsync(&k.ctr->heavy_count, 0, 4, s), "reset heavy count");
+                if (items)
+                {
+                    lanes::expandk_light_items<RowSrc, Op><<<blocks_for(F), kBlock, 0, s>>>(rows, o, k);
+                    hip_check(hipGetLastError(), "launch hybrid expandk_light_items");
+                    lanes::expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(o, k);
+                    hip_check(hipGetLastError(), "launch hybrid expandk_heavy_items");
+                }
+                else
+                {
+                    lanes::expandk_light<RowSrc, KP, Op><<<blocks_for(F), kBlock, 0, s>>>(rows, o, k);
+                    hip_check(hipGetLastError(), "launch hybrid expandk_light");
+                    lanes::expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(o, k);
+                    hip_check(hipGetLastError(), "launch hybrid expandk_heavy");
+                }
+            }
+            lanes::WindowRowsK<OffT> queue_rows() const
+            {
+                lanes::WindowRowsK<OffT> r{};
+                r.off = E.dg.off.template as<OffT>();
+                r.q = E.dg.ws.q.template as<std::uint32_t>();
+                r.qmask = E.lk.qmask.template as<unsigned long long>();
+                return r;
+            }
+            void window(std::uint64_t qa, std::uint32_t F, std::uint64_t lo, std::uint64_t hi,
+                        const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                lanes::WindowRowsK<OffT> r = queue_rows();
+                r.qa = static_cast<std::uint32_t>(qa);
+                r.lo = lo;
+                r.hi = hi;
+                run(r, col, w, F);
+            }
+            void list(const std::uint32_t *rows, const std::uint32_t *vals, std::uint32_t F, std::uint64_t lo,
+                      std::uint64_t hi, const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                lanes::WindowRowsK<OffT> r = queue_rows();
+                r.rows = rows;
+                r.idx = vals;
+                r.lo = lo;
+                r.hi = hi;
+                run(r, col, w, F);
+            }
+            void packed(const std::uint32_t *rows, const std::uint64_t *index, const std::uint32_t *vals,
+                        std::uint32_t F, const std::uint32_t *col, const std::uint32_t *w) const
+            {
+                run(lanes::PackedRowsK{rows, vals, index, E.lk.qmask.template as<unsigned long long>()}, col, w, F);
+            }
+            void cached(const CacheSeg *seg, std::uint32_t nseg, std::uint32_t F, const std::uint32_t *col,
+                        const std::uint32_t *w) const
+            {
+                run(lanes::CachedRowsK<OffT>{E.dg.off.template as<OffT>(), E.dg.ws.q.template as<std::uint32_t>(),
+                                             E.lk.qmask.template as<unsigned long long>(), seg, nseg},
+                    col, w, F);
+            }
+        };
+
+        void prepare_lanes(HybridEngine &E, std::uint32_t K, bool ppr)
+        {
+            DeviceGraph &g = E.dg;
+            LaneState &L = E.lk;
+            const std::uint64_t VK = g.V * K;
+            L.val.ensure(VK * 4, "hipMalloc([V][K] state)");
+            if (ppr)
+            {
+                L.rank.ensure(VK * 4, "hipMalloc([V][K] rank)");
+                L.snap.ensure(VK * 4, "hipMalloc([V][K] snapshot)");
+            }
+            if (L.mark.bytes < (g.V + kPad) * 8)
+            {
+                // mark stays all-zero between runs (the compaction clears what it reads)
+                L.mark.alloc((g.V + kPad) * 8, "hipMalloc(lane marks)");
+                L.qmask.alloc((g.V + kPad) * 8, "hipMalloc(queue masks)");
+                hip_check(hipMemsetAsync(L.mark.ptr, 0, (g.V + kPad) * 8, g.stream), "clear marks");
+            }
+            L.stage_mask.ensure(static_cast<std::uint64_t>(compact_blocks(g.V)) * kCompactRowsPerBlock * 8,
+                                "hipMalloc(stage masks)");
+            const std::uint64_t cap = lanes::lane_chunk_capacity(g.V, g.E, E.light_max, E.chunk);
+            if (L.heavy.bytes < cap * sizeof(lanes::HeavyChunkK))
+                L.heavy.alloc(cap * sizeof(lanes::HeavyChunkK), "hipMalloc(heavy chunks)");
+            L.heavy_cap = L.heavy.bytes / sizeof(lanes::HeavyChunkK);
+            L.seeds.ensure(64 * 4 * 4, "hipMalloc(seeds)");
+            L.lanes.ensure(8, "hipMalloc(lane mask)");
+            L.partial.ensure((1024 + 1) * 64 * sizeof(double), "hipMalloc(partials)");
+        }
+
+        // Mark compaction in vertex order: rows with a non-zero mark -> sorted
+        // queue + masks; ctr->frontier_edges = (row, query) pairs.
+        std::uint32_t ordered_compaction_k(HybridEngine &E, Counters &out)
+        {
+            DeviceGraph &g = E.dg;
+            hipStream_t s = g.stream;
+            Counters *d_ctr = g.ws.ctr.as<Counters>();
+            const unsigned nb = compact_blocks(g.V);
+            lanes::MarkSel sel{E.lk.mark.as<unsigned long long>(), E.lk.lanes.as<unsigned long long>()};
+            compact_rows<unsigned long long, std::uint32_t, lanes::MarkSel><<<nb, kBlock, 0, s>>>(
+                g.V, sel, E.stage_q.as<std::uint32_t>(), E.lk.stage_mask.as<unsigned long long>(), d_ctr, nullptr,
+                nullptr, E.block_counts.as<std::uint32_t>());
+            hip_check(hipGetLastError(), "launch ordered mark compaction");
+            std::size_t temp = E.scan_temp.bytes;
+            hip_check(hipcub::DeviceScan::ExclusiveSum(E.scan_temp.ptr, temp, E.block_counts.as<std::uint32_t>(),
+                                                       E.block_offsets.as<std::uint32_t>(), nb + 1, s),
+                      "scan block counts");
+            gather_ordered<unsigned long long><<<nb, kBlock, 0, s>>>(
+                E.stage_q.as<std::uint32_t>(), E.lk.stage_mask.as<unsigned long long>(),
+                E.block_counts.as<std::uint32_t>(), E.block_offsets.as<std::uint32_t>(), nb,
+                g.ws.q.as<std::uint32_t>(), E.lk.qmask.as<unsigned long long>(), d_ctr);
+            hip_check(hipGetLastError(), "launch gather");
+            Counters *h = g.host_ctr->as<Counters>();
+            hip_check(hipMemcpyAsync(h, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read counters");
+            hip_check(hipStreamSynchronize(s), "hybrid K round");
+            out = *h;
+            return h->next_count;
+        }
+
+        template <class OffT, int KP, class Op>
+        void hybrid_k_rounds(HybridEngine &E, const Op &op, std::uint32_t K, bool ppr, const hyt_run_options &o,
+                             hyt_hybrid_stats &st)
+        {
+            DeviceGraph &g = E.dg;
+            LaneState &L = E.lk;
+            hipStream_t s = g.stream;
+            Counters *d_ctr = g.ws.ctr.as<Counters>();
+            hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
+            Counters c;
+            std::uint32_t F = ordered_compaction_k(E, c);
+            std::uint64_t pairs = c.frontier_edges;
+            lanes::KArgs a{};
+            a.rs = nullptr;
+            a.ctr = d_ctr;
+            a.heavy = L.heavy.as<lanes::HeavyChunkK>();
+            a.heavy_cap = L.heavy_cap;
+            a.light_max = E.light_max;
+            a.chunk = E.chunk;
+            a.K = K;
+            a.mark = L.mark.as<unsigned long long>();
+            const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : (ppr ? 100000 : g.V);
+            Plan plan;
+            st.run.converged = F == 0 ? 1 : 0;
+            for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
+            {
+                plan_round<OffT>(E, F, plan, st);
+                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
+                if (ppr)
+                {
+                    lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
+                        g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
+                        reinterpret_cast<float *>(L.val.ptr), L.rank.as<float>(), L.snap.as<float>());
+                    hip_check(hipGetLastError(), "launch pprk_snapshot");
+                }
+                const LaneExpander<OffT, KP, Op> x{E, op, a, lanes::use_items(o.lane_form, pairs, F, KP)};
+                execute_round<OffT>(E, plan, x, st);
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
+                Counters cc;
+                const std::uint32_t Fn = ordered_compaction_k(E, cc);
+                finish_round_stats(E, st);
+                if (o.time_kernels)
+                {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, g.ev0, g.ev1), "hipEventElapsedTime");
+                    st.run.expand_ms += ms;
+                    st.run.expand_launches += 1;
+                }
+                if (cc.heavy_overflow)
+                    fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
+                if (cc.overflow)
+                {
+                    st.run.overflow = 1;
+                    fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
+                }
+                st.run.frontier_total += F;
+                st.run.edges_total += cc.edges;
+                st.run.updates_total += Fn;
+                st.run.rounds = round + 1;
+                F = Fn;
+                pairs = cc.frontier_edges;
+                if (F == 0)
+                    st.run.converged = 1;
+            }
+        }
+
+        template <class OffT, class Op>
+        void hybrid_k_dispatch(HybridEngine &E, const Op &op, std::uint32_t K, bool ppr, const hyt_run_options &o,
+                               hyt_hybrid_stats &st)
+        {
+            switch (lanes::pow2_at_least(K))
+            {
+            case 2: hybrid_k_rounds<OffT, 2>(E, op, K, ppr, o, st); break;
+            case 4: hybrid_k_rounds<OffT, 4>(E, op, K, ppr, o, st); break;
+            case 8: hybrid_k_rounds<OffT, 8>(E, op, K, ppr, o, st); break;
+            case 16: hybrid_k_rounds<OffT, 16>(E, op, K, ppr, o, st); break;
+            case 32: hybrid_k_rounds<OffT, 32>(E, op, K, ppr, o, st); break;
+            default: hybrid_k_rounds<OffT, 64>(E, op, K, ppr, o, st); break;
+            }
+        }
+
+        template <class Op>
+        void hybrid_k_run(HybridEngine &E, const Op &op, std::uint32_t K, bool ppr, const hyt_run_options &o,
+                          hyt_hybrid_stats &st)
+        {
+            if (E.dg.off64)
+                hybrid_k_dispatch<std::uint64_t>(E, op, K, ppr, o, st);
+            else
+                hybrid_k_dispatch<std::uint32_t>(E, op, K, ppr, o, st);
+        }
+
+        void hybrid_sssp_k(HybridEngine &E, const std::uint32_t *sources, std::uint32_t K, std::uint32_t *dist_out,
+                           const hyt_run_options &o, hyt_hybrid_stats &st)
+        {
+            DeviceGraph &g = E.dg;
+            LaneState &L = E.lk;
+            prepare_lanes(E, K, false);
+            const int wm = g.weight_mode();
+            const std::uint32_t inf = (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
+            launch_fill<std::uint32_t>(L.val.as<std::uint32_t>(), g.V * K, inf, g.stream);
+            const lanes::SeedPlan p = lanes::plan_seeds(sources, K);
+            lanes::upload_seeds(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, sources, K,
+                                L.val.as<std::uint32_t>(), 0U);
+            auto run = [&](auto op) {
+                op.dist = L.val.as<std::uint32_t>();
+                op.K = K;
+                op.overflow = &g.ws.ctr.as<Counters>()->overflow;
+                hybrid_k_run(E, op, K, false, o, st);
+            };
+            switch (wm)
+            {
+            case kWU32: run(lanes::SsspKOp<kWU32>{}); break;
+            case kWF32: run(lanes::SsspKOp<kWF32>{}); break;
+            case kWUnitU32: run(lanes::SsspKOp<kWUnitU32>{}); break;
+            default: run(lanes::SsspKOp<kWUnitF32>{}); break;
+            }
+            hip_check(hipMemcpyAsync(dist_out, L.val.ptr, g.V * K * 4,
+                                     o.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g.stream),
+                      "copy distances");
+            hip_check(hipStreamSynchronize(g.stream), "hybrid sssp output");
+        }
+
+        void hybrid_ppr_k(HybridEngine &E, const std::uint32_t *seeds, std::uint32_t K, float damping, float eps,
+                          float *out, const hyt_run_options &o, hyt_hybrid_stats &st)
+        {
+            DeviceGraph &g = E.dg;
+            LaneState &L = E.lk;
+            prepare_lanes(E, K, true);
+            const std::uint64_t VK = g.V * K;
+            launch_fill<float>(reinterpret_cast<float *>(L.val.ptr), VK, 0.0F, g.stream);
+            launch_fill<float>(L.rank.as<float>(), VK, 0.0F, g.stream);
+            const lanes::SeedPlan p = lanes::plan_seeds(seeds, K);
+            const float r0 = 1.0F - damping;
+            std::uint32_t r0_bits;
+            std::memcpy(&r0_bits, &r0, 4);
+            lanes::upload_seeds(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, seeds, K,
+                                L.val.as<std::uint32_t>(), r0_bits);
+            lanes::PprKOp op{};
+            op.res = reinterpret_cast<float *>(L.val.ptr);
+            op.snap = L.snap.as<float>();
+            op.K = K;
+            op.damping = damping;
+            op.eps = eps;
+            hybrid_k_run(E, op, K, true, o, st);
+            float *dst = o.out_on_device ? out : L.snap.as<float>();
+            lanes::pprk_finish(g.stream, L.rank.as<float>(), op.res, g.V, K, L.partial.as<double>(), dst);
+            if (!o.out_on_device)
+                hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
+            hip_check(hipStreamSynchronize(g.stream), "hybrid ppr output");
+        }
     } // namespace
+} // namespace hyt
+
+namespace hyt
+{
+    // Shared front of the K-lane entry points: checks, split, stats, and a
+    // clean mark array after an error (the next run expects it all zero).
+    template <class Body>
+    void hybrid_k_call(hyt_hybrid_t h, const uint32_t *ids, uint32_t K, bool weights, const hyt_run_options *opts,
+                       hyt_hybrid_stats *stats, Body body)
+    {
+        using namespace hyt;
+        require(h != nullptr && ids != nullptr, "null engine or sources");
+        require(K >= 1 && K <= 64, "K must be in [1, 64]");
+        HybridEngine &E = h->e;
+        for (uint32_t k = 0; k < K; ++k)
+            require(ids[k] < E.dg.V, "source vertex is out of range");
+        hyt_run_options o{};
+        if (opts)
+            o = *opts;
+        require(o.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");
+        hyt_hybrid_stats st{};
+        st.partitions = E.parts.size();
+        st.link_gbs_peak = E.link_gbs;
+        st.cpu_pack_gbs = E.pack_gbs;
+        st.zero_copy_gbs = E.zc_gbs;
+        hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
+        const auto t0 = std::chrono::steady_clock::now();
+        E.begin_run(weights);
+        E.set_split(o, kLaneLightMaxDegree, kLaneHeavyChunk);
+        try
+        {
+            body(E, o, st);
+        }
+        catch (...)
+        {
+            if (E.lk.mark.ptr)
+                (void)hipMemsetAsync(E.lk.mark.ptr, 0, E.lk.mark.bytes, E.dg.stream);
+            (void)hipStreamSynchronize(E.dg.stream);
+            if (stats)
+                *stats = st;
+            throw;
+        }
+        st.device_edge_bytes = E.edge_area_bytes();
+        st.device_state_bytes = E.state_bytes();
+        st.run.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (stats)
+            *stats = st;
+    }
 } // namespace hyt
 
 extern "C" {
@@ -1236,6 +1711,32 @@ int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ran
         st.run.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         if (stats)
             *stats = st;
+    });
+}
+
+int hyt_hybrid_sssp_k(hyt_hybrid_t h, const uint32_t *sources, uint32_t num_sources, uint32_t *dist,
+                      const hyt_run_options *opts, hyt_hybrid_stats *stats)
+{
+    return hyt::guarded("hyt_hybrid_sssp_k", [&] {
+        hyt::require(dist != nullptr, "null dist");
+        hyt::hybrid_k_call(h, sources, num_sources, true, opts, stats,
+                      [&](hyt::HybridEngine &E, const hyt_run_options &o, hyt_hybrid_stats &st) {
+                          hyt::hybrid_sssp_k(E, sources, num_sources, dist, o, st);
+                      });
+    });
+}
+
+int hyt_hybrid_ppr_k(hyt_hybrid_t h, const uint32_t *seeds, uint32_t num_seeds, float damping, float epsilon,
+                     float *out, const hyt_run_options *opts, hyt_hybrid_stats *stats)
+{
+    return hyt::guarded("hyt_hybrid_ppr_k", [&] {
+        hyt::require(out != nullptr, "null output");
+        hyt::require(damping >= 0.0F && damping < 1.0F, "PageRank damping_factor must be in [0, 1)");
+        hyt::require(epsilon > 0.0F && std::isfinite(epsilon), "PageRank epsilon must be finite and positive");
+        hyt::hybrid_k_call(h, seeds, num_seeds, false, opts, stats,
+                      [&](hyt::HybridEngine &E, const hyt_run_options &o, hyt_hybrid_stats &st) {
+                          hyt::hybrid_ppr_k(E, seeds, num_seeds, damping, epsilon, out, o, st);
+                      });
     });
 }
 

@@ -18,6 +18,10 @@
 #include "compact.hpp"
 #include "expand.hpp"
 
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
 namespace hyt
 {
     namespace lanes
@@ -50,8 +54,6 @@ namespace hyt
         struct KArgs
         {
             const RoundState *rs; // non-null: F and the form come from the device
-            const std::uint32_t *q;
-            const unsigned long long *qmask;
             std::uint32_t F;
             Counters *ctr;
             HeavyChunkK *heavy;
@@ -60,6 +62,113 @@ namespace hyt
             std::uint32_t chunk;
             std::uint32_t K;
             unsigned long long *mark;
+        };
+
+        // Row sources: row i of a launch names its vertex u, its queue index
+        // gi (the row of the PPR snapshot), its query mask, the edges to walk
+        // (begin, count; positions relative to the Op's column/weight base)
+        // and its full out-degree (the PPR share).
+        template <class OffT>
+        struct QueueRowsK // whole rows of the queue, positions absolute
+        {
+            using Pos = OffT;
+            const OffT *off;
+            const std::uint32_t *q;
+            const unsigned long long *qmask;
+
+            __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &gi, std::uint64_t &mask, Pos &begin,
+                                std::uint64_t &count, std::uint64_t &deg) const
+            {
+                gi = i;
+                u = q[i];
+                mask = qmask[i];
+                begin = off[u];
+                deg = count = static_cast<std::uint64_t>(off[u + 1] - begin);
+            }
+        };
+
+        // Queue rows [qa, qa + F), or (rows != nullptr) the rows of a list with
+        // their queue indices, clipped to the edge window [lo, hi): one
+        // partition in a device slot, a window of the streaming ring, or the
+        // mapped host graph.
+        template <class OffT>
+        struct WindowRowsK
+        {
+            using Pos = OffT;
+            const OffT *off;
+            const std::uint32_t *q;
+            const unsigned long long *qmask;
+            const std::uint32_t *rows; // optional row list
+            const std::uint32_t *idx;  // queue index of each listed row
+            std::uint32_t qa;
+            std::uint64_t lo, hi;
+
+            __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &gi, std::uint64_t &mask, Pos &begin,
+                                std::uint64_t &count, std::uint64_t &deg) const
+            {
+                if (rows != nullptr)
+                {
+                    u = rows[i];
+                    gi = idx[i];
+                }
+                else
+                {
+                    gi = qa + i;
+                    u = q[gi];
+                }
+                mask = qmask[gi];
+                const std::uint64_t b = off[u], e = off[u + 1];
+                deg = e - b;
+                const std::uint64_t cb = b > lo ? b : lo;
+                const std::uint64_t ce = e < hi ? e : hi;
+                count = ce > cb ? ce - cb : 0;
+                begin = static_cast<Pos>(count ? cb - lo : 0);
+            }
+        };
+
+        // Rows of many cached partitions (segment table, see CacheSeg).
+        template <class OffT>
+        struct CachedRowsK
+        {
+            using Pos = std::uint64_t;
+            const OffT *off;
+            const std::uint32_t *q;
+            const unsigned long long *qmask;
+            const CacheSeg *seg;
+            std::uint32_t nseg;
+
+            __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &gi, std::uint64_t &mask, Pos &begin,
+                                std::uint64_t &count, std::uint64_t &deg) const
+            {
+                const CacheSeg &sg = seg_of(seg, nseg, i);
+                gi = sg.qlo + (i - sg.rows_before);
+                u = q[gi];
+                mask = qmask[gi];
+                const std::uint64_t b = off[u];
+                deg = count = off[u + 1] - b;
+                begin = sg.unit_pos + (b - sg.edge_begin);
+            }
+        };
+
+        // Rows packed on the host: row i is vertex rows[i], queue index idx[i],
+        // edges [index[i], index[i+1]) of the packed arrays.
+        struct PackedRowsK
+        {
+            using Pos = std::uint64_t;
+            const std::uint32_t *rows;
+            const std::uint32_t *idx;
+            const std::uint64_t *index;
+            const unsigned long long *qmask;
+
+            __device__ void get(std::uint32_t i, std::uint32_t &u, std::uint32_t &gi, std::uint64_t &mask, Pos &begin,
+                                std::uint64_t &count, std::uint64_t &deg) const
+            {
+                u = rows[i];
+                gi = idx[i];
+                mask = qmask[gi];
+                begin = index[i];
+                deg = count = index[i + 1] - begin;
+            }
         };
 
         // Work split of item-form rows: a row's work is edges x c (its active
@@ -172,11 +281,12 @@ namespace hyt
             return a.rs->form == form;
         }
 
-        template <class OffT, int KP, class Op>
-        __global__ __launch_bounds__(kBlock) void expandk_light(const OffT *__restrict__ off, Op op, KArgs a)
+        template <class RowSrc, int KP, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_light(RowSrc rsrc, Op op, KArgs a)
         {
+            using Pos = typename RowSrc::Pos;
             __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
-            __shared__ OffT s_begin[kWavesPerBlock][kWave];
+            __shared__ Pos s_begin[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
@@ -194,27 +304,25 @@ namespace hyt
             {
                 const std::uint32_t i = tile * kBlock + threadIdx.x;
                 std::uint32_t deg = 0, nch = 0, row = 0, scale = 0;
-                std::uint64_t mask = 0;
-                OffT begin = 0, full = 0;
+                std::uint64_t mask = 0, full = 0;
+                Pos begin = 0;
                 if (i < F)
                 {
-                    const std::uint32_t u = a.q[i];
-                    begin = off[u];
-                    full = off[u + 1] - begin;
-                    mask = a.qmask[i];
-                    row = Op::kRowIsQueueIndex ? i : u;
-                    scale = op.row_scale(static_cast<std::uint64_t>(full));
+                    std::uint32_t u, gi;
+                    std::uint64_t rdeg;
+                    rsrc.get(i, u, gi, mask, begin, full, rdeg);
+                    row = Op::kRowIsQueueIndex ? gi : u;
+                    scale = op.row_scale(rdeg);
                     if (full <= a.light_max)
                         deg = static_cast<std::uint32_t>(full);
                     else
-                        nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
+                        nch = static_cast<std::uint32_t>((full + ce - 1) / ce);
                 }
                 const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t c) {
                     const std::uint64_t lo = static_cast<std::uint64_t>(c) * ce;
-                    const std::uint64_t full64 = static_cast<std::uint64_t>(full);
                     HeavyChunkK h;
                     h.row = row;
-                    h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
+                    h.count = static_cast<std::uint32_t>((full - lo < ce) ? (full - lo) : ce);
                     h.begin = static_cast<std::uint64_t>(begin) + lo;
                     h.mask = mask;
                     h.scale = scale;
@@ -225,7 +333,7 @@ namespace hyt
                 {
                     // chunk queue full: walk the row here (rows of 2^24 edges or
                     // more cannot be - 64 of them must fit a u32 - and are reported)
-                    if (static_cast<std::uint64_t>(full) >= (1ULL << 24))
+                    if (full >= (1ULL << 24))
                         atomicOr(&a.ctr->heavy_overflow, 1U);
                     else
                         deg = static_cast<std::uint32_t>(full);
@@ -356,11 +464,12 @@ namespace hyt
             }
         }
 
-        template <class OffT, class Op>
-        __global__ __launch_bounds__(kBlock) void expandk_light_items(const OffT *__restrict__ off, Op op, KArgs a)
+        template <class RowSrc, class Op>
+        __global__ __launch_bounds__(kBlock) void expandk_light_items(RowSrc rsrc, Op op, KArgs a)
         {
+            using Pos = typename RowSrc::Pos;
             __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
-            __shared__ OffT s_begin[kWavesPerBlock][kWave];
+            __shared__ Pos s_begin[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_c[kWavesPerBlock][kWave];
@@ -376,32 +485,30 @@ namespace hyt
             {
                 const std::uint32_t i = tile * kBlock + threadIdx.x;
                 std::uint32_t items = 0, deg = 0, nch = 0, row = 0, scale = 0, c = 0, ce = 16;
-                std::uint64_t mask = 0;
-                OffT begin = 0, full = 0;
+                std::uint64_t mask = 0, full = 0;
+                Pos begin = 0;
                 if (i < F)
                 {
-                    const std::uint32_t u = a.q[i];
-                    begin = off[u];
-                    full = off[u + 1] - begin;
-                    mask = a.qmask[i];
+                    std::uint32_t u, gi;
+                    std::uint64_t rdeg;
+                    rsrc.get(i, u, gi, mask, begin, full, rdeg);
                     c = static_cast<std::uint32_t>(__popcll(mask));
-                    row = Op::kRowIsQueueIndex ? i : u;
-                    scale = op.row_scale(static_cast<std::uint64_t>(full));
+                    row = Op::kRowIsQueueIndex ? gi : u;
+                    scale = op.row_scale(rdeg);
                     ce = chunk_edges(c, a);
-                    if (light_work(static_cast<std::uint64_t>(full), c, a))
+                    if (light_work(full, c, a))
                     {
                         deg = static_cast<std::uint32_t>(full);
                         items = deg * c;
                     }
                     else
-                        nch = static_cast<std::uint32_t>((static_cast<std::uint64_t>(full) + ce - 1) / ce);
+                        nch = static_cast<std::uint32_t>((full + ce - 1) / ce);
                 }
                 const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t ch) {
                     const std::uint64_t lo = static_cast<std::uint64_t>(ch) * ce;
-                    const std::uint64_t full64 = static_cast<std::uint64_t>(full);
                     HeavyChunkK h;
                     h.row = row;
-                    h.count = static_cast<std::uint32_t>((full64 - lo < ce) ? (full64 - lo) : ce);
+                    h.count = static_cast<std::uint32_t>((full - lo < ce) ? (full - lo) : ce);
                     h.begin = static_cast<std::uint64_t>(begin) + lo;
                     h.mask = mask;
                     h.scale = scale;
@@ -412,7 +519,7 @@ namespace hyt
                 {
                     // chunk queue full: walk the row here (work of 2^24 items or
                     // more cannot be - 64 rows' work must fit a u32 - and is reported)
-                    const std::uint64_t w = static_cast<std::uint64_t>(full) * c;
+                    const std::uint64_t w = full * c;
                     if (w >= (1ULL << 24))
                         atomicOr(&a.ctr->heavy_overflow, 1U);
                     else
@@ -610,6 +717,135 @@ namespace hyt
             while (kp < static_cast<int>(K))
                 kp <<= 1;
             return kp < 2 ? 2 : kp;
+        }
+
+        __global__ void seed_rows(std::uint32_t nrows, const std::uint32_t *rows, const unsigned long long *masks,
+                                  const std::uint32_t *lane_src, std::uint32_t K, unsigned long long *mark,
+                                  std::uint32_t *val, std::uint32_t seed_value)
+        {
+            const std::uint32_t t = threadIdx.x;
+            if (t < nrows)
+                mark[rows[t]] = masks[t];
+            if (t < K)
+                val[static_cast<std::uint64_t>(lane_src[t]) * K + t] = seed_value;
+        }
+
+        // Column sums of x = rank + res ([V][K]) into partial[block][K] (double).
+        __global__ __launch_bounds__(kBlock) void pprk_fold(float *rank, const float *res, std::uint64_t V,
+                                                           std::uint32_t K, std::uint32_t KP, double *partial)
+        {
+            __shared__ double s[kBlock];
+            const std::uint32_t rows_per = kBlock / KP;
+            const std::uint32_t k = threadIdx.x % KP;
+            const std::uint32_t r0 = threadIdx.x / KP;
+            double acc = 0.0;
+            if (k < K)
+            {
+                for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * rows_per + r0; v < V;
+                     v += static_cast<std::uint64_t>(gridDim.x) * rows_per)
+                {
+                    const std::uint64_t idx = v * K + k;
+                    const float x = rank[idx] + res[idx];
+                    rank[idx] = x;
+                    acc += x;
+                }
+            }
+            s[threadIdx.x] = acc;
+            __syncthreads();
+            if (threadIdx.x < KP && k < K)
+            {
+                double t = 0.0;
+                for (std::uint32_t r = 0; r < rows_per; ++r)
+                    t += s[r * KP + threadIdx.x];
+                partial[static_cast<std::uint64_t>(blockIdx.x) * 64 + threadIdx.x] = t;
+            }
+        }
+
+        __global__ void pprk_total(double *partial, unsigned nblocks, std::uint32_t K, double *total)
+        {
+            const std::uint32_t k = threadIdx.x;
+            if (k >= K)
+                return;
+            double t = 0.0;
+            for (unsigned b = 0; b < nblocks; ++b)
+                t += partial[static_cast<std::uint64_t>(b) * 64 + k];
+            total[k] = t;
+        }
+
+        __global__ void pprk_normalize(const float *x, std::uint64_t n, std::uint32_t K, const double *total,
+                                       float *out)
+        {
+            std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (; i < n; i += stride)
+            {
+                const double t = total[i % K];
+                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
+            }
+        }
+
+        struct SeedPlan
+        {
+            std::vector<std::uint32_t> rows;
+            std::vector<unsigned long long> masks;
+        };
+
+        SeedPlan plan_seeds(const std::uint32_t *src, std::uint32_t K)
+        {
+            SeedPlan p;
+            for (std::uint32_t k = 0; k < K; ++k)
+            {
+                auto it = std::find(p.rows.begin(), p.rows.end(), src[k]);
+                if (it == p.rows.end())
+                {
+                    p.rows.push_back(src[k]);
+                    p.masks.push_back(1ULL << k);
+                }
+                else
+                    p.masks[static_cast<std::size_t>(it - p.rows.begin())] |= 1ULL << k;
+            }
+            return p;
+        }
+
+        // Seeds a batch: the seeds buffer (at least 1 KiB) receives rows u32[64] |
+        // masks u64[64] | lane sources u32[64]; mark[row] takes each row's lane
+        // mask and val[src_k][k] the seed value.
+        inline void upload_seeds(hipStream_t s, DeviceBuffer &seeds, unsigned long long *mark, const SeedPlan &p,
+                                 const std::uint32_t *src, std::uint32_t K, std::uint32_t *val,
+                                 std::uint32_t seed_value)
+        {
+            std::vector<unsigned char> blob(64 * 4 + 64 * 8 + 64 * 4, 0);
+            std::memcpy(blob.data(), p.rows.data(), p.rows.size() * 4);
+            std::memcpy(blob.data() + 256, p.masks.data(), p.masks.size() * 8);
+            std::memcpy(blob.data() + 768, src, K * 4);
+            unsigned char *d = seeds.as<unsigned char>();
+            hip_check(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s), "upload seeds");
+            seed_rows<<<1, 64, 0, s>>>(static_cast<std::uint32_t>(p.rows.size()),
+                                       reinterpret_cast<const std::uint32_t *>(d),
+                                       reinterpret_cast<const unsigned long long *>(d + 256),
+                                       reinterpret_cast<const std::uint32_t *>(d + 768), K, mark, val, seed_value);
+            hip_check(hipGetLastError(), "launch seed_rows");
+            hip_check(hipStreamSynchronize(s), "seed upload");
+        }
+
+        // PPR-K result: x = rank + res per (vertex, query), normalised per query
+        // column into dst (device); partial holds (1024 + 1) x 64 doubles.
+        inline void pprk_finish(hipStream_t s, float *rank, const float *res, std::uint64_t V, std::uint32_t K,
+                                double *partial, float *dst)
+        {
+            const std::uint32_t KP = static_cast<std::uint32_t>(pow2_at_least(K));
+            const std::uint64_t rows_per = kBlock / KP;
+            const std::uint64_t nb64 = (V + rows_per - 1) / rows_per;
+            const unsigned nb = static_cast<unsigned>(nb64 < 1024 ? (nb64 ? nb64 : 1) : 1024);
+            pprk_fold<<<nb, kBlock, 0, s>>>(rank, res, V, K, KP, partial);
+            hip_check(hipGetLastError(), "launch pprk_fold");
+            double *total = partial + static_cast<std::uint64_t>(1024) * 64;
+            pprk_total<<<1, 64, 0, s>>>(partial, nb, K, total);
+            hip_check(hipGetLastError(), "launch pprk_total");
+            const std::uint64_t VK = V * K;
+            const std::uint64_t b = (VK + kBlock - 1) / kBlock;
+            pprk_normalize<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, s>>>(rank, VK, K, total, dst);
+            hip_check(hipGetLastError(), "launch pprk_normalize");
         }
 
         // Rows with a non-zero mark become the next queue; the mark moves into

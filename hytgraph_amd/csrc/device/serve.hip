@@ -310,8 +310,6 @@ namespace hyt
             hip_check(hipMemsetAsync(S.lanes.ptr, 0, 8, s), "reset lane mask");
             KArgs a;
             a.rs = nullptr;
-            a.q = S.q.as<std::uint32_t>();
-            a.qmask = S.qmask.as<unsigned long long>();
             a.F = S.F;
             a.ctr = d_ctr;
             a.heavy = S.heavy.as<HeavyChunkK>();
@@ -320,17 +318,18 @@ namespace hyt
             a.chunk = kLaneHeavyChunk;
             a.K = S.K;
             a.mark = S.mark.as<unsigned long long>();
+            const QueueRowsK<OffT> rows{g.off.as<OffT>(), S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>()};
             if (S.kind == HYT_QUERY_PPR)
             {
                 const std::uint64_t n = static_cast<std::uint64_t>(S.F) * S.K;
                 pprk_snapshot<<<snapshot_blocks(n), kBlock, 0, s>>>(
-                    a.q, S.F, S.K, a.qmask, reinterpret_cast<float *>(S.val.ptr), S.rank.as<float>(),
+                    rows.q, S.F, S.K, rows.qmask, reinterpret_cast<float *>(S.val.ptr), S.rank.as<float>(),
                     S.snap.as<float>());
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
             if (use_items(S.form, S.pairs, S.F, S.KP))
             {
-                expandk_light_items<OffT, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                expandk_light_items<QueueRowsK<OffT>, Op><<<blocks_for(S.F), kBlock, 0, s>>>(rows, op, a);
                 hip_check(hipGetLastError(), "launch expandk_light_items");
                 expandk_heavy_items<Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                 hip_check(hipGetLastError(), "launch expandk_heavy_items");
@@ -338,7 +337,7 @@ namespace hyt
             }
             else
             {
-                expandk_light<OffT, KP, Op><<<blocks_for(S.F), kBlock, 0, s>>>(g.off.as<OffT>(), op, a);
+                expandk_light<QueueRowsK<OffT>, KP, Op><<<blocks_for(S.F), kBlock, 0, s>>>(rows, op, a);
                 hip_check(hipGetLastError(), "launch expandk_light");
                 expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
                 hip_check(hipGetLastError(), "launch expandk_heavy");

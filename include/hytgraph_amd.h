@@ -326,6 +326,21 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
 int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ranks,
                         const hyt_run_options *opts, hyt_hybrid_stats *stats);
 
+/*
+ * K queries per run over the hybrid engine (1 <= K <= 64): multi-source SSSP
+ * and personalized PageRank with [V][K] state in HBM, outputs as hyt_sssp /
+ * hyt_ppr with K sources. The planner sees each frontier row once whatever
+ * its query mask, so every partition that moves (or is read zero-copy) is
+ * walked once for all K queries. The [V][K] state is outside the edge budget,
+ * like the single-query state. No counterpart in the original project, whose
+ * transfer engines plan one query (src/transfer/filter_engine.cpp:146-220,
+ * compaction_engine.cpp:253-493, zero_copy_engine.cpp:119-393).
+ */
+int hyt_hybrid_sssp_k(hyt_hybrid_t h, const uint32_t *sources, uint32_t num_sources, uint32_t *dist,
+                      const hyt_run_options *opts, hyt_hybrid_stats *stats);
+int hyt_hybrid_ppr_k(hyt_hybrid_t h, const uint32_t *seeds, uint32_t num_seeds, float damping,
+                     float epsilon, float *out, const hyt_run_options *opts, hyt_hybrid_stats *stats);
+
 /* ------------------------------------------- continuous batching (serving) */
 /*
  * No counterpart in the original project (which answers one query per call):

@@ -124,3 +124,80 @@ def test_hybrid_weights_move_only_when_read(gpu, abi, orc):
     assert st["parts_cached"] > 0
     r2, _ = ew.pagerank(D, 1e-6)
     assert np.abs(r2 - rw).sum() <= 1e-6
+
+
+# ------------------------------------------------------------ K queries per run
+def _sources(abi, hg, n, seed):
+    s = [int(x) for x in abi.sample_sources(hg.offsets, seed, n)]
+    return s + [s[0]]  # a repeated source shares its row with another lane
+
+
+@pytest.mark.parametrize("engine", [-1, 0, 1, 2])
+def test_hybrid_sssp_k_bitexact(gpu, abi, orc, engine):
+    hg = pinned(abi, scale=13, seed=5)
+    h = abi.HybridEngine(hg, force_engine=engine, **SMALL)
+    srcs = _sources(abi, hg, 8, 7)
+    refs = {s: orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, s) for s in set(srcs)}
+    for form in (0, 1, 2):
+        _say(f"sssp_k engine {engine} form {form} K={len(srcs)}")
+        d, st = h.sssp_k(srcs, lane_form=form)
+        assert d.shape == (hg.V, len(srcs))
+        for k, s in enumerate(srcs):
+            assert np.array_equal(d[:, k], refs[s]), (form, k)
+        assert st["run"]["converged"] == 1
+        if engine == 0:
+            assert st["parts_filter"] > 0
+        if engine == 1:
+            assert st["parts_compaction"] > 0
+        if engine == 2:
+            assert st["parts_zero_copy"] > 0
+
+
+def test_hybrid_sssp_k_oversized_rows_single_lane_and_round_limit(gpu, abi, orc):
+    # hubs above the packed staging go through the listed-row (zero-copy) path
+    hg = pinned(abi, scale=15, seed=9)
+    deg = np.diff(hg.offsets.astype(np.int64))
+    assert deg.max() > 2048
+    h = abi.HybridEngine(hg, partition_edges=512, pack_edges=2048, force_engine=1, calibrate=0,
+                         device_budget_bytes=2048 * 20 + 16 + 4 * 512 * 8 + 8 * 512 * 8)
+    hub = int(np.argmax(deg))
+    srcs = [hub, 0, 17, 4242, 9]
+    d, st = h.sssp_k(srcs)
+    for k, s in enumerate(srcs):
+        assert np.array_equal(d[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, s)), k
+    # K = 1 equals the single-query engine
+    d1, _ = h.sssp_k([hub])
+    ds, _ = h.sssp(hub)
+    assert np.array_equal(d1[:, 0], ds)
+    # a run stopped by the round limit leaves the engine ready for the next one
+    _, st = h.sssp_k(srcs, max_rounds=2)
+    assert st["run"]["converged"] == 0 and st["run"]["rounds"] == 2
+    d2, _ = h.sssp_k(srcs)
+    assert np.array_equal(d2, d)
+    with pytest.raises(abi.InvalidArgument):
+        h.sssp_k([])
+    with pytest.raises(abi.InvalidArgument):
+        h.sssp_k([0] * 65)
+    with pytest.raises(abi.InvalidArgument):
+        h.sssp_k([0, hg.V])
+
+
+@pytest.mark.parametrize("engine", [-1, 0, 1, 2])
+def test_hybrid_ppr_k(gpu, abi, orc, engine):
+    hg = pinned(abi, scale=12, seed=2, wmax=0)
+    h = abi.HybridEngine(hg, force_engine=engine, **SMALL)
+    eps = 1e-8
+    seeds = _sources(abi, hg, 5, 3)
+    g = abi.DeviceGraph.from_host(hg)
+    alone, _ = g.ppr(seeds, D, eps)
+    for form in (0, 2):
+        r, st = h.ppr_k(seeds, D, eps, lane_form=form)
+        assert st["run"]["converged"] == 1
+        worst = 0.0
+        for k, s in enumerate(seeds):
+            same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps, seed=s)
+            worst = max(worst, np.abs(r[:, k] - same).sum())
+            assert np.abs(r[:, k] - same).sum() <= 2e-6, (form, k)
+            assert np.abs(r[:, k] - alone[:, k]).sum() <= 4e-6, (form, k)
+            assert abs(r[:, k].sum() - 1) < 1e-5
+        _say(f"ppr_k engine {engine} form {form}: worst L1 vs same-procedure oracle {worst:.2e}")
