@@ -24,6 +24,7 @@ HYT_ERR_OUT_OF_MEMORY = 4
 HYT_ERR_OVERFLOW = 5
 HYT_ERR_UNSUPPORTED = 6
 HYT_ERR_INTERNAL = 7
+HYT_ERR_IO = 8
 
 HYT_GEN_RMAT = 0
 HYT_GEN_ER = 1
@@ -246,6 +247,10 @@ SIGNATURES = {
     "hyt_select_engine": (
         C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32)]),
     "hyt_combine_tasks": (C.c_int, [_P, C.c_uint64, C.c_uint64, _P, _P, _P, _u64p]),
+    "hyt_load_edge_list": (C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.c_int32, C.POINTER(_P)]),
+    "hyt_host_graph_save": (C.c_int, [_P, C.c_char_p, C.c_uint32]),
+    "hyt_host_graph_load": (C.c_int, [C.c_char_p, C.c_int32, C.c_uint32, C.POINTER(_P), _u32p]),
+    "hyt_host_graph_from_csr": (C.c_int, [C.c_uint64, C.c_uint64, _P, _P, _P, C.c_int32, C.POINTER(_P)]),
     "hyt_hybrid_config_default": (C.c_int, [C.POINTER(HybridConfig)]),
     "hyt_hybrid_create": (C.c_int, [_P, C.c_uint32, C.POINTER(HybridConfig), C.POINTER(_P)]),
     "hyt_hybrid_free": (C.c_int, [_P]),
@@ -322,10 +327,51 @@ class HostGraph:
     """Generator output owned by the library (views stay valid while alive)."""
 
     def __init__(self, kind=HYT_GEN_RMAT, scale=16, edge_factor=16, a=0.57, b=0.19, c=0.19,
-                 seed=1, weight_min=1, weight_max=64, threads=0, memory=HYT_HOST_PAGEABLE):
-        p = GenParams(kind, scale, edge_factor, a, b, c, seed, weight_min, weight_max, threads, memory)
+                 seed=1, weight_min=1, weight_max=64, threads=0, memory=HYT_HOST_PAGEABLE, _handle=None):
+        self.weights_f32 = False
+        if _handle is None:
+            p = GenParams(kind, scale, edge_factor, a, b, c, seed, weight_min, weight_max, threads, memory)
+            _handle = C.c_void_p()
+            check(load().hyt_generate(C.byref(p), C.byref(_handle)))
+        self._attach(_handle)
+
+    @classmethod
+    def load(cls, path, memory=HYT_HOST_PAGEABLE, verify=True):
+        """Binary CSR file written by save() (hyt_host_graph_load)."""
+        h, f32 = C.c_void_p(), C.c_uint32()
+        check(load().hyt_host_graph_load(str(path).encode(), memory, 1 if verify else 0, C.byref(h), C.byref(f32)))
+        g = cls(_handle=h)
+        g.weights_f32 = bool(f32.value)
+        return g
+
+    @classmethod
+    def from_edge_list(cls, path, num_vertices, weighted=False, memory=HYT_HOST_PAGEABLE):
+        """Text edge list with the original loader's rules (hyt_load_edge_list);
+        weights are float bit patterns."""
         h = C.c_void_p()
-        check(load().hyt_generate(C.byref(p), C.byref(h)))
+        check(load().hyt_load_edge_list(str(path).encode(), num_vertices, 1 if weighted else 0, memory, C.byref(h)))
+        g = cls(_handle=h)
+        g.weights_f32 = bool(weighted)
+        return g
+
+    @classmethod
+    def from_csr(cls, offsets, columns, weights=None, memory=HYT_HOST_PAGEABLE, weights_f32=False):
+        """Copy of caller CSR arrays (hyt_host_graph_from_csr)."""
+        off = np.ascontiguousarray(offsets, dtype=np.uint64)
+        col = np.ascontiguousarray(columns, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
+        h = C.c_void_p()
+        check(load().hyt_host_graph_from_csr(len(off) - 1, len(col), _ptr(off), _ptr(col),
+                                             _ptr(w) if w is not None else None, memory, C.byref(h)))
+        g = cls(_handle=h)
+        g.weights_f32 = bool(weights_f32 and w is not None)
+        return g
+
+    def save(self, path, weights_f32=None):
+        f32 = self.weights_f32 if weights_f32 is None else weights_f32
+        check(load().hyt_host_graph_save(self._h, str(path).encode(), 1 if f32 else 0))
+
+    def _attach(self, h):
         self._h = h
         V, E = C.c_uint64(), C.c_uint64()
         off, col, w = _u64p(), _u32p(), _u32p()

@@ -73,11 +73,6 @@ namespace hyt
     }
 } // namespace hyt
 
-struct hyt_host_graph
-{
-    std::unique_ptr<hyt::HostGraph> g;
-};
-
 namespace hyt
 {
     HostGraph *host_graph_of(hyt_host_graph_t h)

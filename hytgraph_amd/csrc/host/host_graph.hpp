@@ -35,3 +35,9 @@ namespace hyt
     void sample_sources(const std::uint64_t *off, std::uint64_t V, std::uint64_t seed,
                         std::uint32_t count, std::uint32_t *out);
 } // namespace hyt
+
+// Handle behind hyt_host_graph_t.
+struct hyt_host_graph
+{
+    std::unique_ptr<hyt::HostGraph> g;
+};

@@ -32,7 +32,8 @@ enum hyt_status {
     HYT_ERR_OUT_OF_MEMORY = 4,    /* allocation failed or budget exceeded */
     HYT_ERR_OVERFLOW = 5,         /* a u32 distance sum exceeded 2^32-2 */
     HYT_ERR_UNSUPPORTED = 6,
-    HYT_ERR_INTERNAL = 7
+    HYT_ERR_INTERNAL = 7,
+    HYT_ERR_IO = 8                /* a file could not be opened, read or written */
 };
 
 /* Thread-local message of the last failing call on this thread. */
@@ -77,6 +78,36 @@ int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices,
                         uint64_t *num_edges, const uint64_t **row_offsets,
                         const uint32_t **columns, const uint32_t **weights);
 int hyt_host_graph_free(hyt_host_graph_t g);
+
+/*
+ * Graph files and caller graphs as host graphs (memory: hyt_host_memory;
+ * pinned graphs can feed the hybrid engine).
+ *
+ * hyt_load_edge_list: the original's text edge list
+ * (src/graph/graph_loader.cpp:39-179, GraphLoader::load_edge_list): one
+ * "src dst" (or "src dst weight") per line, blank and '#' lines skipped, ids in
+ * [0, num_vertices), extra tokens rejected; edges grouped by source in input
+ * order, parallel edges kept. Weights are stored as float bit patterns.
+ * Same messages as the original; an unreadable file is HYT_ERR_IO.
+ *
+ * hyt_host_graph_save / _load: binary CSR file (no counterpart in the
+ * original, which reads text only): a 128-byte header with V, E, flags and a
+ * checksum per array, then offsets u64[V+1], columns u32[E], weights u32[E].
+ * Read and written with positional I/O on several threads; verify = 1
+ * recomputes the checksums and checks the CSR structure. weights_f32 records
+ * whether the weights are float bit patterns.
+ *
+ * hyt_host_graph_from_csr: copies caller CSR arrays (validated as
+ * CSRGraph::validate does, src/graph/csr_graph.cpp:105-169).
+ */
+int hyt_load_edge_list(const char *path, uint64_t num_vertices, uint32_t weighted, int32_t memory,
+                       hyt_host_graph_t *out);
+int hyt_host_graph_save(hyt_host_graph_t g, const char *path, uint32_t weights_f32);
+int hyt_host_graph_load(const char *path, int32_t memory, uint32_t verify, hyt_host_graph_t *out,
+                        uint32_t *weights_f32);
+int hyt_host_graph_from_csr(uint64_t num_vertices, uint64_t num_edges, const uint64_t *row_offsets,
+                            const uint32_t *columns, const uint32_t *weights, int32_t memory,
+                            hyt_host_graph_t *out);
 
 /* Degree-weighted source sampling: pick edge index uniformly with a
  * counter-based stream keyed by seed and return its source vertex. Vertices

@@ -1,10 +1,12 @@
 // Host planning API in the original project's test style: the known answers
 // of its tests/unit_tests.cpp for ExpTM-Filter (918-1121), ExpTM-Compaction
 // (1289-1563), ImpTM zero-copy (1737-2056) and the contribution scheduler
-// (2751-2887), plus seeded random graphs checked against direct per-vertex
-// recomputation (partition walking, per-partition totals, coverage errors).
+// (2751-2887) and the text edge-list loader (373-592), plus seeded random
+// graphs checked against direct per-vertex recomputation (partition walking,
+// per-partition totals, coverage errors) and a binary CSR round trip.
 #include "graph/activity_tracker.hpp"
 #include "graph/csr_graph.hpp"
+#include "graph/graph_loader.hpp"
 #include "graph/partition.hpp"
 #include "scheduling/contribution_scheduler.hpp"
 #include "transfer/compaction_engine.hpp"
@@ -13,6 +15,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <filesystem>
+#include <fstream>
 #include <iostream>
 #include <random>
 #include <stdexcept>
@@ -273,6 +277,73 @@ namespace
                    "random: modeled TLPs");
         }
     }
+    // Writes `text` to a temporary file and returns its path.
+    std::string temp_file(const std::string &name, const std::string &text)
+    {
+        const auto path = std::filesystem::temp_directory_path() / ("hytgraph_amd_" + name);
+        std::ofstream(path) << text;
+        return path.string();
+    }
+
+    void test_graph_loader_known_answers()
+    {
+        using hytgraph::graph::GraphLoader;
+        {
+            const std::string p = temp_file("u.el", "# simple directed graph\n0 1\n0 2\n1 2\n2 0\n");
+            const CSRGraph g = GraphLoader::load_edge_list(p, 4, false);
+            expect(g.num_vertices() == 4 && g.num_edges() == 4 && !g.has_weights(), "loader: unweighted sizes");
+            expect(g.out_degree(0) == 2 && g.out_degree(1) == 1 && g.out_degree(2) == 1 && g.out_degree(3) == 0,
+                   "loader: degrees");
+            expect(g.neighbor_at(0) == 1 && g.neighbor_at(1) == 2 && g.neighbor_at(2) == 2 && g.neighbor_at(3) == 0,
+                   "loader: neighbours");
+            std::filesystem::remove(p);
+        }
+        {
+            const std::string p = temp_file("w.el", "# weighted graph\n0 1 0.5\n0 2 1.25\n1 2 2.5\n");
+            const CSRGraph g = GraphLoader::load_edge_list(p, 3, true);
+            expect(g.has_weights() && g.num_edges() == 3, "loader: weighted sizes");
+            expect(g.weight_at(0) == 0.5F && g.weight_at(1) == 1.25F && g.weight_at(2) == 2.5F, "loader: weights");
+            // binary round trip keeps every array and the float weights
+            const std::string b = temp_file("w.hytcsr", "");
+            GraphLoader::save_binary(g, b);
+            const CSRGraph h = GraphLoader::load_binary(b);
+            expect(h.row_offsets() == g.row_offsets() && h.column_indices() == g.column_indices() &&
+                       h.edge_weights() == g.edge_weights(),
+                   "binary CSR round trip");
+            std::filesystem::remove(p);
+            std::filesystem::remove(b);
+        }
+        {
+            const std::string p = temp_file("par.el", "0 1\n0 1\n0 2\n");
+            const CSRGraph g = GraphLoader::load_edge_list(p, 3, false);
+            expect(g.num_edges() == 3 && g.out_degree(0) == 3, "loader: parallel edges kept");
+            expect(g.neighbor_at(0) == 1 && g.neighbor_at(1) == 1 && g.neighbor_at(2) == 2, "loader: edge order");
+            std::filesystem::remove(p);
+        }
+        const std::string bad_id = temp_file("bad_id.el", "0 3\n");
+        expect_throw<std::invalid_argument>([&] { (void)GraphLoader::load_edge_list(bad_id, 3, false); },
+                                            "loader: destination out of range");
+        const std::string no_w = temp_file("no_w.el", "0 1\n");
+        expect_throw<std::invalid_argument>([&] { (void)GraphLoader::load_edge_list(no_w, 2, true); },
+                                            "loader: missing weight");
+        const std::string junk = temp_file("junk.el", "this is not an edge\n");
+        expect_throw<std::invalid_argument>([&] { (void)GraphLoader::load_edge_list(junk, 2, false); },
+                                            "loader: malformed record");
+        expect_throw<std::runtime_error>(
+            [&] { (void)GraphLoader::load_edge_list("/nonexistent/hytgraph_amd.el", 2, false); },
+            "loader: unreadable file");
+        try
+        {
+            (void)GraphLoader::load_edge_list(junk, 2, false);
+        }
+        catch (const std::invalid_argument &e)
+        {
+            expect(std::string(e.what()) == "GraphLoader::load_edge_list: malformed edge at line 1",
+                   std::string("loader: message: ") + e.what());
+        }
+        for (const auto &p : {bad_id, no_w, junk})
+            std::filesystem::remove(p);
+    }
 } // namespace
 
 int main()
@@ -284,6 +355,7 @@ int main()
         test_zero_copy_known_answers();
         test_scheduler_known_answers();
         test_random_graphs();
+        test_graph_loader_known_answers();
         std::cout << "planning API tests passed.\n";
         return 0;
     }

@@ -201,3 +201,19 @@ def test_hybrid_ppr_k(gpu, abi, orc, engine):
             assert np.abs(r[:, k] - alone[:, k]).sum() <= 4e-6, (form, k)
             assert abs(r[:, k].sum() - 1) < 1e-5
         _say(f"ppr_k engine {engine} form {form}: worst L1 vs same-procedure oracle {worst:.2e}")
+
+
+def test_hybrid_from_file_and_caller_csr(gpu, abi, orc, tmp_path):
+    # a binary CSR file and a caller's own CSR arrays, loaded into pinned
+    # memory, feed the hybrid engine like generated graphs
+    hg = abi.HostGraph(scale=12, seed=4)
+    path = tmp_path / "g.hytcsr"
+    hg.save(path)
+    ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 3)
+    for g in (abi.HostGraph.load(path, memory=abi.HYT_HOST_PINNED),
+              abi.HostGraph.from_csr(hg.offsets, hg.columns, hg.weights, memory=abi.HYT_HOST_PINNED)):
+        h = abi.HybridEngine(g, **SMALL)
+        d, _ = h.sssp(3)
+        assert np.array_equal(d, ref)
+        dk, _ = h.sssp_k([3, 0])
+        assert np.array_equal(dk[:, 0], ref)
