@@ -94,16 +94,17 @@ namespace hyt
             std::uint32_t light_max, chunk;
         };
 
-        // Compaction into the queue (plus frontier bitmap and out-edge sum);
-        // returns the counters after it.
+        // Compaction into the queue; SYNC rounds also take the snapshot values,
+        // and rounds that may pull the frontier bitmap and out-edge sum (ASYNC
+        // push-only rounds read none of them). Returns the counters after it.
         template <class OffT>
-        Counters compact_and_read(DeviceGraph &g, const SsspSel &sel)
+        Counters compact_and_read(DeviceGraph &g, const SsspSel &sel, bool values, bool direction)
         {
             Workspace &ws = g.ws;
             Counters *d_ctr = ws.ctr.as<Counters>();
             compact_rows<std::uint32_t, OffT, SsspSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
-                g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr, g.off.as<OffT>(),
-                ws.active.as<std::uint16_t>());
+                g.V, sel, ws.q.as<std::uint32_t>(), values ? ws.qd.as<std::uint32_t>() : nullptr, d_ctr,
+                direction ? g.off.as<OffT>() : nullptr, direction ? ws.active.as<std::uint16_t>() : nullptr);
             hip_check(hipGetLastError(), "launch sssp compaction");
             Counters *h_ctr = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
@@ -144,19 +145,21 @@ namespace hyt
             };
             SsspSel sel{ws.touch.as<std::uint8_t>(), ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(),
                         bucketed ? bound_after(0U) : 0xFFFFFFFFU};
-            Counters c0 = compact_and_read<OffT>(g, sel);
-            std::uint32_t F = c0.next_count;
-            unsigned long long Fe = c0.frontier_edges;
-            std::uint32_t pending = c0.pending;
-            std::uint32_t pending_min = ~c0.pending_inv_max;
 
             const std::uint32_t dir = L.o.direction;
             const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
             const bool may_pull = dir == HYT_DIR_PULL || (dir == HYT_DIR_AUTO && pct <= 100);
             if (may_pull && g.E > 0 && g.E < (1ULL << 31))
                 ensure_transpose(g, L.light_max, L.chunk);
-            const bool can_pull = g.in_col.ptr != nullptr;
+            // a transpose left by an earlier call is only used when this call may pull
+            const bool can_pull = may_pull && g.in_col.ptr != nullptr;
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
+
+            Counters c0 = compact_and_read<OffT>(g, sel, SYNC, can_pull);
+            std::uint32_t F = c0.next_count;
+            unsigned long long Fe = c0.frontier_edges;
+            std::uint32_t pending = c0.pending;
+            std::uint32_t pending_min = ~c0.pending_inv_max;
             SsspPullOp<WM, SYNC> pop;
             pop.in_col = g.in_col.as<std::uint32_t>();
             pop.in_w = g.in_w.as<std::uint32_t>();
@@ -191,7 +194,7 @@ namespace hyt
                     // bucket exhausted: advance to the nearest held-back row
                     sel.bound = bound_after(pending_min);
                     hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
-                    const Counters cb = compact_and_read<OffT>(g, sel);
+                    const Counters cb = compact_and_read<OffT>(g, sel, SYNC, can_pull);
                     F = cb.next_count;
                     Fe = cb.frontier_edges;
                     pending = cb.pending;
@@ -226,7 +229,7 @@ namespace hyt
                 }
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_and_read<OffT>(g, sel);
+                const Counters c = compact_and_read<OffT>(g, sel, SYNC, can_pull);
                 if (L.o.time_kernels)
                 {
                     float ms = 0;
