@@ -16,6 +16,7 @@
 //
 // Op supplies the per-algorithm parts:
 //   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
+//   void expanded(uint32_t u, uint32_t value, bool whole_row)  row u walks with value
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
 //   void apply(uint32_t v, uint32_t c)                 update destination v
 // load returns false to drop the edge (e.g. u32 overflow, flagged by Op).
@@ -203,6 +204,7 @@ namespace hyt
             std::uint64_t full;
             rs.get(i, u, vi, begin, count, full);
             value = op.row_value(vi, u, full);
+            op.expanded(u, value, count == full);
             if (count > a.light_max)
                 nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
             else

@@ -1099,6 +1099,7 @@ namespace hyt
                 op.touch = ws.touch.as<std::uint8_t>();
                 op.qd = qd;
                 op.overflow = &d_ctr->overflow;
+                op.expanded_at = ws.prev.as<std::uint32_t>();
                 return op;
             };
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : g.V;

@@ -54,10 +54,22 @@ namespace hyt
         std::uint8_t *touch;
         const std::uint32_t *qd; // snapshot values (SYNC)
         std::uint32_t *overflow;
+        // ASYNC, optional: the compaction's last-seen distances. A row read
+        // its live distance here, so recording that value keeps the next
+        // compaction from queuing the row again when it was improved in this
+        // round before its own expansion (its edges already saw the value).
+        std::uint32_t *expanded_at = nullptr;
 
         __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t) const
         {
             return SYNC ? qd[i] : dist[u];
+        }
+        // Called once per row with the value its edges use; whole_row = every
+        // edge of the row is walked with it in this launch.
+        __device__ void expanded(std::uint32_t u, std::uint32_t value, bool whole_row) const
+        {
+            if (!SYNC && whole_row && expanded_at != nullptr)
+                expanded_at[u] = value;
         }
         __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
         {
@@ -165,6 +177,7 @@ namespace hyt
                 return 0U;
             return __float_as_uint(damping * __uint_as_float(qd[i]) / static_cast<float>(deg));
         }
+        __device__ void expanded(std::uint32_t, std::uint32_t, bool) const {}
         __device__ bool load(std::uint64_t pos, std::uint32_t share, std::uint32_t &v, std::uint32_t &c) const
         {
             v = col[pos];

@@ -176,6 +176,7 @@ namespace hyt
             op.touch = ws.touch.as<std::uint8_t>();
             op.qd = ws.qd.as<std::uint32_t>();
             op.overflow = &d_ctr->overflow;
+            op.expanded_at = SYNC ? nullptr : ws.prev.as<std::uint32_t>();
 
             ExpandArgs a;
             a.q = ws.q.as<std::uint32_t>();
