@@ -173,12 +173,17 @@ class ServeStats(C.Structure):
                 ("state_bytes", C.c_uint64), ("frontier_rows", C.c_uint64), ("lane_rows", C.c_uint64),
                 ("item_rounds", C.c_uint64),
                 ("sessions", C.c_uint32), ("lanes", C.c_uint32 * 16),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("host_bytes", C.c_uint64), ("host_zero_copy_bytes", C.c_uint64),
+                ("parts_cached", C.c_uint64), ("parts_moved", C.c_uint64)]
 
     def as_dict(self) -> dict:
         d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("lanes", "reserved")}
         d["lanes"] = list(self.lanes)[: self.sessions]
         return d
+
+
+class ServeGraph(C.Structure):
+    _fields_ = [("resident", C.c_void_p), ("sssp_engine", C.c_void_p), ("ppr_engine", C.c_void_p)]
 
 
 class TuneOptions(C.Structure):
@@ -277,6 +282,10 @@ SIGNATURES = {
     "hyt_serve_trace": (
         C.c_int, [C.POINTER(_P), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint64,
                   C.POINTER(_P), C.POINTER(Completion), C.POINTER(ServeStats)]),
+    "hyt_serve_trace_mixed": (
+        C.c_int, [C.POINTER(ServeGraph), C.c_uint32, C.POINTER(ServeConfig), C.POINTER(Request), C.c_uint64,
+                  C.POINTER(_P), C.POINTER(Completion), C.POINTER(ServeStats)]),
+    "hyt_lanes_create_hybrid": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(_P)]),
 }
 
 _lib = None
@@ -653,12 +662,16 @@ class HybridEngine:
 class LaneSession:
     """K query lanes of one kind over a DeviceGraph (kept alive by this object)."""
 
-    def __init__(self, graph: DeviceGraph, kind=HYT_QUERY_SSSP, lanes=8, damping=0.85, epsilon=1e-6):
+    def __init__(self, graph, kind=HYT_QUERY_SSSP, lanes=8, damping=0.85, epsilon=1e-6):
+        """graph: a DeviceGraph, or a HybridEngine (hyt_lanes_create_hybrid)."""
         self.graph = graph
         self.kind = kind
         self.V = graph.V
         h = C.c_void_p()
-        check(load().hyt_lanes_create(graph._h, kind, lanes, damping, epsilon, C.byref(h)))
+        if isinstance(graph, HybridEngine):
+            check(load().hyt_lanes_create_hybrid(graph._h, kind, lanes, damping, epsilon, C.byref(h)))
+        else:
+            check(load().hyt_lanes_create(graph._h, kind, lanes, damping, epsilon, C.byref(h)))
         self._h = h
 
     def admit(self, lane: int, source: int) -> None:
@@ -726,23 +739,37 @@ def serve_config(**overrides) -> ServeConfig:
 
 def serve_trace(graphs, requests, keep_results=False, **config):
     """Replays requests = [(arrival_s, graph_index, kind, source), ...] (sorted by
-    arrival). Returns (completions as a structured numpy array, stats dict,
-    results list or None)."""
+    arrival). graphs: DeviceGraph, or (sssp_engine, ppr_engine) HybridEngine
+    pairs (either may be None) for graphs served from host memory
+    (hyt_serve_trace_mixed). Returns (completions as a structured numpy
+    array, stats dict, results list or None)."""
     n = len(requests)
     req = (Request * max(n, 1))()
     for i, (t, gi, kind, src) in enumerate(requests):
         req[i] = Request(int(gi), int(kind), int(src), 0, float(t))
-    garr = (C.c_void_p * len(graphs))(*[g._h for g in graphs])
+    mixed = any(not isinstance(g, DeviceGraph) for g in graphs)
+    sizes = [g.V if isinstance(g, DeviceGraph) else next(e for e in g if e is not None).V for g in graphs]
+    if mixed:
+        garr = (ServeGraph * len(graphs))()
+        for i, g in enumerate(graphs):
+            if isinstance(g, DeviceGraph):
+                garr[i] = ServeGraph(g._h, None, None)
+            else:
+                garr[i] = ServeGraph(None, g[0]._h if g[0] is not None else None,
+                                     g[1]._h if g[1] is not None else None)
+    else:
+        garr = (C.c_void_p * len(graphs))(*[g._h for g in graphs])
     comp = (Completion * max(n, 1))()
     st = ServeStats()
     cfg = serve_config(**config)
     results = None
     res_arr = None
     if keep_results:
-        results = [np.empty(graphs[r[1]].V, dtype=np.float32 if r[2] == HYT_QUERY_PPR else np.uint32)
+        results = [np.empty(sizes[r[1]], dtype=np.float32 if r[2] == HYT_QUERY_PPR else np.uint32)
                    for r in requests]
         res_arr = (C.c_void_p * max(n, 1))(*[a.ctypes.data for a in results])
-    check(load().hyt_serve_trace(garr, len(graphs), C.byref(cfg), req, n, res_arr, comp, C.byref(st)))
+    fn = load().hyt_serve_trace_mixed if mixed else load().hyt_serve_trace
+    check(fn(garr, len(graphs), C.byref(cfg), req, n, res_arr, comp, C.byref(st)))
     dt = np.dtype([("admit_s", "f8"), ("done_s", "f8"), ("session", "u4"), ("lane", "u4"), ("rounds", "u4"),
                    ("reserved", "u4"), ("reached", "u8"), ("checksum", "u8")])
     arr = np.frombuffer(bytes(comp), dtype=dt, count=n).copy() if n else np.zeros(0, dt)

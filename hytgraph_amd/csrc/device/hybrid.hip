@@ -29,6 +29,7 @@
 //      staging (while the GPU works on the rest) and copies them once.
 // The expand kernels are the in-HBM ones with a windowed / packed row source.
 #include "host_graph.hpp"
+#include "hybrid_lanes.hpp"
 #include "lanes.hpp"
 #include "ops.hpp"
 #include "transfer_plan.hpp"
@@ -127,6 +128,16 @@ namespace hyt
             }
         }
 
+        // Folds queue rows back into the marks (rows are unique after an
+        // ordered compaction, and nothing else writes mark meanwhile).
+        __global__ void mark_from_queue(const std::uint32_t *q, const unsigned long long *qmask, std::uint32_t F,
+                                        unsigned long long *mark)
+        {
+            const std::uint32_t stride = gridDim.x * blockDim.x;
+            for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < F; i += stride)
+                mark[q[i]] |= qmask[i];
+        }
+
         // Zero-copy bandwidth probe: streams a mapped host range.
         __global__ void zc_probe(const uint4 *src, std::uint64_t n, unsigned *sink)
         {
@@ -159,6 +170,7 @@ namespace hyt
         {
             DeviceBuffer val, rank, snap, mark, qmask, stage_mask, heavy, seeds, lanes, partial;
             std::uint64_t heavy_cap = 0;
+            std::unique_ptr<PinnedBuffer> host_lanes; // read-back of `lanes`
             std::uint64_t bytes() const
             {
                 return val.bytes + rank.bytes + snap.bytes + mark.bytes + qmask.bytes + stage_mask.bytes +
@@ -1325,8 +1337,9 @@ namespace hyt
         }
 
         // Mark compaction in vertex order: rows with a non-zero mark -> sorted
-        // queue + masks; ctr->frontier_edges = (row, query) pairs.
-        std::uint32_t ordered_compaction_k(HybridEngine &E, Counters &out)
+        // queue + masks; ctr->frontier_edges = (row, query) pairs; optionally
+        // the OR of the kept masks (E.lk.lanes accumulates it).
+        std::uint32_t ordered_compaction_k(HybridEngine &E, Counters &out, std::uint64_t *lanes_or = nullptr)
         {
             DeviceGraph &g = E.dg;
             hipStream_t s = g.stream;
@@ -1348,8 +1361,13 @@ namespace hyt
             hip_check(hipGetLastError(), "launch gather");
             Counters *h = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read counters");
+            if (lanes_or)
+                hip_check(hipMemcpyAsync(E.lk.host_lanes->ptr, E.lk.lanes.ptr, 8, hipMemcpyDeviceToHost, s),
+                          "read lanes");
             hip_check(hipStreamSynchronize(s), "hybrid K round");
             out = *h;
+            if (lanes_or)
+                std::memcpy(lanes_or, E.lk.host_lanes->ptr, 8);
             return h->next_count;
         }
 
@@ -1506,7 +1524,170 @@ namespace hyt
                 hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
             hip_check(hipStreamSynchronize(g.stream), "hybrid ppr output");
         }
+
+        // ------------------------------------------------ lane sessions
+        template <class OffT, int KP, class Op>
+        void lane_round_t(HybridEngine &E, const Op &op, std::uint32_t K, bool ppr, std::uint32_t F,
+                          std::uint64_t pairs, std::uint32_t form, hyt_hybrid_stats &st, HybridLaneRound &out)
+        {
+            DeviceGraph &g = E.dg;
+            LaneState &L = E.lk;
+            hipStream_t s = g.stream;
+            Counters *d_ctr = g.ws.ctr.as<Counters>();
+            Plan plan;
+            plan_round<OffT>(E, F, plan, st);
+            hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+            hip_check(hipMemsetAsync(L.lanes.ptr, 0, 8, s), "reset lane mask");
+            if (ppr)
+            {
+                lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
+                    g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
+                    reinterpret_cast<float *>(L.val.ptr), L.rank.as<float>(), L.snap.as<float>());
+                hip_check(hipGetLastError(), "launch pprk_snapshot");
+            }
+            lanes::KArgs a{};
+            a.rs = nullptr;
+            a.ctr = d_ctr;
+            a.heavy = L.heavy.as<lanes::HeavyChunkK>();
+            a.heavy_cap = L.heavy_cap;
+            a.light_max = E.light_max;
+            a.chunk = E.chunk;
+            a.K = K;
+            a.mark = L.mark.as<unsigned long long>();
+            out.items = lanes::use_items(form, pairs, F, KP);
+            const LaneExpander<OffT, KP, Op> x{E, op, a, out.items};
+            execute_round<OffT>(E, plan, x, st);
+            ordered_compaction_k(E, out.c, &out.lanes);
+            finish_round_stats(E, st);
+        }
+
+        template <class OffT, int KP>
+        void lane_round_kind(HybridEngine &E, int kind, std::uint32_t K, std::uint32_t F, std::uint64_t pairs,
+                             float damping, float eps, std::uint32_t form, hyt_hybrid_stats &st,
+                             HybridLaneRound &out)
+        {
+            LaneState &L = E.lk;
+            if (kind == HYT_QUERY_PPR)
+            {
+                lanes::PprKOp op{};
+                op.res = reinterpret_cast<float *>(L.val.ptr);
+                op.snap = L.snap.as<float>();
+                op.K = K;
+                op.damping = damping;
+                op.eps = eps;
+                lane_round_t<OffT, KP>(E, op, K, true, F, pairs, form, st, out);
+                return;
+            }
+            auto run = [&](auto op) {
+                op.dist = L.val.as<std::uint32_t>();
+                op.K = K;
+                op.overflow = &E.dg.ws.ctr.as<Counters>()->overflow;
+                lane_round_t<OffT, KP>(E, op, K, false, F, pairs, form, st, out);
+            };
+            switch (E.dg.weight_mode())
+            {
+            case kWU32: run(lanes::SsspKOp<kWU32>{}); break;
+            case kWF32: run(lanes::SsspKOp<kWF32>{}); break;
+            case kWUnitU32: run(lanes::SsspKOp<kWUnitU32>{}); break;
+            default: run(lanes::SsspKOp<kWUnitF32>{}); break;
+            }
+        }
+
+        template <class OffT>
+        void lane_round_kp(HybridEngine &E, int kind, std::uint32_t K, std::uint32_t F, std::uint64_t pairs,
+                           float damping, float eps, std::uint32_t form, hyt_hybrid_stats &st, HybridLaneRound &out)
+        {
+            switch (lanes::pow2_at_least(K))
+            {
+            case 2: lane_round_kind<OffT, 2>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            case 4: lane_round_kind<OffT, 4>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            case 8: lane_round_kind<OffT, 8>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            case 16: lane_round_kind<OffT, 16>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            case 32: lane_round_kind<OffT, 32>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            default: lane_round_kind<OffT, 64>(E, kind, K, F, pairs, damping, eps, form, st, out); break;
+            }
+        }
     } // namespace
+
+    HybridEngine &hybrid_engine(hyt_hybrid *h)
+    {
+        require(h != nullptr, "null hybrid engine");
+        return h->e;
+    }
+
+    DeviceGraph &hybrid_device_graph(HybridEngine &E) { return E.dg; }
+
+    std::uint64_t hybrid_engine_bytes(HybridEngine &E) { return E.edge_area_bytes() + E.state_bytes() - E.lk.bytes(); }
+
+    std::uint64_t hybrid_lane_state_bytes(HybridEngine &E, int kind, std::uint32_t K)
+    {
+        const std::uint64_t V = E.dg.V, Vp = V + kPad;
+        const std::uint64_t per_lane = kind == HYT_QUERY_PPR ? V * 12 : V * 4;
+        const std::uint64_t fixed = Vp * 16 + static_cast<std::uint64_t>(compact_blocks(V)) * kCompactRowsPerBlock * 8 +
+                                    lanes::lane_chunk_capacity(V, E.dg.E, kLaneLightMaxDegree, kLaneHeavyChunk) *
+                                        sizeof(lanes::HeavyChunkK) +
+                                    (1024 + 1) * 64 * sizeof(double) + 1024 + 64;
+        return fixed + per_lane * K;
+    }
+
+    HybridLaneView hybrid_lanes_open(HybridEngine &E, int kind, std::uint32_t K)
+    {
+        require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
+        require(K >= 1 && K <= 64, "lanes must be in [1, 64]");
+        hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
+        const bool ppr = kind == HYT_QUERY_PPR;
+        E.begin_run(!ppr);
+        E.set_split(hyt_run_options{}, kLaneLightMaxDegree, kLaneHeavyChunk);
+        prepare_lanes(E, K, ppr);
+        LaneState &L = E.lk;
+        if (!L.host_lanes)
+            L.host_lanes = std::make_unique<PinnedBuffer>(8);
+        const std::uint64_t VK = E.dg.V * K;
+        const int wm = E.dg.weight_mode();
+        const std::uint32_t inf = ppr ? 0U : ((wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU);
+        launch_fill<std::uint32_t>(L.val.as<std::uint32_t>(), VK, inf, E.dg.stream);
+        if (ppr)
+            launch_fill<float>(L.rank.as<float>(), VK, 0.0F, E.dg.stream);
+        hip_check(hipMemsetAsync(L.mark.ptr, 0, L.mark.bytes, E.dg.stream), "clear marks");
+        hip_check(hipStreamSynchronize(E.dg.stream), "lane state");
+        return HybridLaneView{L.val.as<std::uint32_t>(), ppr ? L.rank.as<float>() : nullptr,
+                              L.mark.as<unsigned long long>()};
+    }
+
+    HybridLaneRound hybrid_lanes_round(HybridEngine &E, int kind, std::uint32_t K, std::uint32_t F,
+                                       std::uint64_t pairs, bool merge, float damping, float eps,
+                                       std::uint32_t form, hyt_hybrid_stats &st)
+    {
+        hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
+        DeviceGraph &g = E.dg;
+        LaneState &L = E.lk;
+        HybridLaneRound out{};
+        if (merge)
+        {
+            if (F > 0)
+            {
+                mark_from_queue<<<blocks_for(F), kBlock, 0, g.stream>>>(
+                    g.ws.q.as<std::uint32_t>(), L.qmask.as<unsigned long long>(), F, L.mark.as<unsigned long long>());
+                hip_check(hipGetLastError(), "launch mark_from_queue");
+            }
+            hip_check(hipMemsetAsync(g.ws.ctr.ptr, 0, sizeof(Counters), g.stream), "reset counters");
+            Counters c;
+            F = ordered_compaction_k(E, c);
+            pairs = c.frontier_edges;
+        }
+        out.F = F;
+        out.pairs = pairs;
+        if (F == 0)
+            return out;
+        if (g.off64)
+            lane_round_kp<std::uint64_t>(E, kind, K, F, pairs, damping, eps, form, st, out);
+        else
+            lane_round_kp<std::uint32_t>(E, kind, K, F, pairs, damping, eps, form, st, out);
+        st.run.rounds += 1;
+        st.run.frontier_total += F;
+        st.run.edges_total += out.c.edges;
+        return out;
+    }
 } // namespace hyt
 
 namespace hyt

@@ -23,6 +23,7 @@
 // round is launched - each session advances at its own pace, so a small
 // graph's queries are not held back by a large graph's rounds. Each request's
 // admit and completion times are recorded.
+#include "hybrid_lanes.hpp"
 #include "lanes.hpp"
 
 #include <algorithm>
@@ -47,6 +48,18 @@ namespace hyt
             for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
                  v += stride)
                 a[v * K + k] = value;
+        }
+
+        // Seeds lane k at source s and marks its row (sessions over the hybrid
+        // engine: the next round folds the mark into its sorted queue).
+        __global__ void lane_seed_mark(std::uint32_t *val, std::uint32_t K, std::uint32_t k, std::uint32_t s,
+                                       std::uint32_t seed, unsigned long long *mark)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+            {
+                val[static_cast<std::uint64_t>(s) * K + k] = seed;
+                atomicOr(mark + s, 1ULL << k);
+            }
         }
 
         // Seeds lane k at source s and appends (s, bit k) to the queue at F.
@@ -156,6 +169,16 @@ namespace hyt
     struct LaneSession
     {
         DeviceGraph *g = nullptr; // borrowed
+        // A session over the hybrid engine (graph edges in host memory) keeps
+        // its lanes in the engine's state, runs the engine's rounds on the
+        // engine's stream (one step blocks the caller for a round) and admits
+        // through the marks; otherwise the session owns its state.
+        HybridEngine *hy = nullptr;
+        hyt_hybrid_stats hst{};
+        std::uint32_t *val_p = nullptr; // [V][K] lane state (own or the engine's)
+        float *rank_p = nullptr;
+        unsigned long long *mark_p = nullptr;
+        bool own_stream = true;
         int kind = HYT_QUERY_SSSP;
         std::uint32_t K = 0, KP = 0;
         float damping = 0.85F, eps = 1e-6F;
@@ -183,7 +206,8 @@ namespace hyt
             if (stream)
             {
                 (void)hipStreamSynchronize(stream);
-                (void)hipStreamDestroy(stream);
+                if (own_stream)
+                    (void)hipStreamDestroy(stream);
             }
             if (done)
                 (void)hipEventDestroy(done);
@@ -265,6 +289,9 @@ namespace hyt
             S.partial.alloc((kGatherBlocks + 1) * sizeof(double), "hipMalloc(lane partials)");
             S.acc.alloc(16, "hipMalloc(lane checksum)");
             S.host = std::make_unique<PinnedBuffer>(sizeof(Counters) + 8 + 16);
+            S.val_p = S.val.as<std::uint32_t>();
+            S.rank_p = kind == HYT_QUERY_PPR ? S.rank.as<float>() : nullptr;
+            S.mark_p = S.mark.as<unsigned long long>();
             hip_check(hipMemsetAsync(S.mark.ptr, 0, Vp * 8, S.stream), "clear marks");
             hip_check(hipMemsetAsync(S.ctr.ptr, 0, sizeof(Counters), S.stream), "clear counters");
             // every lane starts free with a defined state
@@ -272,6 +299,50 @@ namespace hyt
             if (kind == HYT_QUERY_PPR)
                 launch_fill<float>(S.rank.as<float>(), Vp * K, 0.0F, S.stream);
             hip_check(hipStreamSynchronize(S.stream), "lane session init");
+        }
+
+        void create_hybrid_session(LaneSession &S, HybridEngine &E, int kind, std::uint32_t K, float damping,
+                                   float eps)
+        {
+            require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
+            require(K >= 1 && K <= 64, "lanes must be in [1, 64]");
+            if (kind == HYT_QUERY_PPR)
+            {
+                require(damping >= 0.0F && damping < 1.0F, "PageRank damping_factor must be in [0, 1)");
+                require(eps > 0.0F && std::isfinite(eps), "PageRank epsilon must be finite and positive");
+            }
+            DeviceGraph *g = &hybrid_device_graph(E);
+            S.g = g;
+            S.hy = &E;
+            S.kind = kind;
+            S.K = K;
+            S.KP = static_cast<std::uint32_t>(pow2_at_least(K));
+            S.damping = damping;
+            S.eps = eps;
+            S.wm = g->weight_mode();
+            if (kind == HYT_QUERY_SSSP)
+            {
+                S.inf_bits = (S.wm == kWF32 || S.wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
+                S.seed_bits = 0U;
+            }
+            else
+            {
+                S.inf_bits = 0U;
+                const float r0 = 1.0F - damping;
+                std::memcpy(&S.seed_bits, &r0, 4);
+            }
+            const HybridLaneView v = hybrid_lanes_open(E, kind, K);
+            S.val_p = v.val;
+            S.rank_p = v.rank;
+            S.mark_p = v.mark;
+            S.stream = g->stream;
+            S.own_stream = false;
+            hip_check(hipEventCreateWithFlags(&S.done, hipEventDisableTiming), "hipEventCreate");
+            const std::uint64_t Vp = g->V + kPad;
+            S.colbuf.alloc(Vp * 4, "hipMalloc(lane output)");
+            S.partial.alloc((kGatherBlocks + 1) * sizeof(double), "hipMalloc(lane partials)");
+            S.acc.alloc(16, "hipMalloc(lane checksum)");
+            S.host = std::make_unique<PinnedBuffer>(sizeof(Counters) + 8 + 16);
         }
 
         void admit(LaneSession &S, std::uint32_t lane, std::uint32_t source)
@@ -282,18 +353,27 @@ namespace hyt
             require(!S.launched, "a round is in flight");
             require(S.appended < LaneSession::kAppend, "too many admissions since the last round");
             const std::uint64_t V = S.g->V;
-            lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.val.as<std::uint32_t>(), V, S.K, lane, S.inf_bits);
+            lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.val_p, V, S.K, lane, S.inf_bits);
             hip_check(hipGetLastError(), "launch lane_fill");
             if (S.kind == HYT_QUERY_PPR)
             {
-                lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.rank.as<std::uint32_t>(), V, S.K, lane, 0U);
+                lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(reinterpret_cast<std::uint32_t *>(S.rank_p), V, S.K,
+                                                                lane, 0U);
                 hip_check(hipGetLastError(), "launch lane_fill");
             }
-            lane_seed<<<1, 64, 0, S.stream>>>(S.val.as<std::uint32_t>(), S.K, lane, source, S.seed_bits,
-                                              S.q.as<std::uint32_t>(), S.qmask.as<unsigned long long>(), S.F);
-            hip_check(hipGetLastError(), "launch lane_seed");
-            S.F += 1;
-            S.pairs += 1;
+            if (S.hy)
+            {
+                lane_seed_mark<<<1, 64, 0, S.stream>>>(S.val_p, S.K, lane, source, S.seed_bits, S.mark_p);
+                hip_check(hipGetLastError(), "launch lane_seed_mark");
+            }
+            else
+            {
+                lane_seed<<<1, 64, 0, S.stream>>>(S.val_p, S.K, lane, source, S.seed_bits, S.q.as<std::uint32_t>(),
+                                                  S.qmask.as<unsigned long long>(), S.F);
+                hip_check(hipGetLastError(), "launch lane_seed");
+                S.F += 1;
+                S.pairs += 1;
+            }
             S.appended += 1;
             S.running |= 1ULL << lane;
             S.occupied |= 1ULL << lane;
@@ -400,9 +480,34 @@ namespace hyt
             }
         }
 
+        // A hybrid session's round runs to completion here; its counters and
+        // lane mask land where finish_round reads them.
+        void run_hybrid_round(LaneSession &S)
+        {
+            const HybridLaneRound r = hybrid_lanes_round(*S.hy, S.kind, S.K, S.F, S.pairs, S.appended > 0, S.damping,
+                                                         S.eps, S.form, S.hst);
+            S.F = r.F;
+            S.pairs = r.pairs;
+            if (r.F == 0)
+                return;
+            unsigned char *h = S.host->as<unsigned char>();
+            std::memcpy(h, &r.c, sizeof(Counters));
+            std::memcpy(h + sizeof(Counters), &r.lanes, 8);
+            if (r.items)
+                S.item_rounds += 1;
+            hip_check(hipEventRecord(S.done, S.stream), "hipEventRecord");
+            S.launched = true;
+        }
+
         void launch_round(LaneSession &S)
         {
             S.launched = false;
+            if (S.hy)
+            {
+                if (S.F > 0 || S.appended > 0)
+                    run_hybrid_round(S);
+                return;
+            }
             if (S.F == 0)
                 return;
             hip_check(hipSetDevice(S.g->device), "hipSetDevice");
@@ -470,8 +575,8 @@ namespace hyt
             const bool ppr = S.kind == HYT_QUERY_PPR;
             double *partial = S.partial.as<double>();
             const unsigned nb = grid_for(V);
-            lane_gather<<<nb, kBlock, 0, s>>>(S.val.as<std::uint32_t>(), ppr ? S.rank.as<float>() : nullptr, V, S.K,
-                                              lane, S.colbuf.as<std::uint32_t>(), ppr ? partial : nullptr);
+            lane_gather<<<nb, kBlock, 0, s>>>(S.val_p, ppr ? S.rank_p : nullptr, V, S.K, lane,
+                                              S.colbuf.as<std::uint32_t>(), ppr ? partial : nullptr);
             hip_check(hipGetLastError(), "launch lane_gather");
             if (ppr)
             {
@@ -507,11 +612,15 @@ namespace hyt
             for (std::uint32_t r = 0; r < rounds; ++r)
             {
                 bool any = false;
-                for (std::uint32_t i = 0; i < n; ++i)
-                {
-                    launch_round(*ss[i]);
-                    any = any || ss[i]->launched;
-                }
+                // sessions in HBM launch first; a hybrid session's round then
+                // runs while theirs are in flight
+                for (int pass = 0; pass < 2; ++pass)
+                    for (std::uint32_t i = 0; i < n; ++i)
+                        if ((ss[i]->hy != nullptr) == (pass == 1))
+                        {
+                            launch_round(*ss[i]);
+                            any = any || ss[i]->launched;
+                        }
                 if (!any)
                     break;
                 for (std::uint32_t i = 0; i < n; ++i)
@@ -576,11 +685,30 @@ namespace hyt
             return K;
         }
 
-        void serve_trace(const hyt_graph_t *graphs, std::uint32_t ngraphs, const hyt_serve_config &cfg,
-                         const hyt_request *reqs, std::uint64_t n, void *const *results, hyt_completion *out,
-                         hyt_serve_stats &st)
+        // A served graph: resident in HBM, or in host memory behind hybrid
+        // engines (one per query kind).
+        struct ServeGraph
         {
-            require(graphs != nullptr && ngraphs > 0, "no graphs");
+            DeviceGraph *resident = nullptr;
+            HybridEngine *engine[2] = {nullptr, nullptr};
+            DeviceGraph *shape() const
+            {
+                if (resident)
+                    return resident;
+                for (HybridEngine *e : engine)
+                    if (e)
+                        return &hybrid_device_graph(*e);
+                return nullptr;
+            }
+        };
+
+        void serve_trace(const std::vector<ServeGraph> &graphs, const hyt_serve_config &cfg, const hyt_request *reqs,
+                         std::uint64_t n, void *const *results, hyt_completion *out, hyt_serve_stats &st)
+        {
+            const std::uint32_t ngraphs = static_cast<std::uint32_t>(graphs.size());
+            require(ngraphs > 0, "no graphs");
+            for (const ServeGraph &sg : graphs)
+                require(sg.shape() != nullptr, "null graph");
             require(n == 0 || (reqs != nullptr && out != nullptr), "null requests or completions");
             require(cfg.max_lanes >= 1 && cfg.max_lanes <= 64, "max_lanes must be in [1, 64]");
             require(cfg.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");
@@ -588,8 +716,10 @@ namespace hyt
             {
                 require(reqs[i].graph < ngraphs, "request graph index out of range");
                 require(reqs[i].kind == HYT_QUERY_SSSP || reqs[i].kind == HYT_QUERY_PPR, "request kind");
-                require(graphs[reqs[i].graph] != nullptr, "null graph");
-                require(reqs[i].source < graphs[reqs[i].graph]->g.V, "request source out of range");
+                const ServeGraph &sg = graphs[reqs[i].graph];
+                require(sg.resident != nullptr || sg.engine[reqs[i].kind] != nullptr,
+                        "no engine for this query kind on a host-resident graph");
+                require(reqs[i].source < sg.shape()->V, "request source out of range");
                 require(std::isfinite(reqs[i].arrival_s) && reqs[i].arrival_s >= 0.0, "arrival time");
                 require(i == 0 || reqs[i].arrival_s >= reqs[i - 1].arrival_s, "requests must be sorted by arrival");
             }
@@ -604,17 +734,41 @@ namespace hyt
                 {
                     sid[key] = static_cast<int>(skey.size());
                     skey.emplace_back(reqs[i].graph, static_cast<int>(reqs[i].kind));
-                    const DeviceGraph &g = graphs[reqs[i].graph]->g;
-                    fixed.push_back(LaneSession::fixed_bytes(g.V, g.E, g.max_degree));
-                    per_lane.push_back(LaneSession::lane_bytes(g.V, static_cast<int>(reqs[i].kind)));
+                    const ServeGraph &sg = graphs[reqs[i].graph];
+                    const DeviceGraph &g = *sg.shape();
+                    const int kind = static_cast<int>(reqs[i].kind);
+                    if (sg.resident)
+                    {
+                        fixed.push_back(LaneSession::fixed_bytes(g.V, g.E, g.max_degree));
+                        per_lane.push_back(LaneSession::lane_bytes(g.V, kind));
+                    }
+                    else
+                    {
+                        HybridEngine &E = *sg.engine[kind];
+                        const std::uint64_t one = hybrid_lane_state_bytes(E, kind, 1);
+                        const std::uint64_t lane = hybrid_lane_state_bytes(E, kind, 2) - one;
+                        // + the session's own output column, partials and checksum
+                        fixed.push_back(one - lane + (g.V + kPad) * 4 + (kGatherBlocks + 1) * sizeof(double) + 64);
+                        per_lane.push_back(lane);
+                    }
                     demand.push_back(0);
                 }
                 demand[static_cast<std::size_t>(sid[key])] += 1;
             }
             std::uint64_t graph_bytes = 0;
-            for (std::uint32_t i = 0; i < ngraphs; ++i)
-                if (graphs[i])
-                    graph_bytes += graphs[i]->g.device_bytes();
+            std::vector<HybridEngine *> engines;
+            for (const ServeGraph &sg : graphs)
+            {
+                if (sg.resident)
+                    graph_bytes += sg.resident->device_bytes();
+                else
+                    for (HybridEngine *e : sg.engine)
+                        if (e && std::find(engines.begin(), engines.end(), e) == engines.end())
+                        {
+                            engines.push_back(e);
+                            graph_bytes += hybrid_engine_bytes(*e);
+                        }
+            }
             require(cfg.device_budget_bytes > graph_bytes, "device budget smaller than the resident graphs");
             st = hyt_serve_stats{};
             st.graph_bytes = graph_bytes;
@@ -627,8 +781,12 @@ namespace hyt
                 for (std::size_t i = 0; i < S; ++i)
                 {
                     ses.push_back(std::make_unique<LaneSession>());
-                    create_session(*ses.back(), &graphs[skey[i].first]->g, skey[i].second, K[i], cfg.damping,
-                                   cfg.epsilon);
+                    const ServeGraph &sg = graphs[skey[i].first];
+                    if (sg.resident)
+                        create_session(*ses.back(), sg.resident, skey[i].second, K[i], cfg.damping, cfg.epsilon);
+                    else
+                        create_hybrid_session(*ses.back(), *sg.engine[skey[i].second], skey[i].second, K[i],
+                                              cfg.damping, cfg.epsilon);
                     ses.back()->form = cfg.lane_form;
                     st.state_bytes += fixed[i] + K[i] * per_lane[i];
                     if (i < 16)
@@ -637,6 +795,13 @@ namespace hyt
             }
             st.sessions = static_cast<std::uint32_t>(S);
 
+            // sessions in HBM first: a hybrid session's round blocks the loop
+            // while the others' rounds are in flight
+            std::vector<std::size_t> order;
+            for (int pass = 0; pass < 2; ++pass)
+                for (std::size_t i = 0; i < S; ++i)
+                    if ((ses[i]->hy != nullptr) == (pass == 1))
+                        order.push_back(i);
             std::vector<std::deque<std::uint64_t>> waiting(S);
             std::vector<std::vector<std::int64_t>> lane_req(S);
             for (std::size_t i = 0; i < S; ++i)
@@ -654,7 +819,7 @@ namespace hyt
                     ++next;
                 }
                 bool in_flight = false, progress = false;
-                for (std::size_t i = 0; i < S; ++i)
+                for (const std::size_t i : order)
                 {
                     LaneSession &L = *ses[i];
                     if (L.launched)
@@ -734,6 +899,13 @@ namespace hyt
                 st.frontier_rows += L->frontier_rows;
                 st.lane_rows += L->lane_rows;
                 st.item_rounds += L->item_rounds;
+                if (L->hy)
+                {
+                    st.host_bytes += L->hst.bytes_filter + L->hst.bytes_compaction;
+                    st.host_zero_copy_bytes += L->hst.bytes_zero_copy;
+                    st.parts_cached += L->hst.parts_cached;
+                    st.parts_moved += L->hst.parts_filter + L->hst.parts_compaction + L->hst.parts_zero_copy;
+                }
             }
         }
     } // namespace
@@ -840,9 +1012,9 @@ int hyt_lanes_info(hyt_lanes_t s, uint32_t *lanes, uint64_t *occupied, uint64_t 
         if (frontier)
             *frontier = L.F;
         if (state_bytes)
-            *state_bytes = L.val.bytes + L.rank.bytes + L.snap.bytes + L.mark.bytes + L.qmask.bytes + L.q.bytes +
-                           L.heavy.bytes + L.ctr.bytes + L.lanes.bytes + L.colbuf.bytes + L.partial.bytes +
-                           L.acc.bytes;
+            *state_bytes = (L.hy ? hyt::hybrid_lane_state_bytes(*L.hy, L.kind, L.K) : 0) + L.val.bytes +
+                           L.rank.bytes + L.snap.bytes + L.mark.bytes + L.qmask.bytes + L.q.bytes + L.heavy.bytes +
+                           L.ctr.bytes + L.lanes.bytes + L.colbuf.bytes + L.partial.bytes + L.acc.bytes;
     });
 }
 
@@ -865,10 +1037,57 @@ int hyt_serve_trace(const hyt_graph_t *graphs, uint32_t num_graphs, const hyt_se
 {
     return hyt::guarded("hyt_serve_trace", [&] {
         hyt::require(config != nullptr, "null config");
+        hyt::require(graphs != nullptr || num_graphs == 0, "null graph array");
+        std::vector<hyt::ServeGraph> gs(num_graphs);
+        for (uint32_t i = 0; i < num_graphs; ++i)
+        {
+            hyt::require(graphs[i] != nullptr, "null graph");
+            gs[i].resident = &graphs[i]->g;
+        }
         hyt_serve_stats st{};
-        hyt::serve_trace(graphs, num_graphs, *config, requests, num_requests, results, completions, st);
+        hyt::serve_trace(gs, *config, requests, num_requests, results, completions, st);
         if (stats)
             *stats = st;
+    });
+}
+
+int hyt_serve_trace_mixed(const hyt_serve_graph *graphs, uint32_t num_graphs, const hyt_serve_config *config,
+                          const hyt_request *requests, uint64_t num_requests, void *const *results,
+                          hyt_completion *completions, hyt_serve_stats *stats)
+{
+    return hyt::guarded("hyt_serve_trace_mixed", [&] {
+        hyt::require(config != nullptr, "null config");
+        hyt::require(graphs != nullptr || num_graphs == 0, "null graph array");
+        std::vector<hyt::ServeGraph> gs(num_graphs);
+        for (uint32_t i = 0; i < num_graphs; ++i)
+        {
+            if (graphs[i].resident)
+                gs[i].resident = &graphs[i].resident->g;
+            else
+            {
+                gs[i].engine[HYT_QUERY_SSSP] =
+                    graphs[i].sssp_engine ? &hyt::hybrid_engine(graphs[i].sssp_engine) : nullptr;
+                gs[i].engine[HYT_QUERY_PPR] = graphs[i].ppr_engine ? &hyt::hybrid_engine(graphs[i].ppr_engine) : nullptr;
+                hyt::require(gs[i].engine[0] == nullptr || gs[i].engine[0] != gs[i].engine[1],
+                             "SSSP and PPR need an engine each");
+            }
+        }
+        hyt_serve_stats st{};
+        hyt::serve_trace(gs, *config, requests, num_requests, results, completions, st);
+        if (stats)
+            *stats = st;
+    });
+}
+
+int hyt_lanes_create_hybrid(hyt_hybrid_t h, uint32_t kind, uint32_t lanes, float damping, float epsilon,
+                            hyt_lanes_t *out)
+{
+    return hyt::guarded("hyt_lanes_create_hybrid", [&] {
+        hyt::require(h != nullptr && out != nullptr, "null pointer");
+        *out = nullptr;
+        auto s = std::make_unique<hyt_lanes>();
+        hyt::create_hybrid_session(s->s, hyt::hybrid_engine(h), static_cast<int>(kind), lanes, damping, epsilon);
+        *out = s.release();
     });
 }
 

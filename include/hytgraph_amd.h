@@ -452,6 +452,10 @@ typedef struct hyt_serve_stats {
     uint32_t sessions;
     uint32_t lanes[16];           /* lanes of the first 16 sessions (one per graph and kind, in order of first request) */
     uint32_t reserved;
+    /* sessions over hybrid engines: bytes copied over the host link, bytes
+     * read zero-copy (estimated), partition visits from the HBM cache and
+     * visits that moved or mapped a partition */
+    uint64_t host_bytes, host_zero_copy_bytes, parts_cached, parts_moved;
 } hyt_serve_stats;
 
 int hyt_serve_config_default(hyt_serve_config *out); /* 64 GiB, 64 lanes, 0.85, 1e-6 */
@@ -467,6 +471,33 @@ int hyt_serve_config_default(hyt_serve_config *out); /* 64 GiB, 64 lanes, 0.85, 
 int hyt_serve_trace(const hyt_graph_t *graphs, uint32_t num_graphs, const hyt_serve_config *config,
                     const hyt_request *requests, uint64_t num_requests, void *const *results,
                     hyt_completion *completions, hyt_serve_stats *stats);
+
+/*
+ * The same loop over graphs that stay in host memory: a served graph is
+ * either resident in HBM, or (resident NULL) read through hybrid engines over
+ * its pinned host graph, one engine per query kind the trace uses (their edge
+ * areas and vertex state count against the device budget, like resident
+ * graphs). A session over an engine keeps its lanes in the engine's state and
+ * admits queries between the engine's rounds; each round moves, maps or finds
+ * cached every active partition once for all of its lanes. While it serves,
+ * an engine must not be used otherwise.
+ */
+typedef struct hyt_serve_graph {
+    hyt_graph_t resident;
+    hyt_hybrid_t sssp_engine;
+    hyt_hybrid_t ppr_engine;
+} hyt_serve_graph;
+
+int hyt_serve_trace_mixed(const hyt_serve_graph *graphs, uint32_t num_graphs, const hyt_serve_config *config,
+                          const hyt_request *requests, uint64_t num_requests, void *const *results,
+                          hyt_completion *completions, hyt_serve_stats *stats);
+
+/* A lane session over a hybrid engine (the engine's lane state holds its
+ * lanes; one session per engine at a time); every other hyt_lanes_* call
+ * works on it as on an HBM session, hyt_lanes_step runs its rounds after
+ * launching those of the HBM sessions. */
+int hyt_lanes_create_hybrid(hyt_hybrid_t h, uint32_t kind, uint32_t lanes, float damping, float epsilon,
+                            hyt_lanes_t *out);
 
 /* ------------------------------------------------------------- autotuner */
 /*

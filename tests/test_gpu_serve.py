@@ -181,3 +181,105 @@ def test_serve_budget_too_small(gpu, abi):
     with pytest.raises(abi.HytError) as ei:
         abi.serve_trace([g], [(0.0, 0, abi.HYT_QUERY_SSSP, 0)], device_budget_bytes=gb + 1024)
     assert ei.value.status == abi.HYT_ERR_OUT_OF_MEMORY
+
+
+# ----------------------------------------------- graphs served from host memory
+def _pinned(abi, scale, seed, wmax=255):
+    return abi.HostGraph(scale=scale, edge_factor=16, seed=seed, weight_min=1 if wmax else 0, weight_max=wmax,
+                         memory=abi.HYT_HOST_PINNED)
+
+
+# small partitions and an edge budget of a few cached partitions: every
+# round mixes cached, streamed, packed and zero-copy partitions
+HSMALL = dict(partition_edges=2048, pack_edges=8192, device_budget_bytes=8192 * 20 + 16 + 12 * 2048 * 8,
+              calibrate=0)
+
+
+@pytest.mark.parametrize("form", [0, 2])
+def test_hybrid_sssp_session_staggered_admission(gpu, abi, orc, form):
+    hg = _pinned(abi, 13, 21)
+    eng = abi.HybridEngine(hg, **HSMALL)
+    S = abi.LaneSession(eng, abi.HYT_QUERY_SSSP, lanes=4)
+    S.set_form(form)
+    srcs = [int(s) for s in abi.sample_sources(hg.offsets, 9, 10)] + [0]
+    owners = [[None] * 4]
+    results = [[]]
+    S.admit(0, srcs[0])
+    owners[0][0] = srcs[0]
+    abi.LaneSession.step([S], 2)
+    S.admit(1, srcs[1])
+    owners[0][1] = srcs[1]
+    abi.LaneSession.step([S], 1)
+    for lane, s in ((2, srcs[2]), (3, srcs[3])):
+        S.admit(lane, s)
+        owners[0][lane] = s
+    with pytest.raises(abi.InvalidArgument):
+        S.admit(0, 5)
+    _drain(abi, [S], owners, [srcs[4:]], results)
+    assert sorted(r[0] for r in results[0]) == sorted(srcs)
+    for src, d, reached, checksum, nr in results[0]:
+        ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src)
+        assert np.array_equal(d, ref), src
+        fin = ref != 0xFFFFFFFF
+        assert reached == fin.sum() and checksum == int(ref[fin].astype(np.uint64).sum()) and nr > 0
+
+
+def test_hybrid_ppr_session_and_hbm_session_together(gpu, abi, orc):
+    hp = _pinned(abi, 12, 22, wmax=0)
+    eng = abi.HybridEngine(hp, **HSMALL)
+    A = abi.LaneSession(eng, abi.HYT_QUERY_PPR, lanes=3, damping=D, epsilon=1e-8)
+    hs, gs = _graph(abi, 11, 23)
+    B = abi.LaneSession(gs, abi.HYT_QUERY_SSSP, lanes=4)
+    sa = [int(s) for s in abi.sample_sources(hp.offsets, 5, 5)]
+    sb = [int(s) for s in abi.sample_sources(hs.offsets, 6, 6)]
+    owners = [[None] * 3, [None] * 4]
+    for lane in range(3):
+        A.admit(lane, sa[lane])
+        owners[0][lane] = sa[lane]
+    for lane in range(4):
+        B.admit(lane, sb[lane])
+        owners[1][lane] = sb[lane]
+    results = [[], []]
+    _drain(abi, [A, B], owners, [sa[3:], sb[4:]], results)
+    assert len(results[0]) == 5 and len(results[1]) == 6
+    worst = 0.0
+    for seed, r, _, _, _ in results[0]:
+        same, _, _ = orc.delta_push_f64(hp.offsets, hp.columns, D, 1e-8, seed=seed)
+        worst = max(worst, np.abs(r - same).sum())
+        assert np.abs(r - same).sum() <= 2e-6
+        assert abs(r.sum() - 1) < 1e-5
+    for src, d, _, _, _ in results[1]:
+        assert np.array_equal(d, orc.sssp_dijkstra(hs.offsets, hs.columns, hs.weights, src))
+    _say(f"hybrid ppr session: worst L1 vs same-procedure oracle {worst:.2e}")
+
+
+def test_serve_trace_host_resident_graph(gpu, abi, orc):
+    hs, gs = _graph(abi, 11, 31)
+    hp = _pinned(abi, 13, 32)
+    eng_s = abi.HybridEngine(hp, **HSMALL)
+    eng_p = abi.HybridEngine(hp, **HSMALL)
+    rng = np.random.default_rng(8)
+    n = 40
+    arrivals = np.cumsum(rng.exponential(1.0 / 1500.0, n))
+    reqs = []
+    for i in range(n):
+        gi = int(rng.integers(0, 2))
+        kind = abi.HYT_QUERY_SSSP if rng.random() < 0.7 else abi.HYT_QUERY_PPR
+        offs = (hs if gi == 0 else hp).offsets
+        reqs.append((float(arrivals[i]), gi, kind, int(abi.sample_sources(offs, 200 + i, 1)[0])))
+    comp, st, res = abi.serve_trace([gs, (eng_s, eng_p)], reqs, keep_results=True, epsilon=1e-8,
+                                    device_budget_bytes=8 << 30, max_lanes=6)
+    _say(f"serve host-resident: {st}")
+    assert st["admitted"] == n and st["sessions"] == 4
+    assert st["parts_cached"] + st["parts_moved"] > 0 and st["parts_moved"] > 0
+    for i, (t, gi, kind, src) in enumerate(reqs):
+        hg = hs if gi == 0 else hp
+        assert t - 1e-3 <= comp["admit_s"][i] <= comp["done_s"][i]
+        if kind == abi.HYT_QUERY_SSSP:
+            assert np.array_equal(res[i], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src)), i
+        else:
+            same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, 1e-8, seed=src)
+            assert np.abs(res[i] - same).sum() <= 2e-6, i
+    # a host-resident graph needs an engine for every query kind it receives
+    with pytest.raises(abi.InvalidArgument):
+        abi.serve_trace([gs, (eng_s, None)], [(0.0, 1, abi.HYT_QUERY_PPR, 0)])
