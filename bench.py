@@ -251,6 +251,33 @@ def bench_hybrid(args) -> dict:
     }
 
 
+class _ServedGraph:
+    """One served graph: resident in HBM, or in pinned host memory behind
+    two hybrid engines (SSSP, PPR); single-query paths for the baseline."""
+
+    def __init__(self, graph=None, engines=None):
+        self.graph, self.engines = graph, engines
+
+    def sssp(self, src, out_ptr=None):
+        from hytgraph_amd import abi
+        if self.graph is not None:
+            return self.graph.sssp(src, mode=abi.HYT_SSSP_ASYNC, out_ptr=out_ptr)[0]
+        return self.engines[0].sssp(src, out_ptr=out_ptr)[0]
+
+    def ppr(self, src, eps, out_ptr):
+        if self.graph is not None:
+            self.graph.ppr(src, 0.85, eps, out_ptr=out_ptr)
+        else:
+            self.engines[1].ppr_k([src], 0.85, eps, out_ptr=out_ptr)
+
+    def trim(self):
+        if self.graph is not None:
+            self.graph.trim()
+
+    def served(self):
+        return self.graph if self.graph is not None else self.engines
+
+
 def bench_serve(args) -> dict:
     """Served form: several HBM-resident RMAT graphs, a 70/30 mix of SSSP and
     PPR queries on degree-weighted sources, continuous batching into K-lane
@@ -262,17 +289,24 @@ def bench_serve(args) -> dict:
     from hytgraph_amd import abi
 
     scales = [int(x) for x in args.serve_scales.split(",")]
+    host_scales = {int(x) for x in args.serve_host.split(",") if x}
     graphs, hosts, pools, degs = [], [], [], []
     for i, sc in enumerate(scales):
         t0 = time.perf_counter()
+        on_host = sc in host_scales
         hg = abi.HostGraph(scale=sc, edge_factor=args.edge_factor, seed=args.seed + i, weight_min=1,
-                           weight_max=255)
-        g = abi.DeviceGraph.from_host(hg)
+                           weight_max=255, memory=abi.HYT_HOST_PINNED if on_host else abi.HYT_HOST_PAGEABLE)
+        if on_host:
+            cfg = dict(device_budget_bytes=args.serve_edge_gb << 30, partition_edges=1 << args.part_log2)
+            g = _ServedGraph(engines=(abi.HybridEngine(hg, **cfg), abi.HybridEngine(hg, **cfg)))
+        else:
+            g = _ServedGraph(graph=abi.DeviceGraph.from_host(hg))
         graphs.append(g)
         hosts.append(hg)
         pools.append(abi.sample_sources(hg.offsets, args.seed + 100 + i, 4096))
         degs.append(np.diff(hg.offsets.astype(np.int64)))
-        log(f"graph {i}: RMAT scale {sc} V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s")
+        log(f"graph {i}: RMAT scale {sc} V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s"
+            + (f" (host memory, hybrid engines with {args.serve_edge_gb} GiB edge budget each)" if on_host else ""))
     rng = np.random.default_rng(args.seed)
 
     def trace(n, qps):
@@ -294,18 +328,18 @@ def bench_serve(args) -> dict:
     alone_sum = {}
     for i, (_, gi, kind, src) in enumerate(burst[:min(nb, 16)]):  # warm-up + checksums of SSSP results
         if kind == abi.HYT_QUERY_SSSP:
-            d, _ = graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC)
+            d = graphs[gi].sssp(src)
             fin = d != 0xFFFFFFFF
             alone_sum[i] = (int(fin.sum()), int(d[fin].astype(np.uint64).sum()))
         else:
-            graphs[gi].ppr(src, 0.85, eps, out_ptr=dev_out.data_ptr())
+            graphs[gi].ppr(src, eps, dev_out.data_ptr())
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _, gi, kind, src in burst[:nb]:
         if kind == abi.HYT_QUERY_SSSP:
-            graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC, out_ptr=dev_out.data_ptr())
+            graphs[gi].sssp(src, dev_out.data_ptr())
         else:
-            graphs[gi].ppr(src, 0.85, eps, out_ptr=dev_out.data_ptr())
+            graphs[gi].ppr(src, eps, dev_out.data_ptr())
     torch.cuda.synchronize()
     base_s = time.perf_counter() - t0
     base_qps = nb / base_s
@@ -315,7 +349,7 @@ def bench_serve(args) -> dict:
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _, gi, _, src in sssp_only:
-        graphs[gi].sssp(src, mode=abi.HYT_SSSP_ASYNC, out_ptr=dev_out.data_ptr())
+        graphs[gi].sssp(src, dev_out.data_ptr())
     torch.cuda.synchronize()
     sssp1_qps = len(sssp_only) / (time.perf_counter() - t0)
     log(f"one at a time: {nb} mixed queries in {base_s:.2f} s = {base_qps:.1f} queries/s; "
@@ -326,7 +360,8 @@ def bench_serve(args) -> dict:
         g.trim()
 
     # capacity: the whole burst queued at t=0
-    comp, st, _ = abi.serve_trace(graphs, burst, device_budget_bytes=budget, epsilon=eps)
+    served = [g.served() for g in graphs]
+    comp, st, _ = abi.serve_trace(served, burst, device_budget_bytes=budget, epsilon=eps)
     cap_s = float(comp["done_s"].max())
     cap_qps = len(burst) / cap_s
     for i, (reached, checksum) in alone_sum.items():
@@ -335,12 +370,15 @@ def bench_serve(args) -> dict:
     occ = st["lane_rows"] / max(st["frontier_rows"], 1)
     log(f"burst: {len(burst)} queries in {cap_s:.2f} s = {cap_qps:.1f} queries/s "
         f"({cap_qps / base_qps:.2f}x the mix one at a time, {cap_qps / sssp1_qps:.2f}x plain SSSP one at a time); lanes {st['lanes']}, state {st['state_bytes'] / 2**30:.1f} GiB; "
-        f"queries per frontier row {occ:.2f}; {len(alone_sum)} SSSP checksums equal to one-at-a-time")
+        f"queries per frontier row {occ:.2f}; {len(alone_sum)} SSSP checksums equal to one-at-a-time"
+        + (f"; host memory: {st['host_bytes'] / 1e9:.1f} GB copied, {st['host_zero_copy_bytes'] / 1e9:.1f} GB "
+           f"zero-copy, partition visits cached/moved {st['parts_cached']}/{st['parts_moved']}"
+           if host_scales else ""))
 
     # latency: Poisson arrivals at a fixed offered rate
     qps = args.qps if args.qps > 0 else 0.7 * cap_qps
     pois = trace(args.serve_requests, qps)
-    comp2, st2, _ = abi.serve_trace(graphs, pois, device_budget_bytes=budget, epsilon=eps)
+    comp2, st2, _ = abi.serve_trace(served, pois, device_budget_bytes=budget, epsilon=eps)
     arr = np.array([r[0] for r in pois])
     lat = comp2["done_s"] - arr
     kinds = np.array([r[2] for r in pois])
@@ -370,9 +408,13 @@ def bench_serve(args) -> dict:
         "data": "synthetic",
         "config": {
             "workload": f"served mixed SSSP (70%) / PPR (30%, eps {eps:g}, d 0.85) queries on degree-weighted "
-                        f"sources over RMAT scales {scales} ef{args.edge_factor} (u32 weights [1,255]), all in HBM, "
-                        f"{args.budget_gb} GiB device budget, continuous batching into K-lane sessions",
+                        f"sources over RMAT scales {scales} ef{args.edge_factor} (u32 weights [1,255]), "
+                        + (f"scales {sorted(host_scales)} kept in pinned host memory behind hybrid engines "
+                           f"({args.serve_edge_gb} GiB edge budget each, one per query kind), the rest in HBM, "
+                           if host_scales else "all in HBM, ")
+                        + f"{args.budget_gb} GiB device budget, continuous batching into K-lane sessions",
             "scales": scales,
+            "host_resident_scales": sorted(host_scales),
             "budget_gib": args.budget_gb,
             "requests": len(burst),
         },
@@ -384,7 +426,9 @@ def bench_serve(args) -> dict:
                   "state_gib": st["state_bytes"] / 2**30, "graph_gib": st["graph_bytes"] / 2**30,
                   "session_rounds": st["rounds"], "edges": st["edges"],
                   "queries_per_frontier_row": occ,
-                  "sssp_checksums_equal_alone": len(alone_sum)},
+                  "sssp_checksums_equal_alone": len(alone_sum),
+                  "host_bytes": st["host_bytes"], "host_zero_copy_bytes": st["host_zero_copy_bytes"],
+                  "parts_cached": st["parts_cached"], "parts_moved": st["parts_moved"]},
         "poisson": {"offered_qps": qps, "served_qps": served_qps, "p50_ms": pct(lat, 50), "p99_ms": pct(lat, 99),
                     "sssp_p50_ms": pct(lat[kinds == 0], 50), "sssp_p99_ms": pct(lat[kinds == 0], 99),
                     "ppr_p50_ms": pct(lat[kinds == 1], 50), "ppr_p99_ms": pct(lat[kinds == 1], 99),
@@ -605,6 +649,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-baseline", type=int, default=60, help="serve: queries timed one at a time")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: offered rate (0 = 0.7 x burst capacity)")
+    ap.add_argument("--serve-host", default="", help="serve: scales kept in host memory behind hybrid engines")
+    ap.add_argument("--serve-edge-gb", type=int, default=4, help="serve: HBM edge budget per hybrid engine (GiB)")
     ap.add_argument("--ppr-eps", type=float, default=1e-6)
     ap.add_argument("--ks", default="8,32,64", help="batched: lanes per call")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy: MiB per copy")

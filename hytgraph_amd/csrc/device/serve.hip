@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <future>
 #include <thread>
 #include <vector>
 
@@ -175,6 +176,7 @@ namespace hyt
         // through the marks; otherwise the session owns its state.
         HybridEngine *hy = nullptr;
         hyt_hybrid_stats hst{};
+        std::future<HybridLaneRound> hy_round; // the hybrid round in flight (worker thread)
         std::uint32_t *val_p = nullptr; // [V][K] lane state (own or the engine's)
         float *rank_p = nullptr;
         unsigned long long *mark_p = nullptr;
@@ -203,6 +205,8 @@ namespace hyt
 
         ~LaneSession()
         {
+            if (hy_round.valid())
+                hy_round.wait();
             if (stream)
             {
                 (void)hipStreamSynchronize(stream);
@@ -480,23 +484,38 @@ namespace hyt
             }
         }
 
-        // A hybrid session's round runs to completion here; its counters and
-        // lane mask land where finish_round reads them.
+        // A hybrid session's round runs on a worker thread (it plans on the
+        // host between its kernels and copies); nothing else touches the
+        // session until the round is collected by finish_round.
         void run_hybrid_round(LaneSession &S)
         {
-            const HybridLaneRound r = hybrid_lanes_round(*S.hy, S.kind, S.K, S.F, S.pairs, S.appended > 0, S.damping,
-                                                         S.eps, S.form, S.hst);
+            HybridEngine *E = S.hy;
+            const int kind = S.kind;
+            const std::uint32_t K = S.K, F = S.F, form = S.form;
+            const std::uint64_t pairs = S.pairs;
+            const bool merge = S.appended > 0;
+            const float damping = S.damping, eps = S.eps;
+            hyt_hybrid_stats *st = &S.hst;
+            const int device = S.g->device;
+            S.hy_round = std::async(std::launch::async, [=] {
+                hip_check(hipSetDevice(device), "hipSetDevice");
+                return hybrid_lanes_round(*E, kind, K, F, pairs, merge, damping, eps, form, *st);
+            });
+            S.launched = true;
+        }
+
+        // Collects the worker's round: its counters and lane mask land where
+        // finish_round reads them.
+        void collect_hybrid_round(LaneSession &S)
+        {
+            const HybridLaneRound r = S.hy_round.get();
             S.F = r.F;
             S.pairs = r.pairs;
-            if (r.F == 0)
-                return;
             unsigned char *h = S.host->as<unsigned char>();
             std::memcpy(h, &r.c, sizeof(Counters));
             std::memcpy(h + sizeof(Counters), &r.lanes, 8);
             if (r.items)
                 S.item_rounds += 1;
-            hip_check(hipEventRecord(S.done, S.stream), "hipEventRecord");
-            S.launched = true;
         }
 
         void launch_round(LaneSession &S)
@@ -525,7 +544,10 @@ namespace hyt
             if (!S.launched)
                 return 0;
             S.launched = false;
-            hip_check(hipEventSynchronize(S.done), "lane round");
+            if (S.hy)
+                collect_hybrid_round(S);
+            else
+                hip_check(hipEventSynchronize(S.done), "lane round");
             const unsigned char *h = S.host->as<unsigned char>();
             Counters c;
             std::memcpy(&c, h, sizeof(Counters));
@@ -553,6 +575,8 @@ namespace hyt
         // True when the session's round in flight has completed.
         bool round_ready(LaneSession &S)
         {
+            if (S.hy)
+                return S.hy_round.wait_for(std::chrono::seconds(0)) == std::future_status::ready;
             const hipError_t e = hipEventQuery(S.done);
             if (e == hipErrorNotReady)
                 return false;
