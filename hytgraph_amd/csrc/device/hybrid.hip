@@ -633,8 +633,10 @@ namespace hyt
             const std::uint64_t avail = budget - pack_bytes;
             const std::uint64_t ring_unit = kRingBuffers * E.unit_edges * E.edge_bytes; // ring of one partition each
             require(avail >= ring_unit, "device budget too small for the streaming ring");
-            const std::uint64_t k = cfg.filter_combine_k ? cfg.filter_combine_k : 4;
-            const std::uint64_t ring_k = std::max<std::uint64_t>(1, std::min<std::uint64_t>(k, avail / 2 / ring_unit));
+            // ring windows of up to max(k, 8) partitions, within half of what is left
+            const std::uint64_t k = std::max<std::uint64_t>(cfg.filter_combine_k ? cfg.filter_combine_k : 4, 8);
+            const std::uint64_t ring_k =
+                std::max<std::uint64_t>(1, std::min<std::uint64_t>(k, avail / 2 / ring_unit));
             E.ring_edges = ring_k * E.unit_edges;
             E.ring_mem.alloc(ring_k * ring_unit, "hipMalloc(streaming ring)");
             for (hipEvent_t &ev : E.ring_used)
@@ -932,14 +934,18 @@ namespace hyt
                 st.parts_cached += plan.cached.size();
             }
 
-            // 3. filter: each run of consecutive partitions streams through the
-            // ring in windows of ring_edges (one copy per array per window);
-            // whole partitions of a window are then promoted into the cache
+            // 3. filter: each run of consecutive filter partitions (across the
+            // combiner's tasks, which group at most k of them for the cost
+            // model) streams through the ring in windows of ring_edges (one
+            // copy per array per window); whole partitions of a window are
+            // then promoted into the cache
+            std::vector<std::uint64_t> filter_parts;
             for (const Task &t : plan.tasks)
+                if (t.engine == HYT_ENGINE_FILTER)
+                    filter_parts.insert(filter_parts.end(), t.members.begin(), t.members.end());
+            std::sort(filter_parts.begin(), filter_parts.end());
             {
-                if (t.engine != HYT_ENGINE_FILTER)
-                    continue;
-                for_runs(t.members, [&](std::uint64_t p0, std::uint64_t p1) {
+                for_runs(filter_parts, [&](std::uint64_t p0, std::uint64_t p1) {
                     st.parts_filter += p1 - p0 + 1;
                     const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
                     std::uint64_t pa = p0;
@@ -981,19 +987,33 @@ namespace hyt
                 });
             }
 
-            // 4. zero-copy: one window per run of consecutive partitions
+            // 4. zero-copy: every partition of the zero-copy task in one launch
+            // over the mapped host arrays (segment table after the resident
+            // one; a segment's rows sit at their own edge positions)
             for (const Task &t : plan.tasks)
             {
-                if (t.engine != HYT_ENGINE_ZERO_COPY)
+                if (t.engine != HYT_ENGINE_ZERO_COPY || t.members.empty())
                     continue;
-                for_runs(t.members, [&](std::uint64_t p0, std::uint64_t p1) {
-                    const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
-                    for (std::uint64_t p = p0; p <= p1; ++p) // estimated link bytes: m per modelled request
-                        st.bytes_zero_copy += E.part_stats_h[4 * p + 2] * E.run_params.request_bytes;
-                    x.window(plan.qlo[p0], rows_of(p0, p1), e0, e1, E.z_col + e0,
-                             weighted ? E.z_w + e0 : nullptr);
-                    st.parts_zero_copy += p1 - p0 + 1;
-                });
+                const std::size_t base = plan.cached.size();
+                std::uint64_t rows = 0;
+                std::size_t nz = 0;
+                for (const std::uint64_t p : t.members)
+                {
+                    // estimated link bytes: m per modelled request
+                    st.bytes_zero_copy += E.part_stats_h[4 * p + 2] * E.run_params.request_bytes;
+                    CacheSeg &sg = E.seg_h[base + nz++];
+                    sg.qlo = static_cast<std::uint32_t>(plan.qlo[p]);
+                    sg.rows_before = static_cast<std::uint32_t>(rows);
+                    sg.edge_begin = E.parts[p].edge_begin;
+                    sg.unit_pos = E.parts[p].edge_begin;
+                    rows += plan.qn[p];
+                }
+                hip_check(hipMemcpyAsync(E.seg_d.as<CacheSeg>() + base, E.seg_h.data() + base, nz * sizeof(CacheSeg),
+                                         hipMemcpyHostToDevice, s),
+                          "upload zero-copy segments");
+                x.cached(E.seg_d.as<CacheSeg>() + base, static_cast<std::uint32_t>(nz), static_cast<std::uint32_t>(rows),
+                         E.z_col, weighted ? E.z_w : nullptr);
+                st.parts_zero_copy += nz;
             }
 
             // 5. compaction: pack the rows read back in step 1 on host threads,
