@@ -82,11 +82,13 @@ namespace hyt
         __device__ void apply(std::uint32_t v, std::uint32_t c) const
         {
             // A plain load may return an older, larger value; that only
-            // costs an extra atomic, it never skips a needed one.
+            // costs an extra atomic, it never skips a needed one. Without
+            // touch[] the compaction compares every row's dist with prev.
             if (c < dist[v])
             {
                 atomicMin(&dist[v], c);
-                touch[v] = 1;
+                if (touch != nullptr)
+                    touch[v] = 1;
             }
         }
     };
@@ -128,6 +130,8 @@ namespace hyt
         __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
                                          Aux &aux) const
         {
+            if (touch == nullptr)
+                return scan4(row0, n, vals, aux);
             // touch has spare bytes past V, so the 4-byte word is in range
             std::uint32_t *tw = reinterpret_cast<std::uint32_t *>(touch + row0);
             const std::uint32_t t = *tw;
@@ -160,6 +164,39 @@ namespace hyt
             }
             if (keep != t)
                 *tw = keep;
+            return m;
+        }
+
+        // Touch-free mode: rows improved since the last compaction are the
+        // rows with dist < prev (row0 is a multiple of 4 and both arrays have
+        // spare slots past V, so the 16-byte vector accesses stay in range).
+        __device__ std::uint32_t scan4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4], Aux &aux) const
+        {
+            const uint4 d4 = *reinterpret_cast<const uint4 *>(dist + row0);
+            const uint4 p4 = *reinterpret_cast<const uint4 *>(prev + row0);
+            const std::uint32_t d[4] = {d4.x, d4.y, d4.z, d4.w};
+            std::uint32_t p[4] = {p4.x, p4.y, p4.z, p4.w};
+            std::uint32_t m = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                if (static_cast<std::uint32_t>(j) < n && d[j] < p[j])
+                {
+                    if (d[j] < bound)
+                    {
+                        p[j] = d[j];
+                        vals[j] = d[j];
+                        m |= 1U << j;
+                    }
+                    else
+                    {
+                        aux.pending += 1;
+                        aux.inv_max = (~d[j] > aux.inv_max) ? ~d[j] : aux.inv_max;
+                    }
+                }
+            }
+            if (m)
+                *reinterpret_cast<uint4 *>(prev + row0) = make_uint4(p[0], p[1], p[2], p[3]);
             return m;
         }
     };

@@ -15,7 +15,8 @@
 //   * compaction: a dense scan of touch[] keeps v when dist[v] < prev[v]
 //     (prev = distance at the end of the round v last improved in), so the
 //     next frontier is exactly the improved set, and records dist[v] as the
-//     snapshot value of the next round;
+//     snapshot value of the next round; calls that never pull skip touch[]
+//     and compare dist with prev on every row instead;
 //   * SYNC mode expands with the snapshot values (Jacobi), which reproduces
 //     the original iteration count; ASYNC mode reads live distances;
 //   * direction: a round whose frontier out-edges are a large share of E runs
@@ -83,7 +84,8 @@ namespace hyt
             if (threadIdx.x == 0 && blockIdx.x == 0)
             {
                 dist[src] = zero;
-                touch[src] = 1;
+                if (touch != nullptr)
+                    touch[src] = 1;
             }
         }
 
@@ -122,9 +124,17 @@ namespace hyt
 
             launch_fill<std::uint32_t>(ws.dist.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
             launch_fill<std::uint32_t>(ws.prev.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
-            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), ws.touch.as<std::uint8_t>(), src, 0U);
-            hip_check(hipGetLastError(), "launch seed");
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
+
+            const std::uint32_t dir = L.o.direction;
+            const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
+            const bool may_pull = dir == HYT_DIR_PULL || (dir == HYT_DIR_AUTO && pct <= 100);
+            // Push-only calls find improved rows by comparing dist with prev in
+            // the compaction instead of a byte store per improvement (random
+            // stores that each move a cache line); pull rounds need touch[].
+            std::uint8_t *touch = may_pull ? ws.touch.as<std::uint8_t>() : nullptr;
+            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), touch, src, 0U);
+            hip_check(hipGetLastError(), "launch seed");
 
             // delta-stepping buckets (ASYNC only): admit improved rows below
             // `bound`; when none is left, move the bound past the nearest
@@ -143,12 +153,9 @@ namespace hyt
                 const std::uint64_t b = static_cast<std::uint64_t>(d) + L.o.delta;
                 return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
             };
-            SsspSel sel{ws.touch.as<std::uint8_t>(), ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(),
+            SsspSel sel{touch, ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(),
                         bucketed ? bound_after(0U) : 0xFFFFFFFFU};
 
-            const std::uint32_t dir = L.o.direction;
-            const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
-            const bool may_pull = dir == HYT_DIR_PULL || (dir == HYT_DIR_AUTO && pct <= 100);
             if (may_pull && g.E > 0 && g.E < (1ULL << 31))
                 ensure_transpose(g, L.light_max, L.chunk);
             // a transpose left by an earlier call is only used when this call may pull
@@ -173,7 +180,7 @@ namespace hyt
             op.col = g.col.as<std::uint32_t>();
             op.w = g.w.as<std::uint32_t>();
             op.dist = ws.dist.as<std::uint32_t>();
-            op.touch = ws.touch.as<std::uint8_t>();
+            op.touch = touch;
             op.qd = ws.qd.as<std::uint32_t>();
             op.overflow = &d_ctr->overflow;
             op.expanded_at = SYNC ? nullptr : ws.prev.as<std::uint32_t>();
