@@ -8,6 +8,9 @@
 #include "graph/csr_graph.hpp"
 #include "graph/graph_loader.hpp"
 #include "graph/partition.hpp"
+#include "runtime/config.hpp"
+#include "runtime/logging.hpp"
+#include "runtime/result.hpp"
 #include "scheduling/contribution_scheduler.hpp"
 #include "transfer/compaction_engine.hpp"
 #include "transfer/filter_engine.hpp"
@@ -18,6 +21,7 @@
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <iterator>
 #include <random>
 #include <stdexcept>
 #include <string>
@@ -344,6 +348,49 @@ namespace
         for (const auto &p : {bad_id, no_w, junk})
             std::filesystem::remove(p);
     }
+    // the original's runtime known answers (tests/unit_tests.cpp:154-204)
+    void test_runtime_known_answers()
+    {
+        namespace rt = hytgraph::runtime;
+        rt::Config config;
+        config.set("algorithm", "pagerank");
+        config.set("partition.size_bytes", "33554432");
+        config.set("cache.enabled", "true");
+        config.set("transfer.gamma", "0.625");
+        expect(config.contains("algorithm") && config.get("algorithm") == "pagerank", "config: string lookup");
+        expect(config.get_int("partition.size_bytes") == 33554432, "config: integer");
+        expect(config.get_bool("cache.enabled"), "config: boolean");
+        expect(config.get_double("transfer.gamma") == 0.625, "config: double");
+        expect(config.get_or("missing", "fallback") == "fallback", "config: fallback");
+        expect_throw<std::out_of_range>([&] { (void)config.get("missing"); }, "config: missing key");
+        expect_throw<std::invalid_argument>([&] { config.set("", "x"); }, "config: empty key");
+        config.set("flag", "Maybe");
+        expect_throw<std::invalid_argument>([&] { (void)config.get_bool("flag"); }, "config: bad boolean");
+        config.set("flag", "OFF");
+        expect(!config.get_bool("flag"), "config: OFF");
+
+        rt::Result result;
+        result.experiment_id = "unit-test";
+        result.timestamp_utc = "2026-01-01T00:00:00Z";
+        result.config["algorithm"] = "pagerank";
+        result.measurements["runtime_seconds"] = 1.25;
+        result.measurements["gteps"] = 0.1;
+        const std::string json = rt::result_to_json(result);
+        for (const char *want : {"\"schema_version\": \"0.1\"", "\"experiment_id\": \"unit-test\"",
+                                 "\"runtime_seconds\": 1.25", "\"gteps\": 0.1", "\"status\": \"success\"",
+                                 "\"algorithm\": \"pagerank\""})
+            expect(json.find(want) != std::string::npos, std::string("result json: ") + want);
+        const auto path = std::filesystem::temp_directory_path() / "hytgraph_amd_result.json";
+        expect(rt::write_result_json(result, path.string()), "result file writes");
+        std::ifstream file(path);
+        std::string back((std::istreambuf_iterator<char>(file)), std::istreambuf_iterator<char>());
+        expect(back == json, "result file content");
+        std::filesystem::remove(path);
+        expect(!rt::write_result_json(result, "/nonexistent/dir/r.json"), "result file: unwritable path");
+        rt::set_log_level(rt::LogLevel::Error);
+        rt::log(rt::LogLevel::Info, "filtered");
+        rt::set_log_level(rt::LogLevel::Info);
+    }
 } // namespace
 
 int main()
@@ -356,6 +403,7 @@ int main()
         test_scheduler_known_answers();
         test_random_graphs();
         test_graph_loader_known_answers();
+        test_runtime_known_answers();
         std::cout << "planning API tests passed.\n";
         return 0;
     }
