@@ -12,10 +12,12 @@
 #include "runtime/logging.hpp"
 #include "runtime/result.hpp"
 #include "scheduling/contribution_scheduler.hpp"
+#include "scheduling/hub_sort.hpp"
 #include "transfer/compaction_engine.hpp"
 #include "transfer/filter_engine.hpp"
 #include "transfer/zero_copy_engine.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <filesystem>
@@ -348,6 +350,90 @@ namespace
         for (const auto &p : {bad_id, no_w, junk})
             std::filesystem::remove(p);
     }
+    // the original's hub-sort known answers (tests/unit_tests.cpp:2468-2749),
+    // and on seeded random graphs the ranked prefix equals a full ranking of
+    // every vertex (recomputed here from the degrees)
+    void test_hub_sort_known_answers()
+    {
+        using sc::HubSorter;
+        using sc::HubSortOptions;
+        {
+            const CSRGraph g(3, Offs{0, 2, 3, 4}, Ids{1, 2, 2, 0});
+            const auto r = HubSorter(HubSortOptions{2.0 / 3.0}).sort(g);
+            expect(r.scores.size() == 3 && std::abs(r.scores[0] - 0.5) < 1e-12 && std::abs(r.scores[1] - 0.25) < 1e-12 &&
+                       std::abs(r.scores[2] - 0.5) < 1e-12,
+                   "hub scores 0.5 / 0.25 / 0.5");
+        }
+        {
+            CSRGraph g(5, Offs{0, 3, 4, 5, 6, 7}, Ids{1, 2, 3, 0, 0, 0, 0});
+            const auto r = HubSorter(HubSortOptions{0.4}).sort(g);
+            expect(r.hub_count == 2 && r.vertex_order == Ids({0, 1, 2, 3, 4}), "hubs first, others in order");
+            g.reorder_vertices(r.vertex_order);
+            expect(g.row_offsets() == Offs({0, 3, 4, 5, 6, 7}) && g.column_indices() == Ids({1, 2, 3, 0, 0, 0, 0}),
+                   "hub order applied to the CSR");
+        }
+        {
+            const CSRGraph g(6, Offs{0, 2, 3, 4, 5, 6, 6}, Ids{1, 2, 2, 3, 0, 0});
+            const auto r = HubSorter().sort(g);
+            std::vector<bool> seen(6, false);
+            for (const auto v : r.vertex_order)
+            {
+                expect(v < 6 && !seen[v], "hub order is a permutation");
+                seen[v] = true;
+            }
+            expect(r.vertex_order.size() == 6, "hub order covers every vertex");
+        }
+        {
+            const CSRGraph g(4, Offs{0, 0, 0, 0, 0}, Ids{});
+            const auto r = HubSorter().sort(g);
+            expect(r.hub_count == 1 && r.vertex_order == Ids({0, 1, 2, 3}), "edgeless graph: one hub, natural order");
+            for (const double x : r.scores)
+                expect(x == 0.0, "edgeless graph: zero scores");
+        }
+        expect_throw<std::invalid_argument>([] { HubSorter(HubSortOptions{1.5}); }, "hub fraction above 1");
+        expect_throw<std::invalid_argument>([] { HubSorter(HubSortOptions{std::nan("")}); }, "hub fraction NaN");
+        std::mt19937_64 rng(11);
+        for (int t = 0; t < 20; ++t)
+        {
+            const std::uint32_t V = 50 + static_cast<std::uint32_t>(rng() % 400);
+            Offs off(V + 1, 0);
+            Ids col;
+            for (std::uint32_t v = 0; v < V; ++v)
+            {
+                const std::uint32_t d = static_cast<std::uint32_t>(rng() % 9 == 0 ? rng() % 40 : rng() % 4);
+                for (std::uint32_t k = 0; k < d; ++k)
+                    col.push_back(static_cast<std::uint32_t>(rng() % 7 == 0 ? rng() % 5 : rng() % V));
+                off[v + 1] = col.size();
+            }
+            const CSRGraph g(V, off, col);
+            const double frac = static_cast<double>(rng() % 101) / 100.0;
+            const auto r = HubSorter(HubSortOptions{frac}).sort(g);
+            std::vector<std::uint64_t> in(V, 0);
+            for (auto c : col)
+                ++in[c];
+            std::vector<std::uint32_t> all(V);
+            for (std::uint32_t v = 0; v < V; ++v)
+                all[v] = v;
+            const auto key = [&](std::uint32_t v) {
+                return static_cast<long double>(off[v + 1] - off[v]) * static_cast<long double>(in[v]);
+            };
+            std::stable_sort(all.begin(), all.end(), [&](std::uint32_t a, std::uint32_t b) { return key(a) > key(b); });
+            const double want = V * frac;
+            const std::size_t hubs = std::min<std::size_t>(
+                V, static_cast<std::size_t>(std::fabs(want - std::round(want)) < 1e-12 ? std::round(want) : std::ceil(want)));
+            expect(r.hub_count == hubs, "random: hub count");
+            for (std::size_t i = 0; i < hubs; ++i)
+                expect(r.vertex_order[i] == all[i], "random: hub prefix equals the full ranking");
+            std::size_t at = hubs;
+            std::vector<bool> hub(V, false);
+            for (std::size_t i = 0; i < hubs; ++i)
+                hub[all[i]] = true;
+            for (std::uint32_t v = 0; v < V; ++v)
+                if (!hub[v])
+                    expect(r.vertex_order[at++] == v, "random: others in natural order");
+        }
+    }
+
     // the original's runtime known answers (tests/unit_tests.cpp:154-204)
     void test_runtime_known_answers()
     {
@@ -404,6 +490,7 @@ int main()
         test_random_graphs();
         test_graph_loader_known_answers();
         test_runtime_known_answers();
+        test_hub_sort_known_answers();
         std::cout << "planning API tests passed.\n";
         return 0;
     }
