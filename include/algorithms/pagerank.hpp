@@ -48,6 +48,11 @@ namespace hytgraph::algorithms
     // within max_iterations rounds (0 = unlimited).
     PageRankResult pagerank_gpu(const graph::CSRGraph &graph, const PageRankOptions &options = {});
 
+    // The original's CPU Jacobi iteration (src/algorithms/pagerank.cpp:89-163),
+    // declared for source compatibility only: this library replaces the GPU
+    // path, and the definition comes from the original's CPU sources.
+    PageRankResult pagerank_cpu(const graph::CSRGraph &graph, const PageRankOptions &options = {});
+
     // Personalized PageRank (addition) for up to 64 seeds in one [V][K]
     // traversal; ranks[k] sums to 1. epsilon is the per-vertex residual
     // threshold.

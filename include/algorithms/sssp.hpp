@@ -45,6 +45,11 @@ namespace hytgraph::algorithms
     // no device is available or a HIP call fails.
     SSSPResult sssp_gpu(const graph::CSRGraph &graph, const SSSPOptions &options = {});
 
+    // The original's CPU Bellman-Ford (src/algorithms/sssp.cpp:109-234),
+    // declared for source compatibility only: this library replaces the GPU
+    // path, and the definition comes from the original's CPU sources.
+    SSSPResult sssp_cpu(const graph::CSRGraph &graph, const SSSPOptions &options = {});
+
     // Batched form (addition): K <= 64 sources in one [V][K] traversal.
     // distances[k] holds source k's distances (+inf unreachable).
     struct SSSPBatchResult

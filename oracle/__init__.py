@@ -39,6 +39,7 @@ def build(with_ref: bool = True) -> None:
     if with_ref and REFERENCE_TREE.exists():
         subprocess.run(["make", "-s", "-C", str(HERE), "ref", f"REF={REFERENCE_TREE}"], check=True)
         subprocess.run(["make", "-s", "-C", str(HERE), "sep", f"REF={REFERENCE_TREE}"], check=True)
+        subprocess.run(["make", "-s", "-C", str(HERE), "unit", f"REF={REFERENCE_TREE}"], check=True)
 
 
 def _load_orc():

@@ -1,0 +1,24 @@
+"""The original project's own C++ tests against this repository's C++ API:
+tests/unit_tests.cpp (CSRGraph, loader, partitioner, activity tracker, the
+ExpTM/ImpTM planners, HyTM cost model, task combiner, hub sort, contribution
+scheduler, runtime config/result) and tests/algorithm_tests.cpp (CPU
+algorithms and activity statistics, with the original's CPU algorithm
+sources), compiled where they lie against include/ and linked with
+libhytgraph_cpp.so by `make -C oracle unit`. Skipped where the original tree
+was not available to build them."""
+import subprocess
+from pathlib import Path
+
+import pytest
+
+UNIT = Path(__file__).resolve().parent.parent / "oracle" / "_ref" / "unit"
+
+
+@pytest.mark.parametrize("name", ["unit_tests", "algorithm_tests"])
+def test_original_cpp_tests_pass_against_this_api(name):
+    exe = UNIT / name
+    if not exe.exists():
+        pytest.skip("original C++ tests not built (original tree absent)")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "passed" in r.stdout
