@@ -1,16 +1,19 @@
 // Drop-in GPU algorithms over the C ABI: sssp_gpu / pagerank_gpu with the
-// signatures of the original include/algorithms/{sssp,pagerank}.hpp, plus the
-// batched additions and the host contribution functions.
+// signatures of the original include/algorithms/{sssp,pagerank}.hpp, and the
+// host contribution functions. (The batched additions are in
+// batched_api.cpp.)
 //
-// This translation unit only uses CSRGraph's public accessors, so it builds
-// against this repository's include/ or the original project's include/
-// unchanged (it replaces cuda/sssp.cu and cuda/pagerank.cu there).
+// This translation unit only uses CSRGraph's public accessors and the
+// original's declarations, so it builds against this repository's include/
+// or the original project's include/ unchanged (it replaces cuda/sssp.cu and
+// cuda/pagerank.cu there).
 //
 // Error mapping (original cuda/sssp.cu:18-25,81-98, cuda/pagerank.cu:76-98):
 // invalid graphs and options -> std::invalid_argument with the original's
 // messages; no device or a HIP failure -> std::runtime_error.
 #include "algorithms/pagerank.hpp"
 #include "algorithms/sssp.hpp"
+#include "gpu_call.hpp"
 #include "hytgraph_amd.h"
 
 #include <cmath>
@@ -22,90 +25,7 @@
 
 namespace hytgraph::algorithms
 {
-    namespace
-    {
-        void check(int status, const char *what)
-        {
-            if (status == HYT_OK)
-                return;
-            std::string msg = std::string(what) + ": " + hyt_last_error();
-            if (status == HYT_ERR_INVALID_ARGUMENT)
-                throw std::invalid_argument(msg);
-            throw std::runtime_error(msg);
-        }
-
-        void require_device(const char *what)
-        {
-            int n = 0;
-            check(hyt_device_count(&n), what);
-            if (n == 0)
-                throw std::runtime_error(std::string(what) + ": no HIP device is available");
-        }
-
-        // Owns a device graph for the duration of one call (the original
-        // allocates and frees device memory inside each call as well).
-        struct ScopedGraph
-        {
-            hyt_graph_t g = nullptr;
-            ~ScopedGraph()
-            {
-                if (g)
-                    hyt_graph_free(g);
-            }
-        };
-
-        // Uploads the CSR; float weights travel as IEEE bit patterns.
-        void upload(const graph::CSRGraph &graph, bool with_weights, ScopedGraph &sg, const char *what)
-        {
-            const auto &off = graph.row_offsets();
-            const auto &col = graph.column_indices();
-            const std::uint32_t *w = nullptr;
-            std::vector<std::uint32_t> bits;
-            if (with_weights && graph.has_weights())
-            {
-                const auto &wf = graph.edge_weights();
-                bits.resize(wf.size());
-                if (!wf.empty())
-                    std::memcpy(bits.data(), wf.data(), wf.size() * sizeof(float));
-                w = bits.data();
-            }
-            check(hyt_graph_upload(graph.num_vertices(), graph.num_edges(), off.data(), col.data(), w,
-                                   HYT_GRAPH_WEIGHTS_F32, &sg.g),
-                  what);
-        }
-
-        void validate_sssp(const graph::CSRGraph &graph, const SSSPOptions &options)
-        {
-            if (graph.num_vertices() == 0U)
-                throw std::invalid_argument("SSSP requires at least one vertex");
-            if (static_cast<graph::CSRGraph::offset_type>(options.source) >= graph.num_vertices())
-                throw std::invalid_argument("SSSP source vertex is out of range");
-            if (!(options.tolerance >= 0.0F) || !std::isfinite(options.tolerance))
-                throw std::invalid_argument("SSSP tolerance must be finite and non-negative");
-            for (const float w : graph.edge_weights())
-                if (!std::isfinite(w) || w < 0.0F)
-                    throw std::invalid_argument("SSSP requires finite non-negative edge weights");
-        }
-
-        void validate_pagerank(const graph::CSRGraph &graph, const PageRankOptions &options)
-        {
-            if (graph.num_vertices() == 0U)
-                throw std::invalid_argument("PageRank requires at least one vertex");
-            if (!(options.damping_factor >= 0.0F && options.damping_factor < 1.0F))
-                throw std::invalid_argument("PageRank damping_factor must be in [0, 1)");
-            if (!(options.tolerance > 0.0F) || !std::isfinite(options.tolerance))
-                throw std::invalid_argument("PageRank tolerance must be finite and positive");
-            if (options.max_iterations == 0U)
-                throw std::invalid_argument("PageRank max_iterations must be greater than zero");
-        }
-
-        float bits_to_float(std::uint32_t b)
-        {
-            float f;
-            std::memcpy(&f, &b, 4);
-            return f;
-        }
-    } // namespace
+    using namespace detail;
 
     SSSPResult sssp_gpu(const graph::CSRGraph &graph, const SSSPOptions &options)
     {
@@ -137,40 +57,6 @@ namespace hytgraph::algorithms
         return r;
     }
 
-    SSSPBatchResult sssp_gpu_batch(const graph::CSRGraph &graph,
-                                   const std::vector<graph::CSRGraph::vertex_id> &sources,
-                                   std::size_t max_iterations)
-    {
-        if (sources.empty() || sources.size() > 64)
-            throw std::invalid_argument("sssp_gpu_batch: between 1 and 64 sources are supported");
-        SSSPOptions probe;
-        for (auto s : sources)
-        {
-            probe.source = s;
-            validate_sssp(graph, probe);
-        }
-        require_device("sssp_gpu_batch");
-        ScopedGraph sg;
-        upload(graph, true, sg, "sssp_gpu_batch");
-        const auto n = static_cast<std::size_t>(graph.num_vertices());
-        const std::size_t K = sources.size();
-        std::vector<std::uint32_t> bits(n * K);
-        hyt_run_options o{};
-        o.mode = HYT_SSSP_ASYNC;
-        o.max_rounds = max_iterations;
-        hyt_run_stats st{};
-        check(hyt_sssp(sg.g, sources.data(), static_cast<std::uint32_t>(K), bits.data(), &o, &st),
-              "sssp_gpu_batch");
-        SSSPBatchResult r;
-        r.distances.assign(K, std::vector<float>(n));
-        for (std::size_t v = 0; v < n; ++v)
-            for (std::size_t k = 0; k < K; ++k)
-                r.distances[k][v] = bits_to_float(bits[v * K + k]);
-        r.rounds = static_cast<std::size_t>(st.rounds);
-        r.converged = st.converged != 0;
-        return r;
-    }
-
     PageRankResult pagerank_gpu(const graph::CSRGraph &graph, const PageRankOptions &options)
     {
         validate_pagerank(graph, options);
@@ -191,38 +77,6 @@ namespace hytgraph::algorithms
               "pagerank_gpu");
         r.iterations = static_cast<std::size_t>(st.rounds);
         r.residual = static_cast<float>(st.residual);
-        r.converged = st.converged != 0;
-        return r;
-    }
-
-    PPRResult ppr_gpu(const graph::CSRGraph &graph, const std::vector<graph::CSRGraph::vertex_id> &seeds,
-                      float damping_factor, float epsilon)
-    {
-        if (seeds.empty() || seeds.size() > 64)
-            throw std::invalid_argument("ppr_gpu: between 1 and 64 seeds are supported");
-        PageRankOptions probe;
-        probe.damping_factor = damping_factor;
-        validate_pagerank(graph, probe);
-        for (auto s : seeds)
-            if (static_cast<graph::CSRGraph::offset_type>(s) >= graph.num_vertices())
-                throw std::invalid_argument("PPR seed vertex is out of range");
-        require_device("ppr_gpu");
-        ScopedGraph sg;
-        upload(graph, false, sg, "ppr_gpu");
-        const auto n = static_cast<std::size_t>(graph.num_vertices());
-        const std::size_t K = seeds.size();
-        std::vector<float> out(n * K);
-        hyt_run_options o{};
-        hyt_run_stats st{};
-        check(hyt_ppr(sg.g, seeds.data(), static_cast<std::uint32_t>(K), damping_factor, epsilon, out.data(), &o,
-                      &st),
-              "ppr_gpu");
-        PPRResult r;
-        r.ranks.assign(K, std::vector<float>(n));
-        for (std::size_t v = 0; v < n; ++v)
-            for (std::size_t k = 0; k < K; ++k)
-                r.ranks[k][v] = out[v * K + k];
-        r.rounds = static_cast<std::size_t>(st.rounds);
         r.converged = st.converged != 0;
         return r;
     }

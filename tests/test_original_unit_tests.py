@@ -22,3 +22,10 @@ def test_original_cpp_tests_pass_against_this_api(name):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "passed" in r.stdout
+
+
+def test_drop_in_compiles_against_the_original_headers():
+    obj = UNIT / "drop_in_original_headers.o"
+    if not Path("/root/reference/include").exists() and not obj.exists():
+        pytest.skip("original tree absent")
+    assert obj.exists() and obj.stat().st_size > 0
