@@ -18,7 +18,8 @@
 //   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
 //   void expanded(uint32_t u, uint32_t value, bool whole_row)  row u walks with value
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
-//   void apply(uint32_t v, uint32_t c)                 update destination v
+//   uint32_t peek(uint32_t v)                          read destination v's state
+//   void apply(uint32_t v, uint32_t c, uint32_t seen)  update v (seen = peek(v))
 // load returns false to drop the edge (e.g. u32 overflow, flagged by Op).
 //
 // RowSrc says where a frontier row's edges are: GraphRows reads the CSR
@@ -152,10 +153,14 @@ namespace hyt
                     ok[k] = op.load(pos, value, vv[k], cc[k]);
                 }
             }
+            std::uint32_t seen[kUnroll];
+#pragma unroll
+            for (int k = 0; k < kUnroll; ++k)
+                seen[k] = ok[k] ? op.peek(vv[k]) : 0U;
 #pragma unroll
             for (int k = 0; k < kUnroll; ++k)
                 if (ok[k])
-                    op.apply(vv[k], cc[k]);
+                    op.apply(vv[k], cc[k], seen[k]);
         }
     }
 

@@ -241,8 +241,9 @@ namespace hyt
             const int kl = lane % KP;
             for (std::uint32_t base = 0; base < total; base += G * kUnroll)
             {
-                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll];
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], cand[kUnroll], seen[kUnroll];
                 std::uint64_t m[kUnroll];
+                bool ok[kUnroll];
 #pragma unroll
                 for (int k = 0; k < kUnroll; ++k)
                 {
@@ -256,11 +257,23 @@ namespace hyt
                         op.load(pos, vv[k], ww[k]);
                     }
                 }
+                // reads of every in-flight edge first, then the atomics, so no
+                // read waits behind an earlier edge's atomic to the same array
 #pragma unroll
                 for (int k = 0; k < kUnroll; ++k)
                 {
                     const bool act = static_cast<std::uint32_t>(kl) < a.K && ((m[k] >> kl) & 1ULL);
-                    const bool imp = act && op.update(row[k], scale[k], vv[k], ww[k], kl);
+                    cand[k] = seen[k] = 0;
+                    ok[k] = act && op.prepare(row[k], scale[k], vv[k], ww[k], kl, cand[k], seen[k]);
+                }
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                    if (ok[k])
+                        op.commit(vv[k], kl, cand[k], seen[k]);
+#pragma unroll
+                for (int k = 0; k < kUnroll; ++k)
+                {
+                    const bool imp = ok[k] && op.improved(cand[k], seen[k]);
                     const std::uint64_t gb = group_bits<KP>(__ballot(imp), group);
                     if (kl == 0 && gb != 0)
                         atomicOr(&a.mark[vv[k]], static_cast<unsigned long long>(gb));
@@ -437,7 +450,8 @@ namespace hyt
             const int lane = lane_id();
             for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
             {
-                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], kk[kUnroll];
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], kk[kUnroll], cand[kUnroll],
+                    seen[kUnroll];
                 bool ok[kUnroll];
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u)
@@ -452,15 +466,33 @@ namespace hyt
                         op.load(pos, vv[u], ww[u]);
                     }
                 }
+                // as in walk_edges_k: every read, then every atomic, then the
+                // mark words, then the mark atomics
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u)
-                    if (ok[u] && op.update(row[u], scale[u], vv[u], ww[u], static_cast<int>(kk[u])))
-                    {
-                        unsigned long long *mp = a.mark + vv[u];
-                        const unsigned long long bit = 1ULL << kk[u];
-                        if ((*mp & bit) == 0ULL)
-                            atomicOr(mp, bit);
-                    }
+                {
+                    cand[u] = seen[u] = 0;
+                    ok[u] = ok[u] &&
+                            op.prepare(row[u], scale[u], vv[u], ww[u], static_cast<int>(kk[u]), cand[u], seen[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                    if (ok[u])
+                        op.commit(vv[u], static_cast<int>(kk[u]), cand[u], seen[u]);
+                unsigned long long word[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                {
+                    ok[u] = ok[u] && op.improved(cand[u], seen[u]);
+                    word[u] = ok[u] ? a.mark[vv[u]] : ~0ULL;
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u)
+                {
+                    const unsigned long long bit = 1ULL << kk[u];
+                    if ((word[u] & bit) == 0ULL)
+                        atomicOr(a.mark + vv[u], bit);
+                }
             }
         }
 
@@ -624,10 +656,13 @@ namespace hyt
                 else
                     wt = 0x3F800000U;
             }
-            __device__ bool update(std::uint32_t u, std::uint32_t, std::uint32_t v, std::uint32_t wt, int kl) const
+            // Candidate distance of lane kl for v through row u, and v's
+            // current value (a value read early may be older and larger; that
+            // only costs an extra atomic, it never skips a needed one).
+            __device__ bool prepare(std::uint32_t u, std::uint32_t, std::uint32_t v, std::uint32_t wt, int kl,
+                                    std::uint32_t &cand, std::uint32_t &seen) const
             {
                 const std::uint32_t du = dist[static_cast<std::uint64_t>(u) * K + kl];
-                std::uint32_t c;
                 if constexpr (WM == kWU32 || WM == kWUnitU32)
                 {
                     const std::uint64_t s = static_cast<std::uint64_t>(du) + wt;
@@ -636,18 +671,21 @@ namespace hyt
                         atomicOr(overflow, 1U);
                         return false;
                     }
-                    c = static_cast<std::uint32_t>(s);
+                    cand = static_cast<std::uint32_t>(s);
                 }
                 else
                 {
-                    c = __float_as_uint(__fadd_rn(__uint_as_float(du), __uint_as_float(wt)));
+                    cand = __float_as_uint(__fadd_rn(__uint_as_float(du), __uint_as_float(wt)));
                 }
-                std::uint32_t *p = dist + static_cast<std::uint64_t>(v) * K + kl;
-                if (c >= *p)
-                    return false;
-                atomicMin(p, c);
+                seen = dist[static_cast<std::uint64_t>(v) * K + kl];
                 return true;
             }
+            __device__ void commit(std::uint32_t v, int kl, std::uint32_t cand, std::uint32_t &seen) const
+            {
+                if (cand < seen)
+                    atomicMin(dist + static_cast<std::uint64_t>(v) * K + kl, cand);
+            }
+            __device__ bool improved(std::uint32_t cand, std::uint32_t seen) const { return cand < seen; }
         };
 
         // ----------------------------------------------------------- PPR-K
@@ -670,12 +708,24 @@ namespace hyt
                 v = col[pos];
                 wt = 0;
             }
-            __device__ bool update(std::uint32_t i, std::uint32_t scale, std::uint32_t v, std::uint32_t, int kl) const
+            // The share of lane kl pushed from queue row i; commit adds it and
+            // keeps the residual it found (returning atomic), improved = the
+            // residual crossed eps with this add.
+            __device__ bool prepare(std::uint32_t i, std::uint32_t scale, std::uint32_t, std::uint32_t, int kl,
+                                    std::uint32_t &cand, std::uint32_t &) const
             {
                 const float add = __uint_as_float(scale) * snap[static_cast<std::uint64_t>(i) * K + kl];
-                if (!(add > 0.0F))
-                    return false;
-                const float old = unsafeAtomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl], add);
+                cand = __float_as_uint(add);
+                return add > 0.0F;
+            }
+            __device__ void commit(std::uint32_t v, int kl, std::uint32_t cand, std::uint32_t &seen) const
+            {
+                seen = __float_as_uint(unsafeAtomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl],
+                                                       __uint_as_float(cand)));
+            }
+            __device__ bool improved(std::uint32_t cand, std::uint32_t seen) const
+            {
+                const float old = __uint_as_float(seen), add = __uint_as_float(cand);
                 return old <= eps && old + add > eps;
             }
         };

@@ -79,12 +79,17 @@ namespace hyt
             atomicOr(overflow, 1U);
             return false;
         }
-        __device__ void apply(std::uint32_t v, std::uint32_t c) const
+        // The walk reads the destinations of all its in-flight edges first
+        // (peek) and only then issues their atomics (apply with the value
+        // seen), so the reads overlap instead of each waiting behind the
+        // previous edge's atomic to the same array. A value seen may be older
+        // and larger than the current one; that only costs an extra atomic,
+        // it never skips a needed one. Without touch[] the compaction
+        // compares every row's dist with prev.
+        __device__ std::uint32_t peek(std::uint32_t v) const { return dist[v]; }
+        __device__ void apply(std::uint32_t v, std::uint32_t c, std::uint32_t seen) const
         {
-            // A plain load may return an older, larger value; that only
-            // costs an extra atomic, it never skips a needed one. Without
-            // touch[] the compaction compares every row's dist with prev.
-            if (c < dist[v])
+            if (c < seen)
             {
                 atomicMin(&dist[v], c);
                 if (touch != nullptr)
@@ -221,7 +226,8 @@ namespace hyt
             c = share;
             return true;
         }
-        __device__ void apply(std::uint32_t v, std::uint32_t c) const
+        __device__ std::uint32_t peek(std::uint32_t) const { return 0U; }
+        __device__ void apply(std::uint32_t v, std::uint32_t c, std::uint32_t) const
         {
             unsafeAtomicAdd(&res[v], __uint_as_float(c));
         }
