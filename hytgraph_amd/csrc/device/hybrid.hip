@@ -354,6 +354,31 @@ namespace hyt
             require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
             require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
             ensure_workspace(dg, light_max, chunk);
+            if (stage_val.ptr == nullptr)
+                stage_val.alloc(static_cast<std::uint64_t>(compact_blocks(dg.V)) * kCompactRowsPerBlock * 4,
+                                "hipMalloc(stage values)");
+        }
+
+        // HBM a lane session leaves allocated besides its lane state: the edge
+        // area, offsets, partition tables, compaction staging and the queue.
+        std::uint64_t serving_bytes() const
+        {
+            const std::uint64_t Vp = dg.V + kPad;
+            return edge_area_bytes() + dg.off.bytes + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes +
+                   block_offsets.bytes + scan_temp.bytes + stage_q.bytes + seg_d.bytes + Vp * 4 + sizeof(Counters);
+        }
+
+        // A lane session uses only the queue and the counters of the
+        // single-query workspace; the rest is released (a later single-query
+        // call allocates it again).
+        void release_single_query_state()
+        {
+            Workspace &ws = dg.ws;
+            for (DeviceBuffer *b : {&ws.dist, &ws.rank, &ws.prev, &ws.touch, &ws.qd, &ws.partial, &ws.active,
+                                    &ws.share, &ws.heavy, &stage_val})
+                b->release();
+            ws.heavy_cap = 0;
+            ws.ready = false; // the next ensure_workspace re-creates every array
         }
 
         bool is_cached(std::uint64_t p) const { return res[p].col >= 0 && (!need_w || res[p].w >= 0); }
@@ -1637,7 +1662,7 @@ namespace hyt
 
     DeviceGraph &hybrid_device_graph(HybridEngine &E) { return E.dg; }
 
-    std::uint64_t hybrid_engine_bytes(HybridEngine &E) { return E.edge_area_bytes() + E.state_bytes() - E.lk.bytes(); }
+    std::uint64_t hybrid_engine_bytes(HybridEngine &E) { return E.serving_bytes(); }
 
     std::uint64_t hybrid_lane_state_bytes(HybridEngine &E, int kind, std::uint32_t K)
     {
@@ -1658,6 +1683,8 @@ namespace hyt
         const bool ppr = kind == HYT_QUERY_PPR;
         E.begin_run(!ppr);
         E.set_split(hyt_run_options{}, kLaneLightMaxDegree, kLaneHeavyChunk);
+        hip_check(hipStreamSynchronize(E.dg.stream), "lane session");
+        E.release_single_query_state();
         prepare_lanes(E, K, ppr);
         LaneState &L = E.lk;
         if (!L.host_lanes)

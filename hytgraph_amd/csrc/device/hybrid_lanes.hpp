@@ -18,7 +18,9 @@ namespace hyt
     HybridEngine &hybrid_engine(hyt_hybrid *h);
     // Offsets-only device graph of the engine (V, E, max_degree, stream).
     DeviceGraph &hybrid_device_graph(HybridEngine &E);
-    // HBM the engine holds outside its lane state (edge area + vertex state).
+    // HBM the engine holds while it serves, outside its lane state (edge
+    // area, offsets, partition tables, compaction staging, queue): opening a
+    // session releases the single-query workspace.
     std::uint64_t hybrid_engine_bytes(HybridEngine &E);
     // HBM of the lane state for K lanes of `kind` (planning).
     std::uint64_t hybrid_lane_state_bytes(HybridEngine &E, int kind, std::uint32_t K);
