@@ -377,7 +377,7 @@ def bench_serve(args) -> dict:
 
     # latency: Poisson arrivals at a fixed offered rate
     qps = args.qps if args.qps > 0 else 0.7 * cap_qps
-    pois = trace(args.serve_requests, qps)
+    pois = trace(args.serve_poisson or args.serve_requests, qps)
     comp2, st2, _ = abi.serve_trace(served, pois, device_budget_bytes=budget, epsilon=eps)
     arr = np.array([r[0] for r in pois])
     lat = comp2["done_s"] - arr
@@ -650,6 +650,7 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-baseline", type=int, default=60, help="serve: queries timed one at a time")
     ap.add_argument("--qps", type=float, default=0.0, help="serve: offered rate (0 = 0.7 x burst capacity)")
     ap.add_argument("--serve-host", default="", help="serve: scales kept in host memory behind hybrid engines")
+    ap.add_argument("--serve-poisson", type=int, default=0, help="serve: Poisson-trace requests (0 = --serve-requests)")
     ap.add_argument("--serve-edge-gb", type=int, default=4, help="serve: HBM edge budget per hybrid engine (GiB)")
     ap.add_argument("--ppr-eps", type=float, default=1e-6)
     ap.add_argument("--ks", default="8,32,64", help="batched: lanes per call")

@@ -1092,10 +1092,18 @@ int hyt_serve_trace_mixed(const hyt_serve_graph *graphs, uint32_t num_graphs, co
                 gs[i].engine[HYT_QUERY_SSSP] =
                     graphs[i].sssp_engine ? &hyt::hybrid_engine(graphs[i].sssp_engine) : nullptr;
                 gs[i].engine[HYT_QUERY_PPR] = graphs[i].ppr_engine ? &hyt::hybrid_engine(graphs[i].ppr_engine) : nullptr;
-                hyt::require(gs[i].engine[0] == nullptr || gs[i].engine[0] != gs[i].engine[1],
-                             "SSSP and PPR need an engine each");
             }
         }
+        // a session keeps its lanes in its engine's state: one session per engine
+        std::vector<const void *> seen;
+        for (const auto &g : gs)
+            for (const auto *e : g.engine)
+                if (e)
+                {
+                    hyt::require(std::find(seen.begin(), seen.end(), e) == seen.end(),
+                                 "each hybrid engine may serve one graph and one query kind");
+                    seen.push_back(e);
+                }
         hyt_serve_stats st{};
         hyt::serve_trace(gs, *config, requests, num_requests, results, completions, st);
         if (stats)

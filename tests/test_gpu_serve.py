@@ -280,6 +280,11 @@ def test_serve_trace_host_resident_graph(gpu, abi, orc):
         else:
             same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, 1e-8, seed=src)
             assert np.abs(res[i] - same).sum() <= 2e-6, i
-    # a host-resident graph needs an engine for every query kind it receives
+    # a host-resident graph needs an engine for every query kind it receives,
+    # and an engine serves one graph and one kind
     with pytest.raises(abi.InvalidArgument):
         abi.serve_trace([gs, (eng_s, None)], [(0.0, 1, abi.HYT_QUERY_PPR, 0)])
+    with pytest.raises(abi.InvalidArgument):
+        abi.serve_trace([gs, (eng_s, eng_s)], [(0.0, 1, abi.HYT_QUERY_SSSP, 0)])
+    with pytest.raises(abi.InvalidArgument):
+        abi.serve_trace([(eng_s, None), (eng_s, None)], [(0.0, 1, abi.HYT_QUERY_SSSP, 0)])
