@@ -9,6 +9,8 @@
 //
 // Sel::select4(row0, n, vals[4], aux) -> bit mask of the selected rows among
 // [row0, row0+n), n <= 4; vals[j] is the value of row row0+j when selected.
+// Sel::whole16() true: Sel::select16(row0, V, vals[4][4], masks[4], aux)
+// handles the thread's 16 rows in one call instead.
 // Sel::Aux is per-thread side information (e.g. held-back rows) that
 // Sel::publish(aux, ctr) reduces across the wavefront and publishes.
 // Optionally (non-null pointers) the pass also writes the frontier bitmap
@@ -25,6 +27,7 @@ namespace hyt
     struct NoAux
     {
     };
+
 
     constexpr int kCompactGroups = 4;                         // groups of 4 rows per thread
     constexpr int kCompactRows = 4 * kCompactGroups;          // rows per thread
@@ -56,17 +59,32 @@ namespace hyt
         std::uint32_t gm[kCompactGroups];
         std::uint32_t n = 0;
         typename Sel::Aux aux{};
-#pragma unroll
-        for (int g = 0; g < kCompactGroups; ++g)
+        if (sel.whole16())
         {
-            gm[g] = 0;
-            const std::uint64_t r = row0 + 4 * g;
-            if (r < V)
+            // the selector reads its 16 rows at once (no store between the loads)
+#pragma unroll
+            for (int g = 0; g < kCompactGroups; ++g)
+                gm[g] = 0;
+            if (row0 < V)
+                sel.select16(row0, V, gv, gm, aux);
+#pragma unroll
+            for (int g = 0; g < kCompactGroups; ++g)
+                n += static_cast<std::uint32_t>(__popc(gm[g]));
+        }
+        else
+        {
+#pragma unroll
+            for (int g = 0; g < kCompactGroups; ++g)
             {
-                const std::uint32_t avail = static_cast<std::uint32_t>(V - r < 4 ? V - r : 4);
-                gm[g] = sel.select4(r, avail, gv[g], aux);
+                gm[g] = 0;
+                const std::uint64_t r = row0 + 4 * g;
+                if (r < V)
+                {
+                    const std::uint32_t avail = static_cast<std::uint32_t>(V - r < 4 ? V - r : 4);
+                    gm[g] = sel.select4(r, avail, gv[g], aux);
+                }
+                n += static_cast<std::uint32_t>(__popc(gm[g]));
             }
-            n += static_cast<std::uint32_t>(__popc(gm[g]));
         }
 
         sel.publish(aux, ctr);

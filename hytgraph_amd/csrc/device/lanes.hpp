@@ -912,6 +912,12 @@ namespace hyt
             // (row, lane) pairs selected goes to ctr->frontier_edges
             unsigned long long *lanes_out;
 
+            // no 16-row path
+            __device__ bool whole16() const { return false; }
+            template <class Vals, class A>
+            __device__ void select16(std::uint64_t, std::uint64_t, Vals &, std::uint32_t (&)[4], A &) const
+            {
+            }
             __device__ void publish(const Aux &aux, Counters *ctr) const
             {
                 if (lanes_out == nullptr)

@@ -204,6 +204,53 @@ namespace hyt
                 *reinterpret_cast<uint4 *>(prev + row0) = make_uint4(p[0], p[1], p[2], p[3]);
             return m;
         }
+
+        // Touch-free mode reads the thread's 16 rows of dist and prev before
+        // any store (row0 is a multiple of 16 below V; the arrays have 16
+        // spare slots past V, so the vector reads stay in range, and rows at
+        // or past V are never selected).
+        __device__ bool whole16() const { return touch == nullptr; }
+        __device__ void select16(std::uint64_t row0, std::uint64_t V, std::uint32_t (&vals)[4][4],
+                                 std::uint32_t (&masks)[4], Aux &aux) const
+        {
+            uint4 d4[4], p4[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+            {
+                d4[g] = *reinterpret_cast<const uint4 *>(dist + row0 + 4 * g);
+                p4[g] = *reinterpret_cast<const uint4 *>(prev + row0 + 4 * g);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+            {
+                const std::uint64_t r = row0 + 4 * g;
+                const std::uint32_t n = r >= V ? 0U : static_cast<std::uint32_t>(V - r < 4 ? V - r : 4);
+                const std::uint32_t d[4] = {d4[g].x, d4[g].y, d4[g].z, d4[g].w};
+                std::uint32_t p[4] = {p4[g].x, p4[g].y, p4[g].z, p4[g].w};
+                std::uint32_t m = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    if (static_cast<std::uint32_t>(j) < n && d[j] < p[j])
+                    {
+                        if (d[j] < bound)
+                        {
+                            p[j] = d[j];
+                            vals[g][j] = d[j];
+                            m |= 1U << j;
+                        }
+                        else
+                        {
+                            aux.pending += 1;
+                            aux.inv_max = (~d[j] > aux.inv_max) ? ~d[j] : aux.inv_max;
+                        }
+                    }
+                }
+                masks[g] = m;
+                if (m)
+                    *reinterpret_cast<uint4 *>(prev + r) = make_uint4(p[0], p[1], p[2], p[3]);
+            }
+        }
     };
 
     struct PrOp
@@ -246,6 +293,12 @@ namespace hyt
         float damping;
         float eps;
 
+        // no 16-row path
+        __device__ bool whole16() const { return false; }
+        template <class Vals, class A>
+        __device__ void select16(std::uint64_t, std::uint64_t, Vals &, std::uint32_t (&)[4], A &) const
+        {
+        }
         __device__ void publish(const Aux &, Counters *) const {}
         __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
                                          Aux &) const
