@@ -17,7 +17,8 @@
 // The serving loop replays a request trace (arrival times, graph, kind,
 // source) against the wall clock: arrivals queue per session, lanes per
 // session are sized from the device budget (admission by memory: each lane
-// goes to the session with the most demand per lane that still fits), and the
+// goes to the session with the most work per lane that still fits, work being
+// its requests times its graph's edges), and the
 // loop polls every session: when its round has completed, finished lanes are
 // read and freed, queued queries are admitted into free lanes, and the next
 // round is launched - each session advances at its own pace, so a small
@@ -669,12 +670,15 @@ namespace hyt
 
         // Lanes per session from the budget: every session gets one lane,
         // then each further lane goes to the session with the most queued
-        // demand per lane that still fits (demand = its requests in the
-        // trace), until nothing fits or every session has min(demand,
-        // max_lanes) lanes.
+        // work per lane that still fits (work = its requests in the trace x
+        // its graph's edge count, the cost of one query to first order), until
+        // nothing fits or every session has min(demand, max_lanes) lanes. A
+        // burst is done when the slowest session is, so lanes go where the
+        // work is: a large graph's queries each stream far more edges.
         std::vector<std::uint32_t> plan_lanes(const std::vector<std::uint64_t> &fixed,
                                               const std::vector<std::uint64_t> &per_lane,
-                                              const std::vector<std::uint64_t> &demand, std::uint64_t avail,
+                                              const std::vector<std::uint64_t> &demand,
+                                              const std::vector<double> &cost, std::uint64_t avail,
                                               std::uint32_t max_lanes)
         {
             const std::size_t n = fixed.size();
@@ -694,7 +698,7 @@ namespace hyt
                 {
                     if (K[i] >= max_lanes || K[i] >= demand[i] || used + per_lane[i] > avail)
                         continue;
-                    const double ratio = static_cast<double>(demand[i]) / K[i];
+                    const double ratio = static_cast<double>(demand[i]) * cost[i] / K[i];
                     if (best == n || ratio > best_ratio)
                     {
                         best = i;
@@ -750,6 +754,7 @@ namespace hyt
             // sessions: one per (graph, kind) that appears
             std::vector<int> sid(static_cast<std::size_t>(ngraphs) * 2, -1);
             std::vector<std::uint64_t> fixed, per_lane, demand;
+            std::vector<double> cost;
             std::vector<std::pair<std::uint32_t, int>> skey;
             for (std::uint64_t i = 0; i < n; ++i)
             {
@@ -776,6 +781,7 @@ namespace hyt
                         per_lane.push_back(lane);
                     }
                     demand.push_back(0);
+                    cost.push_back(static_cast<double>(g.E) + 1.0);
                 }
                 demand[static_cast<std::size_t>(sid[key])] += 1;
             }
@@ -801,7 +807,7 @@ namespace hyt
             if (S > 0)
             {
                 const std::vector<std::uint32_t> K =
-                    plan_lanes(fixed, per_lane, demand, cfg.device_budget_bytes - graph_bytes, cfg.max_lanes);
+                    plan_lanes(fixed, per_lane, demand, cost, cfg.device_budget_bytes - graph_bytes, cfg.max_lanes);
                 for (std::size_t i = 0; i < S; ++i)
                 {
                     ses.push_back(std::make_unique<LaneSession>());
