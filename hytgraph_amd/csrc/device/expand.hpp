@@ -196,6 +196,7 @@ namespace hyt
         __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
         __shared__ std::uint64_t s_begin[kWavesPerBlock][kWave];
         __shared__ std::uint32_t s_val[kWavesPerBlock][kWave];
+        __shared__ std::uint32_t s_total[kWavesPerBlock];
 
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
@@ -229,11 +230,22 @@ namespace hyt
         s_ex[wid][lane] = incl - deg;
         s_begin[wid][lane] = begin;
         s_val[wid][lane] = value;
+        if (lane == 0)
+            s_total[wid] = total;
         __syncthreads();
+        // one edge-count atomic per workgroup: per-wavefront atomics on the
+        // one counter serialise (measured: ~9% of an RMAT-24 SSSP round)
+        if (threadIdx.x == 0)
+        {
+            unsigned long long sum = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w)
+                sum += s_total[w];
+            if (sum)
+                atomicAdd(&a.ctr->edges, sum);
+        }
         if (total == 0)
             return;
-        if (lane == 0)
-            atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
 
         const std::uint32_t *ex = s_ex[wid];
         const std::uint64_t *bg = s_begin[wid];

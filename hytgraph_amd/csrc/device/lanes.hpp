@@ -294,6 +294,27 @@ namespace hyt
             return a.rs->form == form;
         }
 
+        // Adds the workgroup's edge count (each wavefront's lane 0 holds its
+        // own) with one atomic: per-wavefront atomics on the one counter
+        // serialise. Every thread of the workgroup must call it.
+        __device__ __forceinline__ void add_block_edges(unsigned long long *s_edges, unsigned long long wedges,
+                                                        Counters *ctr)
+        {
+            const int wid = static_cast<int>(threadIdx.x) / kWave;
+            if (lane_id() == 0)
+                s_edges[wid] = wedges;
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                unsigned long long sum = 0;
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w)
+                    sum += s_edges[w];
+                if (sum)
+                    atomicAdd(&ctr->edges, sum);
+            }
+        }
+
         template <class RowSrc, int KP, class Op>
         __global__ __launch_bounds__(kBlock) void expandk_light(RowSrc rsrc, Op op, KArgs a)
         {
@@ -303,10 +324,12 @@ namespace hyt
             __shared__ std::uint32_t s_row[kWavesPerBlock][kWave];
             __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
+            __shared__ unsigned long long s_edges[kWavesPerBlock];
 
             std::uint32_t F;
             if (!round_rows(a, HYT_LANES_GROUP, F))
                 return;
+            unsigned long long wedges = 0; // this wavefront's edges (lane 0)
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
             const std::uint32_t ce = a.chunk;
@@ -361,8 +384,7 @@ namespace hyt
                 __syncthreads();
                 if (total == 0)
                     continue; // every wavefront still meets the next tile's barrier
-                if (lane == 0)
-                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(total));
+                wedges += total;
                 const std::uint32_t *ex = s_ex[wid];
                 walk_edges_k<KP>(op, a, total,
                                  [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
@@ -374,6 +396,7 @@ namespace hyt
                                      sc = s_scale[wid][j];
                                  });
             }
+            add_block_edges(s_edges, wedges, a.ctr);
         }
 
         template <int KP, class Op>
@@ -507,9 +530,11 @@ namespace hyt
             __shared__ std::uint32_t s_c[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
 
+            __shared__ unsigned long long s_edges[kWavesPerBlock];
             std::uint32_t F;
             if (!round_rows(a, HYT_LANES_ITEMS, F))
                 return;
+            unsigned long long wedges = 0; // this wavefront's edges (lane 0)
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
             const std::uint32_t tiles = (F + kBlock - 1) / kBlock;
@@ -575,8 +600,7 @@ namespace hyt
                 __syncthreads();
                 if (total == 0)
                     continue; // every wavefront still meets the next tile's barrier
-                if (lane == 0)
-                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(dsum));
+                wedges += dsum;
                 const std::uint32_t *ex = s_ex[wid];
                 walk_items_k(op, a, total,
                              [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint32_t &sc,
@@ -591,6 +615,7 @@ namespace hyt
                                  sc = s_scale[wid][j];
                              });
             }
+            add_block_edges(s_edges, wedges, a.ctr);
         }
 
         template <class Op>

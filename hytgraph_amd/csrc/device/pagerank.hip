@@ -211,10 +211,10 @@ namespace hyt
                 if (pull)
                 {
                     pull_light<OffT, PrPullOp><<<blocks_for(g.V), kBlock, 0, s>>>(g.in_off.as<OffT>(), pop, g.V,
-                                                                                   light_max, d_ctr);
+                                                                                   light_max);
                     hip_check(hipGetLastError(), "launch pr_pull_light");
                     pull_heavy<PrPullOp><<<kHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
-                                                                      g.in_heavy_count, d_ctr);
+                                                                      g.in_heavy_count);
                     hip_check(hipGetLastError(), "launch pr_pull_heavy");
                     st.pull_rounds += 1;
                     st.pull_active_edges += Fe;
@@ -240,10 +240,12 @@ namespace hyt
                 if (c.heavy_overflow)
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.frontier_total += F;
-                st.edges_total += c.edges;
+                // a pull round scans every in-edge (its kernels count none)
+                const std::uint64_t round_edges = pull ? g.E : c.edges;
+                st.edges_total += round_edges;
                 if (pull)
                 {
-                    st.pull_edges += c.edges;
+                    st.pull_edges += round_edges;
                     st.pull_frontier += F;
                 }
                 st.updates_total += c.next_count;

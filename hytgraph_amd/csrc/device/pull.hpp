@@ -31,9 +31,12 @@ namespace hyt
         return (bits[u >> 5] >> (u & 31U)) & 1U;
     }
 
+    // A pull round scans every in-edge of the graph (light rows here, the
+    // static chunk list of heavy rows in pull_heavy), so the caller counts E
+    // edges for it; the kernels keep no shared counter.
     template <class OffT, class Op>
     __global__ __launch_bounds__(kBlock) void pull_light(const OffT *__restrict__ in_off, Op op, std::uint64_t V,
-                                                         std::uint32_t light_max, Counters *ctr)
+                                                         std::uint32_t light_max)
     {
         __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
         __shared__ OffT s_begin[kWavesPerBlock][kWave];
@@ -58,8 +61,6 @@ namespace hyt
         s_acc[wid][lane] = op.identity();
         __syncthreads();
         const std::uint32_t *ex = s_ex[wid];
-        if (lane == 0 && total)
-            atomicAdd(&ctr->edges, static_cast<unsigned long long>(total));
         // (no early return: every thread reaches the second barrier)
         for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
         {
@@ -105,8 +106,7 @@ namespace hyt
     }
 
     template <class Op>
-    __global__ __launch_bounds__(kBlock) void pull_heavy(Op op, const HeavyChunk *chunks, std::uint64_t n,
-                                                         Counters *ctr)
+    __global__ __launch_bounds__(kBlock) void pull_heavy(Op op, const HeavyChunk *chunks, std::uint64_t n)
     {
         const int lane = lane_id();
         const std::uint64_t waves = static_cast<std::uint64_t>(gridDim.x) * kWavesPerBlock;
@@ -114,8 +114,6 @@ namespace hyt
              c += waves)
         {
             const HeavyChunk h = chunks[c];
-            if (lane == 0)
-                atomicAdd(&ctr->edges, static_cast<unsigned long long>(h.count));
             std::uint32_t acc = op.identity();
             for (std::uint32_t base = 0; base < h.count; base += kWave * kUnroll)
             {
