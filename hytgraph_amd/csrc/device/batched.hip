@@ -115,6 +115,8 @@ namespace hyt
             ctr->next_count = 0;
             ctr->heavy_count = 0;
             ctr->edges = 0;
+            for (int i = 0; i < kEdgeSlots; ++i)
+                ctr->slot[i].v = 0;
             ctr->frontier_edges = 0;
         }
 
@@ -122,7 +124,7 @@ namespace hyt
         {
             if (threadIdx.x != 0 || blockIdx.x != 0 || rs->F == 0)
                 return;
-            rs->edges_total += ctr->edges;
+            rs->edges_total += ctr->all_edges();
             rs->updates_total += ctr->next_count;
         }
 

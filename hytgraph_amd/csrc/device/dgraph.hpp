@@ -14,6 +14,7 @@
 
 #include "hip_util.hpp"
 
+#include <cstddef>
 #include <cstdint>
 #include <memory>
 
@@ -35,20 +36,45 @@ namespace hyt
         std::uint64_t begin;  // first edge index
     };
 
-    // Per-round counters. The first 16 bytes are zeroed at the start of each
-    // round; the two flags are sticky for the whole call.
-    struct Counters
+    // Edge counts of the expand kernels go to kEdgeSlots counters on separate
+    // 256-byte lines (workgroup b adds to slot b % kEdgeSlots): one counter
+    // taking every workgroup's atomic serialises them.
+    constexpr int kEdgeSlots = 8;
+    struct alignas(256) EdgeSlot
+    {
+        unsigned long long v;
+    };
+
+    // Per-round counters. The bytes before `overflow` (kRoundCounterBytes)
+    // are zeroed at the start of each round; the two flags are sticky for the
+    // whole call.
+    struct alignas(256) Counters
     {
         std::uint32_t next_count;          // rows written by the compaction pass
         std::uint32_t heavy_count;         // heavy chunks queued this round
-        unsigned long long edges;          // edges expanded this round
+        unsigned long long edges;          // edges expanded this round (besides the slots)
         unsigned long long frontier_edges; // out-degree sum of the compacted rows
         std::uint32_t pending;             // improved rows held back (at/above the bucket bound)
         std::uint32_t pending_inv_max;     // max of ~dist over held-back rows (= ~min dist)
+        EdgeSlot slot[kEdgeSlots];         // edges expanded this round, spread
         std::uint32_t overflow;            // a u32 sum overflowed
         std::uint32_t heavy_overflow;      // heavy chunk queue overflowed
+
+        __host__ __device__ unsigned long long all_edges() const
+        {
+            unsigned long long e = edges;
+            for (int i = 0; i < kEdgeSlots; ++i)
+                e += slot[i].v;
+            return e;
+        }
     };
-    constexpr std::size_t kRoundCounterBytes = 32;
+    constexpr std::size_t kRoundCounterBytes = offsetof(Counters, overflow);
+
+    // Adds n expanded edges for workgroup blockIdx.x.
+    __device__ __forceinline__ void add_edges(Counters *ctr, unsigned long long n)
+    {
+        atomicAdd(&ctr->slot[blockIdx.x % kEdgeSlots].v, n);
+    }
 
     struct Workspace
     {

@@ -242,7 +242,7 @@ namespace hyt
             for (int w = 0; w < kWavesPerBlock; ++w)
                 sum += s_total[w];
             if (sum)
-                atomicAdd(&a.ctr->edges, sum);
+                add_edges(a.ctr, sum);
         }
         if (total == 0)
             return;
@@ -275,7 +275,7 @@ namespace hyt
         {
             const HeavyChunk h = a.heavy[c];
             if (lane == 0)
-                atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                add_edges(a.ctr, h.count);
             walk_edges(op, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                 pos = h.begin + e;
                 val = h.value;

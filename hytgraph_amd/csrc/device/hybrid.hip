@@ -1189,7 +1189,7 @@ namespace hyt
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
                 st.run.frontier_total += F;
-                st.run.edges_total += cc.edges;
+                st.run.edges_total += cc.all_edges();
                 st.run.updates_total += Fn;
                 st.run.rounds = round + 1;
                 F = Fn;
@@ -1250,7 +1250,7 @@ namespace hyt
                 if (cc.heavy_overflow)
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.run.frontier_total += F;
-                st.run.edges_total += cc.edges;
+                st.run.edges_total += cc.all_edges();
                 st.run.updates_total += Fn;
                 st.run.rounds = round + 1;
                 F = Fn;
@@ -1475,7 +1475,7 @@ namespace hyt
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
                 st.run.frontier_total += F;
-                st.run.edges_total += cc.edges;
+                st.run.edges_total += cc.all_edges();
                 st.run.updates_total += Fn;
                 st.run.rounds = round + 1;
                 F = Fn;
@@ -1732,7 +1732,7 @@ namespace hyt
             lane_round_kp<std::uint32_t>(E, kind, K, F, pairs, damping, eps, form, st, out);
         st.run.rounds += 1;
         st.run.frontier_total += F;
-        st.run.edges_total += out.c.edges;
+        st.run.edges_total += out.c.all_edges();
         return out;
     }
 } // namespace hyt

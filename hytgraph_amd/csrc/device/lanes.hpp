@@ -311,7 +311,7 @@ namespace hyt
                 for (int w = 0; w < kWavesPerBlock; ++w)
                     sum += s_edges[w];
                 if (sum)
-                    atomicAdd(&ctr->edges, sum);
+                    add_edges(ctr, sum);
             }
         }
 
@@ -413,7 +413,7 @@ namespace hyt
             {
                 const HeavyChunkK h = a.heavy[c];
                 if (lane == 0)
-                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                    add_edges(a.ctr, h.count);
                 walk_edges_k<KP>(op, a, h.count,
                                  [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
                                      std::uint32_t &sc) {
@@ -632,7 +632,7 @@ namespace hyt
             {
                 const HeavyChunkK h = a.heavy[ci];
                 if (lane == 0)
-                    atomicAdd(&a.ctr->edges, static_cast<unsigned long long>(h.count));
+                    add_edges(a.ctr, h.count);
                 const std::uint32_t c = static_cast<std::uint32_t>(__popcll(h.mask));
                 if (c == 0)
                     continue;

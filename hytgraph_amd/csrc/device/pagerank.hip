@@ -241,7 +241,7 @@ namespace hyt
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.frontier_total += F;
                 // a pull round scans every in-edge (its kernels count none)
-                const std::uint64_t round_edges = pull ? g.E : c.edges;
+                const std::uint64_t round_edges = pull ? g.E : c.all_edges();
                 st.edges_total += round_edges;
                 if (pull)
                 {

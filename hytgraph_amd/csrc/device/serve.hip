@@ -562,7 +562,7 @@ namespace hyt
                 if ((S.running >> k) & 1ULL)
                     S.lane_rounds[k] += 1;
             S.rounds += 1;
-            S.edges += c.edges;
+            S.edges += c.all_edges();
             S.frontier_rows += S.F;
             S.lane_rows += S.pairs;
             S.pairs = c.frontier_edges; // (row, lane) pairs of the next frontier
