@@ -49,6 +49,7 @@ namespace hyt
                                                            std::uint32_t *block_counts = nullptr)
     {
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
+        __shared__ unsigned long long s_dsum[kWavesPerBlock];
         __shared__ std::uint32_t s_base;
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
@@ -106,14 +107,24 @@ namespace hyt
 #pragma unroll
             for (int d = kWave / 2; d > 0; d >>= 1)
                 dsum += __shfl_xor(dsum, d, kWave);
-            if (lane == 0 && dsum)
-                atomicAdd(&ctr->frontier_edges, dsum);
+            if (lane == 0)
+                s_dsum[wid] = dsum;
         }
 
         const std::uint32_t incl = wave_incl_scan(n);
         if (lane == kWave - 1)
             s_wsum[wid] = incl;
         __syncthreads();
+        if (off != nullptr && threadIdx.x == 0)
+        {
+            // one atomic per workgroup for the out-degree sum
+            unsigned long long dsum = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w)
+                dsum += s_dsum[w];
+            if (dsum)
+                atomicAdd(&ctr->frontier_edges, dsum);
+        }
         std::uint32_t wbase = 0, total = 0;
 #pragma unroll
         for (int w = 0; w < kWavesPerBlock; ++w)

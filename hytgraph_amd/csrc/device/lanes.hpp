@@ -957,11 +957,32 @@ namespace hyt
                     hi |= __shfl_xor(hi, d, kWave);
                     pairs += __shfl_xor(pairs, d, kWave);
                 }
-                const unsigned long long all = (static_cast<unsigned long long>(hi) << 32) | lo;
-                if (lane_id() == 0 && all != 0ULL)
+                // one pair of atomics per workgroup (every thread of the
+                // workgroup calls publish): per-wavefront atomics on the two
+                // shared words serialise
+                __shared__ unsigned long long s_all[kWavesPerBlock];
+                __shared__ std::uint32_t s_pairs[kWavesPerBlock];
+                const int wid = static_cast<int>(threadIdx.x) / kWave;
+                if (lane_id() == 0)
                 {
-                    atomicOr(lanes_out, all);
-                    atomicAdd(&ctr->frontier_edges, static_cast<unsigned long long>(pairs));
+                    s_all[wid] = (static_cast<unsigned long long>(hi) << 32) | lo;
+                    s_pairs[wid] = pairs;
+                }
+                __syncthreads();
+                if (threadIdx.x == 0)
+                {
+                    unsigned long long all = 0, sum = 0;
+#pragma unroll
+                    for (int w = 0; w < kWavesPerBlock; ++w)
+                    {
+                        all |= s_all[w];
+                        sum += s_pairs[w];
+                    }
+                    if (all != 0ULL)
+                    {
+                        atomicOr(lanes_out, all);
+                        atomicAdd(&ctr->frontier_edges, sum);
+                    }
                 }
             }
             __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n,
