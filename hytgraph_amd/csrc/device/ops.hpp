@@ -316,19 +316,67 @@ namespace hyt
                     r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0.0F;
             }
             std::uint32_t m = 0;
-            float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
 #pragma unroll
             for (int j = 0; j < 4; ++j)
-            {
                 if (r[j] > eps)
-                {
-                    const std::uint64_t v = row0 + j;
-                    res[v] = 0.0F;
-                    rank[v] += r[j];
-                    vals[j] = __float_as_uint(r[j]);
-                    const OffT deg = off[v + 1] - off[v];
-                    sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
                     m |= 1U << j;
+            float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
+            if (m)
+            {
+                // offsets of the 4 rows (+1); a full group reads them as one
+                // vector when the row offsets are 32-bit
+                OffT o[5];
+                if (n == 4 && sizeof(OffT) == 4 && (row0 % 4) == 0)
+                {
+                    const uint4 o4 = *reinterpret_cast<const uint4 *>(off + row0);
+                    o[0] = o4.x, o[1] = o4.y, o[2] = o4.z, o[3] = o4.w;
+                    o[4] = off[row0 + 4];
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 5; ++j)
+                        o[j] = static_cast<std::uint32_t>(j) <= n ? off[row0 + j] : o[j - 1];
+                }
+                float k[4] = {0.0F, 0.0F, 0.0F, 0.0F};
+                if (n == 4)
+                {
+                    const float4 k4 = *reinterpret_cast<const float4 *>(rank + row0);
+                    k[0] = k4.x, k[1] = k4.y, k[2] = k4.z, k[3] = k4.w;
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (static_cast<std::uint32_t>(j) < n)
+                            k[j] = rank[row0 + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                {
+                    if ((m >> j) & 1U)
+                    {
+                        k[j] += r[j];
+                        vals[j] = __float_as_uint(r[j]);
+                        const OffT deg = o[j + 1] - o[j];
+                        sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
+                        r[j] = 0.0F;
+                    }
+                }
+                if (n == 4)
+                {
+                    *reinterpret_cast<float4 *>(res + row0) = make_float4(r[0], r[1], r[2], r[3]);
+                    *reinterpret_cast<float4 *>(rank + row0) = make_float4(k[0], k[1], k[2], k[3]);
+                }
+                else
+                {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if ((m >> j) & 1U)
+                        {
+                            res[row0 + j] = 0.0F;
+                            rank[row0 + j] = k[j];
+                        }
                 }
             }
             // every row's share is rewritten (0 outside the frontier), so a
