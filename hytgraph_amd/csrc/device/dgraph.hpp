@@ -127,13 +127,17 @@ namespace hyt
     // 1024; scales 20-22 are within 3% of each other.
     constexpr std::uint32_t kDefaultLightMaxDegree = 2048;
     constexpr std::uint32_t kDefaultHeavyChunk = 4096;
-    // PageRank (push and pull rounds), measured the same way on RMAT scale 24
-    // (eps 1e-7): 512 / 2048 took 0.716 s against 0.829 s (256 / 1024) and
-    // 0.878 s (2048 / 4096). The K-lane kernels keep 256 / 1024: a K-lane
+    // PageRank (push and pull rounds), eps 1e-7 on one MI355X: with the pull
+    // kernels counting no edges, rows above 32 in-edges go to the chunk
+    // kernel, which gathers at 72% of the HBM peak against 22% for the
+    // 64-rows-per-wavefront walk: 32 / 256 took 0.316 s on RMAT 24 against
+    // 0.553 s (512 / 2048), 0.063 / 0.148 s on RMAT 22, 1.86 / 2.39 s on
+    // RMAT 26, and 0.507 / 0.506 s on ER 24 (16 / 256: within 2% on RMAT,
+    // 3% slower on ER). The K-lane kernels keep 256 / 1024: a K-lane
     // light row carries K queries per edge, so long light rows would leave
     // one wavefront with most of a round's work.
-    constexpr std::uint32_t kPagerankLightMaxDegree = 512;
-    constexpr std::uint32_t kPagerankHeavyChunk = 2048;
+    constexpr std::uint32_t kPagerankLightMaxDegree = 32;
+    constexpr std::uint32_t kPagerankHeavyChunk = 256;
     constexpr std::uint32_t kLaneLightMaxDegree = 256;
     constexpr std::uint32_t kLaneHeavyChunk = 1024;
     // Hybrid engine expansions read windows of partitions, many of them over
