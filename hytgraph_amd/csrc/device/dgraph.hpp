@@ -13,6 +13,7 @@
 #pragma once
 
 #include "hip_util.hpp"
+#include "splits.hpp"
 
 #include <cstddef>
 #include <cstdint>
@@ -119,14 +120,8 @@ namespace hyt
         }
     };
 
-    // Default work-split thresholds (degree binning) of single-source SSSP
-    // push rounds: rows up to light_max edges are walked by the light kernel,
-    // longer rows in heavy_chunk-edge pieces. Chosen by the autotuner on RMAT
-    // scales 24 and 26 (MI355X): 2048 / 4096 ran 25.0 -> 18.2 ms (scale 24)
-    // and 90.7 -> 68.9 ms (scale 26, 2048 / 1024) per query against 256 /
-    // 1024; scales 20-22 are within 3% of each other.
-    constexpr std::uint32_t kDefaultLightMaxDegree = 2048;
-    constexpr std::uint32_t kDefaultHeavyChunk = 4096;
+    // The default work split of single-source SSSP depends on the graph's
+    // size: default_sssp_split (splits.hpp).
     // PageRank (push and pull rounds), eps 1e-7 on one MI355X: with the pull
     // kernels counting no edges, rows above 32 in-edges go to the chunk
     // kernel, which gathers at 72% of the HBM peak against 22% for the

@@ -303,8 +303,9 @@ namespace hyt
     void sssp_single(DeviceGraph &g, std::uint32_t src, std::uint32_t *dist_out, const hyt_run_options &o,
                      hyt_run_stats &st)
     {
-        Launch L{g, o, o.light_max_degree ? o.light_max_degree : kDefaultLightMaxDegree,
-                 o.heavy_chunk ? o.heavy_chunk : kDefaultHeavyChunk};
+        std::uint32_t dl = 0, dc = 0;
+        default_sssp_split(g.E, dl, dc);
+        Launch L{g, o, o.light_max_degree ? o.light_max_degree : dl, o.heavy_chunk ? o.heavy_chunk : dc};
         require(L.light_max <= (1U << 20), "light_max_degree must be <= 2^20");
         require(L.chunk >= 64 && L.chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
         ensure_workspace(g, L.light_max, L.chunk);

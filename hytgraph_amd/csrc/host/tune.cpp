@@ -2,6 +2,7 @@
 // run on the caller's sources and timed on the host around the whole call
 // (the calls synchronise before returning); the fastest mean wins.
 #include "common.hpp"
+#include "splits.hpp"
 
 #include <hip/hip_runtime_api.h>
 
@@ -70,24 +71,28 @@ int hyt_tune(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources, const
         const std::uint32_t reps = opts->reps ? opts->reps : 2;
         const std::uint32_t max_lanes = opts->max_lanes ? opts->max_lanes : 64;
         require(max_lanes >= 2 && max_lanes <= 64, "max_lanes must be in [2, 64]");
-        std::uint64_t V = 0;
-        ok(hyt_graph_info(g, &V, nullptr, nullptr));
+        std::uint64_t V = 0, E = 0;
+        ok(hyt_graph_info(g, &V, &E, nullptr));
         for (std::uint32_t i = 0; i < num_sources; ++i)
             require(sources[i] < V, "source vertex is out of range");
         hyt_tune_result r{};
-        r.light_max_degree = 256;
-        r.heavy_chunk = 1024;
+        std::uint32_t light0 = 0, chunk0 = 0;
+        default_sssp_split(E, light0, chunk0);
+        r.light_max_degree = light0;
+        r.heavy_chunk = chunk0;
         DeviceOut buf(V * max_lanes * 4);
         if (opts->tune_split)
         {
-            // warm-up (workspace allocation), then the default split
-            (void)time_split(g, sources, 1, 256, 1024, 1, buf);
-            r.default_sssp_ms = time_split(g, sources, num_sources, 256, 1024, reps, buf);
+            // warm-up (workspace allocation), then the graph's default split
+            (void)time_split(g, sources, 1, light0, chunk0, 1, buf);
+            r.default_sssp_ms = time_split(g, sources, num_sources, light0, chunk0, reps, buf);
             r.sssp_ms = r.default_sssp_ms;
             r.candidates = 1;
-            for (std::uint32_t light : {64U, 128U, 512U, 1024U, 2048U})
+            for (std::uint32_t light : {64U, 128U, 256U, 512U, 1024U, 2048U})
             {
-                const double t = time_split(g, sources, num_sources, light, 1024, reps, buf);
+                if (light == light0)
+                    continue;
+                const double t = time_split(g, sources, num_sources, light, chunk0, reps, buf);
                 ++r.candidates;
                 if (t < r.sssp_ms)
                 {
@@ -95,8 +100,10 @@ int hyt_tune(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources, const
                     r.light_max_degree = light;
                 }
             }
-            for (std::uint32_t chunk : {256U, 512U, 2048U, 4096U, 8192U})
+            for (std::uint32_t chunk : {256U, 512U, 1024U, 2048U, 4096U, 8192U})
             {
+                if (chunk == chunk0)
+                    continue;
                 const double t = time_split(g, sources, num_sources, r.light_max_degree, chunk, reps, buf);
                 ++r.candidates;
                 if (t < r.sssp_ms)
