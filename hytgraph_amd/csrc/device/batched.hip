@@ -265,8 +265,8 @@ namespace hyt
 
         void split_params(const hyt_run_options &o, std::uint32_t &lm, std::uint32_t &ch)
         {
-            lm = o.light_max_degree ? o.light_max_degree : kLaneLightMaxDegree;
-            ch = o.heavy_chunk ? o.heavy_chunk : kLaneHeavyChunk;
+            lm = o.light_max_degree ? o.light_max_degree : kBatchedLightMaxDegree;
+            ch = o.heavy_chunk ? o.heavy_chunk : kBatchedHeavyChunk;
             require(lm <= (1U << 20), "light_max_degree must be <= 2^20");
             require(ch >= 64 && ch <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
             require(o.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");

@@ -135,6 +135,13 @@ namespace hyt
     constexpr std::uint32_t kPagerankHeavyChunk = 256;
     constexpr std::uint32_t kLaneLightMaxDegree = 256;
     constexpr std::uint32_t kLaneHeavyChunk = 1024;
+    // Batched calls (hyt_sssp / hyt_ppr with K >= 2, graph in HBM): RMAT 24,
+    // one MI355X, 64 / 512 ran K = 8 SSSP at 228 queries/s and K = 64 at 363
+    // against 199 and 328 at 256 / 1024 (PPR 242 / 365 against 230 / 361);
+    // chunks of 256 edges halve K = 64 SSSP. Lane sessions keep 256 / 1024:
+    // their chunk queue is fixed per-session state, which 64 / 512 doubles.
+    constexpr std::uint32_t kBatchedLightMaxDegree = 64;
+    constexpr std::uint32_t kBatchedHeavyChunk = 512;
     // Hybrid engine expansions read windows of partitions, many of them over
     // the host link (zero-copy): short light rows and small chunks keep many
     // wavefronts with reads in flight. RMAT scale 27, 4 GiB budget, one SSSP
