@@ -23,6 +23,7 @@ LIBNAME = "libhytgraph_amd.so"
 CPPLIB = "libhytgraph_cpp.so"
 CPPSRC = PKG / "cpp"
 TESTS_CPP = ROOT / "tests" / "cpp"
+APPS = PKG / "apps"  # executables (hyt_experiment) built beside the libraries
 
 ARCH = "gfx950"
 COMMON = [
@@ -98,13 +99,13 @@ def build_cpp(debug: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"C++ library build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, out)
-    for test_src in sorted(TESTS_CPP.glob("*.cpp")):
+    for test_src in [*sorted(TESTS_CPP.glob("*.cpp")), *sorted(APPS.glob("*.cpp"))]:
         exe = LIBDIR / test_src.stem
         cmd = [cxx, "-std=c++17", "-Wall", "-Wextra", *opt, f"-I{ROOT / 'include'}", str(test_src),
                f"-L{LIBDIR}", "-lhytgraph_cpp", "-lhytgraph_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
-            raise RuntimeError(f"C++ test build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+            raise RuntimeError(f"C++ executable build failed\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     return out
 
 
