@@ -271,15 +271,17 @@ namespace hyt
         const std::uint32_t nh = a.ctr->heavy_count;
         const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
         const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+        unsigned long long wedges = 0; // one edge-count atomic per wavefront
         for (std::uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kWave; c < n; c += waves)
         {
             const HeavyChunk h = a.heavy[c];
-            if (lane == 0)
-                add_edges(a.ctr, h.count);
+            wedges += h.count;
             walk_edges(op, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                 pos = h.begin + e;
                 val = h.value;
             });
         }
+        if (lane == 0 && wedges)
+            add_edges(a.ctr, wedges);
     }
 } // namespace hyt

@@ -409,11 +409,11 @@ namespace hyt
             const std::uint32_t nh = a.ctr->heavy_count;
             const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
             const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+            unsigned long long wedges = 0; // one edge-count atomic per wavefront
             for (std::uint32_t c = (blockIdx.x * kBlock + threadIdx.x) / kWave; c < n; c += waves)
             {
                 const HeavyChunkK h = a.heavy[c];
-                if (lane == 0)
-                    add_edges(a.ctr, h.count);
+                wedges += h.count;
                 walk_edges_k<KP>(op, a, h.count,
                                  [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &r, std::uint64_t &m,
                                      std::uint32_t &sc) {
@@ -423,6 +423,8 @@ namespace hyt
                                      sc = h.scale;
                                  });
             }
+            if (lane == 0 && wedges)
+                add_edges(a.ctr, wedges);
         }
 
         // ------------------------------------------------- sparse lane masks
@@ -628,11 +630,11 @@ namespace hyt
             const std::uint32_t nh = a.ctr->heavy_count;
             const std::uint32_t n = nh < a.heavy_cap ? nh : static_cast<std::uint32_t>(a.heavy_cap);
             const std::uint32_t waves = gridDim.x * kWavesPerBlock;
+            unsigned long long wedges = 0; // one edge-count atomic per wavefront
             for (std::uint32_t ci = (blockIdx.x * kBlock + threadIdx.x) / kWave; ci < n; ci += waves)
             {
                 const HeavyChunkK h = a.heavy[ci];
-                if (lane == 0)
-                    add_edges(a.ctr, h.count);
+                wedges += h.count;
                 const std::uint32_t c = static_cast<std::uint32_t>(__popcll(h.mask));
                 if (c == 0)
                     continue;
@@ -646,6 +648,8 @@ namespace hyt
                                  sc = h.scale;
                              });
             }
+            if (lane == 0 && wedges)
+                add_edges(a.ctr, wedges);
         }
 
         // Item form when asked, or (auto) when the queue's (row, query) pairs
