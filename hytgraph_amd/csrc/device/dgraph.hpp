@@ -35,6 +35,8 @@ namespace hyt
         std::uint32_t value;  // snapshot distance / residual share (bits)
         std::uint32_t count;  // edges in this chunk
         std::uint64_t begin;  // first edge index
+        std::uint32_t row;    // the row's vertex
+        std::uint32_t record; // 1: the chunks keep the row's expansion record
     };
 
     // Edge counts of the expand kernels go to kEdgeSlots counters on separate

@@ -17,6 +17,9 @@
 // Op supplies the per-algorithm parts:
 //   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
 //   void expanded(uint32_t u, uint32_t value, bool whole_row)  row u walks with value
+//   void expanded_chunked(uint32_t u, bool whole_row)  row u goes out as chunks
+//   uint32_t chunk_value(uint32_t u, uint32_t queued, bool record)  value a
+//       chunk of row u walks with (queued = the row's value at queue time)
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
 //   uint32_t peek(uint32_t v)                          read destination v's state
 //   void apply(uint32_t v, uint32_t c, uint32_t seen)  update v (seen = peek(v))
@@ -202,19 +205,27 @@ namespace hyt
         const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
 
-        std::uint32_t deg = 0, value = 0, nch = 0;
+        std::uint32_t deg = 0, value = 0, nch = 0, row = 0;
         std::uint64_t begin = 0, count = 0;
+        bool whole = false;
         if (i < a.F)
         {
             std::uint32_t u, vi;
             std::uint64_t full;
             rs.get(i, u, vi, begin, count, full);
+            row = u;
             value = op.row_value(vi, u, full);
-            op.expanded(u, value, count == full);
+            whole = count == full;
             if (count > a.light_max)
+            {
                 nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
+                op.expanded_chunked(u, whole);
+            }
             else
+            {
                 deg = static_cast<std::uint32_t>(count);
+                op.expanded(u, value, whole);
+            }
         }
         queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
             const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
@@ -222,6 +233,8 @@ namespace hyt
             h.value = value;
             h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
             h.begin = begin + lo;
+            h.row = row;
+            h.record = whole ? 1U : 0U;
             return h;
         });
 
@@ -276,9 +289,10 @@ namespace hyt
         {
             const HeavyChunk h = a.heavy[c];
             wedges += h.count;
+            const std::uint32_t value = op.chunk_value(h.row, h.value, h.record != 0U);
             walk_edges(op, h.count, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                 pos = h.begin + e;
-                val = h.value;
+                val = value;
             });
         }
         if (lane == 0 && wedges)

@@ -71,6 +71,29 @@ namespace hyt
             if (!SYNC && whole_row && expanded_at != nullptr)
                 expanded_at[u] = value;
         }
+        // A row going out as chunks: each chunk reads the row's live distance
+        // when it starts (the light kernel may have lowered it since the row
+        // was queued) and raises the record to the largest value any chunk
+        // used, so the record starts at 0 (ASYNC, whole rows only).
+        __device__ void expanded_chunked(std::uint32_t u, bool whole_row) const
+        {
+            if (!SYNC && whole_row && expanded_at != nullptr)
+                expanded_at[u] = 0U;
+        }
+        __device__ std::uint32_t chunk_value(std::uint32_t u, std::uint32_t queued, bool record) const
+        {
+            if (SYNC)
+                return queued;
+            std::uint32_t v = 0;
+            if (lane_id() == 0)
+            {
+                const std::uint32_t live = dist[u];
+                v = live < queued ? live : queued;
+                if (record && expanded_at != nullptr)
+                    atomicMax(&expanded_at[u], v);
+            }
+            return __shfl(v, 0, kWave);
+        }
         __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
         {
             v = col[pos];
@@ -267,6 +290,8 @@ namespace hyt
             return __float_as_uint(damping * __uint_as_float(qd[i]) / static_cast<float>(deg));
         }
         __device__ void expanded(std::uint32_t, std::uint32_t, bool) const {}
+        __device__ void expanded_chunked(std::uint32_t, bool) const {}
+        __device__ std::uint32_t chunk_value(std::uint32_t, std::uint32_t queued, bool) const { return queued; }
         __device__ bool load(std::uint64_t pos, std::uint32_t share, std::uint32_t &v, std::uint32_t &c) const
         {
             v = col[pos];

@@ -98,6 +98,8 @@ namespace hyt
                 const std::uint64_t lo = static_cast<std::uint64_t>(c) * chunk;
                 h.count = static_cast<std::uint32_t>(deg - lo < chunk ? deg - lo : chunk);
                 h.begin = begin + lo;
+                h.row = static_cast<std::uint32_t>(v);
+                h.record = 0;
                 out[base + incl - nch + c] = h;
             }
         }
