@@ -129,6 +129,9 @@ namespace hyt
         std::uint64_t heavy_cap;
         std::uint32_t light_max;
         std::uint32_t chunk;
+        // 0: every row; 1: only queue the chunks of rows above light_max;
+        // 2: only walk the rows up to light_max
+        std::uint32_t pass = 0;
     };
 
     // Processes `kUnroll` x 64 edges per iteration: all loads first, then all
@@ -214,15 +217,17 @@ namespace hyt
             std::uint64_t full;
             rs.get(i, u, vi, begin, count, full);
             row = u;
-            value = op.row_value(vi, u, full);
             whole = count == full;
-            if (count > a.light_max)
+            const bool chunked = count > a.light_max;
+            if (chunked && a.pass != 2)
             {
+                value = op.row_value(vi, u, full);
                 nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
                 op.expanded_chunked(u, whole);
             }
-            else
+            else if (!chunked && a.pass != 1)
             {
+                value = op.row_value(vi, u, full);
                 deg = static_cast<std::uint32_t>(count);
                 op.expanded(u, value, whole);
             }
