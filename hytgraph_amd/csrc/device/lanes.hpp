@@ -869,7 +869,7 @@ namespace hyt
             std::vector<unsigned long long> masks;
         };
 
-        SeedPlan plan_seeds(const std::uint32_t *src, std::uint32_t K)
+        inline SeedPlan plan_seeds(const std::uint32_t *src, std::uint32_t K)
         {
             SeedPlan p;
             for (std::uint32_t k = 0; k < K; ++k)
