@@ -10,6 +10,11 @@ namespace hyt
     constexpr int kWavesPerBlock = kBlock / kWave;
     constexpr int kUnroll = 4;                  // edges in flight per lane
     constexpr unsigned kHeavyGrid = 2048;       // workgroups of the chunk kernels
+    // Single-source SSSP and PageRank chunk kernels (RMAT 24 SSSP: 1024
+    // workgroups ran 32.5 GTEPS against 31.7 at 2048, 29.9 at 512 and 31.3
+    // at 4096; PageRank 0.305 against 0.310 s). The K-lane chunk kernels keep
+    // 2048: at 1024 batched K = 8 SSSP fell from 284 to 230 queries/s.
+    constexpr unsigned kSingleHeavyGrid = 1024;
 
     template <class T>
     __global__ void fill_kernel(T *p, std::uint64_t n, T value)

@@ -213,7 +213,7 @@ namespace hyt
                     pull_light<OffT, PrPullOp><<<blocks_for(g.V), kBlock, 0, s>>>(g.in_off.as<OffT>(), pop, g.V,
                                                                                    light_max);
                     hip_check(hipGetLastError(), "launch pr_pull_light");
-                    pull_heavy<PrPullOp><<<kHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
+                    pull_heavy<PrPullOp><<<kSingleHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
                                                                       g.in_heavy_count);
                     hip_check(hipGetLastError(), "launch pr_pull_heavy");
                     st.pull_rounds += 1;
@@ -224,7 +224,7 @@ namespace hyt
                     expand_light<GraphRows<OffT>, PrOp><<<blocks_for(F), kBlock, 0, s>>>(whole_graph(g.off.as<OffT>(), a.q),
                                                                                      op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_light");
-                    expand_heavy<PrOp><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    expand_heavy<PrOp><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_heavy");
                 }
                 if (o.time_kernels)

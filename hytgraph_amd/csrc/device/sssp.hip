@@ -221,7 +221,7 @@ namespace hyt
                     pull_light<OffT, SsspPullOp<WM, SYNC>><<<blocks_for(g.V), kBlock, 0, s>>>(
                         g.in_off.as<OffT>(), pop, g.V, L.light_max);
                     hip_check(hipGetLastError(), "launch sssp_pull_light");
-                    pull_heavy<SsspPullOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
+                    pull_heavy<SsspPullOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
                                                                                   g.in_heavy_count);
                     hip_check(hipGetLastError(), "launch sssp_pull_heavy");
                     st.pull_rounds += 1;
@@ -240,7 +240,7 @@ namespace hyt
                     expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
                         whole_graph(g.off.as<OffT>(), a.q), op, a1);
                     hip_check(hipGetLastError(), "launch sssp_expand_light");
-                    expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a1);
+                    expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
                     hip_check(hipGetLastError(), "launch sssp_expand_heavy");
                     expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
                         whole_graph(g.off.as<OffT>(), a.q), op, a2);
@@ -251,7 +251,7 @@ namespace hyt
                     expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
                         whole_graph(g.off.as<OffT>(), a.q), op, a);
                     hip_check(hipGetLastError(), "launch sssp_expand_light");
-                    expand_heavy<SsspOp<WM, SYNC>><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
+                    expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a);
                     hip_check(hipGetLastError(), "launch sssp_expand_heavy");
                 }
                 if (L.o.time_kernels)
