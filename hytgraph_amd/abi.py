@@ -32,6 +32,7 @@ HYT_HOST_PAGEABLE = 0
 HYT_HOST_PINNED = 1
 HYT_GRAPH_WEIGHTS_U32 = 0
 HYT_GRAPH_WEIGHTS_F32 = 1
+HYT_GRAPH_OFF64 = 2
 HYT_SSSP_SYNC = 0
 HYT_SSSP_ASYNC = 1
 HYT_DIR_AUTO = 0
@@ -70,6 +71,7 @@ class RunOptions(C.Structure):
         ("pull_percent", C.c_uint32),
         ("delta", C.c_uint32),
         ("lane_form", C.c_uint32),
+        ("tolerance", C.c_float),
     ]
 
 
@@ -123,7 +125,7 @@ class HybridConfig(C.Structure):
     _fields_ = [
         ("device_budget_bytes", C.c_uint64), ("partition_edges", C.c_uint64), ("pack_edges", C.c_uint64),
         ("filter_combine_k", C.c_uint32), ("force_engine", C.c_int32), ("host_threads", C.c_uint32),
-        ("calibrate", C.c_uint32), ("cache", C.c_uint32), ("reserved", C.c_uint32), ("cost", CostParams),
+        ("calibrate", C.c_uint32), ("cache", C.c_uint32), ("flags", C.c_uint32), ("cost", CostParams),
     ]
 
 
@@ -418,15 +420,15 @@ HYT_LANES_ITEMS = 2
 
 def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
              light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0,
-             lane_form=HYT_LANES_AUTO) -> RunOptions:
+             lane_form=HYT_LANES_AUTO, tolerance=0.0) -> RunOptions:
     return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk,
-                      direction, pull_percent, delta, lane_form)
+                      direction, pull_percent, delta, lane_form, tolerance)
 
 
 class DeviceGraph:
     """A CSR graph resident in HBM (hyt_graph_t)."""
 
-    def __init__(self, offsets, columns, weights=None, weights_f32=False):
+    def __init__(self, offsets, columns, weights=None, weights_f32=False, off64=False):
         offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         columns = np.ascontiguousarray(columns, dtype=np.uint32)
         if weights is not None:
@@ -439,16 +441,18 @@ class DeviceGraph:
         self.weights_f32 = weights_f32
         h = C.c_void_p()
         flags = HYT_GRAPH_WEIGHTS_F32 if weights_f32 else HYT_GRAPH_WEIGHTS_U32
+        if off64:
+            flags |= HYT_GRAPH_OFF64
         check(load().hyt_graph_upload(self.V, len(columns), _ptr(offsets), _ptr(columns),
                                       _ptr(weights), flags, C.byref(h)))
         self._h = h
 
     @classmethod
-    def from_host(cls, hg: HostGraph, weights_f32=False):
+    def from_host(cls, hg: HostGraph, weights_f32=False, off64=False):
         w = hg.weights
         if w is not None and weights_f32:
             w = w.astype(np.float32)
-        return cls(hg.offsets, hg.columns, w, weights_f32=weights_f32)
+        return cls(hg.offsets, hg.columns, w, weights_f32=weights_f32, off64=off64)
 
     def info(self) -> dict:
         V, E, B = C.c_uint64(), C.c_uint64(), C.c_uint64()

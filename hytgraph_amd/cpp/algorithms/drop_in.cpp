@@ -40,13 +40,9 @@ namespace hytgraph::algorithms
         o.max_rounds = options.max_iterations;
         hyt_run_stats st{};
         const std::uint32_t src = options.source;
-        if (options.tolerance > 0.0F)
-        {
-            // The original's tolerance rule (candidate + tol < best, applied
-            // in incoming-edge order) is order-dependent; the frontier kernels
-            // implement the tolerance-0 rule only.
-            throw std::invalid_argument("sssp_gpu: tolerance > 0 is not supported by the HIP path");
-        }
+        // tolerance > 0: the original's candidate + tol < best rule, applied
+        // in incoming-edge order by the ordered pull rounds
+        o.tolerance = options.tolerance;
         check(hyt_sssp(sg.g, &src, 1, bits.data(), &o, &st), "sssp_gpu");
         SSSPResult r;
         r.distances.resize(n);

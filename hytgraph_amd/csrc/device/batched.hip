@@ -5,8 +5,9 @@
 // single-source results): a lane marks v when its candidate beats a plain load
 // of dist[v][k], which cannot miss an improvement (the first lowering atomic
 // of a round comes from a lane whose load passed). PPR-K needs the exact
-// eps-crossing, so it uses returning float atomics, and snapshots the active
-// residuals of the frontier before each round like the single-source form.
+// eps-crossing, so it uses returning integer atomics on 64-bit fixed-point
+// residual words (mass.hpp), and snapshots the active residuals of the
+// frontier before each round like the single-source form.
 #include "lanes.hpp"
 
 #include <algorithm>
@@ -42,11 +43,12 @@ namespace hyt
                      std::uint32_t chunk)
         {
             const std::uint64_t VK = g.V * K;
-            kw.val.ensure(VK * 4, "hipMalloc([V][K] state)");
+            // SSSP: u32 distances; PPR: u64 residual, rank and snapshot words
+            kw.val.ensure(VK * (ppr ? 8 : 4), "hipMalloc([V][K] state)");
             if (ppr)
             {
-                kw.rank.ensure(VK * 4, "hipMalloc([V][K] rank)");
-                kw.snap.ensure(VK * 4, "hipMalloc([V][K] snapshot)");
+                kw.rank.ensure(VK * 8, "hipMalloc([V][K] rank)");
+                kw.snap.ensure(VK * 8, "hipMalloc([V][K] snapshot)");
             }
             if (kw.mark.bytes < (g.V + kPad) * 8)
             {
@@ -173,8 +175,8 @@ namespace hyt
                 round_begin<<<1, 64, 0, s>>>(d_ctr, rs, KP, o.lane_form, max_rounds);
                 if (ppr)
                     pprk_snapshot<<<snapshot_blocks(g.V * static_cast<std::uint64_t>(K)), kBlock, 0, s>>>(
-                        rows.q, 0, K, rows.qmask, reinterpret_cast<float *>(kw.val.ptr), kw.rank.as<float>(),
-                        kw.snap.as<float>(), rs);
+                        rows.q, 0, K, rows.qmask, kw.val.as<unsigned long long>(), kw.rank.as<unsigned long long>(),
+                        kw.snap.as<unsigned long long>(), rs);
                 expandk_light<QueueRowsK<OffT>, KP, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
                 expandk_light_items<QueueRowsK<OffT>, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
                 expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);
@@ -284,7 +286,8 @@ namespace hyt
         const std::uint32_t inf = (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
         launch_fill<std::uint32_t>(kw.val.as<std::uint32_t>(), g.V * K, inf, g.stream);
         const SeedPlan p = plan_seeds(sources, K);
-        upload_seeds(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, sources, K, kw.val.as<std::uint32_t>(), 0U);
+        upload_seeds<std::uint32_t>(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, sources, K,
+                                    kw.val.as<std::uint32_t>(), 0U);
         auto run = [&](auto op) {
             op.col = g.col.as<std::uint32_t>();
             op.w = g.w.as<std::uint32_t>();
@@ -316,24 +319,24 @@ namespace hyt
         KWorkspace &kw = kws_of(g);
         prepare(g, kw, K, true, lm, ch);
         const std::uint64_t VK = g.V * K;
-        launch_fill<float>(reinterpret_cast<float *>(kw.val.ptr), VK, 0.0F, g.stream);
-        launch_fill<float>(kw.rank.as<float>(), VK, 0.0F, g.stream);
+        const MassScale m = mass_scale(1, damping, eps);
+        launch_fill<unsigned long long>(kw.val.as<unsigned long long>(), VK, 0ULL, g.stream);
+        launch_fill<unsigned long long>(kw.rank.as<unsigned long long>(), VK, 0ULL, g.stream);
         const SeedPlan p = plan_seeds(seeds, K);
-        const float r0 = 1.0F - damping;
-        std::uint32_t r0_bits;
-        std::memcpy(&r0_bits, &r0, 4);
-        upload_seeds(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, seeds, K, kw.val.as<std::uint32_t>(), r0_bits);
+        upload_seeds<unsigned long long>(g.stream, kw.seeds, kw.mark.as<unsigned long long>(), p, seeds, K,
+                                         kw.val.as<unsigned long long>(), m.r0);
         PprKOp op;
         op.col = g.col.as<std::uint32_t>();
-        op.res = reinterpret_cast<float *>(kw.val.ptr);
-        op.snap = kw.snap.as<float>();
+        op.res = kw.val.as<unsigned long long>();
+        op.snap = kw.snap.as<unsigned long long>();
         op.K = K;
         op.damping = damping;
-        op.eps = eps;
+        op.eps = m.eps;
         dispatch_off(g, kw, op, K, o, st, lm, ch, true);
 
         float *dst = o.out_on_device ? out : kw.snap.as<float>();
-        pprk_finish(g.stream, kw.rank.as<float>(), op.res, g.V, K, kw.partial.as<double>(), dst);
+        pprk_finish(g.stream, kw.rank.as<unsigned long long>(), op.res, g.V, K, kw.partial.as<unsigned long long>(),
+                    dst);
         if (!o.out_on_device)
             hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
         hip_check(hipStreamSynchronize(g.stream), "ppr output");

@@ -5,7 +5,10 @@
 //   col  u32[E]   destination ids, row-sorted
 //   w    u32[E]   integer weights or float bit patterns (absent if unweighted)
 // Workspace (single-source traversals; every [V] array has 16 spare slots):
-//   dist/res u32[V] (distance or residual bits), rank f32[V] (PageRank),
+//   dist  u32[V]  distances (PageRank: staging of the float output),
+//   res, rank u64[V] PageRank residual and pushed mass, fixed-point words
+//                 (mass.hpp; allocated by the first PageRank call),
+//   share f32[V]  PageRank per-edge share of the last compaction's rows,
 //   prev  u32[V]  distance at the end of the round a vertex last improved in,
 //   touch u8[V]   set (plain store) by expand when a vertex may have improved,
 //   q     u32[V]  frontier rows, qd u32[V] their snapshot values,
@@ -81,7 +84,8 @@ namespace hyt
 
     struct Workspace
     {
-        DeviceBuffer dist, rank, prev, touch, q, qd, heavy, ctr, partial;
+        DeviceBuffer dist, prev, touch, q, qd, heavy, ctr, partial;
+        DeviceBuffer res, rank; // PageRank mass words u64[V] (ensure_mass_state)
         DeviceBuffer active; // frontier bitmap (bit v = row v selected by the last compaction)
         DeviceBuffer share;  // PageRank pull: d * r_u / outdeg(u) of the frontier rows, by vertex
         std::uint64_t heavy_cap = 0;
@@ -111,7 +115,7 @@ namespace hyt
         std::uint64_t device_bytes() const
         {
             return off.bytes + col.bytes + w.bytes + in_off.bytes + in_col.bytes + in_w.bytes + in_heavy.bytes +
-                   ws.dist.bytes + ws.rank.bytes + ws.prev.bytes + ws.touch.bytes + ws.q.bytes + ws.qd.bytes +
+                   ws.dist.bytes + ws.res.bytes + ws.rank.bytes + ws.prev.bytes + ws.touch.bytes + ws.q.bytes + ws.qd.bytes +
                    ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes;
         }
         int weight_mode() const
@@ -165,8 +169,11 @@ namespace hyt
     // Spare slots after every [V] workspace array (vector loads past the end).
     constexpr std::uint64_t kPad = 16;
 
-    // x = rank + residual, normalised to sum 1, into out (host or device);
-    // st.residual = unpushed mass / total.
+    // Allocates the PageRank residual and rank words (u64[V + kPad] each).
+    void ensure_mass_state(DeviceGraph &g);
+
+    // x = rank + residual (mass words), normalised to sum 1, into out (host
+    // or device); st.residual = unpushed mass / total.
     void finalize_ranks(DeviceGraph &g, float *out, bool out_on_device, hyt_run_stats &st);
 
     // Builds the incoming CSR and the static chunk list of its heavy rows.

@@ -185,13 +185,22 @@ namespace hyt
         if (lane == 0)
             hbase = atomicAdd(&ctr->heavy_count, htotal);
         hbase = __shfl(hbase, 0, kWave);
+        const std::uint64_t slot = static_cast<std::uint64_t>(hbase) + hincl - nch;
         if (static_cast<std::uint64_t>(hbase) + htotal > heavy_cap)
         {
+            // the round fails on the host; until then the chunk kernel reads
+            // min(heavy_count, cap) slots, so the reserved ones that exist
+            // hold empty chunks (no edges, no record) rather than stale bytes
             if (lane == 0)
                 atomicOr(&ctr->heavy_overflow, 1U);
+            for (std::uint32_t c = 0; c < nch; ++c)
+                if (slot + c < heavy_cap)
+                {
+                    Chunk h{};
+                    heavy[slot + c] = h;
+                }
             return;
         }
-        const std::uint32_t slot = hbase + hincl - nch;
         for (std::uint32_t c = 0; c < nch; ++c)
             heavy[slot + c] = make(c);
     }

@@ -50,7 +50,6 @@ namespace hyt
         if (!ws.ready)
         {
             ws.dist.alloc(Vp * 4, "hipMalloc(dist)");
-            ws.rank.alloc(Vp * 4, "hipMalloc(rank)");
             ws.prev.alloc(Vp * 4, "hipMalloc(prev)");
             ws.touch.alloc(Vp, "hipMalloc(touch)");
             ws.q.alloc(Vp * 4, "hipMalloc(queue)");
@@ -69,6 +68,13 @@ namespace hyt
         }
         ws.heavy_cap = ws.heavy.bytes / sizeof(HeavyChunk);
     }
+
+    void ensure_mass_state(DeviceGraph &g)
+    {
+        const std::uint64_t bytes = (g.V + kPad) * 8;
+        g.ws.res.ensure(bytes, "hipMalloc(PageRank residuals)");
+        g.ws.rank.ensure(bytes, "hipMalloc(PageRank ranks)");
+    }
 } // namespace hyt
 
 extern "C" {
@@ -81,9 +87,10 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
         using namespace hyt;
         require(out != nullptr, "out is null");
         *out = nullptr;
-        require(flags == HYT_GRAPH_WEIGHTS_U32 || flags == HYT_GRAPH_WEIGHTS_F32, "unknown graph flags");
+        require((flags & ~static_cast<uint32_t>(HYT_GRAPH_WEIGHTS_F32 | HYT_GRAPH_OFF64)) == 0, "unknown graph flags");
+        const bool f32 = (flags & HYT_GRAPH_WEIGHTS_F32) != 0;
         validate_csr(num_vertices, num_edges, row_offsets, columns);
-        if (weights != nullptr && flags == HYT_GRAPH_WEIGHTS_F32)
+        if (weights != nullptr && f32)
         {
             // Float weights must be finite and >= 0 (original sssp_cpu check);
             // -0.0 is allowed and behaves as +0 in the sums.
@@ -102,9 +109,9 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
         g.V = num_vertices;
         g.E = num_edges;
         g.device = dev;
-        g.off64 = num_edges >= 0xFFFFFFFFULL;
+        g.off64 = num_edges >= 0xFFFFFFFFULL || (flags & HYT_GRAPH_OFF64) != 0;
         g.weighted = weights != nullptr;
-        g.weights_f32 = flags == HYT_GRAPH_WEIGHTS_F32;
+        g.weights_f32 = f32;
         hip_check(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&g.ev0), "hipEventCreate");
         hip_check(hipEventCreate(&g.ev1), "hipEventCreate");

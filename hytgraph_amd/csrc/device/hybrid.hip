@@ -374,7 +374,7 @@ namespace hyt
         void release_single_query_state()
         {
             Workspace &ws = dg.ws;
-            for (DeviceBuffer *b : {&ws.dist, &ws.rank, &ws.prev, &ws.touch, &ws.qd, &ws.partial, &ws.active,
+            for (DeviceBuffer *b : {&ws.dist, &ws.res, &ws.rank, &ws.prev, &ws.touch, &ws.qd, &ws.partial, &ws.active,
                                     &ws.share, &ws.heavy, &stage_val})
                 b->release();
             ws.heavy_cap = 0;
@@ -592,7 +592,8 @@ namespace hyt
             g.E = host->E;
             for (std::uint64_t v = 0; v < host->V; ++v)
                 g.max_degree = std::max<std::uint64_t>(g.max_degree, host->off[v + 1] - host->off[v]);
-            g.off64 = host->E >= 0xFFFFFFFFULL;
+            require((cfg.flags & ~static_cast<std::uint32_t>(HYT_GRAPH_OFF64)) == 0, "unknown hybrid flags");
+            g.off64 = host->E >= 0xFFFFFFFFULL || (cfg.flags & HYT_GRAPH_OFF64) != 0;
             g.weighted = host->weighted;
             g.weights_f32 = f32;
             hip_check(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -1210,19 +1211,20 @@ namespace hyt
             Workspace &ws = g.ws;
             hipStream_t s = g.stream;
             Counters *d_ctr = ws.ctr.as<Counters>();
-            launch_fill<float>(ws.dist.as<float>(), g.V, 1.0F - damping, s);
-            launch_fill<float>(ws.rank.as<float>(), g.V, 0.0F, s);
+            ensure_mass_state(g);
+            const MassScale m = mass_scale(g.V, damping, eps);
+            launch_fill<unsigned long long>(ws.res.as<unsigned long long>(), g.V, m.r0, s);
+            launch_fill<unsigned long long>(ws.rank.as<unsigned long long>(), g.V, 0ULL, s);
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            PrSel<OffT> sel{ws.dist.as<float>(), ws.rank.as<float>(), ws.share.as<float>(), g.off.as<OffT>(), damping,
-                            eps};
+            PrSel<OffT> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(), ws.share.as<float>(),
+                            g.off.as<OffT>(), damping, m.eps};
             Counters c;
             std::uint32_t F = ordered_compaction<OffT>(E, sel, c);
             auto make = [&](const std::uint32_t *col, const std::uint32_t *, const std::uint32_t *qd) {
                 PrOp op;
                 op.col = col;
-                op.res = ws.dist.as<float>();
+                op.res = ws.res.as<unsigned long long>();
                 op.qd = qd;
-                op.damping = damping;
                 return op;
             };
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : 100000;
@@ -1357,11 +1359,12 @@ namespace hyt
             DeviceGraph &g = E.dg;
             LaneState &L = E.lk;
             const std::uint64_t VK = g.V * K;
-            L.val.ensure(VK * 4, "hipMalloc([V][K] state)");
+            // SSSP: u32 distances; PPR: u64 residual, rank and snapshot words
+            L.val.ensure(VK * (ppr ? 8 : 4), "hipMalloc([V][K] state)");
             if (ppr)
             {
-                L.rank.ensure(VK * 4, "hipMalloc([V][K] rank)");
-                L.snap.ensure(VK * 4, "hipMalloc([V][K] snapshot)");
+                L.rank.ensure(VK * 8, "hipMalloc([V][K] rank)");
+                L.snap.ensure(VK * 8, "hipMalloc([V][K] snapshot)");
             }
             if (L.mark.bytes < (g.V + kPad) * 8)
             {
@@ -1450,7 +1453,8 @@ namespace hyt
                 {
                     lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
                         g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
-                        reinterpret_cast<float *>(L.val.ptr), L.rank.as<float>(), L.snap.as<float>());
+                        L.val.as<unsigned long long>(), L.rank.as<unsigned long long>(),
+                        L.snap.as<unsigned long long>());
                     hip_check(hipGetLastError(), "launch pprk_snapshot");
                 }
                 const LaneExpander<OffT, KP, Op> x{E, op, a, lanes::use_items(o.lane_form, pairs, F, KP)};
@@ -1520,8 +1524,8 @@ namespace hyt
             const std::uint32_t inf = (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
             launch_fill<std::uint32_t>(L.val.as<std::uint32_t>(), g.V * K, inf, g.stream);
             const lanes::SeedPlan p = lanes::plan_seeds(sources, K);
-            lanes::upload_seeds(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, sources, K,
-                                L.val.as<std::uint32_t>(), 0U);
+            lanes::upload_seeds<std::uint32_t>(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, sources, K,
+                                               L.val.as<std::uint32_t>(), 0U);
             auto run = [&](auto op) {
                 op.dist = L.val.as<std::uint32_t>();
                 op.K = K;
@@ -1548,23 +1552,22 @@ namespace hyt
             LaneState &L = E.lk;
             prepare_lanes(E, K, true);
             const std::uint64_t VK = g.V * K;
-            launch_fill<float>(reinterpret_cast<float *>(L.val.ptr), VK, 0.0F, g.stream);
-            launch_fill<float>(L.rank.as<float>(), VK, 0.0F, g.stream);
+            const MassScale m = mass_scale(1, damping, eps);
+            launch_fill<unsigned long long>(L.val.as<unsigned long long>(), VK, 0ULL, g.stream);
+            launch_fill<unsigned long long>(L.rank.as<unsigned long long>(), VK, 0ULL, g.stream);
             const lanes::SeedPlan p = lanes::plan_seeds(seeds, K);
-            const float r0 = 1.0F - damping;
-            std::uint32_t r0_bits;
-            std::memcpy(&r0_bits, &r0, 4);
-            lanes::upload_seeds(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, seeds, K,
-                                L.val.as<std::uint32_t>(), r0_bits);
+            lanes::upload_seeds<unsigned long long>(g.stream, L.seeds, L.mark.as<unsigned long long>(), p, seeds, K,
+                                                    L.val.as<unsigned long long>(), m.r0);
             lanes::PprKOp op{};
-            op.res = reinterpret_cast<float *>(L.val.ptr);
-            op.snap = L.snap.as<float>();
+            op.res = L.val.as<unsigned long long>();
+            op.snap = L.snap.as<unsigned long long>();
             op.K = K;
             op.damping = damping;
-            op.eps = eps;
+            op.eps = m.eps;
             hybrid_k_run(E, op, K, true, o, st);
             float *dst = o.out_on_device ? out : L.snap.as<float>();
-            lanes::pprk_finish(g.stream, L.rank.as<float>(), op.res, g.V, K, L.partial.as<double>(), dst);
+            lanes::pprk_finish(g.stream, L.rank.as<unsigned long long>(), op.res, g.V, K,
+                               L.partial.as<unsigned long long>(), dst);
             if (!o.out_on_device)
                 hip_check(hipMemcpyAsync(out, dst, VK * 4, hipMemcpyDeviceToHost, g.stream), "copy ppr");
             hip_check(hipStreamSynchronize(g.stream), "hybrid ppr output");
@@ -1586,8 +1589,8 @@ namespace hyt
             if (ppr)
             {
                 lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
-                    g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
-                    reinterpret_cast<float *>(L.val.ptr), L.rank.as<float>(), L.snap.as<float>());
+                    g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(), L.val.as<unsigned long long>(),
+                    L.rank.as<unsigned long long>(), L.snap.as<unsigned long long>());
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
             lanes::KArgs a{};
@@ -1615,11 +1618,11 @@ namespace hyt
             if (kind == HYT_QUERY_PPR)
             {
                 lanes::PprKOp op{};
-                op.res = reinterpret_cast<float *>(L.val.ptr);
-                op.snap = L.snap.as<float>();
+                op.res = L.val.as<unsigned long long>();
+                op.snap = L.snap.as<unsigned long long>();
                 op.K = K;
                 op.damping = damping;
-                op.eps = eps;
+                op.eps = mass_scale(1, damping, eps).eps;
                 lane_round_t<OffT, KP>(E, op, K, true, F, pairs, form, st, out);
                 return;
             }
@@ -1667,7 +1670,8 @@ namespace hyt
     std::uint64_t hybrid_lane_state_bytes(HybridEngine &E, int kind, std::uint32_t K)
     {
         const std::uint64_t V = E.dg.V, Vp = V + kPad;
-        const std::uint64_t per_lane = kind == HYT_QUERY_PPR ? V * 12 : V * 4;
+        // PPR: residual, rank and snapshot words (u64); SSSP: u32 distances
+        const std::uint64_t per_lane = kind == HYT_QUERY_PPR ? V * 24 : V * 4;
         const std::uint64_t fixed = Vp * 16 + static_cast<std::uint64_t>(compact_blocks(V)) * kCompactRowsPerBlock * 8 +
                                     lanes::lane_chunk_capacity(V, E.dg.E, kLaneLightMaxDegree, kLaneHeavyChunk) *
                                         sizeof(lanes::HeavyChunkK) +
@@ -1691,13 +1695,17 @@ namespace hyt
             L.host_lanes = std::make_unique<PinnedBuffer>(8);
         const std::uint64_t VK = E.dg.V * K;
         const int wm = E.dg.weight_mode();
-        const std::uint32_t inf = ppr ? 0U : ((wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU);
-        launch_fill<std::uint32_t>(L.val.as<std::uint32_t>(), VK, inf, E.dg.stream);
         if (ppr)
-            launch_fill<float>(L.rank.as<float>(), VK, 0.0F, E.dg.stream);
+        {
+            launch_fill<unsigned long long>(L.val.as<unsigned long long>(), VK, 0ULL, E.dg.stream);
+            launch_fill<unsigned long long>(L.rank.as<unsigned long long>(), VK, 0ULL, E.dg.stream);
+        }
+        else
+            launch_fill<std::uint32_t>(L.val.as<std::uint32_t>(), VK,
+                                       (wm == kWF32 || wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU, E.dg.stream);
         hip_check(hipMemsetAsync(L.mark.ptr, 0, L.mark.bytes, E.dg.stream), "clear marks");
         hip_check(hipStreamSynchronize(E.dg.stream), "lane state");
-        return HybridLaneView{L.val.as<std::uint32_t>(), ppr ? L.rank.as<float>() : nullptr,
+        return HybridLaneView{L.val.ptr, ppr ? L.rank.as<unsigned long long>() : nullptr,
                               L.mark.as<unsigned long long>()};
     }
 
@@ -1755,6 +1763,7 @@ namespace hyt
         if (opts)
             o = *opts;
         require(o.lane_form <= HYT_LANES_ITEMS, "lane_form must be 0, 1 or 2");
+        require(o.tolerance == 0.0F, "SSSP tolerance > 0 runs on an HBM-resident graph (hyt_sssp)");
         hyt_hybrid_stats st{};
         st.partitions = E.parts.size();
         st.link_gbs_peak = E.link_gbs;
@@ -1867,6 +1876,7 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
         hyt_run_options o{};
         if (opts)
             o = *opts;
+        require(o.tolerance == 0.0F, "SSSP tolerance > 0 runs on an HBM-resident graph (hyt_sssp)");
         hyt_hybrid_stats st{};
         st.partitions = E.parts.size();
         st.link_gbs_peak = E.link_gbs;

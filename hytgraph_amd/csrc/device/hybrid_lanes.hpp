@@ -27,8 +27,8 @@ namespace hyt
 
     struct HybridLaneView
     {
-        std::uint32_t *val;       // [V][K]: distances, or PPR residual bits
-        float *rank;              // [V][K] PPR pushed mass (null for SSSP)
+        void *val;                // [V][K]: u32 distances, or u64 PPR residual words (mass.hpp)
+        unsigned long long *rank; // [V][K] PPR pushed mass words (null for SSSP)
         unsigned long long *mark; // per row: lanes whose row awaits the next round
     };
     // Allocates the lane state for K lanes of `kind` and frees every lane.

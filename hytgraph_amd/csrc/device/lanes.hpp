@@ -17,6 +17,7 @@
 
 #include "compact.hpp"
 #include "expand.hpp"
+#include "mass.hpp"
 
 #include <algorithm>
 #include <cstring>
@@ -241,7 +242,8 @@ namespace hyt
             const int kl = lane % KP;
             for (std::uint32_t base = 0; base < total; base += G * kUnroll)
             {
-                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], cand[kUnroll], seen[kUnroll];
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll];
+                typename Op::Val cand[kUnroll], seen[kUnroll];
                 std::uint64_t m[kUnroll];
                 bool ok[kUnroll];
 #pragma unroll
@@ -475,8 +477,8 @@ namespace hyt
             const int lane = lane_id();
             for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
             {
-                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], kk[kUnroll], cand[kUnroll],
-                    seen[kUnroll];
+                std::uint32_t vv[kUnroll], ww[kUnroll], row[kUnroll], scale[kUnroll], kk[kUnroll];
+                typename Op::Val cand[kUnroll], seen[kUnroll];
                 bool ok[kUnroll];
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u)
@@ -667,6 +669,7 @@ namespace hyt
         template <int WM>
         struct SsspKOp
         {
+            using Val = std::uint32_t; // distance (u32 or float bits)
             static constexpr bool kRowIsQueueIndex = false;
             const std::uint32_t *__restrict__ col;
             const std::uint32_t *__restrict__ w;
@@ -718,50 +721,52 @@ namespace hyt
         };
 
         // ----------------------------------------------------------- PPR-K
+        // Residuals, ranks and snapshots are mass words (mass.hpp): the add
+        // of lane kl through row i is the share of its snapshot word at the
+        // row's scale, computed as in the single-query form, and the
+        // returning integer atomic tells exactly one adder that the residual
+        // crossed eps.
         struct PprKOp
         {
+            using Val = unsigned long long; // mass word
             static constexpr bool kRowIsQueueIndex = true;
             const std::uint32_t *__restrict__ col;
-            float *res;
-            const float *snap; // [F][K] snapshot residuals
+            unsigned long long *res;
+            const unsigned long long *snap; // [F][K] snapshot residuals
             std::uint32_t K;
             float damping;
-            float eps;
+            unsigned long long eps; // threshold word
 
             __device__ std::uint32_t row_scale(std::uint64_t deg) const
             {
-                return deg ? __float_as_uint(damping / static_cast<float>(deg)) : 0U;
+                return __float_as_uint(row_scale_of(damping, deg));
             }
             __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
             {
                 v = col[pos];
                 wt = 0;
             }
-            // The share of lane kl pushed from queue row i; commit adds it and
-            // keeps the residual it found (returning atomic), improved = the
-            // residual crossed eps with this add.
             __device__ bool prepare(std::uint32_t i, std::uint32_t scale, std::uint32_t, std::uint32_t, int kl,
-                                    std::uint32_t &cand, std::uint32_t &) const
+                                    unsigned long long &cand, unsigned long long &) const
             {
-                const float add = __uint_as_float(scale) * snap[static_cast<std::uint64_t>(i) * K + kl];
-                cand = __float_as_uint(add);
-                return add > 0.0F;
+                const unsigned long long r = snap[static_cast<std::uint64_t>(i) * K + kl];
+                cand = r ? share_word(share_of(r, __uint_as_float(scale))) : 0ULL;
+                return cand != 0ULL;
             }
-            __device__ void commit(std::uint32_t v, int kl, std::uint32_t cand, std::uint32_t &seen) const
+            __device__ void commit(std::uint32_t v, int kl, unsigned long long cand, unsigned long long &seen) const
             {
-                seen = __float_as_uint(unsafeAtomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl],
-                                                       __uint_as_float(cand)));
+                seen = atomicAdd(&res[static_cast<std::uint64_t>(v) * K + kl], cand);
             }
-            __device__ bool improved(std::uint32_t cand, std::uint32_t seen) const
+            __device__ bool improved(unsigned long long cand, unsigned long long seen) const
             {
-                const float old = __uint_as_float(seen), add = __uint_as_float(cand);
-                return old <= eps && old + add > eps;
+                return seen <= eps && seen + cand > eps;
             }
         };
 
         // Moves the active residuals of the frontier rows into rank.
         __global__ void pprk_snapshot(const std::uint32_t *q, std::uint32_t F, std::uint32_t K,
-                                      const unsigned long long *qmask, float *res, float *rank, float *snap,
+                                      const unsigned long long *qmask, unsigned long long *res,
+                                      unsigned long long *rank, unsigned long long *snap,
                                       const RoundState *rs = nullptr)
         {
             const std::uint64_t n = static_cast<std::uint64_t>(rs ? rs->F : F) * K;
@@ -772,12 +777,12 @@ namespace hyt
                 const std::uint32_t i = static_cast<std::uint32_t>(t / K);
                 const std::uint32_t k = static_cast<std::uint32_t>(t % K);
                 const std::uint32_t u = q[i];
-                float r = 0.0F;
+                unsigned long long r = 0ULL;
                 if ((qmask[i] >> k) & 1ULL)
                 {
                     const std::uint64_t idx = static_cast<std::uint64_t>(u) * K + k;
                     r = res[idx];
-                    res[idx] = 0.0F;
+                    res[idx] = 0ULL;
                     rank[idx] += r;
                 }
                 snap[t] = r;
@@ -798,9 +803,10 @@ namespace hyt
             return kp < 2 ? 2 : kp;
         }
 
+        template <class T>
         __global__ void seed_rows(std::uint32_t nrows, const std::uint32_t *rows, const unsigned long long *masks,
-                                  const std::uint32_t *lane_src, std::uint32_t K, unsigned long long *mark,
-                                  std::uint32_t *val, std::uint32_t seed_value)
+                                  const std::uint32_t *lane_src, std::uint32_t K, unsigned long long *mark, T *val,
+                                  T seed_value)
         {
             const std::uint32_t t = threadIdx.x;
             if (t < nrows)
@@ -809,22 +815,24 @@ namespace hyt
                 val[static_cast<std::uint64_t>(lane_src[t]) * K + t] = seed_value;
         }
 
-        // Column sums of x = rank + res ([V][K]) into partial[block][K] (double).
-        __global__ __launch_bounds__(kBlock) void pprk_fold(float *rank, const float *res, std::uint64_t V,
-                                                           std::uint32_t K, std::uint32_t KP, double *partial)
+        // Column sums of x = rank + res ([V][K] mass words, written into
+        // rank) into partial[block][k], exact (no column passes 2^62).
+        __global__ __launch_bounds__(kBlock) void pprk_fold(unsigned long long *rank, const unsigned long long *res,
+                                                           std::uint64_t V, std::uint32_t K, std::uint32_t KP,
+                                                           unsigned long long *partial)
         {
-            __shared__ double s[kBlock];
+            __shared__ unsigned long long s[kBlock];
             const std::uint32_t rows_per = kBlock / KP;
             const std::uint32_t k = threadIdx.x % KP;
             const std::uint32_t r0 = threadIdx.x / KP;
-            double acc = 0.0;
+            unsigned long long acc = 0;
             if (k < K)
             {
                 for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * rows_per + r0; v < V;
                      v += static_cast<std::uint64_t>(gridDim.x) * rows_per)
                 {
                     const std::uint64_t idx = v * K + k;
-                    const float x = rank[idx] + res[idx];
+                    const unsigned long long x = rank[idx] + res[idx];
                     rank[idx] = x;
                     acc += x;
                 }
@@ -833,32 +841,33 @@ namespace hyt
             __syncthreads();
             if (threadIdx.x < KP && k < K)
             {
-                double t = 0.0;
+                unsigned long long t = 0;
                 for (std::uint32_t r = 0; r < rows_per; ++r)
                     t += s[r * KP + threadIdx.x];
                 partial[static_cast<std::uint64_t>(blockIdx.x) * 64 + threadIdx.x] = t;
             }
         }
 
-        __global__ void pprk_total(double *partial, unsigned nblocks, std::uint32_t K, double *total)
+        __global__ void pprk_total(const unsigned long long *partial, unsigned nblocks, std::uint32_t K,
+                                   unsigned long long *total)
         {
             const std::uint32_t k = threadIdx.x;
             if (k >= K)
                 return;
-            double t = 0.0;
+            unsigned long long t = 0;
             for (unsigned b = 0; b < nblocks; ++b)
                 t += partial[static_cast<std::uint64_t>(b) * 64 + k];
             total[k] = t;
         }
 
-        __global__ void pprk_normalize(const float *x, std::uint64_t n, std::uint32_t K, const double *total,
-                                       float *out)
+        __global__ void pprk_normalize(const unsigned long long *x, std::uint64_t n, std::uint32_t K,
+                                       const unsigned long long *total, float *out)
         {
             std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (; i < n; i += stride)
             {
-                const double t = total[i % K];
+                const double t = static_cast<double>(total[i % K]);
                 out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
             }
         }
@@ -888,10 +897,10 @@ namespace hyt
 
         // Seeds a batch: the seeds buffer (at least 1 KiB) receives rows u32[64] |
         // masks u64[64] | lane sources u32[64]; mark[row] takes each row's lane
-        // mask and val[src_k][k] the seed value.
+        // mask and val[src_k][k] the seed value (T: u32 distance, u64 mass word).
+        template <class T>
         inline void upload_seeds(hipStream_t s, DeviceBuffer &seeds, unsigned long long *mark, const SeedPlan &p,
-                                 const std::uint32_t *src, std::uint32_t K, std::uint32_t *val,
-                                 std::uint32_t seed_value)
+                                 const std::uint32_t *src, std::uint32_t K, T *val, T seed_value)
         {
             std::vector<unsigned char> blob(64 * 4 + 64 * 8 + 64 * 4, 0);
             std::memcpy(blob.data(), p.rows.data(), p.rows.size() * 4);
@@ -899,7 +908,7 @@ namespace hyt
             std::memcpy(blob.data() + 768, src, K * 4);
             unsigned char *d = seeds.as<unsigned char>();
             hip_check(hipMemcpyAsync(d, blob.data(), blob.size(), hipMemcpyHostToDevice, s), "upload seeds");
-            seed_rows<<<1, 64, 0, s>>>(static_cast<std::uint32_t>(p.rows.size()),
+            seed_rows<T><<<1, 64, 0, s>>>(static_cast<std::uint32_t>(p.rows.size()),
                                        reinterpret_cast<const std::uint32_t *>(d),
                                        reinterpret_cast<const unsigned long long *>(d + 256),
                                        reinterpret_cast<const std::uint32_t *>(d + 768), K, mark, val, seed_value);
@@ -908,9 +917,9 @@ namespace hyt
         }
 
         // PPR-K result: x = rank + res per (vertex, query), normalised per query
-        // column into dst (device); partial holds (1024 + 1) x 64 doubles.
-        inline void pprk_finish(hipStream_t s, float *rank, const float *res, std::uint64_t V, std::uint32_t K,
-                                double *partial, float *dst)
+        // column into dst (device); partial holds (1024 + 1) x 64 words.
+        inline void pprk_finish(hipStream_t s, unsigned long long *rank, const unsigned long long *res,
+                                std::uint64_t V, std::uint32_t K, unsigned long long *partial, float *dst)
         {
             const std::uint32_t KP = static_cast<std::uint32_t>(pow2_at_least(K));
             const std::uint64_t rows_per = kBlock / KP;
@@ -918,7 +927,7 @@ namespace hyt
             const unsigned nb = static_cast<unsigned>(nb64 < 1024 ? (nb64 ? nb64 : 1) : 1024);
             pprk_fold<<<nb, kBlock, 0, s>>>(rank, res, V, K, KP, partial);
             hip_check(hipGetLastError(), "launch pprk_fold");
-            double *total = partial + static_cast<std::uint64_t>(1024) * 64;
+            unsigned long long *total = partial + static_cast<std::uint64_t>(1024) * 64;
             pprk_total<<<1, 64, 0, s>>>(partial, nb, K, total);
             hip_check(hipGetLastError(), "launch pprk_total");
             const std::uint64_t VK = V * K;

@@ -5,6 +5,7 @@
 
 #include "compact.hpp"
 #include "expand.hpp"
+#include "mass.hpp"
 
 namespace hyt
 {
@@ -276,18 +277,19 @@ namespace hyt
         }
     };
 
+    // Delta-push of PageRank / PPR mass (mass.hpp): a row carries its share
+    // per out-edge (float bits, in mass words, written by PrSel into the
+    // queue values), and every edge adds the share's word to the residual of
+    // its destination with an integer atomic (exact, order-independent).
     struct PrOp
     {
         const std::uint32_t *__restrict__ col;
-        float *res;
-        const std::uint32_t *qd; // snapshot residuals (bits)
-        float damping;
+        unsigned long long *res;
+        const std::uint32_t *qd; // share of each queue row (float bits)
 
         __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t, std::uint64_t deg) const
         {
-            if (deg == 0)
-                return 0U;
-            return __float_as_uint(damping * __uint_as_float(qd[i]) / static_cast<float>(deg));
+            return deg ? qd[i] : 0U;
         }
         __device__ void expanded(std::uint32_t, std::uint32_t, bool) const {}
         __device__ void expanded_chunked(std::uint32_t, bool) const {}
@@ -301,22 +303,26 @@ namespace hyt
         __device__ std::uint32_t peek(std::uint32_t) const { return 0U; }
         __device__ void apply(std::uint32_t v, std::uint32_t c, std::uint32_t) const
         {
-            unsafeAtomicAdd(&res[v], __uint_as_float(c));
+            const unsigned long long w = share_word(__uint_as_float(c));
+            if (w != 0ULL)
+                atomicAdd(&res[v], w);
         }
     };
 
-    // Selects r_u > eps, moves r_u into rank_u and zeroes it, and records
-    // the per-edge share d * r_u / outdeg(u) by vertex for pull rounds.
+    // Selects the rows whose residual word is above eps, moves the residual
+    // into rank and zeroes it, and writes the row's share per out-edge
+    // d * r_u / outdeg(u) (float, in words) both as the queue value (push
+    // rounds) and by vertex into share[] (pull rounds; 0 outside the frontier).
     template <class OffT>
     struct PrSel
     {
         using Aux = NoAux;
-        float *res;
-        float *rank;
+        unsigned long long *res;
+        unsigned long long *rank;
         float *share;
         const OffT *off;
         float damping;
-        float eps;
+        unsigned long long eps;
 
         // no 16-row path
         __device__ bool whole16() const { return false; }
@@ -328,17 +334,19 @@ namespace hyt
         __device__ std::uint32_t select4(std::uint64_t row0, std::uint32_t n, std::uint32_t (&vals)[4],
                                          Aux &) const
         {
-            float r[4];
+            // row0 is a multiple of 4, so a full group is two aligned 16-byte words
+            unsigned long long r[4];
             if (n == 4)
             {
-                const float4 v4 = *reinterpret_cast<const float4 *>(res + row0);
-                r[0] = v4.x, r[1] = v4.y, r[2] = v4.z, r[3] = v4.w;
+                const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(res + row0);
+                const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(res + row0 + 2);
+                r[0] = a.x, r[1] = a.y, r[2] = b.x, r[3] = b.y;
             }
             else
             {
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
-                    r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0.0F;
+                    r[j] = static_cast<std::uint32_t>(j) < n ? res[row0 + j] : 0ULL;
             }
             std::uint32_t m = 0;
 #pragma unroll
@@ -351,7 +359,7 @@ namespace hyt
                 // offsets of the 4 rows (+1); a full group reads them as one
                 // vector when the row offsets are 32-bit
                 OffT o[5];
-                if (n == 4 && sizeof(OffT) == 4 && (row0 % 4) == 0)
+                if (n == 4 && sizeof(OffT) == 4)
                 {
                     const uint4 o4 = *reinterpret_cast<const uint4 *>(off + row0);
                     o[0] = o4.x, o[1] = o4.y, o[2] = o4.z, o[3] = o4.w;
@@ -363,11 +371,12 @@ namespace hyt
                     for (int j = 0; j < 5; ++j)
                         o[j] = static_cast<std::uint32_t>(j) <= n ? off[row0 + j] : o[j - 1];
                 }
-                float k[4] = {0.0F, 0.0F, 0.0F, 0.0F};
+                unsigned long long k[4] = {0ULL, 0ULL, 0ULL, 0ULL};
                 if (n == 4)
                 {
-                    const float4 k4 = *reinterpret_cast<const float4 *>(rank + row0);
-                    k[0] = k4.x, k[1] = k4.y, k[2] = k4.z, k[3] = k4.w;
+                    const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(rank + row0);
+                    const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(rank + row0 + 2);
+                    k[0] = a.x, k[1] = a.y, k[2] = b.x, k[3] = b.y;
                 }
                 else
                 {
@@ -382,16 +391,17 @@ namespace hyt
                     if ((m >> j) & 1U)
                     {
                         k[j] += r[j];
-                        vals[j] = __float_as_uint(r[j]);
-                        const OffT deg = o[j + 1] - o[j];
-                        sh[j] = deg ? damping * r[j] / static_cast<float>(deg) : 0.0F;
-                        r[j] = 0.0F;
+                        sh[j] = share_of(r[j], row_scale_of(damping, static_cast<std::uint64_t>(o[j + 1] - o[j])));
+                        vals[j] = __float_as_uint(sh[j]);
+                        r[j] = 0ULL;
                     }
                 }
                 if (n == 4)
                 {
-                    *reinterpret_cast<float4 *>(res + row0) = make_float4(r[0], r[1], r[2], r[3]);
-                    *reinterpret_cast<float4 *>(rank + row0) = make_float4(k[0], k[1], k[2], k[3]);
+                    *reinterpret_cast<ulonglong2 *>(res + row0) = make_ulonglong2(r[0], r[1]);
+                    *reinterpret_cast<ulonglong2 *>(res + row0 + 2) = make_ulonglong2(r[2], r[3]);
+                    *reinterpret_cast<ulonglong2 *>(rank + row0) = make_ulonglong2(k[0], k[1]);
+                    *reinterpret_cast<ulonglong2 *>(rank + row0 + 2) = make_ulonglong2(k[2], k[3]);
                 }
                 else
                 {
@@ -399,7 +409,7 @@ namespace hyt
                     for (int j = 0; j < 4; ++j)
                         if ((m >> j) & 1U)
                         {
-                            res[row0 + j] = 0.0F;
+                            res[row0 + j] = 0ULL;
                             rank[row0 + j] = k[j];
                         }
                 }

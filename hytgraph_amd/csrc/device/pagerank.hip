@@ -1,4 +1,4 @@
-// Delta-push PageRank / personalized PageRank (FP32) on gfx950.
+// Delta-push PageRank / personalized PageRank on gfx950.
 //
 // The original (src/algorithms/pagerank.cpp:89-163, cuda/pagerank.cu:116-153)
 // runs Jacobi power iteration x' = (1-d)/n + d*M/n + d*P^T x with the dangling
@@ -11,12 +11,14 @@
 //          d * r_u / outdeg(u) to the residual of each out-neighbour;
 //   stop when no residual exceeds eps; result (y + r) / |y + r|_1.
 // The frontier of a round is exactly {u : r_u > eps}: the compaction pass
-// scans the residuals, moves the selected ones into rank and records them
-// (the snapshot), so which vertices push in a round does not depend on
-// thread timing. Push rounds use non-returning float atomics; rounds whose
-// frontier out-edges are a large share of E pull instead: each destination
-// sums d * r_u / outdeg(u) over in-edges from frontier sources in LDS and adds
-// the total to its residual with one store.
+// scans the residuals, moves the selected ones into rank and records each
+// row's share, so which vertices push in a round does not depend on thread
+// timing. Residuals and ranks are 64-bit fixed-point words (mass.hpp), so the
+// sums themselves do not depend on it either. Push rounds add shares with
+// non-returning integer atomics; rounds whose frontier out-edges are a large
+// share of E pull instead: each destination sums the shares of its in-edges
+// from frontier sources in registers and LDS and adds the total to its
+// residual with one store.
 #include "ops.hpp"
 #include "pull.hpp"
 
@@ -31,58 +33,54 @@ namespace hyt
         {
             const std::uint32_t *__restrict__ in_col;
             const float *share;
-            float *res;
+            unsigned long long *res;
 
-            __device__ std::uint32_t identity() const { return 0U; }
-            __device__ bool edge(std::uint64_t pos, std::uint32_t &c) const
+            __device__ unsigned long long identity() const { return 0ULL; }
+            __device__ bool edge(std::uint64_t pos, unsigned long long &c) const
             {
-                c = __float_as_uint(share[in_col[pos]]);
-                return c != 0U;
+                c = share_word(share[in_col[pos]]);
+                return c != 0ULL;
             }
-            __device__ std::uint32_t fold(std::uint32_t a, std::uint32_t b) const
+            __device__ unsigned long long fold(unsigned long long a, unsigned long long b) const { return a + b; }
+            __device__ void finish(std::uint32_t v, unsigned long long acc) const
             {
-                return __float_as_uint(__uint_as_float(a) + __uint_as_float(b));
+                if (acc != 0ULL)
+                    res[v] += acc;
             }
-            __device__ void finish(std::uint32_t v, std::uint32_t acc) const
-            {
-                if (acc != 0U)
-                    res[v] += __uint_as_float(acc);
-            }
-            __device__ void finish_atomic(std::uint32_t v, std::uint32_t acc) const
-            {
-                unsafeAtomicAdd(&res[v], __uint_as_float(acc));
-            }
+            __device__ void finish_atomic(std::uint32_t v, unsigned long long acc) const { atomicAdd(&res[v], acc); }
         };
 
-        __global__ void pr_init_all(float *res, float *rank, std::uint64_t V, float r0)
+        __global__ void pr_init_all(unsigned long long *res, unsigned long long *rank, std::uint64_t V,
+                                    unsigned long long r0)
         {
             std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (; i < V; i += stride)
             {
                 res[i] = r0;
-                rank[i] = 0.0F;
+                rank[i] = 0ULL;
             }
         }
 
-        __global__ void pr_seed(float *res, std::uint32_t s, float r0)
+        __global__ void pr_seed(unsigned long long *res, std::uint32_t s, unsigned long long r0)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
                 res[s] = r0;
         }
 
-        // x = rank + res (written into rank); partial sums of x and res.
-        __global__ __launch_bounds__(kBlock) void pr_fold(float *rank, const float *res, std::uint64_t V,
-                                                          double *partial)
+        // x = rank + res (written into rank); exact per-block sums of x and
+        // res (no word sum passes 2^62, mass.hpp).
+        __global__ __launch_bounds__(kBlock) void pr_fold(unsigned long long *rank, const unsigned long long *res,
+                                                          std::uint64_t V, unsigned long long *partial)
         {
-            __shared__ double sx[kBlock], sr[kBlock];
-            double ax = 0.0, ar = 0.0;
+            __shared__ unsigned long long sx[kBlock], sr[kBlock];
+            unsigned long long ax = 0, ar = 0;
             std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (; i < V; i += stride)
             {
-                const float r = res[i];
-                const float x = rank[i] + r;
+                const unsigned long long r = res[i];
+                const unsigned long long x = rank[i] + r;
                 rank[i] = x;
                 ax += x;
                 ar += r;
@@ -107,10 +105,10 @@ namespace hyt
         }
 
         // Single workgroup: totals of the partials into partial[2*nblocks .. +1].
-        __global__ __launch_bounds__(kBlock) void pr_total(double *partial, unsigned nblocks)
+        __global__ __launch_bounds__(kBlock) void pr_total(unsigned long long *partial, unsigned nblocks)
         {
-            __shared__ double sx[kBlock], sr[kBlock];
-            double ax = 0.0, ar = 0.0;
+            __shared__ unsigned long long sx[kBlock], sr[kBlock];
+            unsigned long long ax = 0, ar = 0;
             for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x)
             {
                 ax += partial[2 * b];
@@ -135,17 +133,17 @@ namespace hyt
             }
         }
 
-        __global__ void pr_normalize(const float *x, std::uint64_t V, const double *total, float *out)
+        __global__ void pr_normalize(const unsigned long long *x, std::uint64_t V, const unsigned long long *total,
+                                     float *out)
         {
-            const double t = total[0];
-            const double inv = t > 0.0 ? 1.0 / t : 0.0;
+            const double t = static_cast<double>(total[0]);
             std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (; i < V; i += stride)
-                out[i] = static_cast<float>(static_cast<double>(x[i]) * inv);
+                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
         }
 
-        constexpr unsigned kFoldBlocks = 1024; // partial holds 2*kFoldBlocks+2 doubles
+        constexpr unsigned kFoldBlocks = 1024; // partial holds 2*kFoldBlocks+2 words
 
         template <class OffT>
         Counters compact_and_read(DeviceGraph &g, const PrSel<OffT> &sel)
@@ -163,21 +161,20 @@ namespace hyt
         }
 
         template <class OffT>
-        void run_push(DeviceGraph &g, const hyt_run_options &o, float damping, float eps, hyt_run_stats &st,
-                      std::uint32_t light_max, std::uint32_t chunk)
+        void run_push(DeviceGraph &g, const hyt_run_options &o, float damping, const MassScale &m,
+                      hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk)
         {
             Workspace &ws = g.ws;
             hipStream_t s = g.stream;
             Counters *d_ctr = ws.ctr.as<Counters>();
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : 100000;
 
-            PrSel<OffT> sel{ws.dist.as<float>(), ws.rank.as<float>(), ws.share.as<float>(), g.off.as<OffT>(),
-                            damping, eps};
+            PrSel<OffT> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(), ws.share.as<float>(),
+                            g.off.as<OffT>(), damping, m.eps};
             PrOp op;
             op.col = g.col.as<std::uint32_t>();
-            op.res = ws.dist.as<float>();
+            op.res = ws.res.as<unsigned long long>();
             op.qd = ws.qd.as<std::uint32_t>();
-            op.damping = damping;
 
             ExpandArgs a;
             a.q = ws.q.as<std::uint32_t>();
@@ -193,7 +190,7 @@ namespace hyt
                 ensure_transpose(g, light_max, chunk);
             const bool can_pull = g.in_col.ptr != nullptr;
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
-            PrPullOp pop{g.in_col.as<std::uint32_t>(), ws.share.as<float>(), ws.dist.as<float>()};
+            PrPullOp pop{g.in_col.as<std::uint32_t>(), ws.share.as<float>(), ws.res.as<unsigned long long>()};
 
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
             Counters c0 = compact_and_read<OffT>(g, sel);
@@ -268,23 +265,23 @@ namespace hyt
     {
         Workspace &ws = g.ws;
         hipStream_t s = g.stream;
-        double *partial = ws.partial.as<double>();
+        unsigned long long *partial = ws.partial.as<unsigned long long>();
         std::uint64_t nb64 = (g.V + kBlock - 1) / kBlock;
         const unsigned nb = static_cast<unsigned>(nb64 < kFoldBlocks ? (nb64 ? nb64 : 1) : kFoldBlocks);
-        pr_fold<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), ws.dist.as<float>(), g.V, partial);
+        pr_fold<<<nb, kBlock, 0, s>>>(ws.rank.as<unsigned long long>(), ws.res.as<unsigned long long>(), g.V, partial);
         hip_check(hipGetLastError(), "launch pr_fold");
         pr_total<<<1, kBlock, 0, s>>>(partial, nb);
         hip_check(hipGetLastError(), "launch pr_total");
-        double totals[2] = {0, 0};
+        unsigned long long totals[2] = {0, 0};
         hip_check(hipMemcpyAsync(totals, partial + 2 * nb, sizeof(totals), hipMemcpyDeviceToHost, s),
                   "read totals");
         float *dst = out_on_device ? out : ws.dist.as<float>();
-        pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<float>(), g.V, partial + 2 * nb, dst);
+        pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<unsigned long long>(), g.V, partial + 2 * nb, dst);
         hip_check(hipGetLastError(), "launch pr_normalize");
         if (!out_on_device)
             hip_check(hipMemcpyAsync(out, dst, g.V * 4, hipMemcpyDeviceToHost, s), "copy ranks");
         hip_check(hipStreamSynchronize(s), "pagerank output");
-        st.residual = totals[0] > 0 ? totals[1] / totals[0] : 0.0;
+        st.residual = totals[0] > 0 ? static_cast<double>(totals[1]) / static_cast<double>(totals[0]) : 0.0;
     }
 
 
@@ -298,26 +295,29 @@ namespace hyt
         require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
         require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
         ensure_workspace(g, light_max, chunk);
+        ensure_mass_state(g);
         hipStream_t s = g.stream;
-        const float r0 = 1.0F - damping;
-        if (seed == 0xFFFFFFFFU)
+        const bool global = seed == 0xFFFFFFFFU;
+        const MassScale m = mass_scale(global ? g.V : 1, damping, eps);
+        unsigned long long *res = g.ws.res.as<unsigned long long>();
+        unsigned long long *rank = g.ws.rank.as<unsigned long long>();
+        if (global)
         {
             std::uint64_t b = (g.V + kBlock - 1) / kBlock;
-            pr_init_all<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, s>>>(
-                g.ws.dist.as<float>(), g.ws.rank.as<float>(), g.V, r0);
+            pr_init_all<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, s>>>(res, rank, g.V, m.r0);
             hip_check(hipGetLastError(), "launch pr_init");
         }
         else
         {
-            launch_fill<float>(g.ws.dist.as<float>(), g.V, 0.0F, s);
-            launch_fill<float>(g.ws.rank.as<float>(), g.V, 0.0F, s);
-            pr_seed<<<1, 64, 0, s>>>(g.ws.dist.as<float>(), seed, r0);
+            launch_fill<unsigned long long>(res, g.V, 0ULL, s);
+            launch_fill<unsigned long long>(rank, g.V, 0ULL, s);
+            pr_seed<<<1, 64, 0, s>>>(res, seed, m.r0);
             hip_check(hipGetLastError(), "launch pr_seed");
         }
         if (g.off64)
-            run_push<std::uint64_t>(g, o, damping, eps, st, light_max, chunk);
+            run_push<std::uint64_t>(g, o, damping, m, st, light_max, chunk);
         else
-            run_push<std::uint32_t>(g, o, damping, eps, st, light_max, chunk);
+            run_push<std::uint32_t>(g, o, damping, m, st, light_max, chunk);
         finalize_ranks(g, out, o.out_on_device != 0, st);
     }
 

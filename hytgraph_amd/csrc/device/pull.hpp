@@ -14,12 +14,13 @@
 //     in registers and finishes with one global atomic.
 // Contributions come only from sources set in the frontier bitmap.
 //
-// Op supplies:
-//   uint32_t identity()
-//   bool edge(uint64_t pos, uint32_t &c)        contribution of in-edge pos
-//   uint32_t fold(uint32_t a, uint32_t b)       register fold
-//   void finish(uint32_t v, uint32_t acc)       single writer of row v
-//   void finish_atomic(uint32_t v, uint32_t acc) one of several chunks of v
+// Op supplies (Val = the type identity() returns: u32 SSSP distances, u64
+// PageRank mass words):
+//   Val identity()
+//   bool edge(uint64_t pos, Val &c)             contribution of in-edge pos
+//   Val fold(Val a, Val b)                      register fold
+//   void finish(uint32_t v, Val acc)            single writer of row v
+//   void finish_atomic(uint32_t v, Val acc)     one of several chunks of v
 #pragma once
 
 #include "kern_common.hpp"
@@ -38,9 +39,10 @@ namespace hyt
     __global__ __launch_bounds__(kBlock) void pull_light(const OffT *__restrict__ in_off, Op op, std::uint64_t V,
                                                          std::uint32_t light_max)
     {
+        using Val = decltype(op.identity());
         __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
         __shared__ OffT s_begin[kWavesPerBlock][kWave];
-        __shared__ std::uint32_t s_acc[kWavesPerBlock][kWave];
+        __shared__ Val s_acc[kWavesPerBlock][kWave];
 
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
@@ -64,7 +66,7 @@ namespace hyt
         // (no early return: every thread reaches the second barrier)
         for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
         {
-            std::uint32_t cc[kUnroll];
+            Val cc[kUnroll];
             int jj[kUnroll];
 #pragma unroll
             for (int k = 0; k < kUnroll; ++k)
@@ -76,7 +78,7 @@ namespace hyt
                 {
                     const int j = owner_of(ex, e);
                     jj[k] = j;
-                    std::uint32_t c;
+                    Val c;
                     if (op.edge(static_cast<std::uint64_t>(s_begin[wid][j]) + (e - ex[j]), c))
                         cc[k] = c;
                 }
@@ -84,12 +86,12 @@ namespace hyt
 #pragma unroll
             for (int k = 0; k < kUnroll; ++k)
             {
-                std::uint32_t val = cc[k];
+                Val val = cc[k];
                 const int j = jj[k];
 #pragma unroll
                 for (int d = 1; d < kWave; d <<= 1)
                 {
-                    const std::uint32_t y = __shfl_up(val, d, kWave);
+                    const Val y = __shfl_up(val, d, kWave);
                     const int yj = __shfl_up(j, d, kWave);
                     if (lane >= d && yj == j)
                         val = op.fold(val, y);
@@ -108,22 +110,23 @@ namespace hyt
     template <class Op>
     __global__ __launch_bounds__(kBlock) void pull_heavy(Op op, const HeavyChunk *chunks, std::uint64_t n)
     {
+        using Val = decltype(op.identity());
         const int lane = lane_id();
         const std::uint64_t waves = static_cast<std::uint64_t>(gridDim.x) * kWavesPerBlock;
         for (std::uint64_t c = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) / kWave; c < n;
              c += waves)
         {
             const HeavyChunk h = chunks[c];
-            std::uint32_t acc = op.identity();
+            Val acc = op.identity();
             for (std::uint32_t base = 0; base < h.count; base += kWave * kUnroll)
             {
-                std::uint32_t cc[kUnroll];
+                Val cc[kUnroll];
 #pragma unroll
                 for (int k = 0; k < kUnroll; ++k)
                 {
                     const std::uint32_t e = base + static_cast<std::uint32_t>(k * kWave + lane);
                     cc[k] = op.identity();
-                    std::uint32_t c2;
+                    Val c2;
                     if (e < h.count && op.edge(h.begin + e, c2))
                         cc[k] = c2;
                 }

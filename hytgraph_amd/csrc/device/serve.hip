@@ -34,6 +34,7 @@
 #include <deque>
 #include <future>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace hyt
@@ -43,8 +44,8 @@ namespace hyt
         using namespace lanes;
 
         // a[v*K + k] = value for every v.
-        __global__ void lane_fill(std::uint32_t *a, std::uint64_t V, std::uint32_t K, std::uint32_t k,
-                                  std::uint32_t value)
+        template <class T>
+        __global__ void lane_fill(T *a, std::uint64_t V, std::uint32_t K, std::uint32_t k, T value)
         {
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
@@ -54,8 +55,9 @@ namespace hyt
 
         // Seeds lane k at source s and marks its row (sessions over the hybrid
         // engine: the next round folds the mark into its sorted queue).
-        __global__ void lane_seed_mark(std::uint32_t *val, std::uint32_t K, std::uint32_t k, std::uint32_t s,
-                                       std::uint32_t seed, unsigned long long *mark)
+        template <class T>
+        __global__ void lane_seed_mark(T *val, std::uint32_t K, std::uint32_t k, std::uint32_t s, T seed,
+                                       unsigned long long *mark)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
             {
@@ -65,8 +67,9 @@ namespace hyt
         }
 
         // Seeds lane k at source s and appends (s, bit k) to the queue at F.
-        __global__ void lane_seed(std::uint32_t *val, std::uint32_t K, std::uint32_t k, std::uint32_t s,
-                                  std::uint32_t seed, std::uint32_t *q, unsigned long long *qmask, std::uint32_t F)
+        template <class T>
+        __global__ void lane_seed(T *val, std::uint32_t K, std::uint32_t k, std::uint32_t s, T seed,
+                                  std::uint32_t *q, unsigned long long *qmask, std::uint32_t F)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
             {
@@ -76,30 +79,32 @@ namespace hyt
             }
         }
 
-        // out[v] = val[v][k] (SSSP) or rank[v][k] + res[v][k] (PPR), with
-        // per-block partial sums of x in double (PPR normalisation).
-        __global__ __launch_bounds__(kBlock) void lane_gather(const std::uint32_t *val, const float *rank,
-                                                              std::uint64_t V, std::uint32_t K, std::uint32_t k,
-                                                              std::uint32_t *out, double *partial)
+        // out[v] = val[v][k] (SSSP distances).
+        __global__ void lane_gather(const std::uint32_t *val, std::uint64_t V, std::uint32_t K, std::uint32_t k,
+                                    std::uint32_t *out)
         {
-            __shared__ double s[kBlock];
-            double acc = 0.0;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+                out[v] = val[v * K + k];
+        }
+
+        // PPR lane k: exact per-block sums of the mass words x = rank + res
+        // (no column passes 2^62).
+        __global__ __launch_bounds__(kBlock) void lane_mass_sum(const unsigned long long *res,
+                                                                const unsigned long long *rank, std::uint64_t V,
+                                                                std::uint32_t K, std::uint32_t k,
+                                                                unsigned long long *partial)
+        {
+            __shared__ unsigned long long s[kBlock];
+            unsigned long long acc = 0;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
                  v += stride)
             {
                 const std::uint64_t idx = v * K + k;
-                if (rank != nullptr)
-                {
-                    const float x = rank[idx] + __uint_as_float(val[idx]);
-                    out[v] = __float_as_uint(x);
-                    acc += static_cast<double>(x);
-                }
-                else
-                    out[v] = val[idx];
+                acc += rank[idx] + res[idx];
             }
-            if (partial == nullptr)
-                return;
             s[threadIdx.x] = acc;
             __syncthreads();
             for (int d = kBlock / 2; d > 0; d >>= 1)
@@ -112,25 +117,31 @@ namespace hyt
                 partial[blockIdx.x] = s[0];
         }
 
-        __global__ void lane_total(const double *partial, unsigned n, double *total)
+        __global__ void lane_total(const unsigned long long *partial, unsigned n, unsigned long long *total)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
             {
-                double t = 0.0;
+                unsigned long long t = 0;
                 for (unsigned b = 0; b < n; ++b)
                     t += partial[b];
                 total[0] = t;
             }
         }
 
-        __global__ void lane_normalize(std::uint32_t *x, std::uint64_t V, const double *total)
+        // out[v] = (rank + res)[v][k] / total as float bits.
+        __global__ void lane_mass_out(const unsigned long long *res, const unsigned long long *rank,
+                                      std::uint64_t V, std::uint32_t K, std::uint32_t k,
+                                      const unsigned long long *total, std::uint32_t *out)
         {
-            const double t = total[0];
+            const double t = static_cast<double>(total[0]);
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
                  v += stride)
-                x[v] = __float_as_uint(
-                    static_cast<float>(t > 0.0 ? static_cast<double>(__uint_as_float(x[v])) / t : 0.0));
+            {
+                const std::uint64_t idx = v * K + k;
+                const double x = static_cast<double>(rank[idx] + res[idx]);
+                out[v] = __float_as_uint(static_cast<float>(t > 0.0 ? x / t : 0.0));
+            }
         }
 
         // Entries != skip and the u64 sum of their bit patterns (result checksum).
@@ -178,15 +189,19 @@ namespace hyt
         HybridEngine *hy = nullptr;
         hyt_hybrid_stats hst{};
         std::future<HybridLaneRound> hy_round; // the hybrid round in flight (worker thread)
-        std::uint32_t *val_p = nullptr; // [V][K] lane state (own or the engine's)
-        float *rank_p = nullptr;
+        // [V][K] lane state (own or the engine's): u32 distances, or u64
+        // PPR residual words with rank_p the u64 pushed-mass words (mass.hpp)
+        void *val_p = nullptr;
+        unsigned long long *rank_p = nullptr;
         unsigned long long *mark_p = nullptr;
         bool own_stream = true;
         int kind = HYT_QUERY_SSSP;
         std::uint32_t K = 0, KP = 0;
         float damping = 0.85F, eps = 1e-6F;
         int wm = kWU32;
-        std::uint32_t inf_bits = 0xFFFFFFFFU, seed_bits = 0;
+        std::uint32_t inf_bits = 0xFFFFFFFFU;  // SSSP: distance of an unreached vertex
+        unsigned long long seed_word = 0;      // PPR: initial residual word of the source
+        unsigned long long eps_word = 0;       // PPR: threshold word
         hipStream_t stream = nullptr;
         hipEvent_t done = nullptr;
         DeviceBuffer val, rank, snap, mark, qmask, q, heavy, ctr, lanes, colbuf, partial, acc;
@@ -233,16 +248,30 @@ namespace hyt
             return Vp * 8 + (Vp + kAppend) * 8 + (Vp + kAppend) * 4 + heavy_chunks(V, E) * sizeof(HeavyChunkK) +
                    Vp * 4 + (kGatherBlocks + 1) * sizeof(double) + sizeof(Counters) + 8 + 16;
         }
-        // val [V][K] (+ rank [V][K] and snapshot [V+kAppend][K] for PPR)
+        // SSSP: u32 val [V][K]; PPR: u64 residual and rank [V][K] and
+        // snapshot [V+kAppend][K] words
         static std::uint64_t lane_bytes(std::uint64_t V, int kind)
         {
             const std::uint64_t Vp = V + kPad;
-            return kind == HYT_QUERY_PPR ? Vp * 4 * 2 + (Vp + kAppend) * 4 : Vp * 4;
+            return kind == HYT_QUERY_PPR ? Vp * 8 * 2 + (Vp + kAppend) * 8 : Vp * 4;
         }
     };
 
     namespace
     {
+        // Fill and seed values of a session's lanes (S.kind, S.wm, S.damping
+        // and S.eps set).
+        void set_values(LaneSession &S)
+        {
+            S.inf_bits = (S.wm == kWF32 || S.wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
+            if (S.kind == HYT_QUERY_PPR)
+            {
+                const MassScale m = mass_scale(1, S.damping, S.eps);
+                S.seed_word = m.r0;
+                S.eps_word = m.eps;
+            }
+        }
+
         void create_session(LaneSession &S, DeviceGraph *g, int kind, std::uint32_t K, float damping, float eps)
         {
             require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
@@ -260,27 +289,18 @@ namespace hyt
             S.damping = damping;
             S.eps = eps;
             S.wm = g->weight_mode();
-            if (kind == HYT_QUERY_SSSP)
-            {
-                S.inf_bits = (S.wm == kWF32 || S.wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
-                S.seed_bits = 0U;
-            }
-            else
-            {
-                S.inf_bits = 0U; // residuals start at 0
-                const float r0 = 1.0F - damping;
-                std::memcpy(&S.seed_bits, &r0, 4);
-            }
+            set_values(S);
             hip_check(hipSetDevice(g->device), "hipSetDevice");
             hip_check(hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipEventCreateWithFlags(&S.done, hipEventDisableTiming), "hipEventCreate");
             const std::uint64_t V = g->V, Vp = V + kPad;
             const std::uint64_t QA = LaneSession::kAppend;
-            S.val.alloc(Vp * K * 4, "hipMalloc(lane state)");
-            if (kind == HYT_QUERY_PPR)
+            const bool ppr = kind == HYT_QUERY_PPR;
+            S.val.alloc(Vp * K * (ppr ? 8 : 4), "hipMalloc(lane state)");
+            if (ppr)
             {
-                S.rank.alloc(Vp * K * 4, "hipMalloc(lane ranks)");
-                S.snap.alloc((Vp + QA) * K * 4, "hipMalloc(lane snapshot)"); // [F][K], F <= V + QA
+                S.rank.alloc(Vp * K * 8, "hipMalloc(lane ranks)");
+                S.snap.alloc((Vp + QA) * K * 8, "hipMalloc(lane snapshot)"); // [F][K], F <= V + QA
             }
             S.mark.alloc(Vp * 8, "hipMalloc(lane marks)");
             S.qmask.alloc((Vp + QA) * 8, "hipMalloc(lane queue masks)");
@@ -294,15 +314,19 @@ namespace hyt
             S.partial.alloc((kGatherBlocks + 1) * sizeof(double), "hipMalloc(lane partials)");
             S.acc.alloc(16, "hipMalloc(lane checksum)");
             S.host = std::make_unique<PinnedBuffer>(sizeof(Counters) + 8 + 16);
-            S.val_p = S.val.as<std::uint32_t>();
-            S.rank_p = kind == HYT_QUERY_PPR ? S.rank.as<float>() : nullptr;
+            S.val_p = S.val.ptr;
+            S.rank_p = ppr ? S.rank.as<unsigned long long>() : nullptr;
             S.mark_p = S.mark.as<unsigned long long>();
             hip_check(hipMemsetAsync(S.mark.ptr, 0, Vp * 8, S.stream), "clear marks");
             hip_check(hipMemsetAsync(S.ctr.ptr, 0, sizeof(Counters), S.stream), "clear counters");
             // every lane starts free with a defined state
-            launch_fill<std::uint32_t>(S.val.as<std::uint32_t>(), Vp * K, S.inf_bits, S.stream);
-            if (kind == HYT_QUERY_PPR)
-                launch_fill<float>(S.rank.as<float>(), Vp * K, 0.0F, S.stream);
+            if (ppr)
+            {
+                launch_fill<unsigned long long>(S.val.as<unsigned long long>(), Vp * K, 0ULL, S.stream);
+                launch_fill<unsigned long long>(S.rank.as<unsigned long long>(), Vp * K, 0ULL, S.stream);
+            }
+            else
+                launch_fill<std::uint32_t>(S.val.as<std::uint32_t>(), Vp * K, S.inf_bits, S.stream);
             hip_check(hipStreamSynchronize(S.stream), "lane session init");
         }
 
@@ -325,17 +349,7 @@ namespace hyt
             S.damping = damping;
             S.eps = eps;
             S.wm = g->weight_mode();
-            if (kind == HYT_QUERY_SSSP)
-            {
-                S.inf_bits = (S.wm == kWF32 || S.wm == kWUnitF32) ? 0x7F800000U : 0xFFFFFFFFU;
-                S.seed_bits = 0U;
-            }
-            else
-            {
-                S.inf_bits = 0U;
-                const float r0 = 1.0F - damping;
-                std::memcpy(&S.seed_bits, &r0, 4);
-            }
+            set_values(S);
             const HybridLaneView v = hybrid_lanes_open(E, kind, K);
             S.val_p = v.val;
             S.rank_p = v.rank;
@@ -358,27 +372,33 @@ namespace hyt
             require(!S.launched, "a round is in flight");
             require(S.appended < LaneSession::kAppend, "too many admissions since the last round");
             const std::uint64_t V = S.g->V;
-            lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(S.val_p, V, S.K, lane, S.inf_bits);
-            hip_check(hipGetLastError(), "launch lane_fill");
-            if (S.kind == HYT_QUERY_PPR)
-            {
-                lane_fill<<<grid_for(V), kBlock, 0, S.stream>>>(reinterpret_cast<std::uint32_t *>(S.rank_p), V, S.K,
-                                                                lane, 0U);
+            auto seed = [&](auto *val, auto fill, auto seed_value) {
+                using T = std::remove_pointer_t<decltype(val)>;
+                lane_fill<T><<<grid_for(V), kBlock, 0, S.stream>>>(val, V, S.K, lane, fill);
                 hip_check(hipGetLastError(), "launch lane_fill");
-            }
-            if (S.hy)
-            {
-                lane_seed_mark<<<1, 64, 0, S.stream>>>(S.val_p, S.K, lane, source, S.seed_bits, S.mark_p);
-                hip_check(hipGetLastError(), "launch lane_seed_mark");
-            }
+                if (S.kind == HYT_QUERY_PPR)
+                {
+                    lane_fill<unsigned long long><<<grid_for(V), kBlock, 0, S.stream>>>(S.rank_p, V, S.K, lane, 0ULL);
+                    hip_check(hipGetLastError(), "launch lane_fill");
+                }
+                if (S.hy)
+                {
+                    lane_seed_mark<T><<<1, 64, 0, S.stream>>>(val, S.K, lane, source, seed_value, S.mark_p);
+                    hip_check(hipGetLastError(), "launch lane_seed_mark");
+                }
+                else
+                {
+                    lane_seed<T><<<1, 64, 0, S.stream>>>(val, S.K, lane, source, seed_value, S.q.as<std::uint32_t>(),
+                                                         S.qmask.as<unsigned long long>(), S.F);
+                    hip_check(hipGetLastError(), "launch lane_seed");
+                    S.F += 1;
+                    S.pairs += 1;
+                }
+            };
+            if (S.kind == HYT_QUERY_PPR)
+                seed(static_cast<unsigned long long *>(S.val_p), 0ULL, S.seed_word);
             else
-            {
-                lane_seed<<<1, 64, 0, S.stream>>>(S.val_p, S.K, lane, source, S.seed_bits, S.q.as<std::uint32_t>(),
-                                                  S.qmask.as<unsigned long long>(), S.F);
-                hip_check(hipGetLastError(), "launch lane_seed");
-                S.F += 1;
-                S.pairs += 1;
-            }
+                seed(static_cast<std::uint32_t *>(S.val_p), S.inf_bits, 0U);
             S.appended += 1;
             S.running |= 1ULL << lane;
             S.occupied |= 1ULL << lane;
@@ -408,8 +428,8 @@ namespace hyt
             {
                 const std::uint64_t n = static_cast<std::uint64_t>(S.F) * S.K;
                 pprk_snapshot<<<snapshot_blocks(n), kBlock, 0, s>>>(
-                    rows.q, S.F, S.K, rows.qmask, reinterpret_cast<float *>(S.val.ptr), S.rank.as<float>(),
-                    S.snap.as<float>());
+                    rows.q, S.F, S.K, rows.qmask, S.val.as<unsigned long long>(), S.rank.as<unsigned long long>(),
+                    S.snap.as<unsigned long long>());
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
             if (use_items(S.form, S.pairs, S.F, S.KP))
@@ -446,11 +466,11 @@ namespace hyt
             {
                 PprKOp op;
                 op.col = g.col.as<std::uint32_t>();
-                op.res = reinterpret_cast<float *>(S.val.ptr);
-                op.snap = S.snap.as<float>();
+                op.res = S.val.as<unsigned long long>();
+                op.snap = S.snap.as<unsigned long long>();
                 op.K = S.K;
                 op.damping = S.damping;
-                op.eps = S.eps;
+                op.eps = S.eps_word;
                 launch_round_t<OffT, KP>(S, op);
                 return;
             }
@@ -598,17 +618,24 @@ namespace hyt
             const std::uint64_t V = S.g->V;
             hipStream_t s = S.stream;
             const bool ppr = S.kind == HYT_QUERY_PPR;
-            double *partial = S.partial.as<double>();
+            unsigned long long *partial = S.partial.as<unsigned long long>();
             const unsigned nb = grid_for(V);
-            lane_gather<<<nb, kBlock, 0, s>>>(S.val_p, ppr ? S.rank_p : nullptr, V, S.K, lane,
-                                              S.colbuf.as<std::uint32_t>(), ppr ? partial : nullptr);
-            hip_check(hipGetLastError(), "launch lane_gather");
             if (ppr)
             {
+                const unsigned long long *res = static_cast<const unsigned long long *>(S.val_p);
+                lane_mass_sum<<<nb, kBlock, 0, s>>>(res, S.rank_p, V, S.K, lane, partial);
+                hip_check(hipGetLastError(), "launch lane_mass_sum");
                 lane_total<<<1, 64, 0, s>>>(partial, nb, partial + kGatherBlocks);
                 hip_check(hipGetLastError(), "launch lane_total");
-                lane_normalize<<<nb, kBlock, 0, s>>>(S.colbuf.as<std::uint32_t>(), V, partial + kGatherBlocks);
-                hip_check(hipGetLastError(), "launch lane_normalize");
+                lane_mass_out<<<nb, kBlock, 0, s>>>(res, S.rank_p, V, S.K, lane, partial + kGatherBlocks,
+                                                    S.colbuf.as<std::uint32_t>());
+                hip_check(hipGetLastError(), "launch lane_mass_out");
+            }
+            else
+            {
+                lane_gather<<<nb, kBlock, 0, s>>>(static_cast<const std::uint32_t *>(S.val_p), V, S.K, lane,
+                                                  S.colbuf.as<std::uint32_t>());
+                hip_check(hipGetLastError(), "launch lane_gather");
             }
             std::uint64_t *hacc = reinterpret_cast<std::uint64_t *>(S.host->as<unsigned char>() + sizeof(Counters) + 8);
             if (reached || checksum)

@@ -26,12 +26,22 @@
 // Why touch[] cannot miss an improvement: the first atomicMin that lowers
 // dist[v] in a round came from a thread whose filter load passed, and that
 // thread stored touch[v].
+//
+// Tolerance > 0 (float weights, SYNC): the original's rule - v takes a
+// candidate c only when c + tol < best, best starting at v's distance and
+// moving as the in-edges are read in incoming-CSR order - depends on that
+// order, so these rounds run as ordered pulls: one thread per destination
+// walks its in-edges in the transpose's (source, edge position) order, the
+// order of build_incoming_csr (sssp.cpp:21-80), and applies the rule
+// sequentially; v joins the next frontier when best + tol < its distance.
 #include "ops.hpp"
 #include "pull.hpp"
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
+#include <utility>
 
 namespace hyt
 {
@@ -87,6 +97,147 @@ namespace hyt
                 if (touch != nullptr)
                     touch[src] = 1;
             }
+        }
+
+        // One ordered pull round of the tolerance rule (cuda/sssp.cu:100-154):
+        // cur/next are float bit patterns, act/next_act one byte per vertex;
+        // ctr->next_count counts the vertices of the next frontier.
+        template <class OffT, bool WEIGHTED>
+        __global__ __launch_bounds__(kBlock) void sssp_tol_round(const OffT *__restrict__ in_off,
+                                                                 const std::uint32_t *__restrict__ in_col,
+                                                                 const std::uint32_t *__restrict__ in_w,
+                                                                 const std::uint32_t *cur, const std::uint8_t *act,
+                                                                 std::uint32_t *next, std::uint8_t *next_act,
+                                                                 std::uint64_t V, float tol, Counters *ctr)
+        {
+            __shared__ std::uint32_t s_n[kWavesPerBlock];
+            std::uint32_t n = 0;
+            const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
+            for (std::uint64_t v = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V;
+                 v += stride)
+            {
+                const float dv = __uint_as_float(cur[v]);
+                float best = dv;
+                const std::uint64_t end = in_off[v + 1];
+                for (std::uint64_t pos = in_off[v]; pos < end; ++pos)
+                {
+                    const std::uint32_t u = in_col[pos];
+                    if (act[u] == 0)
+                        continue;
+                    const std::uint32_t du = cur[u];
+                    if ((du & 0x7F800000U) == 0x7F800000U) // not finite
+                        continue;
+                    const float c = __fadd_rn(__uint_as_float(du), WEIGHTED ? __uint_as_float(in_w[pos]) : 1.0F);
+                    if (__fadd_rn(c, tol) < best)
+                        best = c;
+                }
+                next[v] = __float_as_uint(best);
+                const bool improved = __fadd_rn(best, tol) < dv;
+                next_act[v] = improved ? 1 : 0;
+                n += improved ? 1U : 0U;
+            }
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+                n += __shfl_xor(n, d, kWave);
+            if (lane_id() == 0)
+                s_n[threadIdx.x / kWave] = n;
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                std::uint32_t t = 0;
+#pragma unroll
+                for (int w = 0; w < kWavesPerBlock; ++w)
+                    t += s_n[w];
+                if (t)
+                    atomicAdd(&ctr->next_count, t);
+            }
+        }
+
+        __global__ void tol_seed(std::uint32_t *dist, std::uint8_t *act, std::uint32_t src)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+            {
+                dist[src] = 0U; // +0.0f
+                act[src] = 1;
+            }
+        }
+
+        // The original's SYNC rounds with tolerance > 0 (sssp.cpp:140-232):
+        // iterations = rounds run, converged on the first round that changes
+        // nothing; max_rounds 0 means V.
+        template <class OffT>
+        void run_tolerance(DeviceGraph &g, std::uint32_t src, const hyt_run_options &o, std::uint32_t light_max,
+                           std::uint32_t chunk, hyt_run_stats &st)
+        {
+            Workspace &ws = g.ws;
+            hipStream_t s = g.stream;
+            Counters *d_ctr = ws.ctr.as<Counters>();
+            const std::uint64_t Vp = g.V + kPad;
+            DeviceBuffer act_a(Vp, "hipMalloc(tolerance frontier)"), act_b(Vp, "hipMalloc(tolerance frontier)");
+            std::uint32_t *cur = ws.dist.as<std::uint32_t>(), *next = ws.prev.as<std::uint32_t>();
+            std::uint8_t *act = act_a.as<std::uint8_t>(), *next_act = act_b.as<std::uint8_t>();
+            launch_fill<std::uint32_t>(cur, g.V, 0x7F800000U, s);
+            launch_fill<std::uint8_t>(act, g.V, 0, s);
+            tol_seed<<<1, 64, 0, s>>>(cur, act, src);
+            hip_check(hipGetLastError(), "launch tol_seed");
+            if (g.E > 0)
+                ensure_transpose(g, light_max, chunk);
+            const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : g.V;
+            const std::uint64_t b = (g.V + kBlock - 1) / kBlock;
+            const unsigned grid = static_cast<unsigned>(b < 8192 ? (b ? b : 1) : 8192);
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            std::uint64_t frontier = 1;
+            st.converged = 0;
+            for (std::uint64_t round = 0; round < max_rounds; ++round)
+            {
+                if (g.E == 0)
+                {
+                    // no in-edges: the round keeps every distance and changes nothing
+                    st.rounds = round + 1;
+                    st.converged = 1;
+                    break;
+                }
+                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
+                if (g.weighted)
+                    sssp_tol_round<OffT, true><<<grid, kBlock, 0, s>>>(g.in_off.as<OffT>(), g.in_col.as<std::uint32_t>(),
+                                                                       g.in_w.as<std::uint32_t>(), cur, act, next,
+                                                                       next_act, g.V, o.tolerance, d_ctr);
+                else
+                    sssp_tol_round<OffT, false><<<grid, kBlock, 0, s>>>(g.in_off.as<OffT>(), g.in_col.as<std::uint32_t>(),
+                                                                        nullptr, cur, act, next, next_act, g.V,
+                                                                        o.tolerance, d_ctr);
+                hip_check(hipGetLastError(), "launch sssp_tol_round");
+                if (o.time_kernels)
+                    hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
+                hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read counters");
+                hip_check(hipStreamSynchronize(s), "sssp tolerance round");
+                if (o.time_kernels)
+                {
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, g.ev0, g.ev1), "hipEventElapsedTime");
+                    st.expand_ms += ms;
+                    st.expand_launches += 1;
+                }
+                std::swap(cur, next);
+                std::swap(act, next_act);
+                st.rounds = round + 1;
+                st.frontier_total += frontier;
+                st.edges_total += g.E; // every in-edge is read
+                st.pull_rounds += 1;
+                st.pull_edges += g.E;
+                frontier = h_ctr->next_count;
+                st.updates_total += frontier;
+                if (frontier == 0)
+                {
+                    st.converged = 1;
+                    break;
+                }
+            }
+            // the distances of the last round: callers read ws.dist
+            if (cur != ws.dist.as<std::uint32_t>())
+                hip_check(hipMemcpyAsync(ws.dist.ptr, cur, g.V * 4, hipMemcpyDeviceToDevice, s), "copy distances");
         }
 
         struct Launch
@@ -328,19 +479,34 @@ namespace hyt
         require(L.light_max <= (1U << 20), "light_max_degree must be <= 2^20");
         require(L.chunk >= 64 && L.chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
         ensure_workspace(g, L.light_max, L.chunk);
-        try
+        if (o.tolerance != 0.0F)
         {
+            require(o.tolerance > 0.0F && std::isfinite(o.tolerance), "SSSP tolerance must be finite and non-negative");
+            require(g.weights_f32, "SSSP tolerance > 0 needs float weights (HYT_GRAPH_WEIGHTS_F32)");
+            require(o.mode == HYT_SSSP_SYNC && o.delta == 0 && o.direction != HYT_DIR_PUSH,
+                    "SSSP tolerance > 0 runs SYNC pull rounds (no ASYNC mode, buckets or push direction)");
+            require(g.E < (1ULL << 31), "SSSP tolerance > 0 needs fewer than 2^31 edges (transpose)");
             if (g.off64)
-                dispatch_weights<std::uint64_t>(L, src, st);
+                run_tolerance<std::uint64_t>(g, src, o, L.light_max, L.chunk, st);
             else
-                dispatch_weights<std::uint32_t>(L, src, st);
+                run_tolerance<std::uint32_t>(g, src, o, L.light_max, L.chunk, st);
         }
-        catch (...)
+        else
         {
-            // a failed round may leave touch[] marks behind: restore all-zero
-            (void)hipMemsetAsync(g.ws.touch.ptr, 0, g.V + kPad, g.stream);
-            (void)hipStreamSynchronize(g.stream);
-            throw;
+            try
+            {
+                if (g.off64)
+                    dispatch_weights<std::uint64_t>(L, src, st);
+                else
+                    dispatch_weights<std::uint32_t>(L, src, st);
+            }
+            catch (...)
+            {
+                // a failed round may leave touch[] marks behind: restore all-zero
+                (void)hipMemsetAsync(g.ws.touch.ptr, 0, g.V + kPad, g.stream);
+                (void)hipStreamSynchronize(g.stream);
+                throw;
+            }
         }
         hip_check(hipMemcpyAsync(dist_out, g.ws.dist.ptr, g.V * 4,
                                  o.out_on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, g.stream),
@@ -366,6 +532,7 @@ extern "C" int hyt_sssp(hyt_graph_t g, const uint32_t *sources, uint32_t num_sou
         if (opts)
             o = *opts;
         require(o.mode == HYT_SSSP_SYNC || o.mode == HYT_SSSP_ASYNC, "unknown SSSP mode");
+        require(o.tolerance == 0.0F || num_sources == 1, "SSSP tolerance > 0 runs one source per call");
         hyt_run_stats st{};
         hip_check(hipSetDevice(g->g.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();

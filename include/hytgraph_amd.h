@@ -118,7 +118,9 @@ int hyt_sample_sources(const uint64_t *row_offsets, uint64_t num_vertices,
 /* ------------------------------------------------- device-resident graph */
 enum hyt_graph_flags {
     HYT_GRAPH_WEIGHTS_U32 = 0,   /* weights are unsigned integers */
-    HYT_GRAPH_WEIGHTS_F32 = 1    /* weights are IEEE float bit patterns (>= +0) */
+    HYT_GRAPH_WEIGHTS_F32 = 1,   /* weights are IEEE float bit patterns (>= +0) */
+    HYT_GRAPH_OFF64 = 2          /* OR-ed in: u64 device row offsets whatever E is
+                                    (they are always used from E >= 2^32 - 1) */
 };
 
 typedef struct hyt_graph *hyt_graph_t;
@@ -171,6 +173,11 @@ typedef struct hyt_run_options {
     uint32_t delta;            /* ASYNC SSSP: delta-stepping bucket width in weight units
                                   (0 = plain rounds); SYNC mode ignores it */
     uint32_t lane_form;        /* batched (K >= 2): hyt_lane_form */
+    float tolerance;           /* single-source SYNC SSSP on float weights: the
+                                  original's rule, v takes candidate c only when
+                                  c + tolerance < best, in incoming-edge order
+                                  (sssp.cpp:199-216, cuda/sssp.cu:131-153); > 0 runs
+                                  ordered pull rounds over the transpose */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */
@@ -317,7 +324,7 @@ typedef struct hyt_hybrid_config {
     uint32_t host_threads;        /* compaction packers (0: min(hw, 16)) */
     uint32_t calibrate;           /* 1: measure host link, packing, zero-copy; set rtt/cpu */
     uint32_t cache;               /* 1: filtered partitions stay resident (LRU) */
-    uint32_t reserved;
+    uint32_t flags;               /* HYT_GRAPH_OFF64: u64 device row offsets whatever E is */
     hyt_cost_params cost;         /* used as given when calibrate == 0 */
 } hyt_hybrid_config;
 

@@ -67,6 +67,12 @@ def csr_from_edges(V, edges, weights=None):
     return off, col, w
 
 
+def f32(x):
+    """The value a float parameter of the C ABI carries: the FP64 oracles
+    are given the damping and epsilon the GPU computes with."""
+    return float(np.float32(x))
+
+
 @pytest.fixture
 def rng():
     return np.random.default_rng(1234)

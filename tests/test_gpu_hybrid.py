@@ -9,8 +9,10 @@ FP64 oracles (same bars as the in-HBM path).
 import numpy as np
 import pytest
 
+from conftest import f32
+
 pytestmark = pytest.mark.gpu
-D = 0.85
+D = f32(0.85)  # the damping the C ABI carries (float)
 
 
 def _say(msg):
@@ -81,13 +83,41 @@ def test_hybrid_pagerank(gpu, abi, orc, engine):
     h = abi.HybridEngine(hg, force_engine=engine, **SMALL)
     eps = 1e-7
     r, st = h.pagerank(D, eps)
-    same, rounds, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps)
+    same, rounds, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, f32(eps))
     exact = orc.pagerank_f64(hg.offsets, hg.columns, D)
     _say(f"engine {engine}: L1 same {np.abs(r - same).sum():.2e} exact {np.abs(r - exact).sum():.2e} "
          f"rounds {st['run']['rounds']}/{rounds}")
     assert st["run"]["converged"] == 1
-    assert np.abs(r - same).sum() <= 2e-6
+    assert np.abs(r - same).sum() <= 1e-6
     assert np.abs(r - exact).sum() <= 2 * D * eps / (1 - D) ** 2 + 2e-6
+
+
+@pytest.mark.parametrize("engine", [-1, 0, 1, 2])
+def test_hybrid_erdos_renyi(gpu, abi, orc, engine):
+    # the low-skew contrast graph of the out-of-memory configuration: same V
+    # and E as RMAT, uniform (src, dst); SSSP bit-exact, PageRank and K-lane
+    # PPR the in-HBM words, through every mover
+    hg = pinned(abi, scale=14, seed=61, kind=abi.HYT_GEN_ER)
+    h = abi.HybridEngine(hg, force_engine=engine, **SMALL)
+    src = abi.sample_sources(hg.offsets, 62, 3)
+    for s in src:
+        _say(f"ER engine {engine} sssp src {s}")
+        d, st = h.sssp(int(s))
+        assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(s)))
+    dk, _ = h.sssp_k(src)
+    for k, s in enumerate(src):
+        assert np.array_equal(dk[:, k], orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(s))), k
+    hu = pinned(abi, scale=14, seed=63, wmax=0, kind=abi.HYT_GEN_ER)
+    hy = abi.HybridEngine(hu, force_engine=engine, **SMALL)
+    g = abi.DeviceGraph.from_host(hu)
+    eps = 1e-7
+    r, st = hy.pagerank(D, eps)
+    same, _, _ = orc.delta_push_f64(hu.offsets, hu.columns, D, f32(eps))
+    assert st["run"]["converged"] == 1 and np.abs(r - same).sum() <= 1e-6
+    assert np.array_equal(r, g.pagerank(D, eps)[0])
+    rk, _ = hy.ppr_k(src, D, 1e-8)
+    for k, s in enumerate(src):
+        assert np.array_equal(rk[:, k], g.ppr(int(s), D, 1e-8)[0]), k
 
 
 def test_hybrid_reuse_and_errors(gpu, abi, orc):
@@ -195,10 +225,10 @@ def test_hybrid_ppr_k(gpu, abi, orc, engine):
         assert st["run"]["converged"] == 1
         worst = 0.0
         for k, s in enumerate(seeds):
-            same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, eps, seed=s)
+            same, _, _ = orc.delta_push_f64(hg.offsets, hg.columns, D, f32(eps), seed=s)
             worst = max(worst, np.abs(r[:, k] - same).sum())
-            assert np.abs(r[:, k] - same).sum() <= 2e-6, (form, k)
-            assert np.abs(r[:, k] - alone[:, k]).sum() <= 4e-6, (form, k)
+            assert np.abs(r[:, k] - same).sum() <= 1e-6, (form, k)
+            assert np.array_equal(r[:, k], alone[:, k]), (form, k)  # same words through the engine
             assert abs(r[:, k].sum() - 1) < 1e-5
         _say(f"ppr_k engine {engine} form {form}: worst L1 vs same-procedure oracle {worst:.2e}")
 

@@ -43,13 +43,20 @@ def test_reference_cuda_test_graph(gpu, abi, orc, mode):
 def test_golden_sssp_cases(gpu, abi, direction):
     data = json.loads((GOLDEN / "reference_cases.json").read_text())
     for case in data["sssp"]:
-        if case["tolerance"] != 0.0:
-            continue  # order-dependent tolerance rule: not implemented on the GPU path
         _say(case["name"])
         w = None if case["weights"] is None else np.array(case["weights"], np.float32)
         g = abi.DeviceGraph(np.array(case["offsets"], np.uint64), np.array(case["columns"], np.uint32), w,
                             weights_f32=True)
-        d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"], direction=direction)
+        if case["tolerance"] != 0.0:
+            # the order-dependent tolerance rule runs as ordered pull rounds
+            if direction == 1:
+                with pytest.raises(abi.InvalidArgument):
+                    g.sssp(case["source"], mode=0, direction=direction, tolerance=case["tolerance"])
+                continue
+            d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"], direction=direction,
+                           tolerance=case["tolerance"])
+        else:
+            d, st = g.sssp(case["source"], mode=0, max_rounds=case["max_iterations"], direction=direction)
         assert d.tolist() == case["distance_bits"], case["name"]
         assert st["rounds"] == case["iterations"] and bool(st["converged"]) == case["converged"], case["name"]
 
@@ -114,6 +121,38 @@ def test_fractional_float_weights_bitexact(gpu, abi, orc, rng):
             assert np.array_equal(d, ref.view(np.uint32)), (src, mode)
             if mode == 0:
                 assert st["rounds"] == it
+
+
+@pytest.mark.parametrize("tol", [0.25, 0.5, 3.0, 40.0])
+@pytest.mark.parametrize("scale", [10, 13])
+def test_tolerance_rule_bitexact(gpu, abi, orc, rng, tol, scale):
+    """Tolerance > 0: the original's order-dependent rule (candidate + tol <
+    best in incoming-edge order) gives the original's distance bits,
+    iteration count and convergence flag, also when truncated, on float
+    weights with fractions and on an unweighted graph (w = 1)."""
+    hg = rmat(abi, scale, 60 + scale)
+    w = (hg.weights.astype(np.float32) * rng.choice(np.array([0.1, 0.37, 1.75, 1.0], np.float32),
+                                                    size=hg.E)).astype(np.float32)
+    g = abi.DeviceGraph(hg.offsets, hg.columns, w, weights_f32=True)
+    gu = abi.DeviceGraph(hg.offsets, hg.columns, None, weights_f32=True)
+    use_ref = orc.ref_available()
+    for src in [0, 5, 901]:
+        for graph, weights in ((g, w), (gu, None)):
+            ref, it, conv = orc.sssp_jacobi_f32(hg.offsets, hg.columns, weights, src, tol, 0, use_ref=use_ref)
+            d, st = graph.sssp(src, mode=0, tolerance=tol)
+            assert np.array_equal(d, ref.view(np.uint32)), (src, tol)
+            assert st["rounds"] == it and bool(st["converged"]) == conv
+            tr, it2, conv2 = orc.sssp_jacobi_f32(hg.offsets, hg.columns, weights, src, tol, 2, use_ref=use_ref)
+            d2, st2 = graph.sssp(src, mode=0, tolerance=tol, max_rounds=2)
+            assert np.array_equal(d2, tr.view(np.uint32)) and st2["rounds"] == it2
+            assert bool(st2["converged"]) == conv2
+    # integer weights, ASYNC mode, push direction and batches reject it
+    gi = abi.DeviceGraph.from_host(hg)
+    for bad in (lambda: gi.sssp(0, tolerance=tol), lambda: g.sssp(0, mode=1, tolerance=tol),
+                lambda: g.sssp(0, direction=1, tolerance=tol), lambda: g.sssp([0, 5], tolerance=tol),
+                lambda: g.sssp(0, tolerance=-1.0)):
+        with pytest.raises(abi.InvalidArgument):
+            bad()
 
 
 @pytest.mark.parametrize("max_rounds", [1, 2, 3, 5])
