@@ -178,6 +178,29 @@ namespace hyt
 
     // Builds the incoming CSR and the static chunk list of its heavy rows.
     void ensure_transpose(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
+
+    // Device arrays of one single-source SSSP run (every [V] array has kPad
+    // spare slots): the graph's workspace, or caller-provided memory. touch
+    // and active are needed only by calls that may pull (null otherwise).
+    struct SsspBuffers
+    {
+        std::uint32_t *dist, *prev, *q, *qd;
+        std::uint8_t *touch;
+        std::uint16_t *active;
+        HeavyChunk *heavy;
+        std::uint64_t heavy_cap;
+        Counters *ctr;
+    };
+    // The single-source work split (options, else the size-dependent default).
+    void sssp_split(const DeviceGraph &g, const hyt_run_options &o, std::uint32_t &light_max, std::uint32_t &chunk);
+    // The SSSP rounds from src over the given arrays (distances end in b.dist).
+    void sssp_rounds(DeviceGraph &g, std::uint32_t src, const SsspBuffers &b, const hyt_run_options &o,
+                     std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st);
+    // Heavy-chunk capacity of a single-source run at a split.
+    inline std::uint64_t heavy_capacity(std::uint64_t E, std::uint32_t light_max, std::uint32_t chunk)
+    {
+        return E / chunk + E / (static_cast<std::uint64_t>(light_max) + 1) + 64;
+    }
 } // namespace hyt
 
 struct hyt_graph

@@ -46,7 +46,7 @@ namespace hyt
     {
         Workspace &ws = g.ws;
         const std::uint64_t Vp = g.V + kPad;
-        const std::uint64_t cap = g.E / chunk + g.E / (static_cast<std::uint64_t>(light_max) + 1) + 64;
+        const std::uint64_t cap = heavy_capacity(g.E, light_max, chunk);
         if (!ws.ready)
         {
             ws.dist.alloc(Vp * 4, "hipMalloc(dist)");

@@ -37,6 +37,21 @@ namespace hyt
         hip_check(hipGetLastError(), "launch fill");
     }
 
+    // Counter resets by a fill kernel rather than hipMemsetAsync, so that a
+    // sequence of rounds can be captured into a HIP graph (memset nodes have
+    // given wrong results on replays). Counters and its round part are whole
+    // 32-bit words.
+    inline void clear_counters(Counters *c, hipStream_t s)
+    {
+        static_assert(sizeof(Counters) % 4 == 0, "Counters is whole words");
+        launch_fill<std::uint32_t>(reinterpret_cast<std::uint32_t *>(c), sizeof(Counters) / 4, 0U, s);
+    }
+    inline void clear_round_counters(Counters *c, hipStream_t s)
+    {
+        static_assert(kRoundCounterBytes % 4 == 0, "the round counters are whole words");
+        launch_fill<std::uint32_t>(reinterpret_cast<std::uint32_t *>(c), kRoundCounterBytes / 4, 0U, s);
+    }
+
     inline unsigned blocks_for(std::uint64_t items)
     {
         std::uint64_t b = (items + kBlock - 1) / kBlock;

@@ -197,7 +197,7 @@ namespace hyt
                     st.converged = 1;
                     break;
                 }
-                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                clear_round_counters(d_ctr, s);
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
                 if (g.weighted)
@@ -245,19 +245,20 @@ namespace hyt
             DeviceGraph &g;
             const hyt_run_options &o;
             std::uint32_t light_max, chunk;
+            SsspBuffers b;
         };
 
         // Compaction into the queue; SYNC rounds also take the snapshot values,
         // and rounds that may pull the frontier bitmap and out-edge sum (ASYNC
         // push-only rounds read none of them). Returns the counters after it.
         template <class OffT>
-        Counters compact_and_read(DeviceGraph &g, const SsspSel &sel, bool values, bool direction)
+        Counters compact_and_read(DeviceGraph &g, const SsspBuffers &b, const SsspSel &sel, bool values,
+                                  bool direction)
         {
-            Workspace &ws = g.ws;
-            Counters *d_ctr = ws.ctr.as<Counters>();
+            Counters *d_ctr = b.ctr;
             compact_rows<std::uint32_t, OffT, SsspSel><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
-                g.V, sel, ws.q.as<std::uint32_t>(), values ? ws.qd.as<std::uint32_t>() : nullptr, d_ctr,
-                direction ? g.off.as<OffT>() : nullptr, direction ? ws.active.as<std::uint16_t>() : nullptr);
+                g.V, sel, b.q, values ? b.qd : nullptr, d_ctr, direction ? g.off.as<OffT>() : nullptr,
+                direction ? b.active : nullptr);
             hip_check(hipGetLastError(), "launch sssp compaction");
             Counters *h_ctr = g.host_ctr->as<Counters>();
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
@@ -269,13 +270,13 @@ namespace hyt
         void run_rounds(Launch &L, std::uint32_t src, hyt_run_stats &st)
         {
             DeviceGraph &g = L.g;
-            Workspace &ws = g.ws;
+            const SsspBuffers &B = L.b;
             hipStream_t s = g.stream;
-            Counters *d_ctr = ws.ctr.as<Counters>();
+            Counters *d_ctr = B.ctr;
 
-            launch_fill<std::uint32_t>(ws.dist.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
-            launch_fill<std::uint32_t>(ws.prev.as<std::uint32_t>(), g.V, Weights<WM>::kInf, s);
-            hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
+            launch_fill<std::uint32_t>(B.dist, g.V, Weights<WM>::kInf, s);
+            launch_fill<std::uint32_t>(B.prev, g.V, Weights<WM>::kInf, s);
+            clear_counters(d_ctr, s);
 
             const std::uint32_t dir = L.o.direction;
             const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
@@ -283,8 +284,9 @@ namespace hyt
             // Push-only calls find improved rows by comparing dist with prev in
             // the compaction instead of a byte store per improvement (random
             // stores that each move a cache line); pull rounds need touch[].
-            std::uint8_t *touch = may_pull ? ws.touch.as<std::uint8_t>() : nullptr;
-            seed_kernel<<<1, 64, 0, s>>>(ws.dist.as<std::uint32_t>(), touch, src, 0U);
+            require(!may_pull || B.touch != nullptr, "pull rounds need the traversal workspace");
+            std::uint8_t *touch = may_pull ? B.touch : nullptr;
+            seed_kernel<<<1, 64, 0, s>>>(B.dist, touch, src, 0U);
             hip_check(hipGetLastError(), "launch seed");
 
             // delta-stepping buckets (ASYNC only): admit improved rows below
@@ -304,7 +306,7 @@ namespace hyt
                 const std::uint64_t b = static_cast<std::uint64_t>(d) + L.o.delta;
                 return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
             };
-            SsspSel sel{touch, ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(),
+            SsspSel sel{touch, B.dist, B.prev,
                         bucketed ? bound_after(0U) : 0xFFFFFFFFU};
 
             if (may_pull && g.E > 0 && g.E < (1ULL << 31))
@@ -313,7 +315,7 @@ namespace hyt
             const bool can_pull = may_pull && g.in_col.ptr != nullptr;
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
 
-            Counters c0 = compact_and_read<OffT>(g, sel, SYNC, can_pull);
+            Counters c0 = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
             std::uint32_t F = c0.next_count;
             unsigned long long Fe = c0.frontier_edges;
             std::uint32_t pending = c0.pending;
@@ -321,26 +323,26 @@ namespace hyt
             SsspPullOp<WM, SYNC> pop;
             pop.in_col = g.in_col.as<std::uint32_t>();
             pop.in_w = g.in_w.as<std::uint32_t>();
-            pop.active = ws.active.as<std::uint32_t>();
-            pop.prev = ws.prev.as<std::uint32_t>();
-            pop.dist = ws.dist.as<std::uint32_t>();
-            pop.touch = ws.touch.as<std::uint8_t>();
+            pop.active = reinterpret_cast<const std::uint32_t *>(B.active);
+            pop.prev = B.prev;
+            pop.dist = B.dist;
+            pop.touch = B.touch;
             pop.overflow = &d_ctr->overflow;
 
             SsspOp<WM, SYNC> op;
             op.col = g.col.as<std::uint32_t>();
             op.w = g.w.as<std::uint32_t>();
-            op.dist = ws.dist.as<std::uint32_t>();
+            op.dist = B.dist;
             op.touch = touch;
-            op.qd = ws.qd.as<std::uint32_t>();
+            op.qd = B.qd;
             op.overflow = &d_ctr->overflow;
-            op.expanded_at = SYNC ? nullptr : ws.prev.as<std::uint32_t>();
+            op.expanded_at = SYNC ? nullptr : B.prev;
 
             ExpandArgs a;
-            a.q = ws.q.as<std::uint32_t>();
+            a.q = B.q;
             a.ctr = d_ctr;
-            a.heavy = ws.heavy.as<HeavyChunk>();
-            a.heavy_cap = ws.heavy_cap;
+            a.heavy = B.heavy;
+            a.heavy_cap = B.heavy_cap;
             a.light_max = L.light_max;
             a.chunk = L.chunk;
 
@@ -352,8 +354,8 @@ namespace hyt
                 {
                     // bucket exhausted: advance to the nearest held-back row
                     sel.bound = bound_after(pending_min);
-                    hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
-                    const Counters cb = compact_and_read<OffT>(g, sel, SYNC, can_pull);
+                    clear_round_counters(d_ctr, s);
+                    const Counters cb = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
                     F = cb.next_count;
                     Fe = cb.frontier_edges;
                     pending = cb.pending;
@@ -362,7 +364,7 @@ namespace hyt
                 }
                 ++round;
                 a.F = F;
-                hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
+                clear_round_counters(d_ctr, s);
                 const bool pull = can_pull && (dir == HYT_DIR_PULL ||
                                                (dir == HYT_DIR_AUTO && Fe * 100ULL >= pct * g.E));
                 if (L.o.time_kernels)
@@ -407,7 +409,7 @@ namespace hyt
                 }
                 if (L.o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_and_read<OffT>(g, sel, SYNC, can_pull);
+                const Counters c = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
                 if (L.o.time_kernels)
                 {
                     float ms = 0;
@@ -440,11 +442,11 @@ namespace hyt
                 if (F == 0 && pending == 0)
                     st.converged = 1;
             }
-            if (pending > 0)
+            if (pending > 0 && touch != nullptr)
             {
                 // stopped by max_rounds with held-back rows: touch[] must be
-                // all-zero for the next call
-                hip_check(hipMemsetAsync(ws.touch.ptr, 0, g.V + kPad, s), "clear touch");
+                // all-zero for the next call (touch-free calls keep none)
+                hip_check(hipMemsetAsync(touch, 0, g.V + kPad, s), "clear touch");
             }
         }
 
@@ -470,15 +472,36 @@ namespace hyt
         }
     } // namespace
 
-    void sssp_single(DeviceGraph &g, std::uint32_t src, std::uint32_t *dist_out, const hyt_run_options &o,
-                     hyt_run_stats &st)
+    void sssp_split(const DeviceGraph &g, const hyt_run_options &o, std::uint32_t &light_max, std::uint32_t &chunk)
     {
         std::uint32_t dl = 0, dc = 0;
         default_sssp_split(g.E, dl, dc);
-        Launch L{g, o, o.light_max_degree ? o.light_max_degree : dl, o.heavy_chunk ? o.heavy_chunk : dc};
-        require(L.light_max <= (1U << 20), "light_max_degree must be <= 2^20");
-        require(L.chunk >= 64 && L.chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
-        ensure_workspace(g, L.light_max, L.chunk);
+        light_max = o.light_max_degree ? o.light_max_degree : dl;
+        chunk = o.heavy_chunk ? o.heavy_chunk : dc;
+        require(light_max <= (1U << 20), "light_max_degree must be <= 2^20");
+        require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
+    }
+
+    void sssp_rounds(DeviceGraph &g, std::uint32_t src, const SsspBuffers &b, const hyt_run_options &o,
+                     std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st)
+    {
+        Launch L{g, o, light_max, chunk, b};
+        if (g.off64)
+            dispatch_weights<std::uint64_t>(L, src, st);
+        else
+            dispatch_weights<std::uint32_t>(L, src, st);
+    }
+
+    void sssp_single(DeviceGraph &g, std::uint32_t src, std::uint32_t *dist_out, const hyt_run_options &o,
+                     hyt_run_stats &st)
+    {
+        std::uint32_t light_max = 0, chunk = 0;
+        sssp_split(g, o, light_max, chunk);
+        ensure_workspace(g, light_max, chunk);
+        Workspace &ws = g.ws;
+        const SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
+                            ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
+                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>()};
         if (o.tolerance != 0.0F)
         {
             require(o.tolerance > 0.0F && std::isfinite(o.tolerance), "SSSP tolerance must be finite and non-negative");
@@ -487,18 +510,15 @@ namespace hyt
                     "SSSP tolerance > 0 runs SYNC pull rounds (no ASYNC mode, buckets or push direction)");
             require(g.E < (1ULL << 31), "SSSP tolerance > 0 needs fewer than 2^31 edges (transpose)");
             if (g.off64)
-                run_tolerance<std::uint64_t>(g, src, o, L.light_max, L.chunk, st);
+                run_tolerance<std::uint64_t>(g, src, o, light_max, chunk, st);
             else
-                run_tolerance<std::uint32_t>(g, src, o, L.light_max, L.chunk, st);
+                run_tolerance<std::uint32_t>(g, src, o, light_max, chunk, st);
         }
         else
         {
             try
             {
-                if (g.off64)
-                    dispatch_weights<std::uint64_t>(L, src, st);
-                else
-                    dispatch_weights<std::uint32_t>(L, src, st);
+                sssp_rounds(g, src, B, o, light_max, chunk, st);
             }
             catch (...)
             {
