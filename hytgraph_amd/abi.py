@@ -238,6 +238,10 @@ SIGNATURES = {
         C.c_int,
         [_P, _P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
     ),
+    "hyt_sssp_noalloc": (
+        C.c_int,
+        [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
+    ),
     "hyt_pagerank": (
         C.c_int,
         [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
@@ -491,6 +495,14 @@ class DeviceGraph:
         return out, st.as_dict()
 
     # -- PageRank -------------------------------------------------------------
+    def sssp_noalloc(self, source, out_ptr, mode=HYT_SSSP_SYNC, **opt):
+        """hyt_sssp_noalloc: one source into device memory (out_ptr, u32[V]),
+        with the library's static scratch instead of the graph's workspace."""
+        o = _options(mode=mode, out_on_device=1, **opt)
+        st = RunStats()
+        check(load().hyt_sssp_noalloc(self._h, int(source), C.c_void_p(out_ptr), C.byref(o), C.byref(st)))
+        return st.as_dict()
+
     def pagerank(self, damping=0.85, epsilon=1e-7, out_ptr=None, **opt):
         o = _options(out_on_device=1 if out_ptr is not None else 0, **opt)
         st = RunStats()

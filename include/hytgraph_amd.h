@@ -216,6 +216,21 @@ typedef struct hyt_run_stats {
 int hyt_sssp(hyt_graph_t g, const uint32_t *sources, uint32_t num_sources,
              uint32_t *dist, const hyt_run_options *opts, hyt_run_stats *stats);
 
+/*
+ * No counterpart in the original: hyt_sssp for one source for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). It calls no hipMalloc/hipFree: the run's
+ * scratch (distances, record, queue, chunk queue, counters) is a static
+ * 128 MiB device array per GPU, held until the process ends, so graphs up to
+ * about V = 2^22 (E <= 2^28) run; larger ones fail with
+ * HYT_ERR_INVALID_ARGUMENT. dist must be device memory (opts->out_on_device
+ * = 1); rounds are push-only (no pull direction, tolerance 0). Calls on one
+ * GPU share the array: calls that overlap on one GPU race, so serialise them.
+ * Distances and stats equal hyt_sssp's for the same options.
+ */
+int hyt_sssp_noalloc(hyt_graph_t g, uint32_t source, uint32_t *dist,
+                     const hyt_run_options *opts, hyt_run_stats *stats);
+
 /* ------------------------------------------------- PageRank / PPR (FP32) */
 /*
  * Delta-push PageRank. Residuals start at (1-damping) per vertex; a round

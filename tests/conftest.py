@@ -67,6 +67,50 @@ def csr_from_edges(V, edges, weights=None):
     return off, col, w
 
 
+class HipRuntime:
+    """Device buffers for tests that pass device pointers to the C ABI, from
+    the HIP runtime libhytgraph_amd.so itself links (torch bundles its own
+    runtime; the two do not share one process well)."""
+
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        self.lib = C.CDLL("/opt/rocm/lib/libamdhip64.so.7")
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipMemGetInfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+        self.lib.hipDeviceSynchronize.argtypes = []
+
+    def _ok(self, rc, what):
+        assert rc == 0, f"{what} failed: {rc}"
+
+    def malloc(self, nbytes):
+        p = self.C.c_void_p()
+        self._ok(self.lib.hipMalloc(self.C.byref(p), nbytes), "hipMalloc")
+        return p.value
+
+    def free(self, ptr):
+        self._ok(self.lib.hipFree(self.C.c_void_p(ptr)), "hipFree")
+
+    def to_host_u32(self, ptr, n):
+        out = np.empty(n, np.uint32)
+        self._ok(self.lib.hipMemcpy(out.ctypes.data, self.C.c_void_p(ptr), n * 4, 2), "hipMemcpy")
+        return out
+
+    def free_bytes(self):
+        f, t = self.C.c_size_t(), self.C.c_size_t()
+        self._ok(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        self._ok(self.lib.hipMemGetInfo(self.C.byref(f), self.C.byref(t)), "hipMemGetInfo")
+        return f.value
+
+
+@pytest.fixture(scope="session")
+def hip(gpu):
+    return HipRuntime()
+
+
 def f32(x):
     """The value a float parameter of the C ABI carries: the FP64 oracles
     are given the damping and epsilon the GPU computes with."""
