@@ -746,6 +746,8 @@ def main(argv=None) -> int:
     ap.add_argument("--ks", default="8,32,64", help="batched: lanes per call")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy: MiB per copy")
     ap.add_argument("--no-extras", action="store_true", help="plain: skip the batched and served legs")
+    ap.add_argument("--host-rounds", action="store_true",
+                    help="plain: one host read-back per round instead of replays of captured rounds")
     ap.add_argument("--extras-requests", type=int, default=400, help="plain: served-leg requests per trace")
     args = ap.parse_args(argv)
 
@@ -794,7 +796,7 @@ def main(argv=None) -> int:
     mode = abi.HYT_SSSP_ASYNC if args.mode == "async" else abi.HYT_SSSP_SYNC
     direction = {"auto": abi.HYT_DIR_AUTO, "push": abi.HYT_DIR_PUSH, "pull": abi.HYT_DIR_PULL}[args.direction]
     dopt = {"direction": direction, "pull_percent": args.pull_percent}
-    sopt = dict(dopt, delta=args.delta)
+    sopt = dict(dopt, delta=args.delta, host_rounds=1 if args.host_rounds else 0)
 
     tuned = None
     if args.tune:
@@ -878,6 +880,8 @@ def main(argv=None) -> int:
             "edges": hg.E,
             "sources_per_step": len(sources),
             "mode": args.mode,
+            "rounds": "host-driven (one read-back per round)" if args.host_rounds else
+                      "device-driven (captured round graph, one read-back per 4 rounds)",
             "direction": args.direction,
             "delta": args.delta,
             "tuned_split": {"light_max_degree": tuned["light_max_degree"],

@@ -43,11 +43,16 @@ namespace hyt
     // with one atomic per workgroup. Otherwise (ordered mode) workgroup b
     // writes its rows at q[b * kCompactRowsPerBlock ...] and its count to
     // block_counts[b]; gather_ordered() then packs them in vertex order.
+    // gate (optional): the pass does nothing when *gate is 0 (a device-driven
+    // round whose queue was empty).
     template <class Val, class OffT, class Sel>
     __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
                                                            Counters *ctr, const OffT *off, std::uint16_t *active,
-                                                           std::uint32_t *block_counts = nullptr)
+                                                           std::uint32_t *block_counts = nullptr,
+                                                           const std::uint32_t *gate = nullptr)
     {
+        if (gate != nullptr && *gate == 0)
+            return; // uniform: every thread reads the same word
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
         __shared__ unsigned long long s_dsum[kWavesPerBlock];
         __shared__ std::uint32_t s_base;

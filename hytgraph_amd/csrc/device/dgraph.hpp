@@ -82,9 +82,26 @@ namespace hyt
         atomicAdd(&ctr->slot[blockIdx.x % kEdgeSlots].v, n);
     }
 
+    constexpr int kMaxGraphRounds = 8; // rounds per captured round graph, at most
+
+    // Device-side control of device-driven rounds (rounds replayed from a
+    // captured HIP graph without a host round trip): the queue size of the
+    // current round, read by the kernels, and the run's totals.
+    struct RoundCtl
+    {
+        std::uint32_t F;         // rows of the current round's queue (0: no work)
+        std::uint32_t done;      // a round found the queue empty, or the round limit
+        std::uint32_t truncated; // stopped by the round limit with work left
+        std::uint32_t pad;
+        std::uint64_t rounds, frontier_total, edges_total, updates_total;
+        // timed runs: device wall clock before and after the expand kernels
+        // of each round of one replay (round r: t[2r], t[2r+1])
+        std::uint64_t t[2 * kMaxGraphRounds];
+    };
+
     struct Workspace
     {
-        DeviceBuffer dist, prev, touch, q, qd, heavy, ctr, partial;
+        DeviceBuffer dist, prev, touch, q, qd, heavy, ctr, partial, rc;
         DeviceBuffer res, rank; // PageRank mass words u64[V] (ensure_mass_state)
         DeviceBuffer active; // frontier bitmap (bit v = row v selected by the last compaction)
         DeviceBuffer share;  // PageRank pull: d * r_u / outdeg(u) of the frontier rows, by vertex
@@ -108,6 +125,7 @@ namespace hyt
         hipStream_t stream = nullptr;
         Workspace ws;
         std::shared_ptr<void> kws; // workspace of the batched [V][K] calls (owned, type-erased)
+        std::shared_ptr<void> round_graph; // captured single-source rounds (owned, type-erased)
         std::unique_ptr<PinnedBuffer> host_ctr;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -116,7 +134,7 @@ namespace hyt
         {
             return off.bytes + col.bytes + w.bytes + in_off.bytes + in_col.bytes + in_w.bytes + in_heavy.bytes +
                    ws.dist.bytes + ws.res.bytes + ws.rank.bytes + ws.prev.bytes + ws.touch.bytes + ws.q.bytes + ws.qd.bytes +
-                   ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes;
+                   ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes + ws.rc.bytes;
         }
         int weight_mode() const
         {
@@ -190,6 +208,8 @@ namespace hyt
         HeavyChunk *heavy;
         std::uint64_t heavy_cap;
         Counters *ctr;
+        // non-null: push-only rounds run device-driven from a captured graph
+        RoundCtl *rc = nullptr;
     };
     // The single-source work split (options, else the size-dependent default).
     void sssp_split(const DeviceGraph &g, const hyt_run_options &o, std::uint32_t &light_max, std::uint32_t &chunk);

@@ -124,6 +124,9 @@ namespace hyt
     {
         const std::uint32_t *q;   // frontier rows (GraphRows reads them through its own pointer)
         std::uint32_t F;          // their number
+        // device-driven rounds: the number of rows is read here at kernel
+        // start (the grid then loops over 256-row tiles), F is ignored
+        const std::uint32_t *F_dev = nullptr;
         Counters *ctr;
         HeavyChunk *heavy;
         std::uint64_t heavy_cap;
@@ -205,83 +208,95 @@ namespace hyt
             heavy[slot + c] = make(c);
     }
 
+    // One 256-row tile of the queue per workgroup iteration; a grid sized
+    // blocks_for(F) takes each tile once, a smaller grid (device-driven
+    // rounds, F read on the device) loops over them.
     template <class RowSrc, class Op>
     __global__ __launch_bounds__(kBlock) void expand_light(RowSrc rs, Op op, ExpandArgs a)
     {
         __shared__ std::uint32_t s_ex[kWavesPerBlock][kWave];
         __shared__ std::uint64_t s_begin[kWavesPerBlock][kWave];
         __shared__ std::uint32_t s_val[kWavesPerBlock][kWave];
-        __shared__ std::uint32_t s_total[kWavesPerBlock];
+        __shared__ unsigned long long s_edges[kWavesPerBlock];
 
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
-        const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-
-        std::uint32_t deg = 0, value = 0, nch = 0, row = 0;
-        std::uint64_t begin = 0, count = 0;
-        bool whole = false;
-        if (i < a.F)
+        const std::uint32_t F = a.F_dev != nullptr ? *a.F_dev : a.F;
+        const std::uint32_t tiles = (F + kBlock - 1) / kBlock;
+        unsigned long long wedges = 0; // this wavefront's light edges (uniform across its lanes)
+        // a block-uniform loop: every wavefront meets the same barriers; LDS
+        // slices are per wavefront, so a wavefront's next-tile writes only
+        // follow its own reads
+        for (std::uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x)
         {
-            std::uint32_t u, vi;
-            std::uint64_t full;
-            rs.get(i, u, vi, begin, count, full);
-            row = u;
-            whole = count == full;
-            const bool chunked = count > a.light_max;
-            if (chunked && a.pass != 2)
+            const std::uint32_t i = tile * kBlock + threadIdx.x;
+            std::uint32_t deg = 0, value = 0, nch = 0, row = 0;
+            std::uint64_t begin = 0, count = 0;
+            bool whole = false;
+            if (i < F)
             {
-                value = op.row_value(vi, u, full);
-                nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
-                op.expanded_chunked(u, whole);
+                std::uint32_t u, vi;
+                std::uint64_t full;
+                rs.get(i, u, vi, begin, count, full);
+                row = u;
+                whole = count == full;
+                const bool chunked = count > a.light_max;
+                if (chunked && a.pass != 2)
+                {
+                    value = op.row_value(vi, u, full);
+                    nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
+                    op.expanded_chunked(u, whole);
+                }
+                else if (!chunked && a.pass != 1)
+                {
+                    value = op.row_value(vi, u, full);
+                    deg = static_cast<std::uint32_t>(count);
+                    op.expanded(u, value, whole);
+                }
             }
-            else if (!chunked && a.pass != 1)
-            {
-                value = op.row_value(vi, u, full);
-                deg = static_cast<std::uint32_t>(count);
-                op.expanded(u, value, whole);
-            }
-        }
-        queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
-            const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
-            HeavyChunk h;
-            h.value = value;
-            h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
-            h.begin = begin + lo;
-            h.row = row;
-            h.record = whole ? 1U : 0U;
-            return h;
-        });
+            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
+                const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+                HeavyChunk h;
+                h.value = value;
+                h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
+                h.begin = begin + lo;
+                h.row = row;
+                h.record = whole ? 1U : 0U;
+                return h;
+            });
 
-        const std::uint32_t incl = wave_incl_scan(deg);
-        const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
-        s_ex[wid][lane] = incl - deg;
-        s_begin[wid][lane] = begin;
-        s_val[wid][lane] = value;
-        if (lane == 0)
-            s_total[wid] = total;
-        __syncthreads();
+            const std::uint32_t incl = wave_incl_scan(deg);
+            const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+            s_ex[wid][lane] = incl - deg;
+            s_begin[wid][lane] = begin;
+            s_val[wid][lane] = value;
+            __syncthreads();
+            if (total == 0)
+                continue;
+            wedges += total;
+            const std::uint32_t *ex = s_ex[wid];
+            const std::uint64_t *bg = s_begin[wid];
+            const std::uint32_t *vl = s_val[wid];
+            walk_edges(op, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
+                const int j = owner_of(ex, e);
+                pos = bg[j] + (e - ex[j]);
+                val = vl[j];
+            });
+        }
         // one edge-count atomic per workgroup: per-wavefront atomics on the
         // one counter serialise (measured: ~9% of an RMAT-24 SSSP round)
+        if (lane == 0)
+            s_edges[wid] = wedges;
+        __syncthreads();
         if (threadIdx.x == 0)
         {
             unsigned long long sum = 0;
 #pragma unroll
             for (int w = 0; w < kWavesPerBlock; ++w)
-                sum += s_total[w];
+                sum += s_edges[w];
             if (sum)
                 add_edges(a.ctr, sum);
         }
-        if (total == 0)
-            return;
-
-        const std::uint32_t *ex = s_ex[wid];
-        const std::uint64_t *bg = s_begin[wid];
-        const std::uint32_t *vl = s_val[wid];
-        walk_edges(op, total, [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
-            const int j = owner_of(ex, e);
-            pos = bg[j] + (e - ex[j]);
-            val = vl[j];
-        });
     }
 
     // The whole graph resident in HBM: window [0, 2^64).

@@ -58,6 +58,7 @@ namespace hyt
             ws.partial.alloc(4096 * sizeof(double), "hipMalloc(partials)");
             ws.active.alloc(Vp / 8 + 64, "hipMalloc(frontier bitmap)");
             ws.share.alloc(Vp * 4, "hipMalloc(push shares)");
+            ws.rc.alloc(sizeof(RoundCtl), "hipMalloc(round control)");
             // touch must start (and is kept) all-zero: compaction clears what it reads
             hip_check(hipMemsetAsync(ws.touch.ptr, 0, Vp, g.stream), "clear touch");
             ws.ready = true;
@@ -115,7 +116,7 @@ int hyt_graph_upload(uint64_t num_vertices, uint64_t num_edges, const uint64_t *
         hip_check(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipEventCreate(&g.ev0), "hipEventCreate");
         hip_check(hipEventCreate(&g.ev1), "hipEventCreate");
-        g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters));
+        g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters) + sizeof(RoundCtl));
 
         const std::uint64_t V = num_vertices, E = num_edges;
         for (std::uint64_t v = 0; v < V; ++v)
@@ -155,6 +156,7 @@ int hyt_graph_trim(hyt_graph_t g)
         hyt::require(g != nullptr, "null graph");
         hyt::DeviceGraph &d = g->g;
         hyt::hip_check(hipStreamSynchronize(d.stream), "hyt_graph_trim");
+        d.round_graph.reset(); // its nodes name workspace arrays
         d.ws = hyt::Workspace{};
         d.kws.reset();
         d.in_off.release();

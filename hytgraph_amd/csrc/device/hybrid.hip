@@ -601,7 +601,7 @@ namespace hyt
             hip_check(hipEventCreate(&g.ev0), "hipEventCreate");
             hip_check(hipEventCreate(&g.ev1), "hipEventCreate");
             hip_check(hipEventCreateWithFlags(&E.ev_pack, hipEventDisableTiming), "hipEventCreate");
-            g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters));
+            g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters) + sizeof(RoundCtl));
             void *dp = nullptr;
             hip_check(hipHostGetDevicePointer(&dp, host->col, 0), "hipHostGetDevicePointer(columns)");
             E.z_col = static_cast<const std::uint32_t *>(dp);

@@ -40,8 +40,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <string>
 #include <utility>
+#include <vector>
 
 namespace hyt
 {
@@ -240,6 +242,85 @@ namespace hyt
                 hip_check(hipMemcpyAsync(ws.dist.ptr, cur, g.V * 4, hipMemcpyDeviceToDevice, s), "copy distances");
         }
 
+        // ------------------------------------------------ device-driven rounds
+        // Push-only rounds (SYNC or ASYNC, no buckets) run from a captured HIP
+        // graph of kGraphRounds rounds, replayed until a round finds the queue
+        // empty: round_begin moves the last compaction's count into rc->F and
+        // clears the round counters (a kernel, no memset node), the expand
+        // kernels read F on the device and loop over tiles, the compaction is
+        // gated on F, round_end adds the round to the totals. The host reads
+        // rc once per replay instead of the counters once per round.
+        constexpr int kGraphRounds = 4;
+        static_assert(kGraphRounds <= kMaxGraphRounds, "round stamps");
+        constexpr unsigned kDeviceLightGrid = 16384; // tiles of 256 rows per sweep
+
+        __global__ void rc_reset(RoundCtl *rc)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+                *rc = RoundCtl{};
+        }
+
+        __global__ void round_begin(Counters *ctr, RoundCtl *rc, std::uint64_t max_rounds)
+        {
+            if (threadIdx.x != 0 || blockIdx.x != 0)
+                return;
+            std::uint32_t F = ctr->next_count;
+            if (F != 0 && rc->rounds >= max_rounds)
+            {
+                rc->truncated = 1;
+                F = 0;
+            }
+            rc->F = F;
+            if (F != 0)
+            {
+                rc->rounds += 1;
+                rc->frontier_total += F;
+            }
+            else
+                rc->done = 1;
+            ctr->next_count = 0;
+            ctr->heavy_count = 0;
+            ctr->edges = 0;
+            ctr->frontier_edges = 0;
+            ctr->pending = 0;
+            ctr->pending_inv_max = 0;
+            for (int i = 0; i < kEdgeSlots; ++i)
+                ctr->slot[i].v = 0;
+        }
+
+        // Device wall clock into rc->t[slot] (kernels of a stream run in
+        // order, so two stamps bracket the kernels between them).
+        __global__ void stamp(RoundCtl *rc, int slot)
+        {
+            if (threadIdx.x == 0 && blockIdx.x == 0)
+                rc->t[slot] = static_cast<std::uint64_t>(wall_clock64());
+        }
+
+        __global__ void round_end(const Counters *ctr, RoundCtl *rc)
+        {
+            if (threadIdx.x != 0 || blockIdx.x != 0 || rc->F == 0)
+                return;
+            rc->edges_total += ctr->all_edges();
+            rc->updates_total += ctr->next_count;
+        }
+
+        // The instantiated round graph of a device graph, kept across calls
+        // (its nodes name the workspace arrays and the split, not the source);
+        // rebuilt when any of them changes.
+        struct RoundGraph
+        {
+            std::vector<std::uint64_t> key;
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            ~RoundGraph()
+            {
+                if (exec)
+                    (void)hipGraphExecDestroy(exec);
+                if (graph)
+                    (void)hipGraphDestroy(graph);
+            }
+        };
+
         struct Launch
         {
             DeviceGraph &g;
@@ -264,6 +345,115 @@ namespace hyt
             hip_check(hipMemcpyAsync(h_ctr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, g.stream), "read counters");
             hip_check(hipStreamSynchronize(g.stream), "sssp round");
             return *h_ctr;
+        }
+
+        template <class OffT, int WM, bool SYNC>
+        void run_device_driven(Launch &L, const SsspSel &sel, const SsspOp<WM, SYNC> &op, ExpandArgs a,
+                               std::uint64_t max_rounds, hyt_run_stats &st)
+        {
+            DeviceGraph &g = L.g;
+            const SsspBuffers &B = L.b;
+            hipStream_t s = g.stream;
+            const bool timed = L.o.time_kernels != 0;
+            a.F_dev = &B.rc->F;
+            const unsigned light_grid = std::min(blocks_for(g.V), kDeviceLightGrid);
+            const unsigned nb = compact_blocks(g.V);
+            const GraphRows<OffT> rows = whole_graph(g.off.as<OffT>(), B.q);
+            const std::vector<std::uint64_t> key{
+                static_cast<std::uint64_t>(sizeof(OffT)), static_cast<std::uint64_t>(WM), SYNC ? 1ULL : 0ULL,
+                timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap,
+                reinterpret_cast<std::uint64_t>(B.dist), reinterpret_cast<std::uint64_t>(B.prev),
+                reinterpret_cast<std::uint64_t>(B.q), reinterpret_cast<std::uint64_t>(B.qd),
+                reinterpret_cast<std::uint64_t>(B.heavy), reinterpret_cast<std::uint64_t>(B.ctr),
+                reinterpret_cast<std::uint64_t>(B.rc), g.off.ptr ? reinterpret_cast<std::uint64_t>(g.off.ptr) : 0,
+                reinterpret_cast<std::uint64_t>(g.col.ptr), reinterpret_cast<std::uint64_t>(g.w.ptr)};
+            auto *cached = static_cast<RoundGraph *>(g.round_graph.get());
+            if (cached == nullptr || cached->key != key)
+            {
+                g.round_graph.reset();
+                auto rg = std::make_shared<RoundGraph>();
+                rg->key = key;
+                hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin round capture");
+                for (int r = 0; r < kGraphRounds; ++r)
+                {
+                    round_begin<<<1, 64, 0, s>>>(B.ctr, B.rc, max_rounds);
+                    if (timed)
+                        stamp<<<1, 64, 0, s>>>(B.rc, 2 * r);
+                    if (SYNC)
+                    {
+                        expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a);
+                        expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a);
+                    }
+                    else
+                    {
+                        ExpandArgs a1 = a, a2 = a;
+                        a1.pass = 1;
+                        a2.pass = 2;
+                        expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a1);
+                        expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
+                        expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a2);
+                    }
+                    if (timed)
+                        stamp<<<1, 64, 0, s>>>(B.rc, 2 * r + 1);
+                    compact_rows<std::uint32_t, OffT, SsspSel><<<nb, kBlock, 0, s>>>(
+                        g.V, sel, B.q, SYNC ? B.qd : nullptr, B.ctr, static_cast<const OffT *>(nullptr),
+                        static_cast<std::uint16_t *>(nullptr), static_cast<std::uint32_t *>(nullptr), &B.rc->F);
+                    round_end<<<1, 64, 0, s>>>(B.ctr, B.rc);
+                }
+                const hipError_t launch_err = hipGetLastError();
+                const hipError_t cap_err = hipStreamEndCapture(s, &rg->graph);
+                hip_check(launch_err, "launch captured round");
+                hip_check(cap_err, "end round capture");
+                hip_check(hipGraphInstantiate(&rg->exec, rg->graph, nullptr, nullptr, 0), "instantiate round graph");
+                g.round_graph = rg;
+                cached = rg.get();
+            }
+            rc_reset<<<1, 64, 0, s>>>(B.rc);
+            hip_check(hipGetLastError(), "launch rc_reset");
+            double clock_khz = 1.0;
+            if (timed)
+            {
+                int khz = 0;
+                hip_check(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, g.device), "wall clock rate");
+                require(khz > 0, "device wall clock rate unknown");
+                clock_khz = static_cast<double>(khz);
+            }
+            RoundCtl *h_rc = reinterpret_cast<RoundCtl *>(g.host_ctr->as<unsigned char>() + sizeof(Counters));
+            Counters *h_ctr = g.host_ctr->as<Counters>();
+            std::uint64_t seen_rounds = 0;
+            while (true)
+            {
+                hip_check(hipGraphLaunch(cached->exec, s), "launch round graph");
+                hip_check(hipMemcpyAsync(h_rc, B.rc, sizeof(RoundCtl), hipMemcpyDeviceToHost, s), "read rounds");
+                hip_check(hipMemcpyAsync(h_ctr, B.ctr, sizeof(Counters), hipMemcpyDeviceToHost, s), "read counters");
+                hip_check(hipStreamSynchronize(s), "sssp rounds");
+                if (timed)
+                {
+                    // rounds of this replay that had work (the rest exit at
+                    // once); the stamps bracket their expand kernels
+                    const std::uint64_t ran = std::min<std::uint64_t>(h_rc->rounds - seen_rounds, kGraphRounds);
+                    for (std::uint64_t r = 0; r < ran; ++r)
+                    {
+                        st.expand_ms += static_cast<double>(h_rc->t[2 * r + 1] - h_rc->t[2 * r]) / clock_khz;
+                        st.expand_launches += 1;
+                    }
+                }
+                seen_rounds = h_rc->rounds;
+                if (h_ctr->heavy_overflow)
+                    fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
+                if (h_ctr->overflow)
+                {
+                    st.overflow = 1;
+                    fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
+                }
+                if (h_rc->done)
+                    break;
+            }
+            st.rounds = h_rc->rounds;
+            st.frontier_total += h_rc->frontier_total;
+            st.edges_total += h_rc->edges_total;
+            st.updates_total += h_rc->updates_total;
+            st.converged = h_rc->truncated ? 0 : 1;
         }
 
         template <class OffT, int WM, bool SYNC>
@@ -348,6 +538,11 @@ namespace hyt
 
             const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
             st.converged = 0;
+            if (B.rc != nullptr && !bucketed && !can_pull && F > 0)
+            {
+                run_device_driven<OffT, WM, SYNC>(L, sel, op, a, max_rounds, st);
+                return;
+            }
             for (std::uint64_t round = 0; round < max_rounds && (F > 0 || pending > 0);)
             {
                 if (F == 0)
@@ -501,7 +696,8 @@ namespace hyt
         Workspace &ws = g.ws;
         const SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
                             ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
-                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>()};
+                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
+                            o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
         if (o.tolerance != 0.0F)
         {
             require(o.tolerance > 0.0F && std::isfinite(o.tolerance), "SSSP tolerance must be finite and non-negative");

@@ -178,6 +178,10 @@ typedef struct hyt_run_options {
                                   c + tolerance < best, in incoming-edge order
                                   (sssp.cpp:199-216, cuda/sssp.cu:131-153); > 0 runs
                                   ordered pull rounds over the transpose */
+    uint32_t host_rounds;      /* single-source SSSP push rounds: 0 = replays of
+                                  captured rounds with device-side round control
+                                  (one read-back per 4 rounds); 1 = one host
+                                  read-back and launch sequence per round */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */

@@ -123,6 +123,38 @@ def test_fractional_float_weights_bitexact(gpu, abi, orc, rng):
                 assert st["rounds"] == it
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_device_driven_rounds_match_host_rounds(gpu, abi, orc, mode):
+    """Push rounds replayed from the captured round graph (device-side round
+    control, the default) against one host read-back per round: the same
+    distances; in SYNC mode the same round count, frontier, edge and update
+    totals, also when truncated by max_rounds; the cached graph serves other
+    sources and is rebuilt after a trim or a split change."""
+    hg = rmat(abi, 15, 81, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    keys = ("rounds", "converged", "frontier_total", "edges_total", "updates_total")
+    for src in abi.sample_sources(hg.offsets, 9, 5):
+        ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+        dd, sd = g.sssp(int(src), mode=mode)
+        dh, sh = g.sssp(int(src), mode=mode, host_rounds=1)
+        assert np.array_equal(dd, ref) and np.array_equal(dh, ref), src
+        assert sd["converged"] == sh["converged"] == 1
+        if mode == 0:
+            assert all(sd[k] == sh[k] for k in keys), (src, sd, sh)
+            for mr in (1, 2, 3, 4, 5, 7, 9):
+                d1, s1 = g.sssp(int(src), mode=0, max_rounds=mr)
+                d2, s2 = g.sssp(int(src), mode=0, max_rounds=mr, host_rounds=1)
+                assert np.array_equal(d1, d2) and all(s1[k] == s2[k] for k in keys), (src, mr)
+    # timed replays: one timed expansion per round with work
+    _, st = g.sssp(0, mode=mode, time_kernels=1)
+    assert st["expand_launches"] == st["rounds"] and st["expand_ms"] > 0
+    d, _ = g.sssp(0, mode=mode, light_max_degree=8, heavy_chunk=64)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
+    g.trim()
+    d, _ = g.sssp(0, mode=mode)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
+
+
 @pytest.mark.parametrize("tol", [0.25, 0.5, 3.0, 40.0])
 @pytest.mark.parametrize("scale", [10, 13])
 def test_tolerance_rule_bitexact(gpu, abi, orc, rng, tol, scale):
