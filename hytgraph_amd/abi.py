@@ -243,6 +243,11 @@ SIGNATURES = {
         C.c_int,
         [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
     ),
+    "hyt_stepper_create": (
+        C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(RunOptions), C.POINTER(_P)]),
+    "hyt_stepper_step": (C.c_int, [_P, C.c_uint64, C.POINTER(RunStats), C.POINTER(C.c_int32)]),
+    "hyt_stepper_read": (C.c_int, [_P, _P, C.POINTER(RunStats)]),
+    "hyt_stepper_free": (C.c_int, [_P]),
     "hyt_pagerank": (
         C.c_int,
         [_P, C.c_float, C.c_float, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
@@ -666,6 +671,51 @@ class HybridEngine:
     def free(self):
         if getattr(self, "_h", None):
             check(load().hyt_hybrid_free(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ round-by-round runs
+HYT_STEP_SSSP = 0
+HYT_STEP_PAGERANK = 1
+
+
+class Stepper:
+    """A single-source SSSP or PageRank/PPR run advanced round by round
+    (hyt_stepper_*); keeps its DeviceGraph alive."""
+
+    def __init__(self, graph, kind=HYT_STEP_SSSP, vertex=0, damping=0.85, epsilon=1e-7, **opt):
+        self.graph = graph
+        self.kind = kind
+        self.V = graph.V
+        o = _options(**opt)
+        h = C.c_void_p()
+        check(load().hyt_stepper_create(graph._h, kind, int(vertex) & 0xFFFFFFFF, damping, epsilon, C.byref(o),
+                                        C.byref(h)))
+        self._h = h
+
+    def step(self, rounds=1):
+        """Runs up to `rounds` rounds; returns (this call's stats, finished)."""
+        st = RunStats()
+        fin = C.c_int32()
+        check(load().hyt_stepper_step(self._h, rounds, C.byref(st), C.byref(fin)))
+        return st.as_dict(), bool(fin.value)
+
+    def read(self):
+        """(current distances u32[V] or normalised ranks f32[V], the run's stats so far)."""
+        out = np.empty(self.V, np.uint32 if self.kind == HYT_STEP_SSSP else np.float32)
+        st = RunStats()
+        check(load().hyt_stepper_read(self._h, _ptr(out), C.byref(st)))
+        return out, st.as_dict()
+
+    def free(self):
+        if getattr(self, "_h", None):
+            check(load().hyt_stepper_free(self._h))
             self._h = None
 
     def __del__(self):

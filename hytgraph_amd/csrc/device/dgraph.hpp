@@ -216,6 +216,15 @@ namespace hyt
     // The SSSP rounds from src over the given arrays (distances end in b.dist).
     void sssp_rounds(DeviceGraph &g, std::uint32_t src, const SsspBuffers &b, const hyt_run_options &o,
                      std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st);
+    // Round-by-round runs (steppers): `rounds` more rounds of a single-source
+    // SSSP run from src (resume = continue the run the workspace holds), or
+    // of a PageRank / PPR run (seed UINT32_MAX: global); the stats are this
+    // call's. pagerank_read writes the current normalised ranks.
+    void sssp_steps(DeviceGraph &g, std::uint32_t src, const hyt_run_options &o, bool resume, std::uint64_t rounds,
+                    hyt_run_stats &st);
+    void pagerank_steps(DeviceGraph &g, std::uint32_t seed, float damping, float eps, const hyt_run_options &o,
+                        bool resume, std::uint64_t rounds, hyt_run_stats &st);
+
     // Heavy-chunk capacity of a single-source run at a split.
     inline std::uint64_t heavy_capacity(std::uint64_t E, std::uint32_t light_max, std::uint32_t chunk)
     {

@@ -68,9 +68,10 @@ namespace hyt
                 res[s] = r0;
         }
 
-        // x = rank + res (written into rank); exact per-block sums of x and
-        // res (no word sum passes 2^62, mass.hpp).
-        __global__ __launch_bounds__(kBlock) void pr_fold(unsigned long long *rank, const unsigned long long *res,
+        // Exact per-block sums of x = rank + res and of res (no word sum
+        // passes 2^62, mass.hpp); the state is left as it is, so a run can
+        // be read between its rounds.
+        __global__ __launch_bounds__(kBlock) void pr_fold(const unsigned long long *rank, const unsigned long long *res,
                                                           std::uint64_t V, unsigned long long *partial)
         {
             __shared__ unsigned long long sx[kBlock], sr[kBlock];
@@ -80,9 +81,7 @@ namespace hyt
             for (; i < V; i += stride)
             {
                 const unsigned long long r = res[i];
-                const unsigned long long x = rank[i] + r;
-                rank[i] = x;
-                ax += x;
+                ax += rank[i] + r;
                 ar += r;
             }
             sx[threadIdx.x] = ax;
@@ -133,14 +132,14 @@ namespace hyt
             }
         }
 
-        __global__ void pr_normalize(const unsigned long long *x, std::uint64_t V, const unsigned long long *total,
-                                     float *out)
+        __global__ void pr_normalize(const unsigned long long *rank, const unsigned long long *res, std::uint64_t V,
+                                     const unsigned long long *total, float *out)
         {
             const double t = static_cast<double>(total[0]);
             std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
             for (; i < V; i += stride)
-                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(x[i]) / t : 0.0);
+                out[i] = static_cast<float>(t > 0.0 ? static_cast<double>(rank[i] + res[i]) / t : 0.0);
         }
 
         constexpr unsigned kFoldBlocks = 1024; // partial holds 2*kFoldBlocks+2 words
@@ -162,12 +161,13 @@ namespace hyt
 
         template <class OffT>
         void run_push(DeviceGraph &g, const hyt_run_options &o, float damping, const MassScale &m,
-                      hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk)
+                      hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk, bool resume = false,
+                      std::uint64_t limit = 0)
         {
             Workspace &ws = g.ws;
             hipStream_t s = g.stream;
             Counters *d_ctr = ws.ctr.as<Counters>();
-            const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : 100000;
+            const std::uint64_t max_rounds = limit ? limit : (o.max_rounds ? o.max_rounds : 100000);
 
             PrSel<OffT> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(), ws.share.as<float>(),
                             g.off.as<OffT>(), damping, m.eps};
@@ -192,8 +192,20 @@ namespace hyt
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
             PrPullOp pop{g.in_col.as<std::uint32_t>(), ws.share.as<float>(), ws.res.as<unsigned long long>()};
 
-            hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            Counters c0 = compact_and_read<OffT>(g, sel);
+            Counters c0{};
+            if (resume)
+            {
+                // the queue and counters of the last compaction of the run
+                hip_check(hipMemcpyAsync(g.host_ctr->ptr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s),
+                          "read counters");
+                hip_check(hipStreamSynchronize(s), "pagerank resume");
+                c0 = *g.host_ctr->as<Counters>();
+            }
+            else
+            {
+                clear_counters(d_ctr, s);
+                c0 = compact_and_read<OffT>(g, sel);
+            }
             std::uint32_t F = c0.next_count;
             unsigned long long Fe = c0.frontier_edges;
             st.converged = F == 0 ? 1 : 0;
@@ -276,7 +288,8 @@ namespace hyt
         hip_check(hipMemcpyAsync(totals, partial + 2 * nb, sizeof(totals), hipMemcpyDeviceToHost, s),
                   "read totals");
         float *dst = out_on_device ? out : ws.dist.as<float>();
-        pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<unsigned long long>(), g.V, partial + 2 * nb, dst);
+        pr_normalize<<<nb, kBlock, 0, s>>>(ws.rank.as<unsigned long long>(), ws.res.as<unsigned long long>(), g.V,
+                                           partial + 2 * nb, dst);
         hip_check(hipGetLastError(), "launch pr_normalize");
         if (!out_on_device)
             hip_check(hipMemcpyAsync(out, dst, g.V * 4, hipMemcpyDeviceToHost, s), "copy ranks");
@@ -288,6 +301,13 @@ namespace hyt
     // One delta-push solve; seed == UINT32_MAX means global PageRank.
     void pagerank_single(DeviceGraph &g, std::uint32_t seed, float damping, float eps, float *out,
                          const hyt_run_options &o, hyt_run_stats &st)
+    {
+        pagerank_steps(g, seed, damping, eps, o, false, 0, st);
+        finalize_ranks(g, out, o.out_on_device != 0, st);
+    }
+
+    void pagerank_steps(DeviceGraph &g, std::uint32_t seed, float damping, float eps, const hyt_run_options &o,
+                        bool resume, std::uint64_t rounds, hyt_run_stats &st)
     {
         check_params(damping, eps);
         const std::uint32_t light_max = o.light_max_degree ? o.light_max_degree : kPagerankLightMaxDegree;
@@ -301,13 +321,13 @@ namespace hyt
         const MassScale m = mass_scale(global ? g.V : 1, damping, eps);
         unsigned long long *res = g.ws.res.as<unsigned long long>();
         unsigned long long *rank = g.ws.rank.as<unsigned long long>();
-        if (global)
+        if (!resume && global)
         {
             std::uint64_t b = (g.V + kBlock - 1) / kBlock;
             pr_init_all<<<static_cast<unsigned>(b < 65536 ? b : 65536), kBlock, 0, s>>>(res, rank, g.V, m.r0);
             hip_check(hipGetLastError(), "launch pr_init");
         }
-        else
+        else if (!resume)
         {
             launch_fill<unsigned long long>(res, g.V, 0ULL, s);
             launch_fill<unsigned long long>(rank, g.V, 0ULL, s);
@@ -315,10 +335,9 @@ namespace hyt
             hip_check(hipGetLastError(), "launch pr_seed");
         }
         if (g.off64)
-            run_push<std::uint64_t>(g, o, damping, m, st, light_max, chunk);
+            run_push<std::uint64_t>(g, o, damping, m, st, light_max, chunk, resume, rounds);
         else
-            run_push<std::uint32_t>(g, o, damping, m, st, light_max, chunk);
-        finalize_ranks(g, out, o.out_on_device != 0, st);
+            run_push<std::uint32_t>(g, o, damping, m, st, light_max, chunk, resume, rounds);
     }
 
     void ppr_batched(DeviceGraph &g, const std::uint32_t *seeds, std::uint32_t K, float damping, float eps,

@@ -267,8 +267,11 @@ namespace hyt
             std::uint32_t F = ctr->next_count;
             if (F != 0 && rc->rounds >= max_rounds)
             {
+                // the round limit: the counters keep the queue for a resumed run
                 rc->truncated = 1;
-                F = 0;
+                rc->F = 0;
+                rc->done = 1;
+                return;
             }
             rc->F = F;
             if (F != 0)
@@ -327,6 +330,11 @@ namespace hyt
             const hyt_run_options &o;
             std::uint32_t light_max, chunk;
             SsspBuffers b;
+            // resume = continue a run that stopped at its round limit (the
+            // arrays and counters hold its state); limit = rounds of this call
+            // (0: the options' max_rounds, else V)
+            bool resume = false;
+            std::uint64_t limit = 0;
         };
 
         // Compaction into the queue; SYNC rounds also take the snapshot values,
@@ -464,9 +472,12 @@ namespace hyt
             hipStream_t s = g.stream;
             Counters *d_ctr = B.ctr;
 
-            launch_fill<std::uint32_t>(B.dist, g.V, Weights<WM>::kInf, s);
-            launch_fill<std::uint32_t>(B.prev, g.V, Weights<WM>::kInf, s);
-            clear_counters(d_ctr, s);
+            if (!L.resume)
+            {
+                launch_fill<std::uint32_t>(B.dist, g.V, Weights<WM>::kInf, s);
+                launch_fill<std::uint32_t>(B.prev, g.V, Weights<WM>::kInf, s);
+                clear_counters(d_ctr, s);
+            }
 
             const std::uint32_t dir = L.o.direction;
             const std::uint64_t pct = L.o.pull_percent ? L.o.pull_percent : kDefaultPullPercentSssp;
@@ -476,8 +487,11 @@ namespace hyt
             // stores that each move a cache line); pull rounds need touch[].
             require(!may_pull || B.touch != nullptr, "pull rounds need the traversal workspace");
             std::uint8_t *touch = may_pull ? B.touch : nullptr;
-            seed_kernel<<<1, 64, 0, s>>>(B.dist, touch, src, 0U);
-            hip_check(hipGetLastError(), "launch seed");
+            if (!L.resume)
+            {
+                seed_kernel<<<1, 64, 0, s>>>(B.dist, touch, src, 0U);
+                hip_check(hipGetLastError(), "launch seed");
+            }
 
             // delta-stepping buckets (ASYNC only): admit improved rows below
             // `bound`; when none is left, move the bound past the nearest
@@ -505,7 +519,18 @@ namespace hyt
             const bool can_pull = may_pull && g.in_col.ptr != nullptr;
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
 
-            Counters c0 = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
+            require(!(L.resume && bucketed), "a resumed run has no delta-stepping buckets");
+            Counters c0{};
+            if (L.resume)
+            {
+                // the queue and counters of the last compaction of the run
+                hip_check(hipMemcpyAsync(g.host_ctr->ptr, d_ctr, sizeof(Counters), hipMemcpyDeviceToHost, s),
+                          "read counters");
+                hip_check(hipStreamSynchronize(s), "sssp resume");
+                c0 = *g.host_ctr->as<Counters>();
+            }
+            else
+                c0 = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
             std::uint32_t F = c0.next_count;
             unsigned long long Fe = c0.frontier_edges;
             std::uint32_t pending = c0.pending;
@@ -536,7 +561,7 @@ namespace hyt
             a.light_max = L.light_max;
             a.chunk = L.chunk;
 
-            const std::uint64_t max_rounds = L.o.max_rounds ? L.o.max_rounds : g.V;
+            const std::uint64_t max_rounds = L.limit ? L.limit : (L.o.max_rounds ? L.o.max_rounds : g.V);
             st.converged = 0;
             if (B.rc != nullptr && !bucketed && !can_pull && F > 0)
             {
@@ -681,6 +706,25 @@ namespace hyt
                      std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st)
     {
         Launch L{g, o, light_max, chunk, b};
+        if (g.off64)
+            dispatch_weights<std::uint64_t>(L, src, st);
+        else
+            dispatch_weights<std::uint32_t>(L, src, st);
+    }
+
+    void sssp_steps(DeviceGraph &g, std::uint32_t src, const hyt_run_options &o, bool resume, std::uint64_t rounds,
+                    hyt_run_stats &st)
+    {
+        require(o.tolerance == 0.0F && o.delta == 0, "round-by-round SSSP runs have no tolerance or buckets");
+        std::uint32_t light_max = 0, chunk = 0;
+        sssp_split(g, o, light_max, chunk);
+        ensure_workspace(g, light_max, chunk);
+        Workspace &ws = g.ws;
+        const SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
+                            ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
+                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
+                            o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
+        Launch L{g, o, light_max, chunk, B, resume, rounds};
         if (g.off64)
             dispatch_weights<std::uint64_t>(L, src, st);
         else

@@ -253,6 +253,33 @@ int hyt_ppr(hyt_graph_t g, const uint32_t *seeds, uint32_t num_seeds,
             float damping, float epsilon, float *out,
             const hyt_run_options *opts, hyt_run_stats *stats);
 
+/* ------------------------------------------------ round-by-round runs */
+/*
+ * A single-source SSSP run (hyt_sssp with one source) or a PageRank / PPR
+ * run (hyt_pagerank, or hyt_ppr with one seed) advanced `rounds` rounds per
+ * hyt_stepper_step and read between steps: the execution behind the SEP
+ * drivers' execute_step() (include/sep/sep_execution_driver.hpp; the
+ * original's drivers are deferred placeholders, its GPU functions only run
+ * to convergence). Rounds and results equal the one-call forms: a SYNC SSSP
+ * stepped one round at a time passes through the states of the original's
+ * max_iterations = 1, 2, ... runs. A stepper runs on its graph's workspace:
+ * one stepper per graph at a time, no other call on the graph while it lives.
+ * vertex: the SSSP source, or the PPR seed (UINT32_MAX: global PageRank).
+ * opts: mode, direction, split, host_rounds (no tolerance or buckets).
+ */
+typedef struct hyt_stepper *hyt_stepper_t;
+enum hyt_step_kind { HYT_STEP_SSSP = 0, HYT_STEP_PAGERANK = 1 };
+int hyt_stepper_create(hyt_graph_t g, uint32_t kind, uint32_t vertex, float damping, float epsilon,
+                       const hyt_run_options *opts, hyt_stepper_t *out);
+/* stats: this call's rounds; finished = 1 once a round found nothing left to
+ * do (later calls run nothing). */
+int hyt_stepper_step(hyt_stepper_t s, uint64_t rounds, hyt_run_stats *stats, int32_t *finished);
+/* out: u32[V] distances (float bit patterns on F32 graphs) or f32[V]
+ * normalised ranks, host or device per opts->out_on_device; total: the
+ * run's stats so far (PageRank: residual = unpushed mass / total). */
+int hyt_stepper_read(hyt_stepper_t s, void *out, hyt_run_stats *total);
+int hyt_stepper_free(hyt_stepper_t s);
+
 /* ---------------------------------------------- hybrid transfer planning */
 /*
  * The HyTM planning steps of the original project as plain functions over

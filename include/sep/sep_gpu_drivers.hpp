@@ -5,10 +5,14 @@
 // SEPSSSPApplication / SEPPageRankApplication implement the SEP hooks with
 // the min-plus (SSSP) and delta-push (PageRank) semantics, so host code can
 // drive them through any SEP driver; SEPSSSPGpuDriver / SEPPageRankGpuDriver
-// run the same computation on the GPU. The context's graph handle is a
-// `const graph::CSRGraph*`. One execute_step() runs the computation to
-// convergence and reports CONVERGED with the rounds' frontier rows
-// (vertices_processed) and improvements (updates_generated).
+// run the same computation on the GPU, one round per execute_step() (the
+// hyt_stepper_* rounds of the C ABI): SUCCESS with the round's frontier rows
+// (vertices_processed) and improvements (updates_generated) while work is
+// left, CONVERGED on the round that finds none. The context's graph handle is
+// a `const graph::CSRGraph*`; the device graph is uploaded on the first step
+// and kept until the driver is finalised or sees another graph. values() is
+// the state after the last step (distances, or normalised ranks).
+// SEPExecutionFactory hands these drivers out for these applications.
 //
 // Variant mapping (results are the same for every variant; the variant picks
 // how rounds run):
@@ -26,6 +30,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <limits>
+#include <memory>
 #include <vector>
 
 namespace hytgraph::sep
@@ -101,7 +106,12 @@ namespace hytgraph::sep
         SEPExecutionVariant variant_{ExecutionMode::SYNC, MessagePassing::PUSH, SchedulingMode::DATA_DRIVEN};
     };
 
-    // Shared part of the GPU drivers: variant checks, context access, results.
+    // A device graph and the round-by-round run on it (defined in the
+    // library; opaque here).
+    struct SEPGpuRun;
+
+    // Shared part of the GPU drivers: variant checks, context access, the
+    // run kept across steps, results.
     template <typename TWeight>
     class SEPGpuDriverBase : public SEPExecutionDriverBase<float, float, TWeight>
     {
@@ -110,9 +120,15 @@ namespace hytgraph::sep
         using step_result_type = typename base_type::step_result_type;
 
         void set_context(SEPExecutionContext &context) noexcept override { base_type::set_context(context); }
+        void finalize() noexcept override
+        {
+            run_.reset();
+            base_type::finalize();
+        }
 
-        // Per-vertex result of the last execute_step (distances or ranks).
+        // Per-vertex state after the last execute_step (distances or ranks).
         const std::vector<float> &values() const noexcept { return values_; }
+        // Rounds run so far (SYNC SSSP: the original's iteration count).
         std::uint64_t rounds() const noexcept { return rounds_; }
 
     protected:
@@ -122,6 +138,7 @@ namespace hytgraph::sep
                                             : nullptr;
         }
 
+        std::shared_ptr<SEPGpuRun> run_;
         std::vector<float> values_;
         std::uint64_t rounds_ = 0;
     };

@@ -217,8 +217,17 @@ namespace
             drv.initialize(app, sep::SEPExecutionConfig(v));
             sep::SEPExecutionContext ctx(const_cast<CSRGraph *>(&g), nullptr, nullptr, nullptr, &frontier);
             drv.set_context(ctx);
-            const auto r = drv.execute_step();
-            expect(r.reached_convergence() && drv.converged(), "SEP SSSP converges: " + v.to_string());
+            // one round per step until a round finds nothing left
+            int steps = 0;
+            sep::SEPExecutionResult r;
+            do
+            {
+                r = drv.execute_step();
+                ++steps;
+                expect(r.succeeded(), "SEP SSSP step succeeds: " + v.to_string());
+            } while (!r.reached_convergence() && steps < 100);
+            expect(drv.converged() && static_cast<std::uint64_t>(steps) == drv.rounds(),
+                   "SEP SSSP converges one round per step: " + v.to_string());
             expect(drv.values() == std::vector<float>({0.0F, 1.0F, 3.0F, 4.0F}), "SEP SSSP distances: " + v.to_string());
             expect(drv.execute_step().status == sep::SEPExecutionStatus::CONVERGED, "SEP SSSP stays converged");
             sep::SEPPageRankApplication pr(0.85F, 1e-9F);
@@ -226,19 +235,56 @@ namespace
             pd.initialize(pr, sep::SEPExecutionConfig(v));
             sep::SEPExecutionContext pctx(const_cast<CSRGraph *>(&cyc), nullptr, nullptr, nullptr, &frontier);
             pd.set_context(pctx);
-            expect(pd.execute_step().succeeded(), "SEP PageRank runs: " + v.to_string());
+            for (int i = 0; i < 10000 && !pd.converged(); ++i)
+                expect(pd.execute_step().succeeded(), "SEP PageRank runs: " + v.to_string());
+            expect(pd.converged(), "SEP PageRank converges: " + v.to_string());
             for (float x : pd.values())
                 expect(std::fabs(x - 1.0F / 3.0F) <= 1e-5F, "SEP PageRank cycle ranks: " + v.to_string());
         }
-        // SYNC push rounds equal the original's iteration count
+        // SYNC push rounds: the chain's states after 1, 2, 3 steps are the
+        // original's max_iterations = 1, 2, 3 states, and 3 rounds converge
+        // (algorithm_tests.cpp:131-157)
         sep::SEPSSSPApplication app(0);
         sep::SEPSSSPGpuDriver drv;
         drv.initialize(app, sep::SEPExecutionConfig::from_string("SYNC_PUSH_DD"));
         CSRGraph chain(4, {0, 1, 2, 2, 2}, {1, 2});
         sep::SEPExecutionContext ctx(const_cast<CSRGraph *>(&chain), nullptr, nullptr, nullptr, &frontier);
         drv.set_context(ctx);
-        (void)drv.execute_step();
+        const float inf = std::numeric_limits<float>::infinity();
+        const std::vector<std::vector<float>> states = {{0.0F, 1.0F, inf, inf}, {0.0F, 1.0F, 2.0F, inf},
+                                                        {0.0F, 1.0F, 2.0F, inf}};
+        for (std::size_t k = 0; k < states.size(); ++k)
+        {
+            const auto r = drv.execute_step();
+            expect(drv.values() == states[k], "SEP SYNC_PUSH_DD state after round " + std::to_string(k + 1));
+            expect(r.reached_convergence() == (k == 2), "SEP SYNC_PUSH_DD converges on round 3");
+        }
         expect(drv.rounds() == 3, "SEP SYNC_PUSH_DD rounds equal sssp_cpu iterations");
+        // the factory binds SSSP / PageRank applications to the GPU drivers
+        auto fd = sep::SEPExecutionFactory<float, float, float>::create("SYNC_PUSH_DD");
+        sep::SEPSSSPApplication fapp(0);
+        fd->initialize(fapp, sep::SEPExecutionConfig::from_string("SYNC_PUSH_DD"));
+        sep::SEPExecutionContext fctx(const_cast<CSRGraph *>(&g), nullptr, nullptr, nullptr, &frontier);
+        fd->set_context(fctx);
+        int fsteps = 0;
+        while (!fd->converged() && fsteps < 100)
+        {
+            expect(fd->execute_step().succeeded(), "factory SSSP driver steps");
+            ++fsteps;
+        }
+        const auto *fv = sep::gpu_values(*fd);
+        expect(fd->converged() && fv != nullptr && *fv == std::vector<float>({0.0F, 1.0F, 3.0F, 4.0F}),
+               "factory driver runs SSSP on the GPU");
+        auto fp = sep::SEPExecutionFactory<float, float, double>::create("ASYNC_PUSH_DD");
+        sep::SEPPageRankApplication fpr(0.85F, 1e-9F);
+        fp->initialize(fpr, sep::SEPExecutionConfig::from_string("ASYNC_PUSH_DD"));
+        sep::SEPExecutionContext fpctx(const_cast<CSRGraph *>(&cyc), nullptr, nullptr, nullptr, &frontier);
+        fp->set_context(fpctx);
+        for (int i = 0; i < 10000 && !fp->converged(); ++i)
+            expect(fp->execute_step().succeeded(), "factory PageRank driver steps");
+        const auto *pv = sep::gpu_values(*fp);
+        expect(fp->converged() && pv != nullptr && std::fabs((*pv)[0] - 1.0F / 3.0F) <= 1e-5F,
+               "factory driver runs PageRank on the GPU");
     }
 } // namespace
 
