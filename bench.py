@@ -181,7 +181,7 @@ def _hybrid_one(abi, kind: int, args) -> dict:
     for s in sources:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        d, st = eng.sssp(int(s), time_kernels=1)
+        d, st = eng.sssp(int(s), time_kernels=1, task_order=args.task_order)
         sec = time.perf_counter() - t0
         reach = int(deg[d != 0xFFFFFFFF].sum())
         moved = st["bytes_filter"] + st["bytes_compaction"]
@@ -190,6 +190,7 @@ def _hybrid_one(abi, kind: int, args) -> dict:
                                                              st["parts_compaction"], st["parts_zero_copy"]],
                         "bytes_copied": moved, "bytes_zero_copy": st["bytes_zero_copy"],
                         "copy_ms": st["copy_ms"], "pack_ms": st["pack_ms"], "plan_ms": st["plan_ms"],
+                        "edges_relaxed": st["run"]["edges_total"], "runs_reordered": st["runs_reordered"],
                         "copy_gbs": moved / (st["copy_ms"] / 1e3) / 1e9 if st["copy_ms"] > 0 else None})
         dists.append(d)
         log(f"{name} hybrid sssp src {int(s)}: {sec:.2f} s, {reach / sec / 1e9:.2f} GTEPS, rounds "
@@ -210,7 +211,7 @@ def _hybrid_one(abi, kind: int, args) -> dict:
     if not args.no_pagerank:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r, pst = eng.pagerank(0.85, args.pr_eps, time_kernels=1)
+        r, pst = eng.pagerank(0.85, args.pr_eps, time_kernels=1, task_order=args.task_order)
         sec = time.perf_counter() - t0
         moved = pst["bytes_filter"] + pst["bytes_compaction"]
         out["pagerank"] = {"eps": args.pr_eps, "seconds": sec, "rounds": pst["run"]["rounds"],
@@ -735,6 +736,8 @@ def main(argv=None) -> int:
     ap.add_argument("--part-log2", type=int, default=20, help="hybrid: log2 edges per partition")
     ap.add_argument("--pr-eps", type=float, default=1e-7)
     ap.add_argument("--no-check", action="store_true", help="hybrid: skip the in-HBM bit-exact check")
+    ap.add_argument("--task-order", type=int, default=0,
+                    help="hybrid: 1 = filter runs in contribution order (ContributionScheduler), 0 = partition order")
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: RMAT scales of the graphs")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--serve-baseline", type=int, default=60, help="serve: queries timed one at a time")

@@ -73,6 +73,7 @@ class RunOptions(C.Structure):
         ("lane_form", C.c_uint32),
         ("tolerance", C.c_float),
         ("host_rounds", C.c_uint32),
+        ("task_order", C.c_uint32),
     ]
 
 
@@ -137,7 +138,7 @@ class HybridStats(C.Structure):
         ("tasks", C.c_uint64), ("bytes_filter", C.c_uint64), ("bytes_compaction", C.c_uint64),
         ("bytes_zero_copy", C.c_uint64), ("copy_ms", C.c_double), ("pack_ms", C.c_double), ("plan_ms", C.c_double),
         ("link_gbs_peak", C.c_double), ("cpu_pack_gbs", C.c_double), ("zero_copy_gbs", C.c_double),
-        ("device_edge_bytes", C.c_uint64), ("device_state_bytes", C.c_uint64),
+        ("device_edge_bytes", C.c_uint64), ("device_state_bytes", C.c_uint64), ("runs_reordered", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -430,9 +431,9 @@ HYT_LANES_ITEMS = 2
 
 def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
              light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0,
-             lane_form=HYT_LANES_AUTO, tolerance=0.0, host_rounds=0) -> RunOptions:
+             lane_form=HYT_LANES_AUTO, tolerance=0.0, host_rounds=0, task_order=0) -> RunOptions:
     return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk,
-                      direction, pull_percent, delta, lane_form, tolerance, host_rounds)
+                      direction, pull_percent, delta, lane_form, tolerance, host_rounds, task_order)
 
 
 class DeviceGraph:

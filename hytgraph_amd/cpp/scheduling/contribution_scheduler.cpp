@@ -2,6 +2,8 @@
 // src/scheduling/contribution_scheduler.cpp:23-246.
 #include "scheduling/contribution_scheduler.hpp"
 
+#include "contribution_order.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -51,22 +53,14 @@ namespace hytgraph::scheduling
         for (const auto &p : priorities)
             if (std::abs(p.contribution) <= options_.zero_contribution_epsilon)
                 ++plan.metrics.zero_contribution_count;
-        std::vector<std::size_t> order(priorities.size());
-        std::iota(order.begin(), order.end(), std::size_t{0});
-        const bool ties_by_index = options_.deterministic_tie_break;
-        std::stable_sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
-            const double ca = priorities[a].contribution, cb = priorities[b].contribution;
-            if (ca != cb)
-                return ca > cb;
-            return ties_by_index && priorities[a].item_index < priorities[b].item_index;
-        });
+        std::vector<hyt::ContributionItem> items(priorities.size());
+        for (std::size_t i = 0; i < items.size(); ++i)
+            items[i] = hyt::ContributionItem{priorities[i].item_index, priorities[i].contribution};
+        const std::vector<std::size_t> order =
+            hyt::contribution_order(items, options_.deterministic_tie_break, &plan.metrics.reordered_item_count);
         plan.ordered_item_indices.reserve(order.size());
-        for (std::size_t pos = 0; pos < order.size(); ++pos)
-        {
-            plan.ordered_item_indices.push_back(priorities[order[pos]].item_index);
-            if (order[pos] != pos)
-                ++plan.metrics.reordered_item_count;
-        }
+        for (const std::size_t pos : order)
+            plan.ordered_item_indices.push_back(priorities[pos].item_index);
         plan.metrics.scheduling_overhead = estimate_scheduling_overhead(priorities);
         return plan;
     }

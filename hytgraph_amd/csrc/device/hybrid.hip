@@ -32,6 +32,7 @@
 #include "hybrid_lanes.hpp"
 #include "lanes.hpp"
 #include "ops.hpp"
+#include "contribution_order.hpp"
 #include "transfer_plan.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -54,14 +55,29 @@ namespace hyt
             return std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
         }
 
+        // Partition contributions computed beside the activity counts (the
+        // priorities of the contribution order): none; SSSP - the smallest
+        // frontier distance of the partition (value bits, u32 or float >= 0,
+        // whose unsigned order is the value order); PageRank - the mass the
+        // partition pushes, sum of share x out-degree (double).
+        enum PrioMode : int
+        {
+            kPrioNone = 0,
+            kPrioMinValue = 1,
+            kPrioPushedMass = 2
+        };
+
         // Per-partition activity counts from the sorted frontier: rows, active
         // edges, zero-copy requests, alignment extras (u64 each, 4 per
-        // partition). Wavefronts whose rows share one partition add once.
+        // partition), and the partition's contribution (prio, per PrioMode,
+        // from the rows' queue values qd). Wavefronts whose rows share one
+        // partition add once.
         template <class OffT>
         __global__ __launch_bounds__(kBlock) void part_stats(const std::uint32_t *q, std::uint32_t F, const OffT *off,
                                                              const std::uint32_t *part_vbegin, std::uint32_t P,
                                                              std::uint64_t m, std::uint64_t d1, std::uint64_t align,
-                                                             unsigned long long *stats)
+                                                             unsigned long long *stats, const std::uint32_t *qd,
+                                                             int prio_mode, unsigned long long *prio)
         {
             const std::uint32_t i = blockIdx.x * kBlock + threadIdx.x;
             const bool valid = i < F;
@@ -92,6 +108,39 @@ namespace hyt
             }
             const std::uint32_t p0 = __shfl(p, 0, kWave);
             const bool same = __ballot(valid && p != p0) == 0ULL;
+            if (prio_mode == kPrioMinValue)
+            {
+                const unsigned long long v = valid ? qd[i] : ~0ULL;
+                if (same)
+                {
+                    unsigned long long mn = v;
+#pragma unroll
+                    for (int d = kWave / 2; d > 0; d >>= 1)
+                    {
+                        const unsigned long long o = __shfl_xor(mn, d, kWave);
+                        mn = o < mn ? o : mn;
+                    }
+                    if (lane_id() == 0 && mn != ~0ULL)
+                        atomicMin(&prio[p0], mn);
+                }
+                else if (valid)
+                    atomicMin(&prio[p], v);
+            }
+            else if (prio_mode == kPrioPushedMass)
+            {
+                double v = valid ? static_cast<double>(__uint_as_float(qd[i])) * static_cast<double>(edges) : 0.0;
+                double *dp = reinterpret_cast<double *>(prio);
+                if (same)
+                {
+#pragma unroll
+                    for (int d = kWave / 2; d > 0; d >>= 1)
+                        v += __shfl_xor(v, d, kWave);
+                    if (lane_id() == 0 && v != 0.0)
+                        atomicAdd(&dp[p0], v);
+                }
+                else if (valid && v != 0.0)
+                    atomicAdd(&dp[p], v);
+            }
             if (same)
             {
 #pragma unroll
@@ -190,6 +239,11 @@ namespace hyt
         std::vector<hyt_partition> parts;
         DeviceBuffer part_vbegin, part_stats_d, block_counts, block_offsets, scan_temp, stage_q, stage_val;
         std::vector<unsigned long long> part_stats_h;
+        // contribution order of the current run (PrioMode) and the partition
+        // contributions of the current round
+        int prio_mode = kPrioNone;
+        DeviceBuffer part_prio_d;
+        std::vector<unsigned long long> part_prio_h;
         // edge area
         hyt_hybrid_config cfg{};
         hytm::Params params;
@@ -337,8 +391,11 @@ namespace hyt
         }
 
         // Weights move only for runs that read them.
-        void begin_run(bool weights)
+        // prio: the contribution order of a single-query run (kPrioNone: the
+        // round's filter runs go in partition order)
+        void begin_run(bool weights, int prio = kPrioNone)
         {
+            prio_mode = prio;
             need_w = weights && host->weighted;
             run_edge_bytes = need_w ? 8 : 4;
             run_params = params;
@@ -631,6 +688,8 @@ namespace hyt
             hip_check(hipMemcpy(E.part_vbegin.ptr, vb.data(), (P + 1) * 4, hipMemcpyHostToDevice), "copy partitions");
             E.part_stats_d.alloc(P * 4 * 8, "hipMalloc(partition stats)");
             E.part_stats_h.assign(P * 4, 0);
+            E.part_prio_d.alloc(P * 8, "hipMalloc(partition contributions)");
+            E.part_prio_h.assign(P, 0);
             E.res.assign(P, Resident{});
             E.last_active.assign(P, 0);
 
@@ -740,14 +799,20 @@ namespace hyt
             hipStream_t s = g.stream;
             const std::uint64_t P = E.parts.size();
             hip_check(hipMemsetAsync(E.part_stats_d.ptr, 0, P * 32, s), "clear partition stats");
+            if (E.prio_mode != kPrioNone)
+                hip_check(hipMemsetAsync(E.part_prio_d.ptr, E.prio_mode == kPrioMinValue ? 0xFF : 0, P * 8, s),
+                          "clear partition contributions");
             part_stats<OffT><<<blocks_for(F), kBlock, 0, s>>>(
                 g.ws.q.as<std::uint32_t>(), F, g.off.as<OffT>(), E.part_vbegin.as<std::uint32_t>(),
                 static_cast<std::uint32_t>(P), E.run_params.request_bytes, E.run_params.entry_bytes,
-                E.run_params.alignment_bytes,
-                E.part_stats_d.as<unsigned long long>());
+                E.run_params.alignment_bytes, E.part_stats_d.as<unsigned long long>(), g.ws.qd.as<std::uint32_t>(),
+                E.prio_mode, E.part_prio_d.as<unsigned long long>());
             hip_check(hipGetLastError(), "launch part_stats");
             hip_check(hipMemcpyAsync(E.part_stats_h.data(), E.part_stats_d.ptr, P * 32, hipMemcpyDeviceToHost, s),
                       "read partition stats");
+            if (E.prio_mode != kPrioNone)
+                hip_check(hipMemcpyAsync(E.part_prio_h.data(), E.part_prio_d.ptr, P * 8, hipMemcpyDeviceToHost, s),
+                          "read partition contributions");
             hip_check(hipStreamSynchronize(s), "partition stats");
             plan.qlo.assign(P, 0);
             plan.qn.assign(P, 0);
@@ -970,8 +1035,55 @@ namespace hyt
                 if (t.engine == HYT_ENGINE_FILTER)
                     filter_parts.insert(filter_parts.end(), t.members.begin(), t.members.end());
             std::sort(filter_parts.begin(), filter_parts.end());
+            // the runs in partition order, or in contribution order (the
+            // original's ContributionScheduler ordering over the runs' device-
+            // computed contributions: SSSP -min frontier distance of the run,
+            // PageRank the mass it pushes)
+            std::vector<std::pair<std::uint64_t, std::uint64_t>> runs;
+            for_runs(filter_parts, [&](std::uint64_t p0, std::uint64_t p1) { runs.emplace_back(p0, p1); });
+            if (E.prio_mode != kPrioNone && runs.size() > 1)
             {
-                for_runs(filter_parts, [&](std::uint64_t p0, std::uint64_t p1) {
+                std::vector<ContributionItem> items(runs.size());
+                for (std::size_t i = 0; i < runs.size(); ++i)
+                {
+                    double c = 0.0;
+                    if (E.prio_mode == kPrioMinValue)
+                    {
+                        unsigned long long mn = ~0ULL;
+                        for (std::uint64_t p = runs[i].first; p <= runs[i].second; ++p)
+                            mn = std::min(mn, E.part_prio_h[p]);
+                        std::uint32_t bits = static_cast<std::uint32_t>(mn);
+                        if (E.weights_f32)
+                        {
+                            float f;
+                            std::memcpy(&f, &bits, 4);
+                            c = -static_cast<double>(f);
+                        }
+                        else
+                            c = -static_cast<double>(bits);
+                    }
+                    else
+                        for (std::uint64_t p = runs[i].first; p <= runs[i].second; ++p)
+                        {
+                            double v;
+                            std::memcpy(&v, &E.part_prio_h[p], 8);
+                            c += v;
+                        }
+                    items[i] = ContributionItem{static_cast<std::size_t>(runs[i].first), c};
+                }
+                std::size_t moved = 0;
+                const std::vector<std::size_t> order = contribution_order(items, true, &moved);
+                std::vector<std::pair<std::uint64_t, std::uint64_t>> sorted;
+                sorted.reserve(runs.size());
+                for (const std::size_t k : order)
+                    sorted.push_back(runs[k]);
+                runs.swap(sorted);
+                st.runs_reordered += moved;
+            }
+            {
+                for (const auto &run : runs)
+                {
+                    const std::uint64_t p0 = run.first, p1 = run.second;
                     st.parts_filter += p1 - p0 + 1;
                     const std::uint64_t e0 = E.parts[p0].edge_begin, e1 = E.parts[p1].edge_end;
                     std::uint64_t pa = p0;
@@ -1010,7 +1122,7 @@ namespace hyt
                             }
                         E.ring_release(r);
                     }
-                });
+                }
             }
 
             // 4. zero-copy: every partition of the zero-copy task in one launch
@@ -1771,6 +1883,7 @@ namespace hyt
         st.zero_copy_gbs = E.zc_gbs;
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
+        require(o.task_order == 0, "K-query runs keep partition order (task_order 0)");
         E.begin_run(weights);
         E.set_split(o, kLaneLightMaxDegree, kLaneHeavyChunk);
         try
@@ -1887,7 +2000,8 @@ int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_r
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
         const int wm = E.dg.weight_mode();
-        E.begin_run(true);
+        require(o.task_order <= 1, "task_order must be 0 or 1");
+        E.begin_run(true, o.task_order ? kPrioMinValue : kPrioNone);
         E.set_split(o);
         auto run = [&](auto off_tag) {
             using OffT = decltype(off_tag);
@@ -1941,7 +2055,8 @@ int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ran
         st.device_state_bytes = E.state_bytes();
         hip_check(hipSetDevice(E.dg.device), "hipSetDevice");
         const auto t0 = std::chrono::steady_clock::now();
-        E.begin_run(false);
+        require(o.task_order <= 1, "task_order must be 0 or 1");
+        E.begin_run(false, o.task_order ? kPrioPushedMass : kPrioNone);
         E.set_split(o);
         if (E.dg.off64)
             hybrid_pagerank_typed<std::uint64_t>(E, damping, epsilon, ranks, o, st);

@@ -182,6 +182,12 @@ typedef struct hyt_run_options {
                                   captured rounds with device-side round control
                                   (one read-back per 4 rounds); 1 = one host
                                   read-back and launch sequence per round */
+    uint32_t task_order;       /* hybrid single-query runs: 0 = filter runs in
+                                  partition order; 1 = by contribution (the
+                                  original's ContributionScheduler order over
+                                  partition contributions computed on the
+                                  device: SSSP -min frontier distance, PageRank
+                                  the mass the partition pushes) */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */
@@ -391,6 +397,7 @@ typedef struct hyt_hybrid_stats {
     double zero_copy_gbs;         /* measured zero-copy kernel read bandwidth */
     uint64_t device_edge_bytes;   /* HBM used for edge data (<= budget) */
     uint64_t device_state_bytes;  /* HBM used for offsets and vertex state */
+    uint64_t runs_reordered;      /* filter runs moved by the contribution order */
 } hyt_hybrid_stats;
 
 int hyt_hybrid_config_default(hyt_hybrid_config *out);

@@ -120,6 +120,30 @@ def test_hybrid_erdos_renyi(gpu, abi, orc, engine):
         assert np.array_equal(rk[:, k], g.ppr(int(s), D, 1e-8)[0]), k
 
 
+@pytest.mark.parametrize("cache", [0, 1])
+def test_hybrid_contribution_order(gpu, abi, orc, cache):
+    # filter runs ordered by device-computed partition contributions (the
+    # original ContributionScheduler order) against partition order: same
+    # SSSP distances and PageRank words, runs actually reordered
+    hg = pinned(abi, scale=14, seed=64)
+    h = abi.HybridEngine(hg, force_engine=0, cache=cache, **SMALL)
+    for src in abi.sample_sources(hg.offsets, 65, 2):
+        d0, s0 = h.sssp(int(src))
+        d1, s1 = h.sssp(int(src), task_order=1)
+        ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+        assert np.array_equal(d0, ref) and np.array_equal(d1, ref)
+        assert s0["runs_reordered"] == 0 and s1["runs_reordered"] > 0
+        _say(f"contribution order (cache {cache}) src {src}: edges relaxed {s1['run']['edges_total']} vs "
+             f"{s0['run']['edges_total']} in partition order, runs reordered {s1['runs_reordered']}")
+    hu = pinned(abi, scale=14, seed=66, wmax=0)
+    hp = abi.HybridEngine(hu, force_engine=0, cache=cache, **SMALL)
+    r0, p0 = hp.pagerank(D, 1e-7)
+    r1, p1 = hp.pagerank(D, 1e-7, task_order=1)
+    assert np.array_equal(r0, r1) and p1["runs_reordered"] > 0
+    with pytest.raises(abi.InvalidArgument):
+        hp.ppr_k([1, 2], D, 1e-6, task_order=1)
+
+
 def test_hybrid_reuse_and_errors(gpu, abi, orc):
     hg = pinned(abi, scale=12, seed=8)
     h = abi.HybridEngine(hg, **SMALL)
