@@ -7,11 +7,18 @@
 // rows sorted by destination. Weight of the edge at final CSR position p is
 // a pure function of (seed, p). Vertex labels are not scrambled.
 //
-// Layout of the work: edges are bucketed by the top bits of the source
-// (counted, then scattered, generating each edge twice instead of storing it
-// twice), each bucket is sorted and deduplicated independently, and the
-// buckets write disjoint ranges of the CSR. Every phase is parallel over
-// buckets or edge ranges.
+// Layout of the work (the peak is the CSR plus a u32 per vertex: no edge
+// list is held beside the CSR, so an RMAT scale-30 graph, 17 B generated
+// edges, fits in about 150 GB):
+//   1. count each source's generated edges (atomic u32 per vertex);
+//   2. prefix sums give every row a range of the column array, sized for its
+//      generated edges; the edges are generated again and each scattered into
+//      its row (a per-vertex cursor);
+//   3. each row is sorted and deduplicated in place, in parallel;
+//   4. the rows are packed to the left in ascending order (each row only
+//      moves to lower positions), the offsets rewritten;
+//   5. weights are drawn for the final positions into their own array.
+// Generating each edge twice costs less than storing it twice.
 #include "common.hpp"
 #include "host_graph.hpp"
 
@@ -153,96 +160,78 @@ namespace hyt
         const std::uint64_t n_gen = static_cast<std::uint64_t>(p.edge_factor) * V;
         const EdgeSource es(p);
 
-        // Buckets by the top bits of the source vertex.
-        const std::uint32_t bbits = std::min<std::uint32_t>(p.scale, 16);
-        const std::uint32_t bshift = p.scale - bbits;
-        const std::uint64_t nb = 1ULL << bbits;
-
-        // Pass 1: per-thread bucket counts.
-        std::vector<std::vector<std::uint64_t>> cnt(T, std::vector<std::uint64_t>(nb, 0));
-        parallel_for(T, n_gen, [&](int t, std::uint64_t lo, std::uint64_t hi) {
-            auto &c = cnt[t];
+        // 1. generated edges per source (no self-loops)
+        std::unique_ptr<std::uint32_t[]> cnt(new std::uint32_t[V]());
+        parallel_for(T, n_gen, [&](int, std::uint64_t lo, std::uint64_t hi) {
             for (std::uint64_t i = lo; i < hi; ++i)
             {
                 std::uint32_t s, d;
                 if (es.edge(i, s, d))
-                    ++c[s >> bshift];
+                    __atomic_fetch_add(&cnt[s], 1U, __ATOMIC_RELAXED);
             }
         });
-        // Bucket starts and per-thread cursors (bucket-major, thread-minor:
-        // thread t's edges of bucket b precede thread t+1's).
-        std::vector<std::uint64_t> bstart(nb + 1, 0);
-        std::vector<std::vector<std::uint64_t>> cur(T, std::vector<std::uint64_t>(nb, 0));
+
+        // 2. row ranges for the generated edges, then the scatter
+        auto g = std::make_unique<HostGraph>(V, 0, false, p.memory);
+        std::uint64_t run = 0;
+        for (std::uint64_t v = 0; v < V; ++v)
         {
-            std::uint64_t run = 0;
-            for (std::uint64_t b = 0; b < nb; ++b)
-            {
-                bstart[b] = run;
-                for (int t = 0; t < T; ++t)
-                {
-                    cur[t][b] = run;
-                    run += cnt[t][b];
-                }
-            }
-            bstart[nb] = run;
+            g->off[v] = run;
+            run += cnt[v];
         }
-        cnt.clear();
-        const std::uint64_t n_kept = bstart[nb];
-        std::vector<std::uint64_t> keys(n_kept);
-
-        // Pass 2: regenerate and scatter keys (src << 32 | dst).
-        parallel_for(T, n_gen, [&](int t, std::uint64_t lo, std::uint64_t hi) {
-            auto &c = cur[t];
+        g->off[V] = run;
+        const std::uint64_t n_raw = run;
+        host_free(g->col, p.memory);
+        g->col = nullptr;
+        g->col = static_cast<std::uint32_t *>(host_alloc(n_raw * sizeof(std::uint32_t), p.memory));
+        parallel_for(T, n_gen, [&](int, std::uint64_t lo, std::uint64_t hi) {
             for (std::uint64_t i = lo; i < hi; ++i)
             {
                 std::uint32_t s, d;
                 if (es.edge(i, s, d))
-                    keys[c[s >> bshift]++] = (static_cast<std::uint64_t>(s) << 32) | d;
-            }
-        });
-        cur.clear();
-
-        // Sort + dedupe each bucket in place.
-        std::vector<std::uint64_t> bunique(nb, 0);
-        parallel_dynamic(T, nb, [&](std::uint64_t b) {
-            auto first = keys.begin() + static_cast<std::ptrdiff_t>(bstart[b]);
-            auto last = keys.begin() + static_cast<std::ptrdiff_t>(bstart[b + 1]);
-            std::sort(first, last);
-            bunique[b] = static_cast<std::uint64_t>(std::unique(first, last) - first);
-        });
-        std::vector<std::uint64_t> bout(nb + 1, 0);
-        for (std::uint64_t b = 0; b < nb; ++b)
-            bout[b + 1] = bout[b] + bunique[b];
-        const std::uint64_t E = bout[nb];
-
-        auto g = std::make_unique<HostGraph>(V, E, p.weight_max != 0, p.memory);
-
-        // Each bucket writes its rows' offsets and columns.
-        parallel_dynamic(T, nb, [&](std::uint64_t b) {
-            const std::uint64_t vlo = b << bshift;
-            const std::uint64_t vhi = (b + 1) << bshift;
-            std::uint64_t pos = bout[b];
-            const std::uint64_t *k = keys.data() + bstart[b];
-            const std::uint64_t n = bunique[b];
-            std::uint64_t j = 0;
-            for (std::uint64_t v = vlo; v < vhi; ++v)
-            {
-                g->off[v] = pos;
-                while (j < n && (k[j] >> 32) == v)
                 {
-                    g->col[pos++] = static_cast<std::uint32_t>(k[j]);
-                    ++j;
+                    const std::uint32_t left = __atomic_sub_fetch(&cnt[s], 1U, __ATOMIC_RELAXED);
+                    g->col[g->off[s] + left] = d;
                 }
             }
         });
-        g->off[V] = E;
-        keys.clear();
-        keys.shrink_to_fit();
 
-        if (g->weighted)
+        // 3. sort + dedupe each row in place; cnt[v] = its unique edges
+        constexpr std::uint64_t kRowsPerTask = 4096;
+        parallel_dynamic(T, (V + kRowsPerTask - 1) / kRowsPerTask, [&](std::uint64_t task) {
+            const std::uint64_t v0 = task * kRowsPerTask, v1 = std::min(V, v0 + kRowsPerTask);
+            for (std::uint64_t v = v0; v < v1; ++v)
+            {
+                std::uint32_t *first = g->col + g->off[v];
+                std::uint32_t *last = g->col + g->off[v + 1];
+                std::sort(first, last);
+                cnt[v] = static_cast<std::uint32_t>(std::unique(first, last) - first);
+            }
+        });
+
+        // 4. pack the rows to the left in ascending order (a row's final start
+        // never exceeds its generated start)
+        std::uint64_t pos = 0;
+        for (std::uint64_t v = 0; v < V; ++v)
         {
+            const std::uint64_t from = g->off[v];
+            const std::uint64_t n = cnt[v];
+            if (pos != from && n != 0)
+                std::memmove(g->col + pos, g->col + from, n * sizeof(std::uint32_t));
+            g->off[v] = pos;
+            pos += n;
+        }
+        g->off[V] = pos;
+        g->E = pos;
+        cnt.reset();
+
+        // 5. weights of the final positions
+        if (p.weight_max != 0)
+        {
+            g->w = static_cast<std::uint32_t *>(host_alloc(g->E * sizeof(std::uint32_t), p.memory));
+            g->weighted = true;
             const std::uint64_t range = static_cast<std::uint64_t>(p.weight_max) - p.weight_min + 1;
-            parallel_for(T, E, [&](int, std::uint64_t lo, std::uint64_t hi) {
+            parallel_for(T, g->E, [&](int, std::uint64_t lo, std::uint64_t hi) {
                 for (std::uint64_t e = lo; e < hi; ++e)
                 {
                     std::uint64_t r = draw(p.seed, kStreamWeights, e, 0);
