@@ -457,7 +457,7 @@ def bench_serve(args) -> dict:
 
     scales = [int(x) for x in args.serve_scales.split(",")]
     host_scales = {int(x) for x in args.serve_host.split(",") if x}
-    graphs, hosts, pools, degs = [], [], [], []
+    graphs, hosts, pools = [], [], []
     for i, sc in enumerate(scales):
         t0 = time.perf_counter()
         on_host = sc in host_scales
@@ -471,7 +471,6 @@ def bench_serve(args) -> dict:
         graphs.append(g)
         hosts.append(hg)
         pools.append(abi.sample_sources(hg.offsets, args.seed + 100 + i, 4096))
-        degs.append(np.diff(hg.offsets.astype(np.int64)))
         log(f"graph {i}: RMAT scale {sc} V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s"
             + (f" (host memory, hybrid engines with {args.serve_edge_gb} GiB edge budget each)" if on_host else ""))
     core = _serve_core(abi, torch, graphs, hosts, scales, pools, args, host_scales, args.serve_requests,
