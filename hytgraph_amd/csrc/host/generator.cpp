@@ -81,8 +81,10 @@ namespace hyt
         {
             const hyt_gen_params &p;
             std::uint32_t thr_a, thr_ab, thr_abc;
+            std::uint64_t key; // the edge stream's key: draw(seed, edges, i, w) = mix(key + i c1 + w c2)
 
-            explicit EdgeSource(const hyt_gen_params &params) : p(params)
+            explicit EdgeSource(const hyt_gen_params &params)
+                : p(params), key(mix64(params.seed ^ mix64(kStreamEdges + 0x632BE59BD9B4E019ULL)))
             {
                 auto t = [](double x) {
                     double v = std::floor(x * 4294967296.0);
@@ -97,38 +99,36 @@ namespace hyt
                 thr_abc = t(p.a + p.b + p.c);
             }
 
-            // Returns false for a self-loop.
+            // Returns false for a self-loop. The words are those of
+            // draw(seed, kStreamEdges, i, w), with the stream key hoisted.
             bool edge(std::uint64_t i, std::uint32_t &src, std::uint32_t &dst) const
             {
+                const std::uint64_t base = key + i * 0xD1B54A32D192ED03ULL;
                 if (p.kind == HYT_GEN_ER)
                 {
-                    std::uint64_t r = draw(p.seed, kStreamEdges, i, 0);
+                    std::uint64_t r = mix64(base);
                     std::uint64_t V = 1ULL << p.scale;
                     src = static_cast<std::uint32_t>(bounded32(static_cast<std::uint32_t>(r), V));
                     dst = static_cast<std::uint32_t>(bounded32(static_cast<std::uint32_t>(r >> 32), V));
                 }
                 else
                 {
+                    // quadrant per level without branches: u < a -> (0,0),
+                    // a <= u < a+b -> (0,1), a+b <= u < a+b+c -> (1,0), else (1,1)
                     std::uint32_t s = 0, d = 0;
-                    std::uint64_t r = 0;
-                    for (std::uint32_t level = 0; level < p.scale; ++level)
+                    auto level = [&](std::uint32_t u, std::uint32_t bit) {
+                        const std::uint32_t ga = u >= thr_a, gab = u >= thr_ab, gabc = u >= thr_abc;
+                        s |= gab << bit;
+                        d |= (ga ^ gab ^ gabc) << bit;
+                    };
+                    std::uint32_t bit = p.scale - 1U;
+                    for (std::uint32_t w = 0; 2 * w < p.scale; ++w)
                     {
-                        if ((level & 1U) == 0U)
-                            r = draw(p.seed, kStreamEdges, i, level >> 1);
-                        std::uint32_t u = (level & 1U) ? static_cast<std::uint32_t>(r >> 32)
-                                                       : static_cast<std::uint32_t>(r);
-                        std::uint32_t sb, db;
-                        if (u < thr_a)
-                            sb = 0, db = 0;
-                        else if (u < thr_ab)
-                            sb = 0, db = 1;
-                        else if (u < thr_abc)
-                            sb = 1, db = 0;
-                        else
-                            sb = 1, db = 1;
-                        std::uint32_t bit = p.scale - 1U - level;
-                        s |= sb << bit;
-                        d |= db << bit;
+                        const std::uint64_t r = mix64(base + w * 0x8CB92BA72F3D8DD7ULL);
+                        level(static_cast<std::uint32_t>(r), bit);
+                        if (2 * w + 1 < p.scale)
+                            level(static_cast<std::uint32_t>(r >> 32), bit - 1U);
+                        bit -= 2U;
                     }
                     src = s;
                     dst = d;
