@@ -353,7 +353,10 @@ def _serve_core(abi, torch, graphs, hosts, scales, pools, args, host_scales, req
     nb = min(baseline, len(burst))
     dev_out = torch.empty(max(h.V for h in hosts), dtype=torch.int32, device="cuda")
     alone_sum = {}
+    tb = time.perf_counter()
     for i, (_, gi, kind, src) in enumerate(burst[:min(nb, 16)]):  # warm-up + checksums of the results
+        if i % 4 == 3:
+            log(f"alone checksums: {i + 1} queries in {time.perf_counter() - tb:.1f} s")
         if kind == abi.HYT_QUERY_SSSP:
             d = graphs[gi].sssp(src)
             fin = d != 0xFFFFFFFF
@@ -365,11 +368,14 @@ def _serve_core(abi, torch, graphs, hosts, scales, pools, args, host_scales, req
             alone_sum[i] = (int(nz.sum()), int(bits[nz].astype(np.uint64).sum()))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _, gi, kind, src in burst[:nb]:
+    for i, (_, gi, kind, src) in enumerate(burst[:nb]):
         if kind == abi.HYT_QUERY_SSSP:
             graphs[gi].sssp(src, dev_out.data_ptr())
         else:
             graphs[gi].ppr(src, eps, dev_out.data_ptr())
+        if i % 8 == 7:
+            torch.cuda.synchronize()
+            log(f"one at a time: {i + 1} queries in {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     base_s = time.perf_counter() - t0
     base_qps = nb / base_s
@@ -378,8 +384,11 @@ def _serve_core(abi, torch, graphs, hosts, scales, pools, args, host_scales, req
     sssp_only = [r for r in burst if r[2] == abi.HYT_QUERY_SSSP][:nb]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _, gi, _, src in sssp_only:
+    for i, (_, gi, _, src) in enumerate(sssp_only):
         graphs[gi].sssp(src, dev_out.data_ptr())
+        if i % 8 == 7:
+            torch.cuda.synchronize()
+            log(f"plain SSSP one at a time: {i + 1} queries in {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     sssp1_qps = len(sssp_only) / (time.perf_counter() - t0)
     log(f"one at a time: {nb} mixed queries in {base_s:.2f} s = {base_qps:.1f} queries/s; "

@@ -30,6 +30,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <future>
@@ -864,6 +866,10 @@ namespace hyt
             for (std::size_t i = 0; i < S; ++i)
                 lane_req[i].assign(ses[i]->K, -1);
             std::uint64_t next = 0, completed = 0;
+            // progress lines on stderr (HYT_SERVE_PROGRESS set): every
+            // twentieth of the requests
+            const bool progress = std::getenv("HYT_SERVE_PROGRESS") != nullptr;
+            const std::uint64_t progress_every = n >= 20 ? n / 20 : 1;
             const auto t0 = Clock::now();
             double busy_from = -1.0; // start of the current stretch with a round in flight
             while (completed < n)
@@ -901,6 +907,13 @@ namespace hyt
                             L.occupied &= ~(1ULL << lane);
                             lane_req[i][lane] = -1;
                             ++completed;
+                            if (progress && (completed % progress_every == 0 || completed == n))
+                            {
+                                std::fprintf(stderr, "[serve] %llu / %llu requests done at %.1f s\n",
+                                             static_cast<unsigned long long>(completed),
+                                             static_cast<unsigned long long>(n), secs(t0));
+                                std::fflush(stderr);
+                            }
                         }
                     }
                     // admission into free lanes, then the session's next round

@@ -25,6 +25,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -42,6 +45,22 @@ namespace hyt
                 hw = 1;
             return static_cast<int>(std::min(hw, 16U));
         }
+
+        // Progress lines on stderr for long generations (HYT_GEN_PROGRESS set):
+        // thread 0 reports each sixteenth of its share of a pass.
+        struct Progress
+        {
+            bool on = false;
+            std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+            void say(const char *what, double frac) const
+            {
+                if (!on)
+                    return;
+                const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                std::fprintf(stderr, "[generator] %s %3.0f%% at %.1f s\n", what, 100.0 * frac, s);
+                std::fflush(stderr);
+            }
+        };
 
         template <class Fn>
         void parallel_for(int threads, std::uint64_t n, Fn &&fn)
@@ -160,15 +179,31 @@ namespace hyt
         const std::uint64_t n_gen = static_cast<std::uint64_t>(p.edge_factor) * V;
         const EdgeSource es(p);
 
+        Progress pg;
+        pg.on = std::getenv("HYT_GEN_PROGRESS") != nullptr;
+        // thread 0 walks [lo, hi) in sixteenths, reporting after each
+        auto in_steps = [&](int t, std::uint64_t lo, std::uint64_t hi, const char *what, auto body) {
+            const std::uint64_t step = (hi - lo + 15) / 16;
+            for (std::uint64_t a = lo; a < hi; a += step)
+            {
+                const std::uint64_t b = std::min(hi, a + step);
+                body(a, b);
+                if (t == 0)
+                    pg.say(what, static_cast<double>(b - lo) / static_cast<double>(hi - lo));
+            }
+        };
+
         // 1. generated edges per source (no self-loops)
         std::unique_ptr<std::uint32_t[]> cnt(new std::uint32_t[V]());
-        parallel_for(T, n_gen, [&](int, std::uint64_t lo, std::uint64_t hi) {
-            for (std::uint64_t i = lo; i < hi; ++i)
-            {
-                std::uint32_t s, d;
-                if (es.edge(i, s, d))
-                    __atomic_fetch_add(&cnt[s], 1U, __ATOMIC_RELAXED);
-            }
+        parallel_for(T, n_gen, [&](int t, std::uint64_t lo, std::uint64_t hi) {
+            in_steps(t, lo, hi, "count", [&](std::uint64_t a, std::uint64_t b) {
+                for (std::uint64_t i = a; i < b; ++i)
+                {
+                    std::uint32_t s, d;
+                    if (es.edge(i, s, d))
+                        __atomic_fetch_add(&cnt[s], 1U, __ATOMIC_RELAXED);
+                }
+            });
         });
 
         // 2. row ranges for the generated edges, then the scatter
@@ -184,16 +219,18 @@ namespace hyt
         host_free(g->col, p.memory);
         g->col = nullptr;
         g->col = static_cast<std::uint32_t *>(host_alloc(n_raw * sizeof(std::uint32_t), p.memory));
-        parallel_for(T, n_gen, [&](int, std::uint64_t lo, std::uint64_t hi) {
-            for (std::uint64_t i = lo; i < hi; ++i)
-            {
-                std::uint32_t s, d;
-                if (es.edge(i, s, d))
+        parallel_for(T, n_gen, [&](int t, std::uint64_t lo, std::uint64_t hi) {
+            in_steps(t, lo, hi, "scatter", [&](std::uint64_t a, std::uint64_t b) {
+                for (std::uint64_t i = a; i < b; ++i)
                 {
-                    const std::uint32_t left = __atomic_sub_fetch(&cnt[s], 1U, __ATOMIC_RELAXED);
-                    g->col[g->off[s] + left] = d;
+                    std::uint32_t s, d;
+                    if (es.edge(i, s, d))
+                    {
+                        const std::uint32_t left = __atomic_sub_fetch(&cnt[s], 1U, __ATOMIC_RELAXED);
+                        g->col[g->off[s] + left] = d;
+                    }
                 }
-            }
+            });
         });
 
         // 3. sort + dedupe each row in place; cnt[v] = its unique edges
@@ -208,6 +245,7 @@ namespace hyt
                 cnt[v] = static_cast<std::uint32_t>(std::unique(first, last) - first);
             }
         });
+        pg.say("sort", 1.0);
 
         // 4. pack the rows to the left in ascending order (a row's final start
         // never exceeds its generated start)
@@ -224,6 +262,7 @@ namespace hyt
         g->off[V] = pos;
         g->E = pos;
         cnt.reset();
+        pg.say("pack", 1.0);
 
         // 5. weights of the final positions
         if (p.weight_max != 0)
