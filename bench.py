@@ -702,6 +702,18 @@ def bench_hybrid_batched(args) -> dict:
     }
 
 
+def bench_stream(args) -> dict:
+    """Reference kernel for HBM byte counters: a read-only streaming kernel
+    of the library over a buffer of known size (each launch reads exactly
+    that many bytes)."""
+    from hytgraph_amd import abi
+
+    n = args.copy_mib << 20
+    gbs = abi.stream_probe(n, 5)
+    return {"workload": "device stream read (hyt_stream_probe)", "bytes_read_per_launch": n, "launches": 6,
+            "GBps": gbs}
+
+
 def bench_copy(args) -> dict:
     """Reference kernel for HBM traffic counters: a plain device copy of a
     known size (reads N bytes, writes N bytes), so counter readings of the
@@ -725,7 +737,8 @@ def bench_copy(args) -> dict:
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["plain", "hybrid", "batched", "hybrid_batched", "serve", "copy"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "hybrid", "batched", "hybrid_batched", "serve", "copy", "stream"],
+                    default="plain")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -790,6 +803,9 @@ def main(argv=None) -> int:
         return 0
     if args.workload == "copy":
         print(json.dumps(bench_copy(args)), flush=True)
+        return 0
+    if args.workload == "stream":
+        print(json.dumps(bench_stream(args)), flush=True)
         return 0
     if hybrid:
         print(json.dumps(bench_hybrid(args)), flush=True)

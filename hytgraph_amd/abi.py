@@ -244,6 +244,7 @@ SIGNATURES = {
         C.c_int,
         [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
     ),
+    "hyt_stream_probe": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "hyt_stepper_create": (
         C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(RunOptions), C.POINTER(_P)]),
     "hyt_stepper_step": (C.c_int, [_P, C.c_uint64, C.POINTER(RunStats), C.POINTER(C.c_int32)]),
@@ -679,6 +680,13 @@ class HybridEngine:
             self.free()
         except Exception:
             pass
+
+
+def stream_probe(nbytes: int, reps: int = 5) -> float:
+    """GB/s of a read-only streaming kernel over nbytes of device memory."""
+    g = C.c_double()
+    check(load().hyt_stream_probe(nbytes, reps, C.byref(g)))
+    return g.value
 
 
 # ------------------------------------------------------------ round-by-round runs

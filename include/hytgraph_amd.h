@@ -259,6 +259,12 @@ int hyt_ppr(hyt_graph_t g, const uint32_t *seeds, uint32_t num_seeds,
             float damping, float epsilon, float *out,
             const hyt_run_options *opts, hyt_run_stats *stats);
 
+/* Measurement: `reps` launches of a read-only streaming kernel over a
+ * device buffer of `bytes` (allocated and freed by the call); gbs = the read
+ * rate. The reference for rocprofv3 byte counters: each launch reads
+ * exactly `bytes` from HBM. */
+int hyt_stream_probe(uint64_t bytes, uint32_t reps, double *gbs);
+
 /* ------------------------------------------------ round-by-round runs */
 /*
  * A single-source SSSP run (hyt_sssp with one source) or a PageRank / PPR
