@@ -870,6 +870,7 @@ def main(argv=None) -> int:
     F = sum(s["frontier_total"] for s in acc)
     R = sum(s["edges_total"] for s in acc)
     U = sum(s["updates_total"] for s in acc)
+    H = sum(s["heavy_edges"] for s in acc)
     exp_ms = sum(s["expand_ms"] for s in acc)
     launches = sum(s["expand_launches"] for s in acc)
     rounds = sum(s["rounds"] for s in acc)
@@ -877,8 +878,9 @@ def main(argv=None) -> int:
     alg_bytes = sssp_bytes(acc, hg.V)
     achieved = alg_bytes / (exp_ms / 1e3) / 1e9 if exp_ms > 0 else 0.0
 
-    # HBM traffic per relaxation round from the rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs of this workload (scaled by a plain copy's readings)
+    # L2-to-fabric traffic per relaxation round from the rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs of this workload (scaled by what the
+    # counters read for a streaming kernel of known size)
     traffic = None
     tfile = ROOT / "profiles" / "sssp_rmat24_traffic.json"
     if (tfile.exists() and args.scale == 24 and args.edge_factor == 16 and args.mode == "async" and pull_rounds == 0
@@ -922,10 +924,15 @@ def main(argv=None) -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "traffic_GBs": round(traffic / (exp_ms / max(launches, 1) / 1e3) / 1e9, 1) if traffic else None,
-            "traffic_note": "bytes moved between L2 and memory per round (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE "
-                            "x 1.0, factors from a 1 GiB device copy); traffic_GBs = traffic / avg round time",
+            "traffic_note": "L2-to-fabric bytes per round (rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors "
+                            "from the library's 1 GiB stream_read / fill kernels); MALL hits included, so not "
+                            "HBM-only; traffic_GBs = traffic / avg round time",
             "kernel": "SSSP relaxation (expand_light+expand_heavy push / pull_light+pull_heavy)",
             "algorithmic_bytes_per_launch": alg_bytes / max(launches, 1),
+            # the part of a round's bytes that the chunk kernel moves (12 per
+            # edge walked as a chunk of a row above light_max); the light kernel
+            # moves the rest
+            "chunk_kernel_bytes_per_launch": 12 * H / max(launches, 1),
             "avg_launch_ms": exp_ms / max(launches, 1),
             "launches": launches,
         },
@@ -933,6 +940,7 @@ def main(argv=None) -> int:
             "queries_per_s": len(sources) * args.steps / elapsed,
             "rounds_per_query": rounds / max(len(acc), 1),
             "edges_relaxed_per_query": R / max(len(acc), 1),
+            "chunk_edges_per_query": H / max(len(acc), 1),
             "m_reach_per_query": sum(m_reach) / len(m_reach),
             "expand_ms_per_query": exp_ms / max(len(acc), 1),
             "sync_rounds": sync_rounds,

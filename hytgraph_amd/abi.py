@@ -93,6 +93,7 @@ class RunStats(C.Structure):
         ("pull_active_edges", C.c_uint64),
         ("pull_edges", C.c_uint64),
         ("pull_frontier", C.c_uint64),
+        ("heavy_edges", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:
@@ -245,6 +246,11 @@ SIGNATURES = {
         [_P, C.c_uint32, _P, C.POINTER(RunOptions), C.POINTER(RunStats)],
     ),
     "hyt_stream_probe": (C.c_int, [C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
+    "hyt_graph_download": (C.c_int, [_P, _P, _P, _P]),
+    "hyt_hub_sort": (C.c_int, [_P, C.c_double, _P, _P, _u64p, C.c_uint32]),
+    "hyt_graph_reorder": (C.c_int, [_P, _P, C.c_uint32]),
+    "hyt_pagerank_contributions": (C.c_int, [_P, _P, C.c_float, _P, C.c_uint32]),
+    "hyt_sssp_contributions": (C.c_int, [_P, _P, C.c_uint64, C.c_float, _P, C.c_uint32]),
     "hyt_stepper_create": (
         C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.POINTER(RunOptions), C.POINTER(_P)]),
     "hyt_stepper_step": (C.c_int, [_P, C.c_uint64, C.POINTER(RunStats), C.POINTER(C.c_int32)]),
@@ -451,6 +457,7 @@ class DeviceGraph:
         self.V = len(offsets) - 1
         self.E = int(offsets[-1]) if self.V >= 0 and len(offsets) else 0
         self.weights_f32 = weights_f32
+        self.weighted = weights is not None
         h = C.c_void_p()
         flags = HYT_GRAPH_WEIGHTS_F32 if weights_f32 else HYT_GRAPH_WEIGHTS_U32
         if off64:
@@ -485,6 +492,52 @@ class DeviceGraph:
             self.free()
         except Exception:
             pass
+
+    def download(self):
+        """(offsets u64[V+1], columns u32[E], weights u32[E] or None) of the device CSR."""
+        off = np.empty(self.V + 1, np.uint64)
+        col = np.empty(self.E, np.uint32)
+        w = np.empty(self.E, np.uint32) if self.weighted else None
+        check(load().hyt_graph_download(self._h, _ptr(off), _ptr(col), _ptr(w)))
+        return off, col, w
+
+    # -- vertex order -------------------------------------------------------
+    def hub_sort(self, hub_fraction=0.08, out_ptr=None, scores_ptr=None):
+        """hyt_hub_sort: (vertex_order u32[V], scores f64[V], hub_count); with
+        out_ptr (device u32[V], scores_ptr device f64[V] or None) the results
+        stay on the device and (None, None, hub_count) is returned."""
+        hubs = C.c_uint64()
+        if out_ptr is not None:
+            check(load().hyt_hub_sort(self._h, hub_fraction, C.c_void_p(out_ptr),
+                                      C.c_void_p(scores_ptr) if scores_ptr else None, C.byref(hubs), 1))
+            return None, None, hubs.value
+        order = np.empty(self.V, np.uint32)
+        scores = np.empty(self.V, np.float64)
+        check(load().hyt_hub_sort(self._h, hub_fraction, _ptr(order), _ptr(scores), C.byref(hubs), 0))
+        return order, scores, hubs.value
+
+    def reorder(self, vertex_order=None, order_ptr=None):
+        """hyt_graph_reorder (in place): vertex_order[new] = old, host array or
+        device address."""
+        if order_ptr is not None:
+            check(load().hyt_graph_reorder(self._h, C.c_void_p(order_ptr), 1))
+            return
+        order = np.ascontiguousarray(vertex_order, dtype=np.uint32)
+        if len(order) != self.V:
+            raise InvalidArgument(HYT_ERR_INVALID_ARGUMENT,
+                                  "CSRGraph::reorder_vertices: vertex_order must contain exactly one entry per vertex")
+        check(load().hyt_graph_reorder(self._h, _ptr(order), 0))
+
+    # -- contributions ------------------------------------------------------
+    def pagerank_contributions(self, current_ranks, damping=0.85):
+        """hyt_pagerank_contributions: f64[V] from f32[V] ranks (host arrays)."""
+        x = np.ascontiguousarray(current_ranks, dtype=np.float32)
+        if len(x) != self.V:
+            raise InvalidArgument(HYT_ERR_INVALID_ARGUMENT,
+                                  "PageRank current_ranks size must match graph vertex count")
+        out = np.empty(self.V, np.float64)
+        check(load().hyt_pagerank_contributions(self._h, _ptr(x), damping, _ptr(out), 0))
+        return out
 
     # -- SSSP ---------------------------------------------------------------
     def sssp(self, sources, mode=HYT_SSSP_SYNC, out=None, out_ptr=None, **opt):
@@ -687,6 +740,18 @@ def stream_probe(nbytes: int, reps: int = 5) -> float:
     g = C.c_double()
     check(load().hyt_stream_probe(nbytes, reps, C.byref(g)))
     return g.value
+
+
+def sssp_contributions(current, candidate, tolerance=0.0):
+    """hyt_sssp_contributions on the current device: f64[n] from two f32[n]
+    distance arrays (host arrays)."""
+    cur = np.ascontiguousarray(current, dtype=np.float32)
+    cand = np.ascontiguousarray(candidate, dtype=np.float32)
+    if len(cur) != len(cand):
+        raise InvalidArgument(HYT_ERR_INVALID_ARGUMENT, "SSSP contribution distance vectors must have equal size")
+    out = np.empty(len(cur), np.float64)
+    check(load().hyt_sssp_contributions(_ptr(cur), _ptr(cand), len(cur), tolerance, _ptr(out), 0))
+    return out
 
 
 # ------------------------------------------------------------ round-by-round runs

@@ -101,7 +101,8 @@ def build_cpp(debug: bool = False) -> Path:
     os.replace(tmp, out)
     for test_src in [*sorted(TESTS_CPP.glob("*.cpp")), *sorted(APPS.glob("*.cpp"))]:
         exe = LIBDIR / test_src.stem
-        cmd = [cxx, "-std=c++17", "-Wall", "-Wextra", *opt, f"-I{ROOT / 'include'}", str(test_src),
+        cmd = [cxx, "-std=c++17", "-Wall", "-Wextra", *opt, f"-I{ROOT / 'include'}", f"-I{CSRC / 'device'}",
+               str(test_src),
                f"-L{LIBDIR}", "-lhytgraph_cpp", "-lhytgraph_amd", "-Wl,-rpath,$ORIGIN", "-o", str(exe)]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:

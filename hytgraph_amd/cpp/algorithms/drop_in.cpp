@@ -147,4 +147,43 @@ namespace hytgraph::algorithms
         }
         return out;
     }
+
+    std::vector<PageRankContribution> pagerank_contributions_gpu(const graph::CSRGraph &graph,
+                                                                 const std::vector<float> &current,
+                                                                 const PageRankOptions &options)
+    {
+        validate_pagerank(graph, options);
+        const auto n = static_cast<std::size_t>(graph.num_vertices());
+        if (current.size() != n)
+            throw std::invalid_argument("PageRank current_ranks size must match graph vertex count");
+        require_device("pagerank_contributions_gpu");
+        ScopedGraph sg;
+        upload(graph, false, sg, "pagerank_contributions_gpu");
+        std::vector<double> c(n);
+        check(hyt_pagerank_contributions(sg.g, current.data(), options.damping_factor, c.data(), 0),
+              "pagerank_contributions_gpu");
+        std::vector<PageRankContribution> out(n);
+        for (std::size_t v = 0; v < n; ++v)
+            out[v] = PageRankContribution{v, c[v]};
+        return out;
+    }
+
+    std::vector<SSSPContribution> sssp_contributions_gpu(const std::vector<float> &current,
+                                                         const std::vector<float> &candidate, float tolerance)
+    {
+        if (!(tolerance >= 0.0F) || !std::isfinite(tolerance))
+            throw std::invalid_argument("SSSP contribution tolerance must be finite and non-negative");
+        if (current.size() != candidate.size())
+            throw std::invalid_argument("SSSP contribution distance vectors must have equal size");
+        std::vector<SSSPContribution> out(current.size());
+        if (current.empty())
+            return out;
+        require_device("sssp_contributions_gpu");
+        std::vector<double> c(current.size());
+        check(hyt_sssp_contributions(current.data(), candidate.data(), current.size(), tolerance, c.data(), 0),
+              "sssp_contributions_gpu");
+        for (std::size_t v = 0; v < c.size(); ++v)
+            out[v] = SSSPContribution{v, c[v]};
+        return out;
+    }
 } // namespace hytgraph::algorithms

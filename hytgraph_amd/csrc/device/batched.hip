@@ -118,7 +118,7 @@ namespace hyt
             ctr->heavy_count = 0;
             ctr->edges = 0;
             for (int i = 0; i < kEdgeSlots; ++i)
-                ctr->slot[i].v = 0;
+                ctr->slot[i].v = ctr->slot[i].h = 0;
             ctr->frontier_edges = 0;
         }
 

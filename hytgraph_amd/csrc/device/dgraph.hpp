@@ -48,7 +48,8 @@ namespace hyt
     constexpr int kEdgeSlots = 8;
     struct alignas(256) EdgeSlot
     {
-        unsigned long long v;
+        unsigned long long v; // rows walked whole or in windows
+        unsigned long long h; // chunks of rows above light_max (expand_heavy)
     };
 
     // Per-round counters. The bytes before `overflow` (kRoundCounterBytes)
@@ -70,7 +71,15 @@ namespace hyt
         {
             unsigned long long e = edges;
             for (int i = 0; i < kEdgeSlots; ++i)
-                e += slot[i].v;
+                e += slot[i].v + slot[i].h;
+            return e;
+        }
+        // the part of all_edges() that the chunk kernel walked
+        __host__ __device__ unsigned long long heavy_edges() const
+        {
+            unsigned long long e = 0;
+            for (int i = 0; i < kEdgeSlots; ++i)
+                e += slot[i].h;
             return e;
         }
     };
@@ -80,6 +89,11 @@ namespace hyt
     __device__ __forceinline__ void add_edges(Counters *ctr, unsigned long long n)
     {
         atomicAdd(&ctr->slot[blockIdx.x % kEdgeSlots].v, n);
+    }
+    // Adds n edges walked by the chunk kernel (counted apart, same slots).
+    __device__ __forceinline__ void add_heavy_edges(Counters *ctr, unsigned long long n)
+    {
+        atomicAdd(&ctr->slot[blockIdx.x % kEdgeSlots].h, n);
     }
 
     constexpr int kMaxGraphRounds = 8; // rounds per captured round graph, at most
@@ -94,6 +108,7 @@ namespace hyt
         std::uint32_t truncated; // stopped by the round limit with work left
         std::uint32_t pad;
         std::uint64_t rounds, frontier_total, edges_total, updates_total;
+        std::uint64_t heavy_edges_total; // of edges_total: walked by the chunk kernel
         // timed runs: device wall clock before and after the expand kernels
         // of each round of one replay (round r: t[2r], t[2r+1])
         std::uint64_t t[2 * kMaxGraphRounds];

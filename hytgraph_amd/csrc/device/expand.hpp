@@ -325,6 +325,6 @@ namespace hyt
             });
         }
         if (lane == 0 && wedges)
-            add_edges(a.ctr, wedges);
+            add_heavy_edges(a.ctr, wedges);
     }
 } // namespace hyt

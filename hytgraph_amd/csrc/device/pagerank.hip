@@ -252,6 +252,8 @@ namespace hyt
                 // a pull round scans every in-edge (its kernels count none)
                 const std::uint64_t round_edges = pull ? g.E : c.all_edges();
                 st.edges_total += round_edges;
+                if (!pull)
+                    st.heavy_edges += c.heavy_edges();
                 if (pull)
                 {
                     st.pull_edges += round_edges;

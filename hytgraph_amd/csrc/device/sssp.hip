@@ -288,7 +288,7 @@ namespace hyt
             ctr->pending = 0;
             ctr->pending_inv_max = 0;
             for (int i = 0; i < kEdgeSlots; ++i)
-                ctr->slot[i].v = 0;
+                ctr->slot[i].v = ctr->slot[i].h = 0;
         }
 
         // Device wall clock into rc->t[slot] (kernels of a stream run in
@@ -304,6 +304,7 @@ namespace hyt
             if (threadIdx.x != 0 || blockIdx.x != 0 || rc->F == 0)
                 return;
             rc->edges_total += ctr->all_edges();
+            rc->heavy_edges_total += ctr->heavy_edges();
             rc->updates_total += ctr->next_count;
         }
 
@@ -460,6 +461,7 @@ namespace hyt
             st.rounds = h_rc->rounds;
             st.frontier_total += h_rc->frontier_total;
             st.edges_total += h_rc->edges_total;
+            st.heavy_edges += h_rc->heavy_edges_total;
             st.updates_total += h_rc->updates_total;
             st.converged = h_rc->truncated ? 0 : 1;
         }
@@ -643,6 +645,8 @@ namespace hyt
                 // a pull round scans every in-edge (its kernels count none)
                 const std::uint64_t round_edges = pull ? g.E : c.all_edges();
                 st.edges_total += round_edges;
+                if (!pull)
+                    st.heavy_edges += c.heavy_edges();
                 if (pull)
                 {
                     st.pull_edges += round_edges;

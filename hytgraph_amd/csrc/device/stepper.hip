@@ -33,6 +33,7 @@ namespace
         t.pull_active_edges += r.pull_active_edges;
         t.pull_edges += r.pull_edges;
         t.pull_frontier += r.pull_frontier;
+        t.heavy_edges += r.heavy_edges;
         t.overflow |= r.overflow;
         t.converged = r.converged;
     }

@@ -48,6 +48,14 @@ namespace hytgraph::algorithms
     // within max_iterations rounds (0 = unlimited).
     PageRankResult pagerank_gpu(const graph::CSRGraph &graph, const PageRankOptions &options = {});
 
+    // pagerank_contributions computed on the GPU (addition): the same values,
+    // bit for bit (float sums in the original's order; options validated as
+    // the original validates them).
+    [[nodiscard]] std::vector<PageRankContribution>
+    pagerank_contributions_gpu(const graph::CSRGraph &graph,
+                               const std::vector<float> &current_ranks,
+                               const PageRankOptions &options = {});
+
     // The original's CPU Jacobi iteration (src/algorithms/pagerank.cpp:89-163),
     // declared for source compatibility only: this library replaces the GPU
     // path, and the definition comes from the original's CPU sources.

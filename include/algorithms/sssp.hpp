@@ -32,6 +32,12 @@ namespace hytgraph::algorithms
                        const std::vector<float> &candidate_distances,
                        float tolerance = 0.0F);
 
+    // sssp_contributions computed on the GPU (addition): the same values.
+    [[nodiscard]] std::vector<SSSPContribution>
+    sssp_contributions_gpu(const std::vector<float> &current_distances,
+                           const std::vector<float> &candidate_distances,
+                           float tolerance = 0.0F);
+
     struct SSSPResult
     {
         std::vector<float> distances;

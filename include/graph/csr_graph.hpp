@@ -55,4 +55,9 @@ namespace hytgraph::graph
         std::vector<vertex_id> column_indices_;
         std::vector<weight_type> edge_weights_;
     };
+
+    // CSRGraph::reorder_vertices computed on the GPU (addition): the same
+    // graph afterwards, and the same std::invalid_argument for an order that
+    // is not a permutation (the graph is then unchanged).
+    void reorder_vertices_gpu(CSRGraph &graph, const std::vector<CSRGraph::vertex_id> &vertex_order);
 } // namespace hytgraph::graph

@@ -143,6 +143,57 @@ int hyt_graph_trim(hyt_graph_t g);
 int hyt_graph_info(hyt_graph_t g, uint64_t *num_vertices, uint64_t *num_edges,
                    uint64_t *device_bytes);
 
+/* Copies the device CSR to the host: row_offsets u64[V+1], columns u32[E],
+ * weights u32[E] (NULL to skip; an unweighted graph has none). */
+int hyt_graph_download(hyt_graph_t g, uint64_t *row_offsets, uint32_t *columns, uint32_t *weights);
+
+/* ------------------------------------------------- vertex order (hubs) */
+/*
+ * HubSorter::sort of the original (src/scheduling/hub_sort.cpp:28-183) on the
+ * device graph: score[v] = outdeg(v) * indeg(v) / (max outdeg * max indeg),
+ * with the original's long double division (x87 extended precision, then
+ * double) reproduced bit for bit; the first hub_count = V * hub_fraction
+ * (rounded to the nearest integer when within 1e-12 of it, else up)
+ * vertices by (score descending, vertex ascending), then the others in
+ * vertex order. vertex_order: u32[V], scores: f64[V] or NULL, both host or
+ * both device memory (out_on_device). Needs V < 2^31 and max outdeg * max
+ * indeg < 2^64. hub_fraction outside [0, 1] or not finite:
+ * HYT_ERR_INVALID_ARGUMENT with the original's message.
+ */
+int hyt_hub_sort(hyt_graph_t g, double hub_fraction, uint32_t *vertex_order, double *scores,
+                 uint64_t *hub_count, uint32_t out_on_device);
+/*
+ * CSRGraph::reorder_vertices of the original (src/graph/csr_graph.cpp:
+ * 171-283) on the device graph, in place: vertex_order[new] = old (u32[V],
+ * host or device memory per order_on_device); new row `new` is old row
+ * `old` with each column c renamed to its new id, weights carried, edge
+ * order kept. An order that is not a permutation fails with the original's
+ * message for the first offending position (out-of-range id or duplicate)
+ * and leaves the graph unchanged. Drops the transpose and workspaces. Needs
+ * V < 2^31.
+ */
+int hyt_graph_reorder(hyt_graph_t g, const uint32_t *vertex_order, uint32_t order_on_device);
+
+/* -------------------------------------------------------- contributions */
+/*
+ * pagerank_contributions of the original (src/algorithms/pagerank.cpp:
+ * 165-243) on the device: contributions[v] = |next(v) - current[v]| for one
+ * synchronous PageRank step from current (float sums in the original's
+ * order, so the values equal the original's bit for bit). current: f32[V],
+ * contributions: f64[V], both host or both device memory (on_device).
+ * Builds the graph's transpose on first use (E < 2^31).
+ */
+int hyt_pagerank_contributions(hyt_graph_t g, const float *current_ranks, float damping,
+                               double *contributions, uint32_t on_device);
+/*
+ * sssp_contributions of the original (src/algorithms/sssp.cpp:236-290) on
+ * the current device: contributions[v] = current - candidate when both are
+ * finite and the gain exceeds tolerance, 1 when only the candidate is
+ * finite, else 0. Arrays of n elements, host or device memory (on_device).
+ */
+int hyt_sssp_contributions(const float *current, const float *candidate, uint64_t n, float tolerance,
+                           double *contributions, uint32_t on_device);
+
 /* ------------------------------------------------------------------ SSSP */
 enum hyt_sssp_mode {
     /* Synchronous rounds: a round relaxes the out-edges of the vertices
@@ -213,6 +264,8 @@ typedef struct hyt_run_stats {
     uint64_t pull_active_edges; /* pull rounds: in-edges whose source was active */
     uint64_t pull_edges;      /* pull rounds: in-edges scanned (part of edges_total) */
     uint64_t pull_frontier;   /* pull rounds: frontier rows (part of frontier_total) */
+    uint64_t heavy_edges;     /* push rounds: edges walked as chunks of rows above
+                                 light_max (part of edges_total) */
 } hyt_run_stats;
 
 /*

@@ -40,6 +40,10 @@ namespace hytgraph::scheduling
 
         [[nodiscard]] HubSortResult sort(const graph::CSRGraph &graph) const;
 
+        // The same result computed on the GPU (addition): scores bit for bit
+        // (the long double division is reproduced exactly), same order.
+        [[nodiscard]] HubSortResult sort_gpu(const graph::CSRGraph &graph) const;
+
     private:
         HubSortOptions options_;
     };

@@ -53,6 +53,10 @@ def _load_orc():
         lib.orc_pagerank_jacobi_f32.argtypes = [_u64, _P, _P, C.c_float, C.c_float, _u64, _P, _P, _P, _P]
         lib.orc_pagerank_f64.argtypes = [_u64, _P, _P, C.c_double, C.c_int64, C.c_double, _u64, _P, _P]
         lib.orc_delta_push_f64.argtypes = [_u64, _P, _P, C.c_double, C.c_double, C.c_int64, _u64, _P, _P, _P]
+        lib.orc_hub_sort.argtypes = [_u64, _P, _P, C.c_double, _P, _P, _P]
+        lib.orc_reorder.argtypes = [_u64, _P, _P, _P, _P, _P, _P, _P, _P]
+        lib.orc_pagerank_contributions.argtypes = [_u64, _P, _P, C.c_float, _P, _P]
+        lib.orc_sssp_contributions.argtypes = [_u64, _P, _P, C.c_float, _P]
         _orc = lib
     return _orc
 
@@ -72,6 +76,10 @@ def _load_ref():
         lib.ref_partition.argtypes = [_u64, _P, _P, _u64, _P, _P, _P, _P, _u64, _P]
         lib.ref_cost_evaluate.argtypes = [_u64, _P, _P, _P, _u64, _P, _P, _P, _P, _P, _u64, _P]
         lib.ref_combine.argtypes = [_P, _u64, _u64, _P, _P, _P, _P]
+        lib.ref_hub_sort.argtypes = [_u64, _P, _P, C.c_double, _P, _P, _P]
+        lib.ref_reorder.argtypes = [_u64, _P, _P, _P, _P, _P, _P, _P]
+        lib.ref_pagerank_contributions.argtypes = [_u64, _P, _P, C.c_float, _P, _P]
+        lib.ref_sssp_contributions.argtypes = [_u64, _P, _P, C.c_float, _P]
         _ref = lib
     return _ref
 
@@ -203,3 +211,56 @@ def ref_combine(engines, k=4):
     nt = C.c_uint64()
     _check(_load_ref().ref_combine(_p(e), n, k, _p(te), _p(to), _p(mem), C.byref(nt)), "ref_combine")
     return [(int(te[t]), mem[int(to[t]):int(to[t + 1])].tolist()) for t in range(nt.value)]
+
+
+def hub_sort(off, col, fraction=0.08, use_ref=False):
+    """HubSorter::sort: (vertex_order u32[V], scores f64[V], hub_count)."""
+    off, col = _arr(off, np.uint64), _arr(col, np.uint32)
+    V = len(off) - 1
+    order = np.empty(V, np.uint32)
+    scores = np.empty(V, np.float64)
+    hubs = C.c_uint64()
+    fn = _load_ref().ref_hub_sort if use_ref else _load_orc().orc_hub_sort
+    _check(fn(V, _p(off), _p(col), C.c_double(fraction), _p(order), _p(scores), C.byref(hubs)), "hub_sort")
+    return order, scores, hubs.value
+
+
+def reorder(off, col, w, order, use_ref=False):
+    """CSRGraph::reorder_vertices: (offsets u64[V+1], columns, weights or
+    None). w: u32 bit patterns (float bits for the compiled original) or None. OracleError with code 1 on an invalid
+    order; .kind = 1 (out of range) or 2 (duplicate) from the restatement."""
+    off, col, order = _arr(off, np.uint64), _arr(col, np.uint32), _arr(order, np.uint32)
+    w = _arr(w, np.uint32)
+    V = len(off) - 1
+    off2 = np.empty(V + 1, np.uint64)
+    col2 = np.empty(len(col), np.uint32)
+    w2 = None if w is None else np.empty(len(col), np.uint32)
+    if use_ref:
+        rc = _load_ref().ref_reorder(V, _p(off), _p(col), _p(w), _p(order), _p(off2), _p(col2), _p(w2))
+        _check(rc, "reorder")
+    else:
+        kind = C.c_int()
+        rc = _load_orc().orc_reorder(V, _p(off), _p(col), _p(w), _p(order), _p(off2), _p(col2), _p(w2),
+                                     C.byref(kind))
+        if rc != 0:
+            err = OracleError(rc, "reorder")
+            err.kind = kind.value
+            raise err
+    return off2, col2, w2
+
+
+def pagerank_contributions(off, col, x, damping=0.85, use_ref=False):
+    off, col, x = _arr(off, np.uint64), _arr(col, np.uint32), _arr(x, np.float32)
+    V = len(off) - 1
+    out = np.empty(V, np.float64)
+    fn = _load_ref().ref_pagerank_contributions if use_ref else _load_orc().orc_pagerank_contributions
+    _check(fn(V, _p(off), _p(col), C.c_float(damping), _p(x), _p(out)), "pagerank_contributions")
+    return out
+
+
+def sssp_contributions(cur, cand, tol=0.0, use_ref=False):
+    cur, cand = _arr(cur, np.float32), _arr(cand, np.float32)
+    out = np.empty(len(cur), np.float64)
+    fn = _load_ref().ref_sssp_contributions if use_ref else _load_orc().orc_sssp_contributions
+    _check(fn(len(cur), _p(cur), _p(cand), C.c_float(tol), _p(out)), "sssp_contributions")
+    return out

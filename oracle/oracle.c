@@ -24,6 +24,23 @@
  *                         procedure the GPU runs. Pinned by convergence to
  *                         orc_pagerank_f64 as eps -> 0 and by its mass
  *                         balance invariant.
+ *   orc_hub_sort          restates HubSorter::sort (src/scheduling/
+ *                         hub_sort.cpp:28-183). Pinned by the original's
+ *                         known answers (tests/unit_tests.cpp:2468-2602,
+ *                         2710-2750) and bit-exact against oracle/_ref.
+ *   orc_reorder           restates CSRGraph::reorder_vertices (src/graph/
+ *                         csr_graph.cpp:171-283). Pinned by the original's
+ *                         known answers (tests/unit_tests.cpp:2604-2708),
+ *                         against oracle/_ref, and by the invariant that
+ *                         SSSP distances permute with the vertex ids.
+ *   orc_pagerank_contributions restates pagerank_contributions (src/
+ *                         algorithms/pagerank.cpp:165-243). Bit-exact against
+ *                         oracle/_ref, and equal to |step - x| of
+ *                         orc_pagerank_jacobi_f32's first iteration from x =
+ *                         1/n (the same float step).
+ *   orc_sssp_contributions restates sssp_contributions (src/algorithms/
+ *                         sssp.cpp:236-290). Bit-exact against oracle/_ref
+ *                         and by hand-worked cases (inf, NaN, tolerance).
  */
 #include <math.h>
 #include <stdint.h>
@@ -405,5 +422,179 @@ int orc_delta_push_f64(uint64_t V, const uint64_t *off, const uint32_t *col, dou
     for (uint64_t v = 0; v < V; ++v)
         out[v] = s > 0.0 ? (rank[v] + res[v]) / s : 0.0;
     free(res), free(rank), free(snap), free(q);
+    return ORC_OK;
+}
+
+/* HubSorter::sort (src/scheduling/hub_sort.cpp:28-183): in-degrees by
+ * counting; score = outdeg * indeg / (max outdeg * max indeg) evaluated in
+ * long double (x87 extended on x86-64) and rounded to double, 0 when the
+ * denominator is 0; hub_count = V * fraction rounded to the nearest integer
+ * when within 1e-12 of it, else up, at most V; order = the first hub_count
+ * vertices by (score desc, id asc), then the rest in id order. */
+static const double *hub_score_key;
+
+static int hub_before(const void *pa, const void *pb)
+{
+    const uint32_t a = *(const uint32_t *)pa, b = *(const uint32_t *)pb;
+    const double sa = hub_score_key[a], sb = hub_score_key[b];
+    if (sa != sb)
+        return sa > sb ? -1 : 1;
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+int orc_hub_sort(uint64_t V, const uint64_t *off, const uint32_t *col, double fraction, uint32_t *order,
+                 double *scores, uint64_t *hub_count)
+{
+    if (!isfinite(fraction) || fraction < 0.0 || fraction > 1.0)
+        return ORC_EINVAL;
+    *hub_count = 0;
+    if (V == 0)
+        return ORC_OK;
+    const uint64_t E = off[V];
+    uint64_t *indeg = calloc(V, sizeof(uint64_t));
+    uint32_t *rank = malloc(V * sizeof(uint32_t));
+    unsigned char *hub = calloc(V, 1);
+    if (!indeg || !rank || !hub) {
+        free(indeg), free(rank), free(hub);
+        return ORC_ENOMEM;
+    }
+    for (uint64_t e = 0; e < E; ++e)
+        indeg[col[e]] += 1;
+    uint64_t max_out = 0, max_in = 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        const uint64_t d = off[v + 1] - off[v];
+        if (d > max_out)
+            max_out = d;
+        if (indeg[v] > max_in)
+            max_in = indeg[v];
+    }
+    const long double den = (long double)max_out * (long double)max_in;
+    for (uint64_t v = 0; v < V; ++v)
+        scores[v] = den > 0.0L ? (double)((long double)(off[v + 1] - off[v]) * (long double)indeg[v] / den) : 0.0;
+    const long double want = (long double)V * (long double)fraction;
+    const long double nearest = roundl(want);
+    const long double count = fabsl(want - nearest) < 1e-12L ? nearest : ceill(want);
+    uint64_t hubs = (uint64_t)count;
+    if (hubs > V)
+        hubs = V;
+    for (uint64_t v = 0; v < V; ++v)
+        rank[v] = (uint32_t)v;
+    hub_score_key = scores;
+    qsort(rank, V, sizeof(uint32_t), hub_before);
+    uint64_t at = 0;
+    for (uint64_t r = 0; r < hubs; ++r) {
+        hub[rank[r]] = 1;
+        order[at++] = rank[r];
+    }
+    for (uint64_t v = 0; v < V; ++v)
+        if (!hub[v])
+            order[at++] = (uint32_t)v;
+    *hub_count = hubs;
+    free(indeg), free(rank), free(hub);
+    return ORC_OK;
+}
+
+/* CSRGraph::reorder_vertices (src/graph/csr_graph.cpp:171-283): order[new] =
+ * old must be a permutation (the first position holding an id >= V or an id
+ * seen before decides the error: *error_kind 1 = out of range, 2 =
+ * duplicate); new row `new` is old row order[new] with columns renamed by
+ * the inverse permutation, weights (u32 bit patterns, may be NULL) carried
+ * in edge order. */
+int orc_reorder(uint64_t V, const uint64_t *off, const uint32_t *col, const uint32_t *w, const uint32_t *order,
+                uint64_t *off2, uint32_t *col2, uint32_t *w2, int *error_kind)
+{
+    *error_kind = 0;
+    unsigned char *seen = calloc(V ? V : 1, 1);
+    uint32_t *inv = malloc((V ? V : 1) * sizeof(uint32_t));
+    if (!seen || !inv) {
+        free(seen), free(inv);
+        return ORC_ENOMEM;
+    }
+    for (uint64_t i = 0; i < V; ++i) {
+        const uint32_t o = order[i];
+        if (o >= V) {
+            *error_kind = 1;
+            break;
+        }
+        if (seen[o]) {
+            *error_kind = 2;
+            break;
+        }
+        seen[o] = 1;
+        inv[o] = (uint32_t)i;
+    }
+    if (*error_kind) {
+        free(seen), free(inv);
+        return ORC_EINVAL;
+    }
+    uint64_t at = 0;
+    off2[0] = 0;
+    for (uint64_t n = 0; n < V; ++n) {
+        const uint32_t u = order[n];
+        for (uint64_t e = off[u]; e < off[u + 1]; ++e) {
+            col2[at] = inv[col[e]];
+            if (w)
+                w2[at] = w[e];
+            ++at;
+        }
+        off2[n + 1] = at;
+    }
+    free(seen), free(inv);
+    return ORC_OK;
+}
+
+/* pagerank_contributions (src/algorithms/pagerank.cpp:165-243): one
+ * synchronous step from x with the float arithmetic of pagerank_cpu (dangling
+ * mass in ascending vertex order, in-edge sums in incoming order); out[v] =
+ * |next(v) - x[v]| as double. */
+int orc_pagerank_contributions(uint64_t V, const uint64_t *off, const uint32_t *col, float damping, const float *x,
+                               double *out)
+{
+    if (V == 0 || !(damping >= 0.0f && damping < 1.0f))
+        return ORC_EINVAL;
+    uint64_t *io, *ie;
+    uint32_t *is;
+    if (build_incoming(V, off, col, &io, &is, &ie))
+        return ORC_ENOMEM;
+    const float n = (float)V;
+    const float base = (1.0f - damping) / n;
+    float dangling = 0.0f;
+    for (uint64_t u = 0; u < V; ++u)
+        if (off[u + 1] == off[u])
+            dangling += x[u];
+    const float share = damping * dangling / n;
+    for (uint64_t v = 0; v < V; ++v) {
+        float sum = 0.0f;
+        for (uint64_t p = io[v]; p < io[v + 1]; ++p) {
+            const uint32_t s = is[p];
+            const uint64_t deg = off[s + 1] - off[s];
+            if (deg != 0)
+                sum += x[s] / (float)deg;
+        }
+        const float next = base + share + damping * sum;
+        out[v] = (double)fabsf(next - x[v]);
+    }
+    free(io), free(is), free(ie);
+    return ORC_OK;
+}
+
+/* sssp_contributions (src/algorithms/sssp.cpp:236-290): current - candidate
+ * (float) when both are finite and the gain exceeds tol; 1 when only the
+ * candidate is finite; else 0. */
+int orc_sssp_contributions(uint64_t n, const float *cur, const float *cand, float tol, double *out)
+{
+    if (!(tol >= 0.0f) || !isfinite(tol))
+        return ORC_EINVAL;
+    for (uint64_t i = 0; i < n; ++i) {
+        double value = 0.0;
+        if (isfinite(cur[i]) && isfinite(cand[i])) {
+            const float gain = cur[i] - cand[i];
+            if (gain > tol)
+                value = (double)gain;
+        } else if (!isfinite(cur[i]) && isfinite(cand[i])) {
+            value = 1.0;
+        }
+        out[i] = value;
+    }
     return ORC_OK;
 }

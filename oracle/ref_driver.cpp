@@ -7,12 +7,17 @@
 // (src/algorithms/pagerank.cpp:89), via CSRGraph (src/graph/csr_graph.cpp:11);
 // LogicalPartitioner::partition (src/graph/partition.cpp:105),
 // HyTMCostModel::evaluate_partitions (src/transfer/hytm_cost_model.cpp:617),
-// TaskCombiner::combine (src/scheduling/task_combiner.cpp:67).
+// TaskCombiner::combine (src/scheduling/task_combiner.cpp:67);
+// HubSorter::sort (src/scheduling/hub_sort.cpp:28), CSRGraph::reorder_vertices
+// (src/graph/csr_graph.cpp:171), pagerank_contributions
+// (src/algorithms/pagerank.cpp:165), sssp_contributions
+// (src/algorithms/sssp.cpp:236).
 #include "algorithms/pagerank.hpp"
 #include "algorithms/sssp.hpp"
 #include "graph/activity_tracker.hpp"
 #include "graph/csr_graph.hpp"
 #include "graph/partition.hpp"
+#include "scheduling/hub_sort.hpp"
 #include "scheduling/task_combiner.hpp"
 #include "transfer/hytm_cost_model.hpp"
 
@@ -208,6 +213,84 @@ int ref_combine(const std::int32_t *engines, std::uint64_t n, std::uint64_t k, s
                 members[at++] = m;
         }
         task_offset[plan.tasks.size()] = at;
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+// HubSorter::sort: order u32[V], scores f64[V], hub count.
+int ref_hub_sort(std::uint64_t V, const std::uint64_t *off, const std::uint32_t *col, double fraction,
+                 std::uint32_t *order, double *scores, std::uint64_t *hub_count)
+{
+    try
+    {
+        const CSRGraph g = make_graph(V, off, col, nullptr);
+        hytgraph::scheduling::HubSortOptions o;
+        o.hub_fraction = fraction;
+        const auto r = hytgraph::scheduling::HubSorter(o).sort(g);
+        for (std::size_t i = 0; i < r.vertex_order.size(); ++i)
+            order[i] = r.vertex_order[i];
+        std::memcpy(scores, r.scores.data(), r.scores.size() * sizeof(double));
+        *hub_count = r.hub_count;
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+// CSRGraph::reorder_vertices on a copy: new offsets u64[V+1], columns, and
+// float weights (w may be NULL).
+int ref_reorder(std::uint64_t V, const std::uint64_t *off, const std::uint32_t *col, const float *w,
+                const std::uint32_t *order, std::uint64_t *off2, std::uint32_t *col2, float *w2)
+{
+    try
+    {
+        CSRGraph g = make_graph(V, off, col, w);
+        g.reorder_vertices(std::vector<CSRGraph::vertex_id>(order, order + V));
+        std::memcpy(off2, g.row_offsets().data(), (V + 1) * sizeof(std::uint64_t));
+        std::memcpy(col2, g.column_indices().data(), g.column_indices().size() * sizeof(std::uint32_t));
+        if (w)
+            std::memcpy(w2, g.edge_weights().data(), g.edge_weights().size() * sizeof(float));
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+int ref_pagerank_contributions(std::uint64_t V, const std::uint64_t *off, const std::uint32_t *col, float damping,
+                               const float *x, double *out)
+{
+    try
+    {
+        const CSRGraph g = make_graph(V, off, col, nullptr);
+        hytgraph::algorithms::PageRankOptions o;
+        o.damping_factor = damping;
+        const auto c = hytgraph::algorithms::pagerank_contributions(g, std::vector<float>(x, x + V), o);
+        for (std::size_t i = 0; i < c.size(); ++i)
+            out[c[i].vertex] = c[i].contribution;
+        return 0;
+    }
+    catch (...)
+    {
+        return code_of(std::current_exception());
+    }
+}
+
+int ref_sssp_contributions(std::uint64_t n, const float *cur, const float *cand, float tol, double *out)
+{
+    try
+    {
+        const auto c = hytgraph::algorithms::sssp_contributions(std::vector<float>(cur, cur + n),
+                                                                std::vector<float>(cand, cand + n), tol);
+        for (std::size_t i = 0; i < c.size(); ++i)
+            out[c[i].vertex] = c[i].contribution;
         return 0;
     }
     catch (...)

@@ -99,6 +99,18 @@ class HipRuntime:
         self._ok(self.lib.hipMemcpy(out.ctypes.data, self.C.c_void_p(ptr), n * 4, 2), "hipMemcpy")
         return out
 
+    def to_host(self, ptr, n, dtype):
+        out = np.empty(n, dtype)
+        self._ok(self.lib.hipMemcpy(out.ctypes.data, self.C.c_void_p(ptr), out.nbytes, 2), "hipMemcpy")
+        return out
+
+    def from_host(self, arr):
+        """A device copy of a host array (free it with free())."""
+        arr = np.ascontiguousarray(arr)
+        ptr = self.malloc(max(arr.nbytes, 4))
+        self._ok(self.lib.hipMemcpy(self.C.c_void_p(ptr), arr.ctypes.data, arr.nbytes, 1), "hipMemcpy")
+        return ptr
+
     def free_bytes(self):
         f, t = self.C.c_size_t(), self.C.c_size_t()
         self._ok(self.lib.hipDeviceSynchronize(), "hipDeviceSynchronize")

@@ -6,6 +6,7 @@
 #include "algorithms/sssp.hpp"
 #include "graph/csr_graph.hpp"
 #include "hytgraph_amd.h"
+#include "scheduling/hub_sort.hpp"
 #include "sep/sep_execution_factory.hpp"
 #include "sep/sep_execution_variant_registry.hpp"
 #include "sep/sep_gpu_drivers.hpp"
@@ -17,6 +18,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace
 {
@@ -286,6 +288,84 @@ namespace
         expect(fp->converged() && pv != nullptr && std::fabs((*pv)[0] - 1.0F / 3.0F) <= 1e-5F,
                "factory driver runs PageRank on the GPU");
     }
+
+    // device hub sort / reorder / contributions against the host forms
+    void test_device_preprocess_gpu()
+    {
+        namespace sch = hytgraph::scheduling;
+        // the original's unit_tests.cpp:2498-2537, 2710-2750 graph, then a
+        // larger pseudo-random graph with parallel edges and self-loops
+        std::vector<CSRGraph> graphs;
+        graphs.emplace_back(5, std::vector<CSRGraph::offset_type>{0, 3, 4, 5, 6, 7},
+                            std::vector<CSRGraph::vertex_id>{1, 2, 3, 0, 0, 0, 0});
+        {
+            const std::size_t n = 3000;
+            std::vector<CSRGraph::offset_type> off(n + 1, 0);
+            std::vector<CSRGraph::vertex_id> col;
+            std::vector<float> w;
+            std::uint64_t x = 88172645463325252ULL;
+            for (std::size_t u = 0; u < n; ++u)
+            {
+                x ^= x << 13, x ^= x >> 7, x ^= x << 17;
+                const std::size_t deg = (u % 7 == 0) ? 0 : (x % 40) * ((u % 13 == 0) ? 8 : 1);
+                for (std::size_t k = 0; k < deg; ++k)
+                {
+                    x ^= x << 13, x ^= x >> 7, x ^= x << 17;
+                    col.push_back(static_cast<CSRGraph::vertex_id>((x % 5 == 0) ? x % 50 : x % n));
+                    w.push_back(static_cast<float>(x % 1000) * 0.125F);
+                }
+                off[u + 1] = col.size();
+            }
+            graphs.emplace_back(n, off, col, w);
+        }
+        for (const CSRGraph &g : graphs)
+        {
+            for (double frac : {0.0, 0.08, 0.4, 1.0})
+            {
+                sch::HubSortOptions o;
+                o.hub_fraction = frac;
+                const auto a = sch::HubSorter(o).sort(g);
+                const auto b = sch::HubSorter(o).sort_gpu(g);
+                expect(a.vertex_order == b.vertex_order && a.hub_count == b.hub_count,
+                       "device hub order equals the host order");
+                expect(std::memcmp(a.scores.data(), b.scores.data(), a.scores.size() * sizeof(double)) == 0,
+                       "device hub scores equal the host scores bit for bit");
+            }
+            const auto order = sch::HubSorter().sort(g).vertex_order;
+            CSRGraph h = g, d = g;
+            h.reorder_vertices(order);
+            hytgraph::graph::reorder_vertices_gpu(d, order);
+            expect(h.row_offsets() == d.row_offsets() && h.column_indices() == d.column_indices() &&
+                       h.edge_weights() == d.edge_weights(),
+                   "device reorder equals CSRGraph::reorder_vertices");
+            std::vector<CSRGraph::vertex_id> dup(order);
+            dup.back() = dup.front();
+            CSRGraph e = g;
+            expect_throw<std::invalid_argument>([&] { hytgraph::graph::reorder_vertices_gpu(e, dup); },
+                                                "device reorder rejects a duplicate id");
+            expect(e.column_indices() == g.column_indices(), "a rejected reorder leaves the graph");
+            std::vector<float> x(static_cast<std::size_t>(g.num_vertices()));
+            for (std::size_t v = 0; v < x.size(); ++v)
+                x[v] = 1.0F / static_cast<float>(x.size()) * static_cast<float>(1 + v % 5);
+            const auto pc = alg::pagerank_contributions(g, x);
+            const auto pg = alg::pagerank_contributions_gpu(g, x);
+            bool same = pc.size() == pg.size();
+            for (std::size_t v = 0; same && v < pc.size(); ++v)
+                same = pc[v].vertex == pg[v].vertex &&
+                       std::memcmp(&pc[v].contribution, &pg[v].contribution, sizeof(double)) == 0;
+            expect(same, "device PageRank contributions equal the host ones bit for bit");
+        }
+        const float inf = std::numeric_limits<float>::infinity();
+        const std::vector<float> cur{5.0F, 5.0F, inf, inf, 3.0F, 2.0F}, cand{3.0F, 4.75F, 7.0F, inf, inf, 2.0F};
+        const auto sc = alg::sssp_contributions(cur, cand, 0.25F);
+        const auto sg = alg::sssp_contributions_gpu(cur, cand, 0.25F);
+        bool same = sc.size() == sg.size();
+        for (std::size_t v = 0; same && v < sc.size(); ++v)
+            same = sc[v].contribution == sg[v].contribution;
+        expect(same, "device SSSP contributions equal the host ones");
+        expect_throw<std::invalid_argument>([&] { (void)alg::sssp_contributions_gpu(cur, cand, -1.0F); },
+                                            "device SSSP contributions reject a negative tolerance");
+    }
 } // namespace
 
 int main(int argc, char **argv)
@@ -311,6 +391,7 @@ int main(int argc, char **argv)
             test_sssp_gpu_known_answers();
             test_pagerank_gpu_known_answers();
             test_sep_gpu();
+            test_device_preprocess_gpu();
         }
         std::cout << "C++ API tests (" << what << ") passed.\n";
         return 0;

@@ -155,6 +155,42 @@ def test_device_driven_rounds_match_host_rounds(gpu, abi, orc, mode):
     assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
 
 
+@pytest.mark.parametrize("device_rounds", [True, False])
+def test_sync_edge_counts_and_chunk_split(gpu, abi, orc, device_rounds):
+    """SYNC push rounds relax exactly the out-edges of the rows that improved
+    in the previous round: edges_total is the degree sum of those rows over
+    all rounds, and heavy_edges (edges walked by the chunk kernel) the part
+    in rows above light_max. The per-round frontiers come from the oracle's
+    distances after 0, 1, 2, ... iterations."""
+    hg = rmat(abi, 12, 71, wmax=64)
+    g = abi.DeviceGraph.from_host(hg)
+    deg = np.diff(hg.offsets.astype(np.int64))
+    w = hg.weights.astype(np.float32)
+    for src in (0, int(abi.sample_sources(hg.offsets, 3, 1)[0])):
+        ref, it, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, w, src)
+        prev = np.full(hg.V, np.inf, np.float32)
+        prev[src] = 0
+        frontier = np.zeros(hg.V, bool)
+        frontier[src] = True
+        rows = []
+        for k in range(1, it + 1):
+            rows.append(frontier)
+            cur, _, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, w, src, 0.0, k)
+            frontier = cur != prev
+            prev = cur
+        assert np.array_equal(prev, ref) and not frontier.any()
+        for light_max in (8, 64, 1 << 20):
+            d, st = g.sssp(src, mode=abi.HYT_SSSP_SYNC, direction=abi.HYT_DIR_PUSH, light_max_degree=light_max,
+                           heavy_chunk=64, host_rounds=0 if device_rounds else 1)
+            assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src))
+            edges = sum(int(deg[f].sum()) for f in rows if f.any())
+            heavy = sum(int(deg[f & (deg > light_max)].sum()) for f in rows if f.any())
+            _say(f"src {src} light_max {light_max}: edges {st['edges_total']} (oracle {edges}), "
+                 f"chunk edges {st['heavy_edges']} (oracle {heavy})")
+            assert st["edges_total"] == edges and st["heavy_edges"] == heavy
+            assert (heavy > 0) == (light_max < deg.max())
+
+
 @pytest.mark.parametrize("tol", [0.25, 0.5, 3.0, 40.0])
 @pytest.mark.parametrize("scale", [10, 13])
 def test_tolerance_rule_bitexact(gpu, abi, orc, rng, tol, scale):
