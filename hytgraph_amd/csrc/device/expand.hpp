@@ -173,28 +173,43 @@ namespace hyt
         }
     }
 
-    // Queues the chunks of this wavefront's heavy rows (nch chunks for a lane's
-    // row, 0 for light rows); one atomic per wavefront.
+    // Queues the chunks of this workgroup's heavy rows (nch chunks for a
+    // thread's row, 0 for light rows) with one atomic per workgroup: every
+    // thread of the block calls it (s_h: kWavesPerBlock + 1 LDS words). One
+    // atomic per wavefront on the single chunk counter serialised in the
+    // light kernel's chunk-queuing pass (RMAT 24: ~100 K same-address
+    // atomics in a large round).
     template <class Chunk, class MakeFn>
     __device__ __forceinline__ void queue_heavy(std::uint32_t nch, Counters *ctr, Chunk *heavy,
-                                                std::uint64_t heavy_cap, MakeFn make)
+                                                std::uint64_t heavy_cap, std::uint32_t *s_h, MakeFn make)
     {
         const int lane = lane_id();
+        const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint32_t hincl = wave_incl_scan(nch);
-        const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
-        if (htotal == 0)
-            return;
-        std::uint32_t hbase = 0;
-        if (lane == 0)
-            hbase = atomicAdd(&ctr->heavy_count, htotal);
-        hbase = __shfl(hbase, 0, kWave);
-        const std::uint64_t slot = static_cast<std::uint64_t>(hbase) + hincl - nch;
-        if (static_cast<std::uint64_t>(hbase) + htotal > heavy_cap)
+        if (lane == kWave - 1)
+            s_h[wid] = hincl;
+        __syncthreads();
+        std::uint32_t before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kWavesPerBlock; ++w)
+        {
+            if (w < wid)
+                before += s_h[w];
+            total += s_h[w];
+        }
+        if (total == 0)
+            return; // uniform across the workgroup
+        if (threadIdx.x == 0)
+            s_h[kWavesPerBlock] = atomicAdd(&ctr->heavy_count, total);
+        __syncthreads();
+        const std::uint32_t hbase = s_h[kWavesPerBlock];
+        const std::uint64_t slot = static_cast<std::uint64_t>(hbase) + before + hincl - nch;
+        if (static_cast<std::uint64_t>(hbase) + total > heavy_cap)
         {
             // the round fails on the host; until then the chunk kernel reads
             // min(heavy_count, cap) slots, so the reserved ones that exist
             // hold empty chunks (no edges, no record) rather than stale bytes
-            if (lane == 0)
+            if (threadIdx.x == 0)
                 atomicOr(&ctr->heavy_overflow, 1U);
             for (std::uint32_t c = 0; c < nch; ++c)
                 if (slot + c < heavy_cap)
@@ -218,6 +233,7 @@ namespace hyt
         __shared__ std::uint64_t s_begin[kWavesPerBlock][kWave];
         __shared__ std::uint32_t s_val[kWavesPerBlock][kWave];
         __shared__ unsigned long long s_edges[kWavesPerBlock];
+        __shared__ std::uint32_t s_h[kWavesPerBlock + 1];
 
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
@@ -254,16 +270,18 @@ namespace hyt
                     op.expanded(u, value, whole);
                 }
             }
-            queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, [&](std::uint32_t c) {
-                const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
-                HeavyChunk h;
-                h.value = value;
-                h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
-                h.begin = begin + lo;
-                h.row = row;
-                h.record = whole ? 1U : 0U;
-                return h;
-            });
+            // the chunk-queuing passes (block-uniform: the pass is a kernel argument)
+            if (a.pass != 2)
+                queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, s_h, [&](std::uint32_t c) {
+                    const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
+                    HeavyChunk h;
+                    h.value = value;
+                    h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
+                    h.begin = begin + lo;
+                    h.row = row;
+                    h.record = whole ? 1U : 0U;
+                    return h;
+                });
 
             const std::uint32_t incl = wave_incl_scan(deg);
             const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
