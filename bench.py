@@ -750,6 +750,9 @@ def main(argv=None) -> int:
     ap.add_argument("--direction", choices=["auto", "push", "pull"], default="auto")
     ap.add_argument("--pull-percent", type=int, default=0, help="auto: pull when frontier edges >= pct%% of E")
     ap.add_argument("--delta", type=int, default=0, help="async SSSP delta-stepping bucket width (0 = off)")
+    ap.add_argument("--heavy-delta", type=int, default=0,
+                    help="async SSSP: rows above light_max wait while their distance exceeds the queue's "
+                         "smallest by more than this (0 = the library default, 2; 4294967295 = no waiting)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pagerank", action="store_true")
     ap.add_argument("--tune", action="store_true", help="plain: autotune the degree-bin thresholds first")
@@ -823,7 +826,7 @@ def main(argv=None) -> int:
     mode = abi.HYT_SSSP_ASYNC if args.mode == "async" else abi.HYT_SSSP_SYNC
     direction = {"auto": abi.HYT_DIR_AUTO, "push": abi.HYT_DIR_PUSH, "pull": abi.HYT_DIR_PULL}[args.direction]
     dopt = {"direction": direction, "pull_percent": args.pull_percent}
-    sopt = dict(dopt, delta=args.delta, host_rounds=1 if args.host_rounds else 0)
+    sopt = dict(dopt, delta=args.delta, host_rounds=1 if args.host_rounds else 0, heavy_delta=args.heavy_delta)
 
     tuned = None
     if args.tune:
@@ -913,6 +916,7 @@ def main(argv=None) -> int:
                       "device-driven (captured round graph, one read-back per 4 rounds)",
             "direction": args.direction,
             "delta": args.delta,
+            "heavy_delta": args.heavy_delta,
             "tuned_split": {"light_max_degree": tuned["light_max_degree"],
                             "heavy_chunk": tuned["heavy_chunk"]} if tuned else None,
         },

@@ -50,6 +50,7 @@ namespace hyt
     {
         unsigned long long v; // rows walked whole or in windows
         unsigned long long h; // chunks of rows above light_max (expand_heavy)
+        std::uint32_t kinv;   // max of ~dist over the rows the compaction kept (waiting hubs)
     };
 
     // Per-round counters. The bytes before `overflow` (kRoundCounterBytes)
@@ -73,6 +74,14 @@ namespace hyt
             for (int i = 0; i < kEdgeSlots; ++i)
                 e += slot[i].v + slot[i].h;
             return e;
+        }
+        // smallest distance the compaction kept (0xFFFFFFFF: none recorded)
+        __host__ __device__ std::uint32_t kept_min() const
+        {
+            std::uint32_t m = 0;
+            for (int i = 0; i < kEdgeSlots; ++i)
+                m = slot[i].kinv > m ? slot[i].kinv : m;
+            return ~m;
         }
         // the part of all_edges() that the chunk kernel walked
         __host__ __device__ unsigned long long heavy_edges() const
@@ -106,7 +115,7 @@ namespace hyt
         std::uint32_t F;         // rows of the current round's queue (0: no work)
         std::uint32_t done;      // a round found the queue empty, or the round limit
         std::uint32_t truncated; // stopped by the round limit with work left
-        std::uint32_t pad;
+        std::uint32_t heavy_bound; // rows above light_max with a larger distance wait (this round)
         std::uint64_t rounds, frontier_total, edges_total, updates_total;
         std::uint64_t heavy_edges_total; // of edges_total: walked by the chunk kernel
         // timed runs: device wall clock before and after the expand kernels
@@ -194,6 +203,19 @@ namespace hyt
     // (one filtered atomic per edge) beats pull (bitmap probe + gather per
     // in-edge) even at 50%, so SSSP pulls only when asked.
     constexpr std::uint64_t kDefaultPullPercentPagerank = 25;
+    // ASYNC single-source SSSP: rows above light_max wait while their
+    // distance exceeds the queue's smallest by more than this (weight units).
+    // A row within w_min of the smallest queued distance is final, so the
+    // hubs - most of RMAT's edges - go out once instead of ~2 times. One
+    // MI355X, 8 sources, u32 weights [1,255]: RMAT 22 / 24 / 26 34.2 / 114.6
+    // / 486.9 ms -> 30.1 / 102.4 / 415.4 ms at 2 (relaxations per query 153
+    // / 505 / 1914 M -> 108 / 345 / 1291 M, rounds 13 -> 15-19); 1 and 4
+    // within 3% of 2; ER 24 (no rows above light_max) unchanged.
+    constexpr std::uint32_t kDefaultHeavyDelta = 2;
+    inline std::uint32_t heavy_delta_of(std::uint32_t option)
+    {
+        return option == 0 ? kDefaultHeavyDelta : (option == 0xFFFFFFFFU ? 0U : option);
+    }
     constexpr std::uint64_t kDefaultPullPercentSssp = 101;
 
     // Allocates the single-source workspace if needed (sized for the graph).

@@ -18,6 +18,7 @@
 //   uint32_t row_value(uint32_t i, uint32_t u, uint64_t deg)  value carried by row
 //   void expanded(uint32_t u, uint32_t value, bool whole_row)  row u walks with value
 //   void expanded_chunked(uint32_t u, bool whole_row)  row u goes out as chunks
+//   void deferred(uint32_t u)  row u (above light_max) waits this round
 //   uint32_t chunk_value(uint32_t u, uint32_t queued, bool record)  value a
 //       chunk of row u walks with (queued = the row's value at queue time)
 //   bool load(uint64_t pos, uint32_t value, uint32_t& v, uint32_t& c) read edge
@@ -135,6 +136,11 @@ namespace hyt
         // 0: every row; 1: only queue the chunks of rows above light_max;
         // 2: only walk the rows up to light_max
         std::uint32_t pass = 0;
+        // rows above light_max whose value exceeds the bound wait (Op::
+        // deferred) instead of going out as chunks; the bound is read on the
+        // device when heavy_bound_dev is set (device-driven rounds)
+        std::uint32_t heavy_bound = 0xFFFFFFFFU;
+        const std::uint32_t *heavy_bound_dev = nullptr;
     };
 
     // Processes `kUnroll` x 64 edges per iteration: all loads first, then all
@@ -238,6 +244,7 @@ namespace hyt
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint32_t F = a.F_dev != nullptr ? *a.F_dev : a.F;
+        const std::uint32_t hbound = a.heavy_bound_dev != nullptr ? *a.heavy_bound_dev : a.heavy_bound;
         const std::uint32_t tiles = (F + kBlock - 1) / kBlock;
         unsigned long long wedges = 0; // this wavefront's light edges (uniform across its lanes)
         // a block-uniform loop: every wavefront meets the same barriers; LDS
@@ -260,8 +267,13 @@ namespace hyt
                 if (chunked && a.pass != 2)
                 {
                     value = op.row_value(vi, u, full);
-                    nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
-                    op.expanded_chunked(u, whole);
+                    if (value > hbound)
+                        op.deferred(u);
+                    else
+                    {
+                        nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
+                        op.expanded_chunked(u, whole);
+                    }
                 }
                 else if (!chunked && a.pass != 1)
                 {

@@ -190,24 +190,39 @@ namespace hyt
             return ce < 16U ? 16U : ce;
         }
 
-        // Queues this wavefront's heavy-row chunks (nch per lane) with one
-        // atomic. When the queue cannot take them all, the reserved slots that
-        // exist are written as empty chunks and false is returned: the rows
-        // are then walked by this wavefront itself (slower, never wrong).
+        // Queues this workgroup's heavy-row chunks (nch per thread) with one
+        // atomic per workgroup; every thread of the block calls it (s_h:
+        // kWavesPerBlock + 1 LDS words). Per-wavefront atomics on the one
+        // chunk counter serialise (see queue_heavy). When the queue cannot
+        // take them all, the reserved slots that exist are written as empty
+        // chunks and false is returned (uniformly): the rows are then walked
+        // by their own wavefronts (slower, never wrong).
         template <class MakeFn>
-        __device__ __forceinline__ bool queue_chunks_k(std::uint32_t nch, const KArgs &a, MakeFn make)
+        __device__ __forceinline__ bool queue_chunks_k(std::uint32_t nch, const KArgs &a, std::uint32_t *s_h,
+                                                       MakeFn make)
         {
             const int lane = lane_id();
+            const int wid = static_cast<int>(threadIdx.x) / kWave;
             const std::uint32_t hincl = wave_incl_scan(nch);
-            const std::uint32_t htotal = __shfl(hincl, kWave - 1, kWave);
-            if (htotal == 0)
-                return true;
-            std::uint32_t hbase = 0;
-            if (lane == 0)
-                hbase = atomicAdd(&a.ctr->heavy_count, htotal);
-            hbase = __shfl(hbase, 0, kWave);
-            const std::uint64_t first = static_cast<std::uint64_t>(hbase) + hincl - nch;
-            if (static_cast<std::uint64_t>(hbase) + htotal <= a.heavy_cap)
+            if (lane == kWave - 1)
+                s_h[wid] = hincl;
+            __syncthreads();
+            std::uint32_t before = 0, total = 0;
+#pragma unroll
+            for (int w = 0; w < kWavesPerBlock; ++w)
+            {
+                if (w < wid)
+                    before += s_h[w];
+                total += s_h[w];
+            }
+            if (total == 0)
+                return true; // uniform across the workgroup
+            if (threadIdx.x == 0)
+                s_h[kWavesPerBlock] = atomicAdd(&a.ctr->heavy_count, total);
+            __syncthreads();
+            const std::uint32_t hbase = s_h[kWavesPerBlock];
+            const std::uint64_t first = static_cast<std::uint64_t>(hbase) + before + hincl - nch;
+            if (static_cast<std::uint64_t>(hbase) + total <= a.heavy_cap)
             {
                 for (std::uint32_t c = 0; c < nch; ++c)
                     a.heavy[first + c] = make(c);
@@ -327,6 +342,7 @@ namespace hyt
             __shared__ std::uint32_t s_scale[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
             __shared__ unsigned long long s_edges[kWavesPerBlock];
+            __shared__ std::uint32_t s_h[kWavesPerBlock + 1];
 
             std::uint32_t F;
             if (!round_rows(a, HYT_LANES_GROUP, F))
@@ -356,7 +372,7 @@ namespace hyt
                     else
                         nch = static_cast<std::uint32_t>((full + ce - 1) / ce);
                 }
-                const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t c) {
+                const bool queued = queue_chunks_k(nch, a, s_h, [&](std::uint32_t c) {
                     const std::uint64_t lo = static_cast<std::uint64_t>(c) * ce;
                     HeavyChunkK h;
                     h.row = row;
@@ -535,6 +551,7 @@ namespace hyt
             __shared__ unsigned long long s_mask[kWavesPerBlock][kWave];
 
             __shared__ unsigned long long s_edges[kWavesPerBlock];
+            __shared__ std::uint32_t s_h[kWavesPerBlock + 1];
             std::uint32_t F;
             if (!round_rows(a, HYT_LANES_ITEMS, F))
                 return;
@@ -565,7 +582,7 @@ namespace hyt
                     else
                         nch = static_cast<std::uint32_t>((full + ce - 1) / ce);
                 }
-                const bool queued = queue_chunks_k(nch, a, [&](std::uint32_t ch) {
+                const bool queued = queue_chunks_k(nch, a, s_h, [&](std::uint32_t ch) {
                     const std::uint64_t lo = static_cast<std::uint64_t>(ch) * ce;
                     HeavyChunkK h;
                     h.row = row;

@@ -81,6 +81,14 @@ namespace hyt
             if (!SYNC && whole_row && expanded_at != nullptr)
                 expanded_at[u] = 0U;
         }
+        // A row above light_max that waits this round (its distance is above
+        // the round's bound): its record becomes "never expanded", so the
+        // next compaction queues it again whatever its distance does.
+        __device__ void deferred(std::uint32_t u) const
+        {
+            if (!SYNC && expanded_at != nullptr)
+                expanded_at[u] = 0xFFFFFFFFU;
+        }
         __device__ std::uint32_t chunk_value(std::uint32_t u, std::uint32_t queued, bool record) const
         {
             if (SYNC)
@@ -131,14 +139,30 @@ namespace hyt
         {
             std::uint32_t pending = 0;
             std::uint32_t inv_max = 0;
+            std::uint32_t kept_inv = 0; // max of ~dist over kept rows (track_min)
         };
         std::uint8_t *touch;
         const std::uint32_t *dist;
         std::uint32_t *prev;
         std::uint32_t bound;
+        // record the smallest kept distance (for rows above light_max that
+        // wait while their distance exceeds it by more than a margin)
+        bool track_min = false;
 
         __device__ void publish(const Aux &aux, Counters *ctr) const
         {
+            if (track_min)
+            {
+                std::uint32_t k = aux.kept_inv;
+#pragma unroll
+                for (int d = kWave / 2; d > 0; d >>= 1)
+                {
+                    const std::uint32_t o = __shfl_xor(k, d, kWave);
+                    k = o > k ? o : k;
+                }
+                if (lane_id() == 0 && k)
+                    atomicMax(&ctr->slot[blockIdx.x % kEdgeSlots].kinv, k);
+            }
             if (bound == 0xFFFFFFFFU)
                 return;
             std::uint32_t p = aux.pending, m = aux.inv_max;
@@ -180,6 +204,7 @@ namespace hyt
                         {
                             prev[v] = d;
                             vals[j] = d;
+                            aux.kept_inv = (~d > aux.kept_inv) ? ~d : aux.kept_inv;
                             m |= 1U << j;
                         }
                         else
@@ -215,6 +240,7 @@ namespace hyt
                     {
                         p[j] = d[j];
                         vals[j] = d[j];
+                        aux.kept_inv = (~d[j] > aux.kept_inv) ? ~d[j] : aux.kept_inv;
                         m |= 1U << j;
                     }
                     else
@@ -261,6 +287,7 @@ namespace hyt
                         {
                             p[j] = d[j];
                             vals[g][j] = d[j];
+                            aux.kept_inv = (~d[j] > aux.kept_inv) ? ~d[j] : aux.kept_inv;
                             m |= 1U << j;
                         }
                         else
@@ -293,6 +320,7 @@ namespace hyt
         }
         __device__ void expanded(std::uint32_t, std::uint32_t, bool) const {}
         __device__ void expanded_chunked(std::uint32_t, bool) const {}
+        __device__ void deferred(std::uint32_t) const {}
         __device__ std::uint32_t chunk_value(std::uint32_t, std::uint32_t queued, bool) const { return queued; }
         __device__ bool load(std::uint64_t pos, std::uint32_t share, std::uint32_t &v, std::uint32_t &c) const
         {

@@ -260,7 +260,30 @@ namespace hyt
                 *rc = RoundCtl{};
         }
 
-        __global__ void round_begin(Counters *ctr, RoundCtl *rc, std::uint64_t max_rounds)
+        // Distance bound of the rows above light_max this round: the smallest
+        // distance the last compaction kept plus heavy_delta (float weights:
+        // added as a float); 0xFFFFFFFF (no waiting) when heavy_delta is 0 or
+        // no distance was recorded.
+        __host__ __device__ inline std::uint32_t heavy_bound_of(std::uint32_t kept_min, std::uint32_t heavy_delta,
+                                                               bool float_dist)
+        {
+            if (heavy_delta == 0 || kept_min == 0xFFFFFFFFU)
+                return 0xFFFFFFFFU;
+            if (float_dist)
+            {
+                float f;
+                std::memcpy(&f, &kept_min, 4);
+                f += static_cast<float>(heavy_delta);
+                std::uint32_t b;
+                std::memcpy(&b, &f, 4);
+                return b;
+            }
+            const std::uint64_t b = static_cast<std::uint64_t>(kept_min) + heavy_delta;
+            return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
+        }
+
+        __global__ void round_begin(Counters *ctr, RoundCtl *rc, std::uint64_t max_rounds, std::uint32_t heavy_delta,
+                                    bool float_dist)
         {
             if (threadIdx.x != 0 || blockIdx.x != 0)
                 return;
@@ -274,6 +297,7 @@ namespace hyt
                 return;
             }
             rc->F = F;
+            rc->heavy_bound = heavy_bound_of(ctr->kept_min(), heavy_delta, float_dist);
             if (F != 0)
             {
                 rc->rounds += 1;
@@ -288,7 +312,10 @@ namespace hyt
             ctr->pending = 0;
             ctr->pending_inv_max = 0;
             for (int i = 0; i < kEdgeSlots; ++i)
+            {
                 ctr->slot[i].v = ctr->slot[i].h = 0;
+                ctr->slot[i].kinv = 0;
+            }
         }
 
         // Device wall clock into rc->t[slot] (kernels of a stream run in
@@ -364,13 +391,15 @@ namespace hyt
             const SsspBuffers &B = L.b;
             hipStream_t s = g.stream;
             const bool timed = L.o.time_kernels != 0;
+            const std::uint32_t heavy_delta = SYNC ? 0U : heavy_delta_of(L.o.heavy_delta);
+            constexpr bool kFloatDist = WM == kWF32 || WM == kWUnitF32;
             a.F_dev = &B.rc->F;
             const unsigned light_grid = std::min(blocks_for(g.V), kDeviceLightGrid);
             const unsigned nb = compact_blocks(g.V);
             const GraphRows<OffT> rows = whole_graph(g.off.as<OffT>(), B.q);
             const std::vector<std::uint64_t> key{
                 static_cast<std::uint64_t>(sizeof(OffT)), static_cast<std::uint64_t>(WM), SYNC ? 1ULL : 0ULL,
-                timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap,
+                timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap, heavy_delta,
                 reinterpret_cast<std::uint64_t>(B.dist), reinterpret_cast<std::uint64_t>(B.prev),
                 reinterpret_cast<std::uint64_t>(B.q), reinterpret_cast<std::uint64_t>(B.qd),
                 reinterpret_cast<std::uint64_t>(B.heavy), reinterpret_cast<std::uint64_t>(B.ctr),
@@ -385,7 +414,7 @@ namespace hyt
                 hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin round capture");
                 for (int r = 0; r < kGraphRounds; ++r)
                 {
-                    round_begin<<<1, 64, 0, s>>>(B.ctr, B.rc, max_rounds);
+                    round_begin<<<1, 64, 0, s>>>(B.ctr, B.rc, max_rounds, heavy_delta, kFloatDist);
                     if (timed)
                         stamp<<<1, 64, 0, s>>>(B.rc, 2 * r);
                     if (SYNC)
@@ -397,6 +426,7 @@ namespace hyt
                     {
                         ExpandArgs a1 = a, a2 = a;
                         a1.pass = 1;
+                        a1.heavy_bound_dev = &B.rc->heavy_bound;
                         a2.pass = 2;
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a1);
                         expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
@@ -514,6 +544,11 @@ namespace hyt
             };
             SsspSel sel{touch, B.dist, B.prev,
                         bucketed ? bound_after(0U) : 0xFFFFFFFFU};
+            // rows above light_max wait while far above the queue's smallest
+            // distance (ASYNC): the compaction records that distance
+            const std::uint32_t heavy_delta = SYNC ? 0U : heavy_delta_of(L.o.heavy_delta);
+            constexpr bool kFloatDist = WM == kWF32 || WM == kWUnitF32;
+            sel.track_min = heavy_delta > 0;
 
             if (may_pull && g.E > 0 && g.E < (1ULL << 31))
                 ensure_transpose(g, L.light_max, L.chunk);
@@ -534,6 +569,7 @@ namespace hyt
             else
                 c0 = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
             std::uint32_t F = c0.next_count;
+            std::uint32_t kept_min = c0.kept_min();
             unsigned long long Fe = c0.frontier_edges;
             std::uint32_t pending = c0.pending;
             std::uint32_t pending_min = ~c0.pending_inv_max;
@@ -579,6 +615,7 @@ namespace hyt
                     clear_round_counters(d_ctr, s);
                     const Counters cb = compact_and_read<OffT>(g, B, sel, SYNC, can_pull);
                     F = cb.next_count;
+                    kept_min = cb.kept_min();
                     Fe = cb.frontier_edges;
                     pending = cb.pending;
                     pending_min = ~cb.pending_inv_max;
@@ -611,6 +648,7 @@ namespace hyt
                     // relaxations per query, 0.565 -> 0.548 ms per round)
                     ExpandArgs a1 = a, a2 = a;
                     a1.pass = 1;
+                    a1.heavy_bound = heavy_bound_of(kept_min, heavy_delta, kFloatDist);
                     a2.pass = 2;
                     expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
                         whole_graph(g.off.as<OffT>(), a.q), op, a1);
@@ -660,6 +698,7 @@ namespace hyt
                     fail(HYT_ERR_OVERFLOW, "u32 distance sum exceeded 2^32-2");
                 }
                 F = c.next_count;
+                kept_min = c.kept_min();
                 Fe = c.frontier_edges;
                 pending = c.pending;
                 pending_min = ~c.pending_inv_max;

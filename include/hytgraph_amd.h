@@ -239,6 +239,14 @@ typedef struct hyt_run_options {
                                   partition contributions computed on the
                                   device: SSSP -min frontier distance, PageRank
                                   the mass the partition pushes) */
+    uint32_t heavy_delta;      /* ASYNC single-source SSSP: rows above light_max
+                                  (the hubs) wait while their distance exceeds
+                                  the smallest distance of the round's queue by
+                                  more than heavy_delta weight units, so they
+                                  go out (nearly) once, with their final
+                                  distance; light rows never wait. 0 = the
+                                  default, 2; UINT32_MAX = no waiting. Same
+                                  distances either way */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */

@@ -156,6 +156,37 @@ def test_device_driven_rounds_match_host_rounds(gpu, abi, orc, mode):
 
 
 @pytest.mark.parametrize("device_rounds", [True, False])
+def test_waiting_hubs_same_distances(gpu, abi, orc, device_rounds):
+    """ASYNC rounds in which rows above light_max wait while their distance
+    exceeds the queue's smallest by more than heavy_delta give Dijkstra's
+    distances, for any margin (1, the most waiting; 0, the default; UINT32_MAX,
+    no waiting), on integer and float weights."""
+    hg = rmat(abi, 14, 91, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    wf = (hg.weights.astype(np.float32) * np.float32(0.37)).astype(np.float32)
+    gf = abi.DeviceGraph(hg.offsets, hg.columns, wf, weights_f32=True)
+    hr = 0 if device_rounds else 1
+    for src in abi.sample_sources(hg.offsets, 92, 3):
+        ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+        reff, _, _ = orc.sssp_jacobi_f32(hg.offsets, hg.columns, wf, int(src))
+        base = g.sssp(int(src), mode=abi.HYT_SSSP_ASYNC, light_max_degree=16, heavy_chunk=64, host_rounds=hr,
+                      heavy_delta=0xFFFFFFFF)[1]
+        for hd in (0, 1, 64, 300, 100000, 0xFFFFFFFF):
+            d, st = g.sssp(int(src), mode=abi.HYT_SSSP_ASYNC, light_max_degree=16, heavy_chunk=64, heavy_delta=hd,
+                           host_rounds=hr)
+            _say(f"src {src} heavy_delta {hd}: rounds {st['rounds']} (no waiting {base['rounds']}), "
+                 f"chunk edges {st['heavy_edges']} ({base['heavy_edges']})")
+            assert np.array_equal(d, ref), (src, hd)
+            assert st["converged"] == 1
+            df, _ = gf.sssp(int(src), mode=abi.HYT_SSSP_ASYNC, light_max_degree=16, heavy_chunk=64,
+                            heavy_delta=hd, host_rounds=hr)
+            assert np.array_equal(df, reff.view(np.uint32)), (src, hd)
+        # SYNC mode ignores the option: the original's rounds
+        d, st = g.sssp(int(src), mode=abi.HYT_SSSP_SYNC, heavy_delta=5, host_rounds=hr)
+        assert np.array_equal(d, ref) and st["rounds"] == g.sssp(int(src), mode=abi.HYT_SSSP_SYNC)[1]["rounds"]
+
+
+@pytest.mark.parametrize("device_rounds", [True, False])
 def test_sync_edge_counts_and_chunk_split(gpu, abi, orc, device_rounds):
     """SYNC push rounds relax exactly the out-edges of the rows that improved
     in the previous round: edges_total is the degree sum of those rows over
