@@ -775,6 +775,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-extras", action="store_true", help="plain: skip the batched and served legs")
     ap.add_argument("--host-rounds", action="store_true",
                     help="plain: one host read-back per round instead of replays of captured rounds")
+    ap.add_argument("--persistent", action="store_true",
+                    help="plain: every round of a query in one persistent cooperative launch")
     ap.add_argument("--extras-requests", type=int, default=400, help="plain: served-leg requests per trace")
     args = ap.parse_args(argv)
 
@@ -826,7 +828,8 @@ def main(argv=None) -> int:
     mode = abi.HYT_SSSP_ASYNC if args.mode == "async" else abi.HYT_SSSP_SYNC
     direction = {"auto": abi.HYT_DIR_AUTO, "push": abi.HYT_DIR_PUSH, "pull": abi.HYT_DIR_PULL}[args.direction]
     dopt = {"direction": direction, "pull_percent": args.pull_percent}
-    sopt = dict(dopt, delta=args.delta, host_rounds=1 if args.host_rounds else 0, heavy_delta=args.heavy_delta)
+    sopt = dict(dopt, delta=args.delta, host_rounds=1 if args.host_rounds else 0, heavy_delta=args.heavy_delta,
+                persistent=1 if args.persistent else 0)
 
     tuned = None
     if args.tune:
@@ -887,7 +890,7 @@ def main(argv=None) -> int:
     traffic = None
     tfile = ROOT / "profiles" / "sssp_rmat24_traffic.json"
     if (tfile.exists() and args.scale == 24 and args.edge_factor == 16 and args.mode == "async" and pull_rounds == 0
-            and not args.tune and args.delta == 0):
+            and not args.tune and args.delta == 0 and not args.persistent):
         traffic = json.loads(tfile.read_text())["traffic_bytes_per_round"]
 
     result = {
@@ -912,7 +915,8 @@ def main(argv=None) -> int:
             "edges": hg.E,
             "sources_per_step": len(sources),
             "mode": args.mode,
-            "rounds": "host-driven (one read-back per round)" if args.host_rounds else
+            "rounds": "persistent (one cooperative launch per query, grid barriers between rounds)"
+                      if args.persistent else "host-driven (one read-back per round)" if args.host_rounds else
                       "device-driven (captured round graph, one read-back per 4 rounds)",
             "direction": args.direction,
             "delta": args.delta,

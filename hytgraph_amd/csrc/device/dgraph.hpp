@@ -150,6 +150,7 @@ namespace hyt
         Workspace ws;
         std::shared_ptr<void> kws; // workspace of the batched [V][K] calls (owned, type-erased)
         std::shared_ptr<void> round_graph; // captured single-source rounds (owned, type-erased)
+        std::shared_ptr<void> pws;         // persistent-kernel queues and stamps (owned, type-erased)
         std::unique_ptr<PinnedBuffer> host_ctr;
         hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -253,6 +254,11 @@ namespace hyt
     // The SSSP rounds from src over the given arrays (distances end in b.dist).
     void sssp_rounds(DeviceGraph &g, std::uint32_t src, const SsspBuffers &b, const hyt_run_options &o,
                      std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st);
+    // Every round of a single-source push run in one persistent cooperative
+    // launch (data-driven queues; distances end in ws.dist). Needs the
+    // workspace (ensure_workspace at the split).
+    void sssp_persistent_rounds(DeviceGraph &g, std::uint32_t src, const hyt_run_options &o, std::uint32_t light_max,
+                                std::uint32_t chunk, hyt_run_stats &st);
     // Round-by-round runs (steppers): `rounds` more rounds of a single-source
     // SSSP run from src (resume = continue the run the workspace holds), or
     // of a PageRank / PPR run (seed UINT32_MAX: global); the stats are this

@@ -157,6 +157,7 @@ int hyt_graph_trim(hyt_graph_t g)
         hyt::DeviceGraph &d = g->g;
         hyt::hip_check(hipStreamSynchronize(d.stream), "hyt_graph_trim");
         d.round_graph.reset(); // its nodes name workspace arrays
+        d.pws.reset();
         d.ws = hyt::Workspace{};
         d.kws.reset();
         d.in_off.release();

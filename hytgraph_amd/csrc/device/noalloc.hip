@@ -94,6 +94,7 @@ extern "C" int hyt_sssp_noalloc(hyt_graph_t g, uint32_t source, uint32_t *dist, 
         require(o.out_on_device == 1, "hyt_sssp_noalloc writes device memory only (out_on_device = 1)");
         require(o.mode == HYT_SSSP_SYNC || o.mode == HYT_SSSP_ASYNC, "unknown SSSP mode");
         require(o.tolerance == 0.0F, "hyt_sssp_noalloc runs tolerance 0 only (tolerance > 0 builds a transpose)");
+        require(o.persistent == 0, "hyt_sssp_noalloc runs the captured rounds (persistent runs allocate their queues)");
         const std::uint64_t pct = o.pull_percent ? o.pull_percent : kDefaultPullPercentSssp;
         require(o.direction == HYT_DIR_PUSH || (o.direction == HYT_DIR_AUTO && pct > 100),
                 "hyt_sssp_noalloc runs push rounds only (pull rounds build a transpose)");

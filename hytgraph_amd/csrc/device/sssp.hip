@@ -797,6 +797,12 @@ namespace hyt
             else
                 run_tolerance<std::uint32_t>(g, src, o, light_max, chunk, st);
         }
+        else if (o.persistent)
+        {
+            require(o.delta == 0 && o.direction != HYT_DIR_PULL,
+                    "persistent SSSP runs push rounds without buckets (direction PUSH or AUTO, delta 0)");
+            sssp_persistent_rounds(g, src, o, light_max, chunk, st);
+        }
         else
         {
             try

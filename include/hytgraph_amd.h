@@ -247,6 +247,14 @@ typedef struct hyt_run_options {
                                   distance; light rows never wait. 0 = the
                                   default, 2; UINT32_MAX = no waiting. Same
                                   distances either way */
+    uint32_t persistent;       /* single-source SSSP push rounds without
+                                  tolerance or buckets: 1 = one persistent
+                                  cooperative launch runs every round (the
+                                  next queue built by the relaxations, grid
+                                  barriers between rounds, no compaction pass
+                                  and no host read-back); SYNC keeps the
+                                  original's iteration count. 0 = the rounds
+                                  host_rounds selects */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */
