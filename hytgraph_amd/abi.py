@@ -226,6 +226,7 @@ SIGNATURES = {
     "hyt_version": (C.c_char_p, []),
     "hyt_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "hyt_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(_P)]),
+    "hyt_generate_device": (C.c_int, [C.POINTER(GenParams), C.c_uint64, C.POINTER(_P)]),
     "hyt_host_graph_view": (
         C.c_int,
         [_P, _u64p, _u64p, C.POINTER(_u64p), C.POINTER(_u32p), C.POINTER(_u32p)],
@@ -358,12 +359,18 @@ class HostGraph:
     """Generator output owned by the library (views stay valid while alive)."""
 
     def __init__(self, kind=HYT_GEN_RMAT, scale=16, edge_factor=16, a=0.57, b=0.19, c=0.19,
-                 seed=1, weight_min=1, weight_max=64, threads=0, memory=HYT_HOST_PAGEABLE, _handle=None):
+                 seed=1, weight_min=1, weight_max=64, threads=0, memory=HYT_HOST_PAGEABLE, _handle=None,
+                 on_device=False, device_budget=0):
+        """on_device: the same graph built on the current HIP device
+        (hyt_generate_device, key buckets within device_budget bytes of HBM)."""
         self.weights_f32 = False
         if _handle is None:
             p = GenParams(kind, scale, edge_factor, a, b, c, seed, weight_min, weight_max, threads, memory)
             _handle = C.c_void_p()
-            check(load().hyt_generate(C.byref(p), C.byref(_handle)))
+            if on_device:
+                check(load().hyt_generate_device(C.byref(p), C.c_uint64(device_budget), C.byref(_handle)))
+            else:
+                check(load().hyt_generate(C.byref(p), C.byref(_handle)))
         self._attach(_handle)
 
     @classmethod

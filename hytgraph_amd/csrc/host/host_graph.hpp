@@ -31,6 +31,9 @@ namespace hyt
     void host_free(void *p, int memory) noexcept;
 
     std::unique_ptr<HostGraph> generate(const hyt_gen_params &p);
+    // The same graph built on the current HIP device, the key buckets within
+    // budget bytes of HBM (0: 40 GiB); params.threads is not used.
+    std::unique_ptr<HostGraph> generate_device(const hyt_gen_params &p, std::uint64_t budget);
     HostGraph *host_graph_of(hyt_host_graph_t h);
     void sample_sources(const std::uint64_t *off, std::uint64_t V, std::uint64_t seed,
                         std::uint32_t count, std::uint32_t *out);

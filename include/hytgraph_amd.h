@@ -72,6 +72,19 @@ typedef struct hyt_host_graph *hyt_host_graph_t;
 
 int hyt_generate(const hyt_gen_params *params, hyt_host_graph_t *out);
 
+/*
+ * The same graph as hyt_generate, bit for bit, built on the current HIP
+ * device (no counterpart in the original): edges are drawn on the device in
+ * source-range buckets that fit device_budget_bytes of HBM (0 = 40 GiB, about
+ * 17 bytes per generated edge of the largest bucket), each bucket sorted and
+ * deduplicated there and copied into a host graph of kind params->memory.
+ * params->threads is not used. RMAT scale 30 (17 B generated edges) takes
+ * seconds where the host generator takes tens of minutes. Fails with
+ * HYT_ERR_OUT_OF_MEMORY when one source bin (1/16384 of the vertex range)
+ * holds more edges than a bucket.
+ */
+int hyt_generate_device(const hyt_gen_params *params, uint64_t device_budget_bytes, hyt_host_graph_t *out);
+
 /* Borrowed views of the host CSR: row_offsets u64[V+1], columns u32[E],
  * weights u32[E] (NULL when unweighted). Valid until hyt_host_graph_free. */
 int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices,
