@@ -2,6 +2,13 @@
 // edges stay in pinned, device-mapped host memory (HyTGraph's hybrid transfer
 // management, executed rather than modelled).
 //
+// Pageable host graphs (too large to pin: the boxes pin at most 64 GiB in
+// total, an RMAT scale-30 CSR is 146 GB) run without zero-copy: the cost
+// model chooses between filter and compaction only (its zero-copy cost taken
+// as infinite), filter copies are staged through pinned bounce buffers by the
+// host threads while the DMA of the previous chunk runs, and a row too long
+// for the packed staging streams through the ring in windows.
+//
 // Resident in HBM: row offsets, vertex state, the frontier, a partition table,
 // and a budgeted edge area holding
 //   * a streaming ring: a few buffers of k partitions' edges each; a filter
@@ -207,6 +214,11 @@ namespace hyt
         };
 
         constexpr std::size_t kRingBuffers = 4;
+        // staged copies from pageable host graphs: bounce buffers of
+        // kBounceBytes each, filled by the host threads while the copy engine
+        // moves the previous one
+        constexpr std::size_t kBounceBuffers = 4;
+        constexpr std::uint64_t kBounceBytes = 64ULL << 20;
 
         struct EventPair
         {
@@ -230,10 +242,16 @@ namespace hyt
 
     struct HybridEngine
     {
-        // host graph (borrowed, pinned + mapped)
+        // host graph (borrowed; pinned + mapped, or pageable)
         HostGraph *host = nullptr;
         bool weights_f32 = false;
-        const std::uint32_t *z_col = nullptr, *z_w = nullptr; // device views of host arrays
+        const std::uint32_t *z_col = nullptr, *z_w = nullptr; // device views of host arrays (pinned graphs)
+        // pageable graphs: pinned bounce buffers for the staged filter copies
+        bool pageable = false;
+        std::uint32_t *bounce[kBounceBuffers] = {};
+        hipEvent_t bounce_done[kBounceBuffers] = {};
+        bool bounce_busy[kBounceBuffers] = {};
+        std::size_t bounce_next = 0;
         // device graph: offsets + vertex workspace (no edges)
         DeviceGraph dg;
         std::vector<hyt_partition> parts;
@@ -310,6 +328,13 @@ namespace hyt
             }
             if (ev_pack)
                 (void)hipEventDestroy(ev_pack);
+            for (std::size_t b = 0; b < kBounceBuffers; ++b)
+            {
+                if (bounce_done[b])
+                    (void)hipEventDestroy(bounce_done[b]);
+                if (bounce[b])
+                    (void)hipHostFree(bounce[b]);
+            }
             for (void *p : {static_cast<void *>(h_rows), static_cast<void *>(h_col), static_cast<void *>(h_w),
                             static_cast<void *>(h_val), static_cast<void *>(h_index), static_cast<void *>(h_q),
                             static_cast<void *>(h_qd)})
@@ -363,6 +388,48 @@ namespace hyt
             hip_check(hipEventRecord(e.a, copy), "hipEventRecord");
             hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, copy), "copy partition");
             hip_check(hipEventRecord(e.b, copy), "hipEventRecord");
+        }
+
+        // Host graph range -> device: one DMA from a pinned graph; from a
+        // pageable graph, kBounceBytes chunks copied by the host threads into
+        // the next free bounce buffer, each then moved by the copy engine
+        // (the copy stream's last event covers the last chunk).
+        void copy_host_graph(void *dst, const void *src, std::uint64_t bytes)
+        {
+            if (!pageable)
+            {
+                copy_h2d(dst, src, bytes);
+                return;
+            }
+            for (std::uint64_t off = 0; off < bytes; off += kBounceBytes)
+            {
+                const std::uint64_t n = std::min<std::uint64_t>(kBounceBytes, bytes - off);
+                const std::size_t b = bounce_next;
+                bounce_next = (bounce_next + 1) % kBounceBuffers;
+                if (bounce_busy[b])
+                    hip_check(hipEventSynchronize(bounce_done[b]), "wait bounce buffer");
+                const char *from = static_cast<const char *>(src) + off;
+                char *to = reinterpret_cast<char *>(bounce[b]);
+                const int T = static_cast<int>(std::max<std::uint64_t>(
+                    1, std::min<std::uint64_t>(static_cast<std::uint64_t>(threads), n >> 22)));
+                if (T == 1)
+                    std::memcpy(to, from, n);
+                else
+                {
+                    std::vector<std::thread> pool;
+                    for (int t = 0; t < T; ++t)
+                        pool.emplace_back([=] {
+                            const std::uint64_t lo = (n * t / T) & ~63ULL;
+                            const std::uint64_t hi = t + 1 == T ? n : ((n * (t + 1) / T) & ~63ULL);
+                            std::memcpy(to + lo, from + lo, hi - lo);
+                        });
+                    for (auto &th : pool)
+                        th.join();
+                }
+                copy_h2d(static_cast<char *>(dst) + off, bounce[b], n);
+                hip_check(hipEventRecord(bounce_done[b], copy), "hipEventRecord");
+                bounce_busy[b] = true;
+            }
         }
 
         std::uint32_t *unit_ptr(std::int64_t u) const
@@ -550,10 +617,25 @@ namespace hyt
         void calibrate(HybridEngine &E)
         {
             hipStream_t s = E.dg.stream;
-            // pinned H2D peak: copy up to 256 MiB of the host column array
+            // H2D peak: copy up to 256 MiB of the host column array (a
+            // pageable graph: through the staged path its filter copies take,
+            // timed on the host)
             const std::uint64_t bytes = std::min<std::uint64_t>(256ULL << 20, E.host->E * 4);
             DeviceBuffer &dst = E.unit_mem.bytes >= bytes ? E.unit_mem : E.ring_mem;
-            if (bytes >= (4ULL << 20) && dst.bytes >= bytes)
+            if (E.pageable && bytes >= (4ULL << 20) && dst.bytes >= bytes)
+            {
+                double sec = 0;
+                for (int rep = 0; rep < 2; ++rep)
+                {
+                    const auto t0 = Clock::now();
+                    E.copy_host_graph(dst.ptr, E.host->col, bytes);
+                    hip_check(hipStreamSynchronize(E.copy), "sync");
+                    sec = std::chrono::duration<double>(Clock::now() - t0).count();
+                }
+                (void)E.drain_copy_time();
+                E.link_gbs = bytes / sec / 1e9;
+            }
+            else if (bytes >= (4ULL << 20) && dst.bytes >= bytes)
             {
                 hipEvent_t a, b;
                 hip_check(hipEventCreate(&a), "hipEventCreate");
@@ -632,7 +714,10 @@ namespace hyt
         void create(HybridEngine &E, HostGraph *host, bool f32, const hyt_hybrid_config &cfg)
         {
             require(host != nullptr, "null host graph");
-            require(host->memory == HYT_HOST_PINNED, "the hybrid engine needs a pinned host graph");
+            require(host->memory == HYT_HOST_PINNED || host->memory == HYT_HOST_PAGEABLE, "unknown host memory kind");
+            E.pageable = host->memory == HYT_HOST_PAGEABLE;
+            require(!E.pageable || cfg.force_engine != HYT_ENGINE_ZERO_COPY,
+                    "zero-copy needs a pinned host graph (a pageable graph runs filter and compaction only)");
             require(host->V > 0 && host->V <= 0xFFFFFFFFULL, "vertex count must be in [1, 2^32-1]");
             require(cfg.partition_edges > 0, "partition_edges must be > 0");
             E.host = host;
@@ -659,13 +744,25 @@ namespace hyt
             hip_check(hipEventCreate(&g.ev1), "hipEventCreate");
             hip_check(hipEventCreateWithFlags(&E.ev_pack, hipEventDisableTiming), "hipEventCreate");
             g.host_ctr = std::make_unique<PinnedBuffer>(sizeof(Counters) + sizeof(RoundCtl));
-            void *dp = nullptr;
-            hip_check(hipHostGetDevicePointer(&dp, host->col, 0), "hipHostGetDevicePointer(columns)");
-            E.z_col = static_cast<const std::uint32_t *>(dp);
-            if (host->weighted)
+            if (E.pageable)
             {
-                hip_check(hipHostGetDevicePointer(&dp, host->w, 0), "hipHostGetDevicePointer(weights)");
-                E.z_w = static_cast<const std::uint32_t *>(dp);
+                for (std::size_t b = 0; b < kBounceBuffers; ++b)
+                {
+                    hip_check(hipHostMalloc(reinterpret_cast<void **>(&E.bounce[b]), kBounceBytes, hipHostMallocDefault),
+                              "hipHostMalloc(bounce buffer)");
+                    hip_check(hipEventCreateWithFlags(&E.bounce_done[b], hipEventDisableTiming), "hipEventCreate");
+                }
+            }
+            else
+            {
+                void *dp = nullptr;
+                hip_check(hipHostGetDevicePointer(&dp, host->col, 0), "hipHostGetDevicePointer(columns)");
+                E.z_col = static_cast<const std::uint32_t *>(dp);
+                if (host->weighted)
+                {
+                    hip_check(hipHostGetDevicePointer(&dp, host->w, 0), "hipHostGetDevicePointer(weights)");
+                    E.z_w = static_cast<const std::uint32_t *>(dp);
+                }
             }
             if (g.off64)
                 upload_offsets<std::uint64_t>(E);
@@ -848,7 +945,13 @@ namespace hyt
                     m.zc_requests = c[2];
                     m.zc_alignment = c[3];
                     hytm::finish_metrics(m, E.run_params);
-                    engine = hytm::evaluate(m, E.run_params).engine;
+                    const hytm::Costs cst = hytm::evaluate(m, E.run_params);
+                    engine = cst.engine;
+                    // a pageable graph has no zero-copy: the strict selector
+                    // with its cost infinite (compaction when T_c < alpha T_f)
+                    if (E.pageable && engine == HYT_ENGINE_ZERO_COPY)
+                        engine = cst.compaction < E.run_params.alpha * cst.filter ? HYT_ENGINE_COMPACTION
+                                                                                  : HYT_ENGINE_FILTER;
                 }
                 engines.push_back(engine);
                 members.push_back(p);
@@ -1096,9 +1199,9 @@ namespace hyt
                         while (pb < p1 && E.parts[pb + 1].edge_begin < b)
                             ++pb;
                         const std::size_t r = E.ring_take();
-                        E.copy_h2d(E.ring_col(r), E.host->col + a, (b - a) * 4);
+                        E.copy_host_graph(E.ring_col(r), E.host->col + a, (b - a) * 4);
                         if (weighted)
-                            E.copy_h2d(E.ring_w(r), E.host->w + a, (b - a) * 4);
+                            E.copy_host_graph(E.ring_w(r), E.host->w + a, (b - a) * 4);
                         st.bytes_filter += (b - a) * E.run_edge_bytes;
                         hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait copy");
                         x.window(plan.qlo[pa], rows_of(pa, pb), a, b, E.ring_col(r), E.ring_w(r));
@@ -1168,8 +1271,34 @@ namespace hyt
                 hip_check(hipEventSynchronize(E.ev_pack), "wait packed staging");
                 {
                     // a row larger than the packed staging is read zero-copy
+                    // (a pageable graph: streamed through the ring in windows)
                     const std::uint32_t u = E.h_q[r0];
                     const std::uint64_t d = E.host->off[u + 1] - E.host->off[u];
+                    if (d > E.pack_cap && E.pageable)
+                    {
+                        E.h_rows[0] = u;
+                        E.h_val[0] = E.h_qd[r0];
+                        E.copy_h2d(E.pk_rows.ptr, E.h_rows, 4);
+                        E.copy_h2d(E.pk_val.ptr, E.h_val, 4);
+                        const std::uint64_t e0 = E.host->off[u], e1 = E.host->off[u + 1];
+                        for (std::uint64_t a = e0; a < e1; a += E.ring_edges)
+                        {
+                            const std::uint64_t b = std::min<std::uint64_t>(e1, a + E.ring_edges);
+                            const std::size_t r = E.ring_take();
+                            E.copy_host_graph(E.ring_col(r), E.host->col + a, (b - a) * 4);
+                            if (weighted)
+                                E.copy_host_graph(E.ring_w(r), E.host->w + a, (b - a) * 4);
+                            hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0),
+                                      "wait row copy");
+                            x.list(E.pk_rows.as<std::uint32_t>(), E.pk_val.as<std::uint32_t>(), 1U, a, b,
+                                   E.ring_col(r), weighted ? E.ring_w(r) : nullptr);
+                            E.ring_release(r);
+                        }
+                        hip_check(hipEventRecord(E.ev_pack, s), "hipEventRecord");
+                        st.bytes_compaction += d * E.run_edge_bytes;
+                        ++r0;
+                        continue;
+                    }
                     if (d > E.pack_cap)
                     {
                         E.h_rows[0] = u;

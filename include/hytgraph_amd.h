@@ -490,8 +490,12 @@ typedef struct hyt_hybrid_stats {
 
 int hyt_hybrid_config_default(hyt_hybrid_config *out);
 
-/* The host graph must be pinned (HYT_HOST_PINNED) and outlive the engine.
- * weights_f32: weights are float bit patterns. */
+/* The host graph must outlive the engine. Pinned graphs (HYT_HOST_PINNED)
+ * get all three movers; pageable graphs (HYT_HOST_PAGEABLE: graphs larger
+ * than what can be pinned) get filter and compaction only - the cost model
+ * takes zero-copy as infinitely expensive, filter copies are staged through
+ * pinned bounce buffers, and force_engine = HYT_ENGINE_ZERO_COPY is
+ * rejected. weights_f32: weights are float bit patterns. */
 int hyt_hybrid_create(hyt_host_graph_t host, uint32_t weights_f32, const hyt_hybrid_config *config,
                       hyt_hybrid_t *out);
 int hyt_hybrid_free(hyt_hybrid_t h);
