@@ -466,12 +466,17 @@ def bench_serve(args) -> dict:
 
     scales = [int(x) for x in args.serve_scales.split(",")]
     host_scales = {int(x) for x in args.serve_host.split(",") if x}
+    pageable_scales = {int(x) for x in args.serve_pageable.split(",") if x}
     graphs, hosts, pools = [], [], []
     for i, sc in enumerate(scales):
         t0 = time.perf_counter()
         on_host = sc in host_scales
+        # generated on the device (the host generator's graphs bit for bit);
+        # host-resident graphs pinned unless listed as pageable (too large to pin)
+        pinned = on_host and sc not in pageable_scales
         hg = abi.HostGraph(scale=sc, edge_factor=args.edge_factor, seed=args.seed + i, weight_min=1,
-                           weight_max=255, memory=abi.HYT_HOST_PINNED if on_host else abi.HYT_HOST_PAGEABLE)
+                           weight_max=255, memory=abi.HYT_HOST_PINNED if pinned else abi.HYT_HOST_PAGEABLE,
+                           on_device=True)
         if on_host:
             cfg = dict(device_budget_bytes=args.serve_edge_gb << 30, partition_edges=1 << args.part_log2)
             g = _ServedGraph(engines=(abi.HybridEngine(hg, **cfg), abi.HybridEngine(hg, **cfg)))
@@ -481,7 +486,8 @@ def bench_serve(args) -> dict:
         hosts.append(hg)
         pools.append(abi.sample_sources(hg.offsets, args.seed + 100 + i, 4096))
         log(f"graph {i}: RMAT scale {sc} V={hg.V} E={hg.E} ready in {time.perf_counter() - t0:.1f} s"
-            + (f" (host memory, hybrid engines with {args.serve_edge_gb} GiB edge budget each)" if on_host else ""))
+            + (f" ({'pinned' if pinned else 'pageable'} host memory, hybrid engines with {args.serve_edge_gb} "
+               f"GiB edge budget each)" if on_host else ""))
     core = _serve_core(abi, torch, graphs, hosts, scales, pools, args, host_scales, args.serve_requests,
                        args.serve_poisson or args.serve_requests, args.serve_baseline)
     eps = core["eps"]
@@ -501,12 +507,14 @@ def bench_serve(args) -> dict:
         "config": {
             "workload": f"served mixed SSSP (70%) / PPR (30%, eps {eps:g}, d 0.85) queries on degree-weighted "
                         f"sources over RMAT scales {scales} ef{args.edge_factor} (u32 weights [1,255]), "
-                        + (f"scales {sorted(host_scales)} kept in pinned host memory behind hybrid engines "
-                           f"({args.serve_edge_gb} GiB edge budget each, one per query kind), the rest in HBM, "
+                        + (f"scales {sorted(host_scales)} kept in host memory behind hybrid engines "
+                           f"({args.serve_edge_gb} GiB edge budget each, one per query kind; pageable: "
+                           f"{sorted(pageable_scales & host_scales)}, the others pinned), the rest in HBM, "
                            if host_scales else "all in HBM, ")
                         + f"{args.budget_gb} GiB device budget, continuous batching into K-lane sessions",
             "scales": scales,
             "host_resident_scales": sorted(host_scales),
+            "pageable_scales": sorted(pageable_scales & host_scales),
             "budget_gib": args.budget_gb,
             "requests": core["burst_requests"],
         },
@@ -769,6 +777,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-host", default="", help="serve: scales kept in host memory behind hybrid engines")
     ap.add_argument("--serve-poisson", type=int, default=0, help="serve: Poisson-trace requests (0 = --serve-requests)")
     ap.add_argument("--serve-edge-gb", type=int, default=4, help="serve: HBM edge budget per hybrid engine (GiB)")
+    ap.add_argument("--serve-pageable", default="",
+                    help="serve: host-resident scales kept in pageable memory (graphs too large to pin)")
     ap.add_argument("--ppr-eps", type=float, default=1e-6)
     ap.add_argument("--ks", default="8,32,64", help="batched: lanes per call")
     ap.add_argument("--copy-mib", type=int, default=1024, help="copy: MiB per copy")
