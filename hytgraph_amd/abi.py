@@ -76,6 +76,7 @@ class RunOptions(C.Structure):
         ("task_order", C.c_uint32),
         ("heavy_delta", C.c_uint32),
         ("persistent", C.c_uint32),
+        ("wait_degree", C.c_uint32),
     ]
 
 
@@ -448,10 +449,10 @@ HYT_LANES_ITEMS = 2
 def _options(mode=HYT_SSSP_SYNC, out_on_device=0, max_rounds=0, time_kernels=0,
              light_max_degree=0, heavy_chunk=0, direction=HYT_DIR_AUTO, pull_percent=0, delta=0,
              lane_form=HYT_LANES_AUTO, tolerance=0.0, host_rounds=0, task_order=0,
-             heavy_delta=0, persistent=0) -> RunOptions:
+             heavy_delta=0, persistent=0, wait_degree=0) -> RunOptions:
     return RunOptions(mode, out_on_device, max_rounds, time_kernels, light_max_degree, heavy_chunk,
                       direction, pull_percent, delta, lane_form, tolerance, host_rounds, task_order,
-                      heavy_delta, persistent)
+                      heavy_delta, persistent, wait_degree)
 
 
 class DeviceGraph:

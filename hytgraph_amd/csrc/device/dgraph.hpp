@@ -218,6 +218,13 @@ namespace hyt
         return option == 0 ? kDefaultHeavyDelta : (option == 0xFFFFFFFFU ? 0U : option);
     }
     constexpr std::uint64_t kDefaultPullPercentSssp = 101;
+    // ASYNC single-source SSSP: light rows of more than this many edges wait
+    // under the same bound as the rows above light_max (option 0: the default,
+    // light_max - only chunked rows wait).
+    inline std::uint32_t wait_degree_of(std::uint32_t option, std::uint32_t light_max)
+    {
+        return option == 0 ? light_max : option;
+    }
 
     // Allocates the single-source workspace if needed (sized for the graph).
     void ensure_workspace(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);

@@ -141,6 +141,9 @@ namespace hyt
         // device when heavy_bound_dev is set (device-driven rounds)
         std::uint32_t heavy_bound = 0xFFFFFFFFU;
         const std::uint32_t *heavy_bound_dev = nullptr;
+        // light rows (pass 2) of more than wait_degree edges wait under the
+        // same bound (0xFFFFFFFF: light rows never wait)
+        std::uint32_t wait_degree = 0xFFFFFFFFU;
     };
 
     // Processes `kUnroll` x 64 edges per iteration: all loads first, then all
@@ -278,8 +281,13 @@ namespace hyt
                 else if (!chunked && a.pass != 1)
                 {
                     value = op.row_value(vi, u, full);
-                    deg = static_cast<std::uint32_t>(count);
-                    op.expanded(u, value, whole);
+                    if (count > a.wait_degree && value > hbound)
+                        op.deferred(u);
+                    else
+                    {
+                        deg = static_cast<std::uint32_t>(count);
+                        op.expanded(u, value, whole);
+                    }
                 }
             }
             // the chunk-queuing passes (block-uniform: the pass is a kernel argument)

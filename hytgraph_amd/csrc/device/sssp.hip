@@ -392,6 +392,7 @@ namespace hyt
             hipStream_t s = g.stream;
             const bool timed = L.o.time_kernels != 0;
             const std::uint32_t heavy_delta = SYNC ? 0U : heavy_delta_of(L.o.heavy_delta);
+            const std::uint32_t wait_degree = wait_degree_of(L.o.wait_degree, L.light_max);
             constexpr bool kFloatDist = WM == kWF32 || WM == kWUnitF32;
             a.F_dev = &B.rc->F;
             const unsigned light_grid = std::min(blocks_for(g.V), kDeviceLightGrid);
@@ -399,7 +400,7 @@ namespace hyt
             const GraphRows<OffT> rows = whole_graph(g.off.as<OffT>(), B.q);
             const std::vector<std::uint64_t> key{
                 static_cast<std::uint64_t>(sizeof(OffT)), static_cast<std::uint64_t>(WM), SYNC ? 1ULL : 0ULL,
-                timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap, heavy_delta,
+                timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap, heavy_delta, wait_degree,
                 reinterpret_cast<std::uint64_t>(B.dist), reinterpret_cast<std::uint64_t>(B.prev),
                 reinterpret_cast<std::uint64_t>(B.q), reinterpret_cast<std::uint64_t>(B.qd),
                 reinterpret_cast<std::uint64_t>(B.heavy), reinterpret_cast<std::uint64_t>(B.ctr),
@@ -428,6 +429,8 @@ namespace hyt
                         a1.pass = 1;
                         a1.heavy_bound_dev = &B.rc->heavy_bound;
                         a2.pass = 2;
+                        a2.heavy_bound_dev = &B.rc->heavy_bound;
+                        a2.wait_degree = wait_degree;
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a1);
                         expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a2);
@@ -547,6 +550,7 @@ namespace hyt
             // rows above light_max wait while far above the queue's smallest
             // distance (ASYNC): the compaction records that distance
             const std::uint32_t heavy_delta = SYNC ? 0U : heavy_delta_of(L.o.heavy_delta);
+            const std::uint32_t wait_degree = wait_degree_of(L.o.wait_degree, L.light_max);
             constexpr bool kFloatDist = WM == kWF32 || WM == kWUnitF32;
             sel.track_min = heavy_delta > 0;
 
@@ -650,6 +654,8 @@ namespace hyt
                     a1.pass = 1;
                     a1.heavy_bound = heavy_bound_of(kept_min, heavy_delta, kFloatDist);
                     a2.pass = 2;
+                    a2.heavy_bound = a1.heavy_bound;
+                    a2.wait_degree = wait_degree;
                     expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<blocks_for(F), kBlock, 0, s>>>(
                         whole_graph(g.off.as<OffT>(), a.q), op, a1);
                     hip_check(hipGetLastError(), "launch sssp_expand_light");

@@ -268,6 +268,10 @@ typedef struct hyt_run_options {
                                   and no host read-back); SYNC keeps the
                                   original's iteration count. 0 = the rounds
                                   host_rounds selects */
+    uint32_t wait_degree;      /* ASYNC single-source SSSP: light rows with more
+                                  edges than this wait under the heavy_delta
+                                  bound as well (0 = light_max: only the rows
+                                  above light_max wait). Same distances */
 } hyt_run_options;
 
 /* Work split of the K-lane rounds (batched queries and lane sessions). */

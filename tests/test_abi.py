@@ -108,9 +108,11 @@ def test_hybrid_config_defaults_and_host_checks(abi):
     c = abi.hybrid_config()
     assert c.device_budget_bytes == 4 << 30 and c.partition_edges == 1 << 20 and c.force_engine == -1
     assert c.cost.alpha == 0.8 and c.cost.entry_bytes == 8
+    # a pageable host graph runs without zero-copy: forcing it is refused
+    # before any device work
     pageable = abi.HostGraph(scale=8, edge_factor=4, seed=1)
     with pytest.raises(abi.InvalidArgument):
-        abi.HybridEngine(pageable)
+        abi.HybridEngine(pageable, force_engine=abi.HYT_ENGINE_ZERO_COPY)
 
 
 def test_struct_layouts_match_header(abi, tmp_path):

@@ -351,3 +351,26 @@ def test_scale20_invariants(gpu, abi, orc):
     reached[src] = False
     assert np.all(tight[reached])
     assert np.array_equal(d, orc.sssp_dijkstra(off, col, w, src))
+
+
+@pytest.mark.parametrize("wait_degree", [8, 32, 100])
+def test_waiting_light_rows_same_distances(gpu, abi, orc, wait_degree):
+    """ASYNC rounds where light rows above wait_degree edges also wait under
+    the heavy_delta bound: the same distances (RMAT, ER, float weights;
+    device-driven and host-driven rounds)."""
+    for kind in (0, 1):
+        hg = rmat(abi, 14, 17, wmax=255, kind=kind)
+        g = abi.DeviceGraph.from_host(hg)
+        for src in abi.sample_sources(hg.offsets, 18, 2):
+            ref = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+            for hd in (1, 2, 8):
+                _say(f"kind {kind} src {src} wait_degree {wait_degree} heavy_delta {hd}")
+                d, st = g.sssp(int(src), mode=1, heavy_delta=hd, wait_degree=wait_degree)
+                assert np.array_equal(d, ref) and st["converged"] == 1
+                dh, _ = g.sssp(int(src), mode=1, heavy_delta=hd, wait_degree=wait_degree, host_rounds=1)
+                assert np.array_equal(dh, ref)
+    hf = rmat(abi, 12, 19)
+    gf = abi.DeviceGraph.from_host(hf, weights_f32=True)
+    ref, _, _ = orc.sssp_jacobi_f32(hf.offsets, hf.columns, hf.weights.astype(np.float32), 0)
+    d, _ = gf.sssp(0, mode=1, wait_degree=wait_degree)
+    assert np.array_equal(d, ref.view(np.uint32))
