@@ -162,10 +162,12 @@ def _hybrid_one(abi, kind: int, args) -> dict:
 
     name = "RMAT" if kind == abi.HYT_GEN_RMAT else "ER"
     t0 = time.perf_counter()
+    # generated on the device (the host generator's graphs bit for bit)
     hg = abi.HostGraph(kind=kind, scale=args.scale, edge_factor=args.edge_factor, seed=args.seed, weight_min=1,
-                       weight_max=255, memory=abi.HYT_HOST_PINNED)
-    log(f"{name} scale {args.scale}: V={hg.V} E={hg.E} generated into pinned memory in "
-        f"{time.perf_counter() - t0:.1f} s")
+                       weight_max=255, memory=abi.HYT_HOST_PAGEABLE if args.host_pageable else abi.HYT_HOST_PINNED,
+                       on_device=True)
+    log(f"{name} scale {args.scale}: V={hg.V} E={hg.E} generated into "
+        f"{'pageable' if args.host_pageable else 'pinned'} memory in {time.perf_counter() - t0:.1f} s")
     deg = np.diff(hg.offsets.astype(np.int64))
     sources = abi.sample_sources(hg.offsets, args.seed + 1, args.sources)
     t0 = time.perf_counter()
@@ -207,6 +209,22 @@ def _hybrid_one(abi, kind: int, args) -> dict:
     moved = sum(q["bytes_copied"] for q in queries)
     copy_ms = sum(q["copy_ms"] for q in queries)
     out["sssp_copy_gbs"] = moved / (copy_ms / 1e3) / 1e9 if copy_ms > 0 else None
+
+    if args.hybrid_k > 1:
+        # K sources per run: each moved partition serves every lane active in it
+        ks = abi.sample_sources(hg.offsets, args.seed + 3, args.hybrid_k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dk, kst = eng.sssp_k(ks)
+        sec = time.perf_counter() - t0
+        moved_k = kst["bytes_filter"] + kst["bytes_compaction"]
+        out["sssp_k"] = {"K": int(args.hybrid_k), "seconds": sec, "queries_per_s": args.hybrid_k / sec,
+                         "speedup_vs_one_at_a_time": (args.hybrid_k / sec) / (len(queries) / tot_sec),
+                         "rounds": kst["run"]["rounds"], "bytes_copied": moved_k,
+                         "bytes_zero_copy": kst["bytes_zero_copy"], "copy_ms": kst["copy_ms"]}
+        log(f"{name} hybrid sssp K={args.hybrid_k}: {sec:.2f} s = {args.hybrid_k / sec:.3f} queries/s "
+            f"({out['sssp_k']['speedup_vs_one_at_a_time']:.2f}x one at a time), rounds {kst['run']['rounds']}, "
+            f"copied {moved_k / 1e9:.1f} GB")
 
     if not args.no_pagerank:
         torch.cuda.synchronize()
@@ -265,7 +283,8 @@ def bench_hybrid(args) -> dict:
     cost model (recalibrated on this host link) picks, HBM edge budget capped."""
     from hytgraph_amd import abi
 
-    graphs = [_hybrid_one(abi, k, args) for k in (abi.HYT_GEN_RMAT, abi.HYT_GEN_ER)]
+    kinds = {"rmat": abi.HYT_GEN_RMAT, "er": abi.HYT_GEN_ER}
+    graphs = [_hybrid_one(abi, kinds[k], args) for k in args.hybrid_kinds.split(",") if k]
     rm = graphs[0]
     peak = rm["link_gbs_peak"]
     copy_gbs = rm["sssp_copy_gbs"] or 0.0
@@ -283,8 +302,10 @@ def bench_hybrid(args) -> dict:
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"hybrid out-of-HBM SSSP + PageRank, RMAT and ER scale-{args.scale} ef{args.edge_factor}, "
-                        f"u32 weights [1,255], graph in pinned host memory, {args.budget_gb} GiB HBM edge budget, "
+            "workload": f"hybrid out-of-HBM SSSP" + ("" if args.no_pagerank else " + PageRank")
+                        + f", {args.hybrid_kinds} scale-{args.scale} ef{args.edge_factor}, "
+                        f"u32 weights [1,255], graph in {'pageable' if args.host_pageable else 'pinned'} host "
+                        f"memory, {args.budget_gb} GiB HBM edge budget, "
                         f"2^{args.part_log2}-edge partitions, {args.sources} degree-weighted sources (first cold)",
             "scale": args.scale,
             "budget_gib": args.budget_gb,
@@ -768,6 +789,10 @@ def main(argv=None) -> int:
     ap.add_argument("--part-log2", type=int, default=20, help="hybrid: log2 edges per partition")
     ap.add_argument("--pr-eps", type=float, default=1e-7)
     ap.add_argument("--no-check", action="store_true", help="hybrid: skip the in-HBM bit-exact check")
+    ap.add_argument("--host-pageable", action="store_true",
+                    help="hybrid: host graphs in pageable memory (graphs too large to pin; no zero-copy)")
+    ap.add_argument("--hybrid-kinds", default="rmat,er", help="hybrid: graph kinds, comma list of rmat / er")
+    ap.add_argument("--hybrid-k", type=int, default=0, help="hybrid: also run one K-source SSSP batch")
     ap.add_argument("--task-order", type=int, default=0,
                     help="hybrid: 1 = filter runs in contribution order (ContributionScheduler), 0 = partition order")
     ap.add_argument("--serve-scales", default="22,24,26", help="serve: RMAT scales of the graphs")
