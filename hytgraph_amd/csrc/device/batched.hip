@@ -57,7 +57,7 @@ namespace hyt
                 kw.qmask.alloc((g.V + kPad) * 8, "hipMalloc(queue masks)");
                 hip_check(hipMemsetAsync(kw.mark.ptr, 0, (g.V + kPad) * 8, g.stream), "clear marks");
             }
-            const std::uint64_t cap = lane_chunk_capacity(g.V, g.E, light_max, chunk);
+            const std::uint64_t cap = lane_chunk_capacity_k(g.V, g.E, light_max, chunk, K);
             if (kw.heavy.bytes < cap * sizeof(HeavyChunkK))
                 kw.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(heavy chunks)");
             kw.heavy_cap = kw.heavy.bytes / sizeof(HeavyChunkK);

@@ -185,12 +185,14 @@ namespace hyt
     constexpr std::uint32_t kLaneLightMaxDegree = 256;
     constexpr std::uint32_t kLaneHeavyChunk = 1024;
     // Batched calls (hyt_sssp / hyt_ppr with K >= 2, graph in HBM): RMAT 24,
-    // one MI355X, 64 / 512 ran K = 8 SSSP at 228 queries/s and K = 64 at 363
-    // against 199 and 328 at 256 / 1024 (PPR 242 / 365 against 230 / 361);
-    // chunks of 256 edges halve K = 64 SSSP. Lane sessions keep 256 / 1024:
-    // their chunk queue is fixed per-session state, which 64 / 512 doubles.
-    constexpr std::uint32_t kBatchedLightMaxDegree = 64;
-    constexpr std::uint32_t kBatchedHeavyChunk = 512;
+    // one MI355X, against the current kernels: 32 / 256 ran K = 8 SSSP at
+    // 308 queries/s and K = 64 at 402 against 290 and 385 at 64 / 512
+    // (16 / 128 and 8 / 128 within 1%, 128 / 1024 and 256 / 1024 slower:
+    // 264 and 256 at K = 8); PPR within noise of every split. Lane sessions
+    // keep 256 / 1024: their chunk queue is fixed per-session state, which
+    // smaller splits enlarge.
+    constexpr std::uint32_t kBatchedLightMaxDegree = 32;
+    constexpr std::uint32_t kBatchedHeavyChunk = 256;
     // Hybrid engine expansions read windows of partitions, many of them over
     // the host link (zero-copy): short light rows and small chunks keep many
     // wavefronts with reads in flight. RMAT scale 27, 4 GiB budget, one SSSP

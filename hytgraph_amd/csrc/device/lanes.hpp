@@ -463,6 +463,23 @@ namespace hyt
             return E / chunk + E / (static_cast<std::uint64_t>(light_max) + 1) + E / 256 + V / 16 + 64;
         }
 
+        // Capacity that holds every chunk of a round of K lanes: group-form
+        // chunks (E / chunk + one per row above light_max) or item-form
+        // chunks, which are max(16, 4 chunk / c) edges for c active queries,
+        // so at most E K / (4 chunk) + one per row. Batched calls size their
+        // queue with it; a queue below it moves hub rows back into the light
+        // walk, where one wavefront takes a whole hub (measured: K = 64 SSSP
+        // on RMAT 24 at split 32 / 256, 98 instead of 400 queries/s).
+        inline std::uint64_t lane_chunk_capacity_k(std::uint64_t V, std::uint64_t E, std::uint32_t light_max,
+                                                   std::uint32_t chunk, std::uint32_t K)
+        {
+            const std::uint64_t rows = E / (static_cast<std::uint64_t>(light_max) + 1) + 1;
+            const std::uint64_t group = E / chunk + rows;
+            // (+1/32: the chunk length is 4 chunk / c rounded down)
+            const std::uint64_t items = E * K / (4ULL * chunk) + E * K / (128ULL * chunk) + rows;
+            return std::max(group, items) + V / 16 + 64;
+        }
+
         // Position of the r-th set bit (0-based) of m; m has more than r set bits.
         __device__ __forceinline__ std::uint32_t nth_set_bit(std::uint64_t m, std::uint32_t r)
         {
