@@ -1616,7 +1616,8 @@ namespace hyt
             }
             L.stage_mask.ensure(static_cast<std::uint64_t>(compact_blocks(g.V)) * kCompactRowsPerBlock * 8,
                                 "hipMalloc(stage masks)");
-            const std::uint64_t cap = lanes::lane_chunk_capacity(g.V, g.E, E.light_max, E.chunk);
+            const std::uint64_t cap = lanes::lane_chunk_capacity(g.V, g.E, E.light_max, E.chunk) +
+                                      static_cast<std::uint64_t>(K) * lanes::lane_item_chunks_per_lane(g.E, E.chunk);
             if (L.heavy.bytes < cap * sizeof(lanes::HeavyChunkK))
                 L.heavy.alloc(cap * sizeof(lanes::HeavyChunkK), "hipMalloc(heavy chunks)");
             L.heavy_cap = L.heavy.bytes / sizeof(lanes::HeavyChunkK);
@@ -1912,7 +1913,9 @@ namespace hyt
     {
         const std::uint64_t V = E.dg.V, Vp = V + kPad;
         // PPR: residual, rank and snapshot words (u64); SSSP: u32 distances
-        const std::uint64_t per_lane = kind == HYT_QUERY_PPR ? V * 24 : V * 4;
+        const std::uint64_t per_lane = (kind == HYT_QUERY_PPR ? V * 24 : V * 4) +
+                                       lanes::lane_item_chunks_per_lane(E.dg.E, kLaneHeavyChunk) *
+                                           sizeof(lanes::HeavyChunkK);
         const std::uint64_t fixed = Vp * 16 + static_cast<std::uint64_t>(compact_blocks(V)) * kCompactRowsPerBlock * 8 +
                                     lanes::lane_chunk_capacity(V, E.dg.E, kLaneLightMaxDegree, kLaneHeavyChunk) *
                                         sizeof(lanes::HeavyChunkK) +

@@ -463,6 +463,14 @@ namespace hyt
             return E / chunk + E / (static_cast<std::uint64_t>(light_max) + 1) + E / 256 + V / 16 + 64;
         }
 
+        // Item-form chunks one more lane can add at a chunk length (see
+        // lane_chunk_capacity_k): lane sessions and hybrid K-lane runs size
+        // their queue as lane_chunk_capacity + K of these.
+        inline std::uint64_t lane_item_chunks_per_lane(std::uint64_t E, std::uint32_t chunk)
+        {
+            return E / (4ULL * chunk) + E / (128ULL * chunk) + 1;
+        }
+
         // Capacity that holds every chunk of a round of K lanes: group-form
         // chunks (E / chunk + one per row above light_max) or item-form
         // chunks, which are max(16, 4 chunk / c) edges for c active queries,

@@ -239,10 +239,16 @@ namespace hyt
         // rows appended by admissions (a vertex may then appear twice).
         static constexpr std::uint64_t kAppend = 64;
 
-        // Heavy-chunk queue (a full queue only slows a round down).
+        // Heavy-chunk queue: the rows above light_max plus, per lane, the
+        // item-form chunks a lane can add (a full queue walks hub rows in the
+        // light kernel, one wavefront per hub).
         static std::uint64_t heavy_chunks(std::uint64_t V, std::uint64_t E)
         {
             return lanes::lane_chunk_capacity(V, E, kLaneLightMaxDegree, kLaneHeavyChunk);
+        }
+        static std::uint64_t heavy_chunks_per_lane(std::uint64_t E)
+        {
+            return lanes::lane_item_chunks_per_lane(E, kLaneHeavyChunk);
         }
         static std::uint64_t fixed_bytes(std::uint64_t V, std::uint64_t E, std::uint64_t /*max_degree*/)
         {
@@ -251,11 +257,13 @@ namespace hyt
                    Vp * 4 + (kGatherBlocks + 1) * sizeof(double) + sizeof(Counters) + 8 + 16;
         }
         // SSSP: u32 val [V][K]; PPR: u64 residual and rank [V][K] and
-        // snapshot [V+kAppend][K] words
-        static std::uint64_t lane_bytes(std::uint64_t V, int kind)
+        // snapshot [V+kAppend][K] words; plus the lane's share of the chunk
+        // queue
+        static std::uint64_t lane_bytes(std::uint64_t V, std::uint64_t E, int kind)
         {
             const std::uint64_t Vp = V + kPad;
-            return kind == HYT_QUERY_PPR ? Vp * 8 * 2 + (Vp + kAppend) * 8 : Vp * 4;
+            return (kind == HYT_QUERY_PPR ? Vp * 8 * 2 + (Vp + kAppend) * 8 : Vp * 4) +
+                   heavy_chunks_per_lane(E) * sizeof(HeavyChunkK);
         }
     };
 
@@ -307,7 +315,8 @@ namespace hyt
             S.mark.alloc(Vp * 8, "hipMalloc(lane marks)");
             S.qmask.alloc((Vp + QA) * 8, "hipMalloc(lane queue masks)");
             S.q.alloc((Vp + QA) * 4, "hipMalloc(lane queue)");
-            const std::uint64_t cap = LaneSession::heavy_chunks(g->V, g->E);
+            const std::uint64_t cap = LaneSession::heavy_chunks(g->V, g->E) +
+                                      static_cast<std::uint64_t>(K) * LaneSession::heavy_chunks_per_lane(g->E);
             S.heavy.alloc(cap * sizeof(HeavyChunkK), "hipMalloc(lane heavy chunks)");
             S.heavy_cap = cap;
             S.ctr.alloc(sizeof(Counters), "hipMalloc(lane counters)");
@@ -798,7 +807,7 @@ namespace hyt
                     if (sg.resident)
                     {
                         fixed.push_back(LaneSession::fixed_bytes(g.V, g.E, g.max_degree));
-                        per_lane.push_back(LaneSession::lane_bytes(g.V, kind));
+                        per_lane.push_back(LaneSession::lane_bytes(g.V, g.E, kind));
                     }
                     else
                     {
@@ -990,7 +999,8 @@ int hyt_lanes_state_bytes(uint64_t num_vertices, uint64_t num_edges, uint64_t ma
         hyt::require(bytes != nullptr, "null output");
         hyt::require(kind == HYT_QUERY_SSSP || kind == HYT_QUERY_PPR, "query kind must be SSSP or PPR");
         *bytes = hyt::LaneSession::fixed_bytes(num_vertices, num_edges, max_degree) +
-                 static_cast<std::uint64_t>(lanes) * hyt::LaneSession::lane_bytes(num_vertices, static_cast<int>(kind));
+                 static_cast<std::uint64_t>(lanes) *
+                     hyt::LaneSession::lane_bytes(num_vertices, num_edges, static_cast<int>(kind));
     });
 }
 
