@@ -109,6 +109,7 @@ namespace hytgraph::sep
         run_.reset();
         values_.clear();
         rounds_ = 0;
+        persistent_ = false;
     }
 
     SEPExecutionResult SEPSSSPGpuDriver::execute_step()
@@ -138,6 +139,56 @@ namespace hytgraph::sep
         if (finished)
             mark_converged();
         return round_result(st, finished);
+    }
+
+    SEPExecutionResult SEPSSSPGpuDriver::run_to_convergence()
+    {
+        persistent_ = false;
+        if (!initialized())
+            return SEPExecutionResult::not_initialized();
+        const graph::CSRGraph *g = graph_or_null();
+        if (g == nullptr)
+            return SEPExecutionResult::invalid_context();
+        if (converged())
+            return SEPExecutionResult::converged_result();
+        const SEPExecutionVariant &v = config().variant();
+        if ((!run_ || run_->host != g) && !v.is_pull() && !v.is_topology_driven())
+        {
+            // a fresh run: every round in one persistent launch
+            const auto &app = static_cast<const SEPSSSPApplication &>(application());
+            if (app.source() >= g->num_vertices())
+                throw std::invalid_argument("SSSP source vertex is out of range");
+            for (const float w : g->edge_weights())
+                if (!std::isfinite(w) || w < 0.0F)
+                    throw std::invalid_argument("SSSP requires finite non-negative edge weights");
+            hyt_run_options o = options_for(v);
+            run_ = std::make_shared<SEPGpuRun>(*g, true, HYT_STEP_SSSP, static_cast<std::uint32_t>(app.source()),
+                                               0.0F, 0.0F, o, "SEPSSSPGpuDriver");
+            o.persistent = 1;
+            const std::uint32_t src = static_cast<std::uint32_t>(app.source());
+            std::vector<std::uint32_t> bits(static_cast<std::size_t>(g->num_vertices()));
+            hyt_run_stats st{};
+            check(hyt_sssp(run_->g, &src, 1, bits.data(), &o, &st), "SEPSSSPGpuDriver");
+            values_.resize(bits.size());
+            if (!bits.empty())
+                std::memcpy(values_.data(), bits.data(), bits.size() * 4);
+            rounds_ = st.rounds;
+            persistent_ = true;
+            // the stepper of run_ never ran: a later step finds the run converged
+            mark_converged();
+            return round_result(st, true);
+        }
+        hyt_run_stats total{};
+        SEPExecutionResult r;
+        do
+        {
+            r = execute_step();
+            if (!r.succeeded())
+                return r;
+            total.frontier_total += r.vertices_processed;
+            total.updates_total += r.updates_generated;
+        } while (!r.reached_convergence());
+        return round_result(total, true);
     }
 
     void SEPPageRankGpuDriver::initialize(application_type &application, const execution_config_type &config)

@@ -14,6 +14,9 @@
 // the state after the last step (distances, or normalised ranks).
 // SEPExecutionFactory hands these drivers out for these applications.
 //
+// run_to_convergence() executes every remaining SSSP round at once; under
+// the DATA_DRIVEN PUSH variants it is one persistent kernel launch.
+//
 // Variant mapping (results are the same for every variant; the variant picks
 // how rounds run):
 //   SYNC  / ASYNC -> synchronous rounds (the original's Jacobi semantics and
@@ -148,6 +151,20 @@ namespace hytgraph::sep
     public:
         void initialize(application_type &application, const execution_config_type &config) override;
         step_result_type execute_step() override;
+        // Every remaining round in one call (no counterpart in the original's
+        // driver interface). A run not yet stepped under a DATA_DRIVEN PUSH
+        // variant (SYNC_PUSH_DD, ASYNC_PUSH_DD) executes as one persistent
+        // cooperative launch (hyt_run_options.persistent): the next queue is
+        // built on the device and no round returns to the host; SYNC keeps
+        // the original's iteration count (rounds()). Other variants, or a
+        // run already stepped, take the remaining rounds through the stepper.
+        // CONVERGED with the run's frontier rows and improvements.
+        step_result_type run_to_convergence();
+        // True when the last run_to_convergence() ran the persistent kernel.
+        bool ran_persistent() const noexcept { return persistent_; }
+
+    private:
+        bool persistent_ = false;
     };
 
     class SEPPageRankGpuDriver final : public SEPGpuDriverBase<double>

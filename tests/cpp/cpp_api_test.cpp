@@ -262,6 +262,52 @@ namespace
             expect(r.reached_convergence() == (k == 2), "SEP SYNC_PUSH_DD converges on round 3");
         }
         expect(drv.rounds() == 3, "SEP SYNC_PUSH_DD rounds equal sssp_cpu iterations");
+        // run_to_convergence: the DATA_DRIVEN PUSH variants run every round in
+        // one persistent launch (SYNC with the original's iteration count),
+        // the others through the stepper; the same distances either way
+        CSRGraph big = [] {
+            // a 3000-vertex graph with a hub and a long chain (many rounds)
+            std::vector<CSRGraph::offset_type> off{0};
+            std::vector<CSRGraph::vertex_id> col;
+            std::vector<float> w;
+            for (std::uint32_t u = 0; u < 3000; ++u)
+            {
+                if (u == 0)
+                    for (std::uint32_t v = 1; v < 1500; ++v)
+                    {
+                        col.push_back(v);
+                        w.push_back(static_cast<float>((v * 37) % 50 + 1));
+                    }
+                if (u + 1 < 3000)
+                {
+                    col.push_back(u + 1);
+                    w.push_back(static_cast<float>((u * 13) % 7 + 1));
+                }
+                off.push_back(col.size());
+            }
+            return CSRGraph(3000, off, col, w);
+        }();
+        hytgraph::algorithms::SSSPOptions so;
+        so.source = 0;
+        // sssp_gpu: the original's distances and iteration count (pinned by
+        // the golden cases)
+        const auto ref = hytgraph::algorithms::sssp_gpu(big, so);
+        for (const char *name : {"SYNC_PUSH_DD", "ASYNC_PUSH_DD", "SYNC_PULL_TD", "ASYNC_PUSH_TD"})
+        {
+            sep::SEPSSSPApplication rapp(0);
+            sep::SEPSSSPGpuDriver rd;
+            rd.initialize(rapp, sep::SEPExecutionConfig::from_string(name));
+            sep::SEPExecutionContext rctx(const_cast<CSRGraph *>(&big), nullptr, nullptr, nullptr, &frontier);
+            rd.set_context(rctx);
+            const auto r = rd.run_to_convergence();
+            const bool dd_push = std::string(name).find("PUSH_DD") != std::string::npos;
+            expect(r.reached_convergence() && rd.converged(), std::string("run_to_convergence converges: ") + name);
+            expect(rd.ran_persistent() == dd_push, std::string("persistent launch for the DD push variants: ") + name);
+            expect(rd.values() == ref.distances, std::string("run_to_convergence distances: ") + name);
+            if (std::string(name) == "SYNC_PUSH_DD")
+                expect(rd.rounds() == ref.iterations, "persistent SYNC rounds equal the original's iterations");
+            expect(rd.execute_step().status == sep::SEPExecutionStatus::CONVERGED, "converged run stays converged");
+        }
         // the factory binds SSSP / PageRank applications to the GPU drivers
         auto fd = sep::SEPExecutionFactory<float, float, float>::create("SYNC_PUSH_DD");
         sep::SEPSSSPApplication fapp(0);
