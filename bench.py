@@ -1067,10 +1067,10 @@ def main(argv=None) -> int:
     return 0
 
 
-def hybrid_leg(abi, torch, args, scale=25, budget_gib=1, sources=3) -> dict:
-    """The hybrid transfer engine inside the default run: RMAT scale 25
-    (u32 weights [1,255], about 4 GB of columns + weights) in pinned host memory
-    under a 1 GiB HBM edge budget, 2^20-edge partitions, SSSP from
+def hybrid_leg(abi, torch, args, scale=27, budget_gib=4, sources=2) -> dict:
+    """The hybrid transfer engine inside the default run: RMAT scale 27
+    (u32 weights [1,255], about 17 GB of columns + weights) in pinned host
+    memory under a 4 GiB HBM edge budget, 2^20-edge partitions, SSSP from
     degree-weighted sources (the first on a cold cache); host-link GB/s of
     the filter + compaction copies against the measured pinned H2D peak, and
     the distances checked bit for bit against the same graph in HBM."""
