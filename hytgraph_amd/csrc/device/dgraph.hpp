@@ -20,6 +20,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 
 namespace hyt
@@ -220,6 +221,27 @@ namespace hyt
         return option == 0 ? kDefaultHeavyDelta : (option == 0xFFFFFFFFU ? 0U : option);
     }
     constexpr std::uint64_t kDefaultPullPercentSssp = 101;
+    // Distance bound of the waiting rows this round: the smallest distance in
+    // the round's queue plus heavy_delta (float weights: added as a float);
+    // 0xFFFFFFFF (no waiting) when heavy_delta is 0 or no distance was
+    // recorded.
+    __host__ __device__ inline std::uint32_t heavy_bound_of(std::uint32_t queue_min, std::uint32_t heavy_delta,
+                                                           bool float_dist)
+    {
+        if (heavy_delta == 0 || queue_min == 0xFFFFFFFFU)
+            return 0xFFFFFFFFU;
+        if (float_dist)
+        {
+            float f;
+            std::memcpy(&f, &queue_min, 4);
+            f += static_cast<float>(heavy_delta);
+            std::uint32_t b;
+            std::memcpy(&b, &f, 4);
+            return b;
+        }
+        const std::uint64_t b = static_cast<std::uint64_t>(queue_min) + heavy_delta;
+        return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
+    }
     // ASYNC single-source SSSP: light rows of more than this many edges wait
     // under the same bound as the rows above light_max (option 0: the default,
     // light_max - only chunked rows wait).

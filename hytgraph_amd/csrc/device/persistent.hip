@@ -22,16 +22,24 @@
 // 1.29 / 1.12; RMAT 20 2.73 / 2.62 vs 2.56 / 2.35; RMAT 24 16.0 / 22.5 vs
 // 12.9 / 18.6. A round costs ~60 us of barrier, fences and queue handling
 // on small graphs, and relaxation runs at ~60 G edges/s against ~88 G/s in
-// the captured kernels (the stamp read of every candidate); ASYNC rounds
-// here have no waiting hubs (RMAT 24: 540 M relaxations per query against
-// 345 M). The captured rounds stay the default; this kernel runs when a
-// caller asks for it (hyt_run_options.persistent).
+// the captured kernels (the stamp read of every candidate). ASYNC rounds
+// wait rows above light_max only when hyt_run_options.heavy_delta is set:
+// with heavy_delta 2, RMAT 24 relaxes 457 M edges per query instead of 553 M
+// but runs 24.8 rounds instead of 20.3, in the same kernel time (9.4 ms),
+// and RMAT 16 / 20 take 1.89 / 3.22 ms instead of 1.37 / 2.50 (captured
+// rounds: 345 M relaxations, 16.1 rounds). The captured rounds stay the
+// default; this kernel runs when a caller asks for it
+// (hyt_run_options.persistent).
 //
 // ASYNC round r (one barrier): walk the chunks queued in round r-1 (each
 // reads its row's live distance), then the rows of queue r with their live
 // distances; rows above light_max queue their chunks for round r+1. A row
 // whose distance equals the value of its last expansion (rec) is skipped:
 // its edges already saw that value (chunks keep rec by atomic max, from 0).
+// With waiting rows, a row above the wait degree whose distance exceeds the
+// queue's smallest by more than heavy_delta joins queue r+1 again instead;
+// the smallest distance of queue r+1 is the smallest candidate of round r or
+// of a waiting row, gathered by one atomic per workgroup.
 //
 // SYNC round r (two barriers, the original's Jacobi rounds and iteration
 // count): phase A records each queued row's distance - the value it held at
@@ -77,10 +85,10 @@ namespace hyt
         {
             std::uint32_t rows[3];   // queue sizes by round slot
             std::uint32_t chunks[3]; // chunk-queue sizes by round slot
+            std::uint32_t qmin[3];   // ASYNC: smallest distance in the queue, by round slot
             std::uint32_t overflow;  // a u32 distance sum overflowed
             std::uint32_t queue_overflow;
             std::uint32_t truncated; // stopped by the round limit with work left
-            std::uint32_t pad;
             unsigned long long rounds, frontier, light_edges, chunk_edges;
             // grid barrier: arrivals per group, groups arrived, generation
             BarrierLine arrive[kGroups];
@@ -113,6 +121,10 @@ namespace hyt
             std::uint64_t queue_cap, chunk_cap;
             PersistCtl *ctl;
             std::uint32_t light_max, chunk;
+            // ASYNC waiting rows: a queued row of more than wait_min edges
+            // whose distance exceeds the queue's smallest by more than
+            // heavy_delta waits for a later round (heavy_delta 0: none waits)
+            std::uint32_t wait_min, heavy_delta;
             std::uint64_t max_rounds;
         };
 
@@ -249,6 +261,19 @@ namespace hyt
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 n = 0;
             }
+            // v from every lane whose flag is set (called by the whole wavefront)
+            __device__ __forceinline__ void append(bool flag, std::uint32_t v)
+            {
+                const std::uint32_t incl = wave_incl_scan(flag ? 1U : 0U);
+                const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
+                if (total == 0)
+                    return;
+                if (n + total > kPushCap)
+                    flush();
+                if (flag)
+                    buf[n + incl - 1] = v;
+                n += total;
+            }
         };
 
         // The relaxation of kUnroll x 64 edges per iteration: all edge loads,
@@ -260,12 +285,16 @@ namespace hyt
         // most c, below its start. v is queued for round `next` unless its
         // stamp already says so: a plain read filters, and only the first
         // candidates of v in a round issue the returning atomicMax that
-        // decides. Appends take one atomic per wavefront.
+        // decides. Appends take one atomic per wavefront. qmin keeps the
+        // lane's smallest candidate: the smallest distance of the next queue
+        // is the smallest candidate of the round (every vertex lowered in the
+        // round is queued, and ends it at its smallest candidate).
         template <int WM, class PosFn>
         __device__ __forceinline__ void relax_edges(const std::uint32_t *__restrict__ col,
                                                     const std::uint32_t *__restrict__ w, std::uint32_t *dist,
                                                     std::uint32_t *stamp, std::uint32_t next, PushBuf &pb,
-                                                    PersistCtl *ctl, std::uint32_t total, PosFn pos_of)
+                                                    PersistCtl *ctl, std::uint32_t total, std::uint32_t &qmin,
+                                                    PosFn pos_of)
         {
             const int lane = lane_id();
             for (std::uint32_t base = 0; base < total; base += kWave * kUnroll)
@@ -300,7 +329,10 @@ namespace hyt
                 {
                     cand[k] = ok[k] && cc[k] < seen[k];
                     if (cand[k])
+                    {
                         atomicMin(&dist[vv[k]], cc[k]);
+                        qmin = cc[k] < qmin ? cc[k] : qmin;
+                    }
                 }
                 std::uint32_t prior[kUnroll];
 #pragma unroll
@@ -338,7 +370,9 @@ namespace hyt
             __shared__ std::uint64_t s_begin[kPWaves][kWave];
             __shared__ std::uint32_t s_val[kPWaves][kWave];
             __shared__ std::uint32_t s_push[kPWaves][kPushCap];
-            __shared__ std::uint32_t s_round[2];
+            __shared__ std::uint32_t s_round[3];
+            __shared__ std::uint32_t s_min[kPWaves];
+            constexpr bool kFloatDist = WM == kWF32 || WM == kWUnitF32;
 
             const int lane = lane_id();
             const int wid = static_cast<int>(threadIdx.x) / kWave;
@@ -358,10 +392,12 @@ namespace hyt
                 {
                     s_round[0] = ld_agent(&ctl->rows[s0]);
                     s_round[1] = SYNC ? 0U : ld_agent(&ctl->chunks[s0]);
+                    s_round[2] = SYNC ? 0xFFFFFFFFU : ld_agent(&ctl->qmin[s0]);
                 }
                 __syncthreads();
                 const std::uint32_t F = s_round[0];
                 const std::uint32_t H = s_round[1];
+                const std::uint32_t hbound = SYNC ? 0xFFFFFFFFU : heavy_bound_of(s_round[2], a.heavy_delta, kFloatDist);
                 __syncthreads(); // s_round is rewritten after the next barrier
                 if (F == 0 && H == 0)
                     break;
@@ -376,6 +412,8 @@ namespace hyt
                     // counters of round r+1 (read, or taken, before the last barrier)
                     st_agent(&ctl->rows[s2], 0U);
                     st_agent(&ctl->chunks[SYNC ? s1 : s2], 0U);
+                    if (!SYNC)
+                        st_agent(&ctl->qmin[s2], 0xFFFFFFFFU);
                     for (int p = 0; p < 2; ++p)
                         for (int g = 0; g < kGroups; ++g)
                             st_agent(&ctl->grab[p][s1][g].n, 0U);
@@ -385,6 +423,7 @@ namespace hyt
                 const std::uint32_t *qin = a.q[r & 1];
                 const std::uint32_t next = r + 1;
                 PushBuf pb{s_push[wid], 0U, a.q[(r + 1) & 1], &ctl->rows[s1], a.queue_cap, &ctl->queue_overflow};
+                std::uint32_t qmin = 0xFFFFFFFFU; // this lane's smallest distance queued for round r+1
 
                 std::uint32_t Hw = H;                    // chunks walked this round
                 const HeavyChunk *hin = a.chunks[r & 1]; // ASYNC: queued in round r-1
@@ -445,7 +484,7 @@ namespace hyt
                         value = __shfl(v, 0, kWave);
                     }
                     chunk_e += h.count;
-                    relax_edges<WM>(a.col, a.w, a.dist, a.stamp, next, pb, ctl, h.count,
+                    relax_edges<WM>(a.col, a.w, a.dist, a.stamp, next, pb, ctl, h.count, qmin,
                                     [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                                         pos = h.begin + e;
                                         val = value;
@@ -457,6 +496,7 @@ namespace hyt
                     const std::uint32_t i = lo + static_cast<std::uint32_t>(lane);
                     std::uint32_t deg = 0, value = 0, nch = 0, row = 0;
                     std::uint64_t begin = 0, count = 0;
+                    bool wait = false; // ASYNC: the row waits, queued again for round r+1
                     if (i < F)
                     {
                         row = qin[i];
@@ -477,7 +517,14 @@ namespace hyt
                             {
                                 begin = a.off[row];
                                 count = a.off[row + 1] - begin;
-                                if (count > a.light_max)
+                                if (count > a.wait_min && value > hbound)
+                                {
+                                    // its record stays below, so the row is
+                                    // not skipped when its turn comes
+                                    wait = a.stamp[row] < next && atomicMax(&a.stamp[row], next) < next;
+                                    qmin = value < qmin ? value : qmin;
+                                }
+                                else if (count > a.light_max)
                                 {
                                     nch = static_cast<std::uint32_t>((count + a.chunk - 1) / a.chunk);
                                     a.rec[row] = 0U;
@@ -491,6 +538,8 @@ namespace hyt
                         }
                     }
                     if (!SYNC)
+                    {
+                        pb.append(wait, row);
                         append_chunks(nch, hcount, hout, a.chunk_cap, &ctl->queue_overflow, [&](std::uint32_t c) {
                             const std::uint64_t clo = static_cast<std::uint64_t>(c) * a.chunk;
                             HeavyChunk h;
@@ -501,6 +550,7 @@ namespace hyt
                             h.record = 1;
                             return h;
                         });
+                    }
                     const std::uint32_t incl = wave_incl_scan(deg);
                     const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
                     if (total == 0)
@@ -515,7 +565,7 @@ namespace hyt
                     const std::uint32_t *ex = s_ex[wid];
                     const std::uint64_t *bg = s_begin[wid];
                     const std::uint32_t *vl = s_val[wid];
-                    relax_edges<WM>(a.col, a.w, a.dist, a.stamp, next, pb, ctl, total,
+                    relax_edges<WM>(a.col, a.w, a.dist, a.stamp, next, pb, ctl, total, qmin,
                                     [&](std::uint32_t e, std::uint64_t &pos, std::uint32_t &val) {
                                         const int j = owner_of(ex, e);
                                         pos = bg[j] + (e - ex[j]);
@@ -527,6 +577,26 @@ namespace hyt
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 });
                 pb.flush();
+                if (!SYNC)
+                {
+                    // the next queue's smallest distance: one atomic per workgroup
+                    for (int o = kWave / 2; o > 0; o >>= 1)
+                    {
+                        const std::uint32_t other = __shfl_xor(qmin, o, kWave);
+                        qmin = other < qmin ? other : qmin;
+                    }
+                    if (lane == 0)
+                        s_min[wid] = qmin;
+                    __syncthreads();
+                    if (threadIdx.x == 0)
+                    {
+                        std::uint32_t m = s_min[0];
+                        for (int k = 1; k < kPWaves; ++k)
+                            m = s_min[k] < m ? s_min[k] : m;
+                        if (m != 0xFFFFFFFFU)
+                            atomicMin(&ctl->qmin[s1], m);
+                    }
+                }
                 grid_barrier(ctl, ++barriers);
             }
             if (lane == 0)
@@ -546,6 +616,9 @@ namespace hyt
                 a.dist[src] = 0U;
                 a.q[0][0] = src;
                 a.ctl->rows[0] = 1U;
+                a.ctl->qmin[0] = 0U;
+                a.ctl->qmin[1] = 0xFFFFFFFFU;
+                a.ctl->qmin[2] = 0xFFFFFFFFU;
             }
         }
 
@@ -594,6 +667,10 @@ namespace hyt
             a.ctl = p.ctl.as<PersistCtl>();
             a.light_max = light_max;
             a.chunk = chunk;
+            // waiting rows only when the caller sets heavy_delta (measured
+            // slower here by default, unlike the captured rounds)
+            a.heavy_delta = SYNC || o.heavy_delta == 0 ? 0U : heavy_delta_of(o.heavy_delta);
+            a.wait_min = std::min(light_max, wait_degree_of(o.wait_degree, light_max));
             a.max_rounds = o.max_rounds ? o.max_rounds : g.V;
             if (a.max_rounds > 0x7FFFFFF0ULL)
                 a.max_rounds = 0x7FFFFFF0ULL; // round numbers (stamps) and barrier generations are u32

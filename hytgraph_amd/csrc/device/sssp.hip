@@ -260,28 +260,6 @@ namespace hyt
                 *rc = RoundCtl{};
         }
 
-        // Distance bound of the rows above light_max this round: the smallest
-        // distance the last compaction kept plus heavy_delta (float weights:
-        // added as a float); 0xFFFFFFFF (no waiting) when heavy_delta is 0 or
-        // no distance was recorded.
-        __host__ __device__ inline std::uint32_t heavy_bound_of(std::uint32_t kept_min, std::uint32_t heavy_delta,
-                                                               bool float_dist)
-        {
-            if (heavy_delta == 0 || kept_min == 0xFFFFFFFFU)
-                return 0xFFFFFFFFU;
-            if (float_dist)
-            {
-                float f;
-                std::memcpy(&f, &kept_min, 4);
-                f += static_cast<float>(heavy_delta);
-                std::uint32_t b;
-                std::memcpy(&b, &f, 4);
-                return b;
-            }
-            const std::uint64_t b = static_cast<std::uint64_t>(kept_min) + heavy_delta;
-            return b >= 0xFFFFFFFFULL ? 0xFFFFFFFFU : static_cast<std::uint32_t>(b);
-        }
-
         __global__ void round_begin(Counters *ctr, RoundCtl *rc, std::uint64_t max_rounds, std::uint32_t heavy_delta,
                                     bool float_dist)
         {

@@ -258,8 +258,9 @@ typedef struct hyt_run_options {
                                   more than heavy_delta weight units, so they
                                   go out (nearly) once, with their final
                                   distance; light rows never wait. 0 = the
-                                  default, 2; UINT32_MAX = no waiting. Same
-                                  distances either way */
+                                  default, 2 (with persistent: no waiting);
+                                  UINT32_MAX = no waiting. Same distances
+                                  either way */
     uint32_t persistent;       /* single-source SSSP push rounds without
                                   tolerance or buckets: 1 = one persistent
                                   cooperative launch runs every round (the

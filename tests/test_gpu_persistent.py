@@ -146,6 +146,38 @@ def test_star_hub_and_edge_cases(gpu, abi, orc):
         assert d.tolist() == [0] and st["converged"] == 1 and st["rounds"] == 1
 
 
+@pytest.mark.parametrize("heavy_delta,wait_degree", [(2, 0), (1, 8), (40, 0), (0xFFFFFFFF, 0)])
+def test_waiting_rows_same_distances(gpu, abi, orc, heavy_delta, wait_degree):
+    """ASYNC waiting rows (rows above the wait degree stay queued while their
+    distance exceeds the queue's smallest by more than heavy_delta): the same
+    distances for u32 weights with small chunks (hubs over many chunks),
+    float weights, and a star hub with a chain behind it."""
+    hg = rmat(abi, 15, 33, wmax=255)
+    g = abi.DeviceGraph.from_host(hg)
+    for src in abi.sample_sources(hg.offsets, 4, 3):
+        _say(f"delta {heavy_delta} wait {wait_degree} source {int(src)}")
+        dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+        for lm, ch in ((0, 0), (16, 64)):
+            d, st = g.sssp(int(src), mode=1, persistent=1, heavy_delta=heavy_delta, wait_degree=wait_degree,
+                           light_max_degree=lm, heavy_chunk=ch)
+            assert np.array_equal(d, dj), (int(src), lm)
+            assert st["converged"] == 1
+    gf = abi.DeviceGraph.from_host(hg, weights_f32=True)
+    dj = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0)
+    d, _ = gf.sssp(0, mode=1, persistent=1, heavy_delta=heavy_delta, wait_degree=wait_degree)
+    want = np.where(dj == 0xFFFFFFFF, np.inf, dj.astype(np.float64)).astype(np.float32)
+    assert np.array_equal(d.view(np.float32), want)
+    V = 6000
+    edges = [(0, v) for v in range(1, 5001)] + [(5000 + i, 5001 + i) for i in range(998)] + [(3, 0), (4999, 2)]
+    weights = [((v * 7919) % 50) + 1 for v in range(1, 5001)] + [3] * 998 + [1, 1]
+    off, col, w = csr_from_edges(V, edges, weights)
+    gs = abi.DeviceGraph(off, col, w.astype(np.uint32))
+    for src in (0, 3, 4999):
+        d, _ = gs.sssp(src, mode=1, persistent=1, heavy_delta=heavy_delta, wait_degree=wait_degree,
+                       light_max_degree=32, heavy_chunk=256)
+        assert np.array_equal(d, orc.sssp_dijkstra(off, col, w.astype(np.uint32), src)), src
+
+
 def test_options_and_overflow(gpu, abi):
     """Bucketed or pull-direction runs are rejected with the persistent kernel;
     a u32 distance overflow is reported."""
