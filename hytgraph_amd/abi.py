@@ -143,6 +143,7 @@ class HybridStats(C.Structure):
         ("bytes_zero_copy", C.c_uint64), ("copy_ms", C.c_double), ("pack_ms", C.c_double), ("plan_ms", C.c_double),
         ("link_gbs_peak", C.c_double), ("cpu_pack_gbs", C.c_double), ("zero_copy_gbs", C.c_double),
         ("device_edge_bytes", C.c_uint64), ("device_state_bytes", C.c_uint64), ("runs_reordered", C.c_uint64),
+        ("transpose_ms", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -232,6 +233,7 @@ SIGNATURES = {
         C.c_int,
         [_P, _u64p, _u64p, C.POINTER(_u64p), C.POINTER(_u32p), C.POINTER(_u32p)],
     ),
+    "hyt_host_graph_transpose": (C.c_int, [_P, C.c_uint32, C.POINTER(_u64p), C.POINTER(_u32p)]),
     "hyt_host_graph_free": (C.c_int, [_P]),
     "hyt_sample_sources": (C.c_int, [_P, C.c_uint64, C.c_uint64, C.c_uint32, _P]),
     "hyt_graph_upload": (
@@ -420,6 +422,14 @@ class HostGraph:
         self.columns = np.ctypeslib.as_array(col, shape=(max(self.E, 1),))[: self.E]
         self.weights = (np.ctypeslib.as_array(w, shape=(max(self.E, 1),))[: self.E]
                         if bool(w) else None)
+
+    def transpose(self, threads=0):
+        """The incoming CSR (hyt_host_graph_transpose): (in_offsets u64[V+1],
+        in_sources u32[E]) views, valid until free()."""
+        off, col = _u64p(), _u32p()
+        check(load().hyt_host_graph_transpose(self._h, threads, C.byref(off), C.byref(col)))
+        return (np.ctypeslib.as_array(off, shape=(self.V + 1,)),
+                np.ctypeslib.as_array(col, shape=(max(self.E, 1),))[: self.E])
 
     def free(self):
         if self._h:

@@ -35,11 +35,22 @@
 //      compaction packs the active rows' edges on host threads into pinned
 //      staging (while the GPU works on the rest) and copies them once.
 // The expand kernels are the in-HBM ones with a windowed / packed row source.
+//
+// PageRank rounds whose frontier out-edges reach pull_percent of E pull
+// instead, as in HBM: they read the host graph's incoming CSR (built on host
+// threads on first use, kept with the host graph), partitioned by destination
+// ranges like the out-edges. Every in-partition is read: resident ones from
+// their units in one launch per group, the rest streamed through the ring in
+// windows and promoted into the cache (resident ids P + q, so in- and
+// out-partitions share the units and the least-recently-active rule). The
+// cost model has nothing to choose in such a round: every in-edge is needed,
+// so a partition is either resident or copied whole.
 #include "host_graph.hpp"
 #include "hybrid_lanes.hpp"
 #include "lanes.hpp"
 #include "ops.hpp"
 #include "contribution_order.hpp"
+#include "pull_seg.hpp"
 #include "transfer_plan.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -308,6 +319,14 @@ namespace hyt
         // degree-bin split of the current run's expansions
         std::uint32_t light_max = kHybridLightMaxDegree, chunk = kHybridHeavyChunk;
         LaneState lk;
+        // PageRank pull rounds: partitions of the incoming CSR (destination
+        // ranges; resident ids parts.size() + q), its offsets on the device,
+        // the segment table and chunk queue of the pull kernels
+        std::vector<hyt_partition> in_parts;
+        DeviceBuffer in_off_d, pull_seg_d, pull_heavy, pull_ctr;
+        std::vector<PullSeg> pull_seg_h;
+        std::uint64_t pull_heavy_cap = 0;
+        double transpose_ms = 0; // host transpose build, reported once
 
         ~HybridEngine()
         {
@@ -346,7 +365,7 @@ namespace hyt
         {
             return dg.device_bytes() + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes +
                    block_offsets.bytes + scan_temp.bytes + stage_q.bytes + stage_val.bytes + seg_d.bytes +
-                   lk.bytes();
+                   lk.bytes() + in_off_d.bytes + pull_seg_d.bytes + pull_heavy.bytes + pull_ctr.bytes;
         }
         std::uint64_t edge_area_bytes() const
         {
@@ -1373,6 +1392,177 @@ namespace hyt
             st.copy_ms += E.drain_copy_time();
         }
 
+        // --------------------------------------------- PageRank pull rounds
+        // A launch of the pull kernels covers at most this many in-edges (or a
+        // ring window, if larger), so their chunk queue has a fixed size.
+        constexpr std::uint64_t kPullLaunchEdges = 1ULL << 26;
+
+        // The incoming CSR on first use: host transpose, in-partitions,
+        // incoming offsets on the device, pull kernel buffers. True when built
+        // by this call.
+        template <class OffT>
+        bool ensure_pull(HybridEngine &E)
+        {
+            if (!E.in_parts.empty())
+                return false;
+            const auto t0 = Clock::now();
+            ensure_host_transpose(*E.host, E.threads);
+            E.transpose_ms = ms_since(t0);
+            const std::uint64_t V = E.host->V;
+            E.in_parts = partition_graph(E.host->in_off, V, E.cfg.partition_edges * 4);
+            const std::uint64_t P = E.parts.size(), Pin = E.in_parts.size();
+            E.in_off_d.alloc((V + 1) * sizeof(OffT), "hipMalloc(incoming offsets)");
+            if (sizeof(OffT) == 8)
+                hip_check(hipMemcpy(E.in_off_d.ptr, E.host->in_off, (V + 1) * 8, hipMemcpyHostToDevice),
+                          "copy incoming offsets");
+            else
+            {
+                std::vector<std::uint32_t> o(V + 1);
+                for (std::uint64_t v = 0; v <= V; ++v)
+                    o[v] = static_cast<std::uint32_t>(E.host->in_off[v]);
+                hip_check(hipMemcpy(E.in_off_d.ptr, o.data(), (V + 1) * 4, hipMemcpyHostToDevice),
+                          "copy incoming offsets");
+            }
+            E.res.resize(P + Pin, Resident{});
+            E.last_active.resize(P + Pin, 0);
+            E.pull_seg_h.resize(Pin);
+            E.pull_seg_d.alloc(std::max<std::uint64_t>(Pin, 1) * sizeof(PullSeg), "hipMalloc(pull segments)");
+            const std::uint64_t L =
+                std::max<std::uint64_t>(E.ring_edges, std::min<std::uint64_t>(kPullLaunchEdges, E.host->E));
+            // a row of d > light_max in-edges makes ceil(d / chunk) <= d / chunk
+            // + d / (light_max + 1) chunks
+            E.pull_heavy_cap = L / kPagerankHeavyChunk + L / (kPagerankLightMaxDegree + 1) + 64;
+            E.pull_heavy.alloc(E.pull_heavy_cap * sizeof(HeavyChunk), "hipMalloc(pull chunks)");
+            E.pull_ctr.alloc(256, "hipMalloc(pull counters)");
+            hip_check(hipMemset(E.pull_ctr.ptr, 0, 256), "clear pull counters");
+            return true;
+        }
+
+        // One pull round: every in-partition with in-edges adds the shares of
+        // its in-edges to its rows' residuals; resident ones from their units,
+        // the rest through the ring (then promoted into the cache).
+        template <class OffT>
+        void pull_round(HybridEngine &E, hyt_hybrid_stats &st)
+        {
+            DeviceGraph &g = E.dg;
+            hipStream_t s = g.stream;
+            ++E.stamp;
+            const std::uint64_t P = E.parts.size();
+            const OffT *in_off = E.in_off_d.as<OffT>();
+            const float *share = g.ws.share.as<float>();
+            unsigned long long *res = g.ws.res.as<unsigned long long>();
+            std::uint32_t *ctr = E.pull_ctr.as<std::uint32_t>();
+            const PullQueue pq{E.pull_heavy.as<HeavyChunk>(), ctr, ctr + 1, E.pull_heavy_cap};
+            const std::uint64_t launch_edges =
+                std::max<std::uint64_t>(E.ring_edges, std::min<std::uint64_t>(kPullLaunchEdges, E.host->E));
+            auto launch = [&](const PullSeg *segs, const PullSeg &one, std::uint64_t nseg, std::uint64_t rows) {
+                if (rows == 0 || nseg == 0)
+                    return;
+                hip_check(hipMemsetAsync(ctr, 0, 4, s), "reset pull chunk count");
+                const dim3 grid(static_cast<unsigned>((rows + kBlock - 1) / kBlock), static_cast<unsigned>(nseg));
+                pr_pull_seg<OffT><<<grid, kBlock, 0, s>>>(in_off, segs, one, share, res, kPagerankLightMaxDegree,
+                                                          kPagerankHeavyChunk, pq);
+                hip_check(hipGetLastError(), "launch pr_pull_seg");
+                pr_pull_seg_heavy<<<kHeavyGrid, kBlock, 0, s>>>(segs, one, share, res, pq);
+                hip_check(hipGetLastError(), "launch pr_pull_seg_heavy");
+            };
+
+            std::vector<std::uint64_t> cached, streamed;
+            for (std::uint64_t q = 0; q < E.in_parts.size(); ++q)
+            {
+                const hyt_partition &pt = E.in_parts[q];
+                if (pt.edge_end == pt.edge_begin)
+                    continue;
+                E.last_active[P + q] = E.stamp;
+                (E.is_cached(P + q) ? cached : streamed).push_back(q);
+            }
+
+            // resident in-partitions: one segment table, launches of at most
+            // launch_edges in-edges (and 65535 segments)
+            if (!cached.empty())
+            {
+                for (std::size_t k = 0; k < cached.size(); ++k)
+                {
+                    const hyt_partition &pt = E.in_parts[cached[k]];
+                    E.pull_seg_h[k] = PullSeg{pt.vertex_begin, pt.vertex_end, pt.edge_begin, pt.edge_end,
+                                              E.unit_ptr(E.res[P + cached[k]].col)};
+                }
+                // pull_seg_h is rewritten only after the round's final synchronisation
+                hip_check(hipMemcpyAsync(E.pull_seg_d.ptr, E.pull_seg_h.data(), cached.size() * sizeof(PullSeg),
+                                         hipMemcpyHostToDevice, s),
+                          "upload pull segments");
+                std::size_t k0 = 0;
+                while (k0 < cached.size())
+                {
+                    std::size_t k1 = k0;
+                    std::uint64_t edges = 0, rows = 0;
+                    while (k1 < cached.size() && k1 - k0 < 65535)
+                    {
+                        const hyt_partition &pt = E.in_parts[cached[k1]];
+                        const std::uint64_t n = pt.edge_end - pt.edge_begin;
+                        if (k1 > k0 && edges + n > launch_edges)
+                            break;
+                        edges += n;
+                        rows = std::max<std::uint64_t>(rows, pt.vertex_end - pt.vertex_begin);
+                        ++k1;
+                    }
+                    launch(E.pull_seg_d.as<PullSeg>() + k0, PullSeg{}, k1 - k0, rows);
+                    k0 = k1;
+                }
+                st.parts_cached += cached.size();
+            }
+
+            // the rest: runs of consecutive in-partitions in ring windows that
+            // end at a partition boundary (a partition longer than a window is
+            // split over several)
+            std::size_t i = 0;
+            while (i < streamed.size())
+            {
+                std::size_t j = i + 1;
+                while (j < streamed.size() && streamed[j] == streamed[j - 1] + 1)
+                    ++j;
+                const std::uint64_t q0 = streamed[i], q1 = streamed[j - 1];
+                i = j;
+                st.parts_filter += q1 - q0 + 1;
+                const std::uint64_t e1 = E.in_parts[q1].edge_end;
+                std::uint64_t qa = q0;
+                for (std::uint64_t a = E.in_parts[q0].edge_begin; a < e1;)
+                {
+                    while (E.in_parts[qa].edge_end <= a)
+                        ++qa;
+                    const std::uint64_t lim = std::min<std::uint64_t>(e1, a + E.ring_edges);
+                    std::uint64_t qb = qa, b = lim;
+                    if (E.in_parts[qa].edge_end <= lim)
+                    {
+                        while (qb < q1 && E.in_parts[qb + 1].edge_end <= lim)
+                            ++qb;
+                        b = E.in_parts[qb].edge_end;
+                    }
+                    const std::size_t r = E.ring_take();
+                    E.copy_host_graph(E.ring_col(r), E.host->in_col + a, (b - a) * 4);
+                    st.bytes_filter += (b - a) * 4;
+                    hip_check(hipStreamWaitEvent(s, E.copy_events[E.copy_events_used - 1].b, 0), "wait copy");
+                    const PullSeg one{E.in_parts[qa].vertex_begin, E.in_parts[qb].vertex_end, a, b, E.ring_col(r)};
+                    launch(nullptr, one, 1, one.v1 - one.v0);
+                    if (E.cfg.cache)
+                        for (std::uint64_t qq = qa; qq <= qb; ++qq)
+                        {
+                            const hyt_partition &pt = E.in_parts[qq];
+                            const std::uint64_t n = pt.edge_end - pt.edge_begin;
+                            if (pt.edge_begin < a || pt.edge_end > b || n > E.unit_edges || n == 0)
+                                continue;
+                            if (!E.admit(P + qq))
+                                break; // every resident partition is in use this round
+                            hip_check(hipMemcpyAsync(E.unit_ptr(E.res[P + qq].col), E.ring_col(r) + (pt.edge_begin - a),
+                                                     n * 4, hipMemcpyDeviceToDevice, s),
+                                      "promote incoming columns");
+                        }
+                    E.ring_release(r);
+                    a = b;
+                }
+            }
+        }
+
         template <class OffT, int WM>
         void hybrid_sssp_typed(HybridEngine &E, std::uint32_t src, std::uint32_t *dist_out, const hyt_run_options &o,
                                hyt_hybrid_stats &st)
@@ -1469,20 +1659,49 @@ namespace hyt
                 return op;
             };
             const std::uint64_t max_rounds = o.max_rounds ? o.max_rounds : 100000;
+            // direction per round as in HBM: pull when the frontier's out-edges
+            // reach pull_percent of E (pull rounds read every in-edge)
+            const std::uint32_t dir = o.direction;
+            require(dir <= HYT_DIR_PULL, "direction must be auto (0), push (1) or pull (2)");
+            const std::uint64_t pct = o.pull_percent ? o.pull_percent : kDefaultPullPercentPagerank;
+            const bool can_pull = dir != HYT_DIR_PUSH && g.E > 0;
+            unsigned long long Fe = c.frontier_edges;
             Plan plan;
             st.run.converged = F == 0 ? 1 : 0;
             for (std::uint64_t round = 0; round < max_rounds && F > 0; ++round)
             {
-                plan_round<OffT>(E, F, plan, st);
+                const bool pull = can_pull && (dir == HYT_DIR_PULL || Fe * 100ULL >= pct * g.E);
+                if (pull)
+                {
+                    if (ensure_pull<OffT>(E))
+                        st.transpose_ms = E.transpose_ms;
+                }
+                else
+                    plan_round<OffT>(E, F, plan, st);
                 hip_check(hipMemsetAsync(d_ctr, 0, kRoundCounterBytes, s), "reset counters");
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
-                execute_round<OffT>(E, plan, single_expander<OffT>(E, make), st);
+                if (pull)
+                    pull_round<OffT>(E, st);
+                else
+                    execute_round<OffT>(E, plan, single_expander<OffT>(E, make), st);
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
                 Counters cc;
                 const std::uint32_t Fn = ordered_compaction<OffT>(E, sel, cc);
                 finish_round_stats(E, st);
+                if (pull)
+                {
+                    std::uint32_t pov = 0;
+                    hip_check(hipMemcpy(&pov, E.pull_ctr.as<std::uint32_t>() + 1, 4, hipMemcpyDeviceToHost),
+                              "read pull overflow");
+                    if (pov)
+                        fail(HYT_ERR_INTERNAL, "pull chunk queue overflow");
+                    st.run.pull_rounds += 1;
+                    st.run.pull_edges += g.E;
+                    st.run.pull_frontier += F;
+                    st.run.pull_active_edges += Fe;
+                }
                 if (o.time_kernels)
                 {
                     float ms = 0;
@@ -1493,10 +1712,11 @@ namespace hyt
                 if (cc.heavy_overflow)
                     fail(HYT_ERR_INTERNAL, "heavy chunk queue overflow");
                 st.run.frontier_total += F;
-                st.run.edges_total += cc.all_edges();
+                st.run.edges_total += pull ? g.E : cc.all_edges();
                 st.run.updates_total += Fn;
                 st.run.rounds = round + 1;
                 F = Fn;
+                Fe = cc.frontier_edges;
                 if (F == 0)
                     st.run.converged = 1;
             }

@@ -70,6 +70,8 @@ namespace hyt
         host_free(off, memory);
         host_free(col, memory);
         host_free(w, memory);
+        host_free(in_off, memory);
+        host_free(in_col, memory);
     }
 } // namespace hyt
 
@@ -125,6 +127,19 @@ int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices, uint64_t *nu
             *columns = g->g->col;
         if (weights)
             *weights = g->g->w;
+    });
+}
+
+int hyt_host_graph_transpose(hyt_host_graph_t g, uint32_t threads, const uint64_t **in_offsets,
+                             const uint32_t **in_sources)
+{
+    return hyt::guarded("hyt_host_graph_transpose", [&] {
+        hyt::require(g != nullptr && g->g, "null host graph");
+        hyt::ensure_host_transpose(*g->g, static_cast<int>(threads));
+        if (in_offsets)
+            *in_offsets = g->g->in_off;
+        if (in_sources)
+            *in_sources = g->g->in_col;
     });
 }
 

@@ -19,6 +19,10 @@ namespace hyt
         std::uint64_t *off = nullptr;
         std::uint32_t *col = nullptr;
         std::uint32_t *w = nullptr;
+        // incoming CSR (the transpose, sources of each destination in
+        // ascending order), same memory kind; built on first use
+        std::uint64_t *in_off = nullptr;
+        std::uint32_t *in_col = nullptr;
 
         HostGraph(std::uint64_t v, std::uint64_t e, bool weights, int mem);
         ~HostGraph();
@@ -35,6 +39,8 @@ namespace hyt
     // budget bytes of HBM (0: 40 GiB); params.threads is not used.
     std::unique_ptr<HostGraph> generate_device(const hyt_gen_params &p, std::uint64_t budget);
     HostGraph *host_graph_of(hyt_host_graph_t h);
+    // Builds g.in_off / g.in_col if absent, on up to `threads` host threads.
+    void ensure_host_transpose(HostGraph &g, int threads);
     void sample_sources(const std::uint64_t *off, std::uint64_t V, std::uint64_t seed,
                         std::uint32_t count, std::uint32_t *out);
 } // namespace hyt

@@ -90,6 +90,15 @@ int hyt_generate_device(const hyt_gen_params *params, uint64_t device_budget_byt
 int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices,
                         uint64_t *num_edges, const uint64_t **row_offsets,
                         const uint32_t **columns, const uint32_t **weights);
+/* The incoming CSR of a host graph (the transpose the original builds for
+ * its pull kernels, build_incoming_csr, src/algorithms/sssp.cpp:21-80 and
+ * cuda/pagerank.cu): in_offsets u64[V+1], in_sources u32[E], the sources of
+ * each destination in ascending order (edge order within a source), in the
+ * graph's memory kind. Built on first call with up to `threads` host threads
+ * (0: up to 16) and kept with the graph; the hybrid engine's PageRank pull
+ * rounds build it the same way. */
+int hyt_host_graph_transpose(hyt_host_graph_t g, uint32_t threads, const uint64_t **in_offsets,
+                             const uint32_t **in_sources);
 int hyt_host_graph_free(hyt_host_graph_t g);
 
 /*
@@ -491,6 +500,8 @@ typedef struct hyt_hybrid_stats {
     uint64_t device_edge_bytes;   /* HBM used for edge data (<= budget) */
     uint64_t device_state_bytes;  /* HBM used for offsets and vertex state */
     uint64_t runs_reordered;      /* filter runs moved by the contribution order */
+    double transpose_ms;          /* PageRank pull rounds: host build of the incoming
+                                     CSR, in the call that built it (else 0) */
 } hyt_hybrid_stats;
 
 int hyt_hybrid_config_default(hyt_hybrid_config *out);
@@ -508,7 +519,11 @@ int hyt_hybrid_info(hyt_hybrid_t h, uint64_t *partitions, hyt_cost_params *param
                     hyt_hybrid_stats *calibration);
 
 /* Single-source SSSP (async rounds) and delta-push PageRank over the hybrid
- * engine; outputs as hyt_sssp / hyt_pagerank. */
+ * engine; outputs as hyt_sssp / hyt_pagerank. PageRank honours
+ * opts->direction and pull_percent as hyt_pagerank does: a pull round reads
+ * the host graph's incoming CSR (hyt_host_graph_transpose, built on the first
+ * pull round) - every in-partition, resident or copied whole - and adds the
+ * same words as a push round, so the ranks are the same bit for bit. */
 int hyt_hybrid_sssp(hyt_hybrid_t h, uint32_t source, uint32_t *dist, const hyt_run_options *opts,
                     hyt_hybrid_stats *stats);
 int hyt_hybrid_pagerank(hyt_hybrid_t h, float damping, float epsilon, float *ranks,
