@@ -1477,8 +1477,17 @@ namespace hyt
                 (E.is_cached(P + q) ? cached : streamed).push_back(q);
             }
 
-            // resident in-partitions: one segment table, launches of at most
-            // launch_edges in-edges (and 65535 segments)
+            // resident in-partitions: one segment table, in groups of about a
+            // ring window's in-edges (at most launch_edges, 65535 segments),
+            // launched one after each streamed window: the compute stream
+            // then frees ring buffers as fast as the copies fill them, where
+            // one long launch first would leave the copy stream waiting
+            struct Group
+            {
+                std::size_t k0, k1;
+                std::uint64_t rows;
+            };
+            std::vector<Group> groups;
             if (!cached.empty())
             {
                 for (std::size_t k = 0; k < cached.size(); ++k)
@@ -1491,6 +1500,7 @@ namespace hyt
                 hip_check(hipMemcpyAsync(E.pull_seg_d.ptr, E.pull_seg_h.data(), cached.size() * sizeof(PullSeg),
                                          hipMemcpyHostToDevice, s),
                           "upload pull segments");
+                const std::uint64_t group_edges = std::min<std::uint64_t>(E.ring_edges, launch_edges);
                 std::size_t k0 = 0;
                 while (k0 < cached.size())
                 {
@@ -1500,17 +1510,22 @@ namespace hyt
                     {
                         const hyt_partition &pt = E.in_parts[cached[k1]];
                         const std::uint64_t n = pt.edge_end - pt.edge_begin;
-                        if (k1 > k0 && edges + n > launch_edges)
+                        if (k1 > k0 && edges + n > group_edges)
                             break;
                         edges += n;
                         rows = std::max<std::uint64_t>(rows, pt.vertex_end - pt.vertex_begin);
                         ++k1;
                     }
-                    launch(E.pull_seg_d.as<PullSeg>() + k0, PullSeg{}, k1 - k0, rows);
+                    groups.push_back(Group{k0, k1, rows});
                     k0 = k1;
                 }
                 st.parts_cached += cached.size();
             }
+            std::size_t next_group = 0;
+            auto launch_group = [&] {
+                const Group &gr = groups[next_group++];
+                launch(E.pull_seg_d.as<PullSeg>() + gr.k0, PullSeg{}, gr.k1 - gr.k0, gr.rows);
+            };
 
             // the rest: runs of consecutive in-partitions in ring windows that
             // end at a partition boundary (a partition longer than a window is
@@ -1558,9 +1573,14 @@ namespace hyt
                                       "promote incoming columns");
                         }
                     E.ring_release(r);
+                    // after the release, so the buffer's next copy does not wait for it
+                    if (next_group < groups.size())
+                        launch_group();
                     a = b;
                 }
             }
+            while (next_group < groups.size())
+                launch_group();
         }
 
         template <class OffT, int WM>
