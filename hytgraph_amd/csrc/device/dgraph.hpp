@@ -265,6 +265,8 @@ namespace hyt
 
     // Builds the incoming CSR and the static chunk list of its heavy rows.
     void ensure_transpose(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk);
+    // g.in_off / g.in_col only (E < 2^31; sources of each row ascending).
+    void build_incoming_csr(DeviceGraph &g);
 
     // Device arrays of one single-source SSSP run (every [V] array has kPad
     // spare slots): the graph's workspace, or caller-provided memory. touch

@@ -1397,16 +1397,84 @@ namespace hyt
         // ring window, if larger), so their chunk queue has a fixed size.
         constexpr std::uint64_t kPullLaunchEdges = 1ULL << 26;
 
-        // The incoming CSR on first use: host transpose, in-partitions,
-        // incoming offsets on the device, pull kernel buffers. True when built
-        // by this call.
+        // The host graph's incoming CSR built on the device, when the graph
+        // and the sort's working arrays fit in free HBM (about 44 bytes per
+        // edge, E < 2^31): the in-HBM transpose (a radix sort of (destination,
+        // source) keys, so each destination's sources come out ascending, the
+        // order of the host build) copied back into the graph's memory kind.
+        // False when it does not fit; the host build runs instead.
+        template <class OffT>
+        bool device_transpose(HybridEngine &E)
+        {
+            HostGraph &h = *E.host;
+            const std::uint64_t V = h.V, Ed = h.E;
+            if (h.in_off != nullptr || Ed == 0 || Ed >= (1ULL << 31))
+                return false;
+            std::size_t free_b = 0, total_b = 0;
+            hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+            const std::uint64_t need = Ed * 44 + (V + 1) * 16 + (256ULL << 20);
+            if (need > free_b / 10 * 9)
+                return false;
+            DeviceGraph t;
+            t.device = E.dg.device;
+            t.V = V;
+            t.E = Ed;
+            t.off64 = sizeof(OffT) == 8;
+            hip_check(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking), "hipStreamCreate");
+            t.off.alloc((V + 1) * sizeof(OffT), "hipMalloc(transpose offsets)");
+            if (sizeof(OffT) == 8)
+                hip_check(hipMemcpy(t.off.ptr, h.off, (V + 1) * 8, hipMemcpyHostToDevice), "copy offsets");
+            else
+            {
+                std::vector<std::uint32_t> o(V + 1);
+                for (std::uint64_t v = 0; v <= V; ++v)
+                    o[v] = static_cast<std::uint32_t>(h.off[v]);
+                hip_check(hipMemcpy(t.off.ptr, o.data(), (V + 1) * 4, hipMemcpyHostToDevice), "copy offsets");
+            }
+            t.col.alloc(Ed * 4, "hipMalloc(transpose columns)");
+            hip_check(hipMemcpy(t.col.ptr, h.col, Ed * 4, hipMemcpyHostToDevice), "copy columns");
+            build_incoming_csr(t);
+            t.col.release();
+            std::uint64_t *in_off = nullptr;
+            std::uint32_t *in_col = nullptr;
+            try
+            {
+                in_off = static_cast<std::uint64_t *>(host_alloc((V + 1) * 8, h.memory));
+                in_col = static_cast<std::uint32_t *>(host_alloc(Ed * 4, h.memory));
+                if (sizeof(OffT) == 8)
+                    hip_check(hipMemcpy(in_off, t.in_off.ptr, (V + 1) * 8, hipMemcpyDeviceToHost), "read in offsets");
+                else
+                {
+                    std::vector<std::uint32_t> o(V + 1);
+                    hip_check(hipMemcpy(o.data(), t.in_off.ptr, (V + 1) * 4, hipMemcpyDeviceToHost),
+                              "read in offsets");
+                    for (std::uint64_t v = 0; v <= V; ++v)
+                        in_off[v] = o[v];
+                }
+                hip_check(hipMemcpy(in_col, t.in_col.ptr, Ed * 4, hipMemcpyDeviceToHost), "read in columns");
+            }
+            catch (...)
+            {
+                host_free(in_off, h.memory);
+                host_free(in_col, h.memory);
+                throw;
+            }
+            h.in_off = in_off;
+            h.in_col = in_col;
+            return true;
+        }
+
+        // The incoming CSR on first use: transpose (on the device when it
+        // fits, else on host threads), in-partitions, incoming offsets on the
+        // device, pull kernel buffers. True when built by this call.
         template <class OffT>
         bool ensure_pull(HybridEngine &E)
         {
             if (!E.in_parts.empty())
                 return false;
             const auto t0 = Clock::now();
-            ensure_host_transpose(*E.host, E.threads);
+            if (!device_transpose<OffT>(E))
+                ensure_host_transpose(*E.host, E.threads);
             E.transpose_ms = ms_since(t0);
             const std::uint64_t V = E.host->V;
             E.in_parts = partition_graph(E.host->in_off, V, E.cfg.partition_edges * 4);

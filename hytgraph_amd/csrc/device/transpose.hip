@@ -184,6 +184,14 @@ namespace hyt
         }
     } // namespace
 
+    void build_incoming_csr(DeviceGraph &g)
+    {
+        if (g.off64)
+            build_typed<std::uint64_t>(g);
+        else
+            build_typed<std::uint32_t>(g);
+    }
+
     void ensure_transpose(DeviceGraph &g, std::uint32_t light_max, std::uint32_t chunk)
     {
         if (!g.in_col.ptr)

@@ -95,8 +95,9 @@ int hyt_host_graph_view(hyt_host_graph_t g, uint64_t *num_vertices,
  * cuda/pagerank.cu): in_offsets u64[V+1], in_sources u32[E], the sources of
  * each destination in ascending order (edge order within a source), in the
  * graph's memory kind. Built on first call with up to `threads` host threads
- * (0: up to 16) and kept with the graph; the hybrid engine's PageRank pull
- * rounds build it the same way. */
+ * (0: up to 16) and kept with the graph. The hybrid engine's PageRank pull
+ * rounds build the same arrays on the device when the graph (E < 2^31) and
+ * the sort fit in free HBM, else this way. */
 int hyt_host_graph_transpose(hyt_host_graph_t g, uint32_t threads, const uint64_t **in_offsets,
                              const uint32_t **in_sources);
 int hyt_host_graph_free(hyt_host_graph_t g);

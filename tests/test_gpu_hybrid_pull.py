@@ -23,6 +23,18 @@ def _say(msg):
     print(f"[gpu-hybrid-pull] {msg}", flush=True)
 
 
+def check_transpose(hg, V):
+    # the incoming CSR the engine built (on the device when it fits): the
+    # stable sort of the edges by destination, sources ascending
+    in_off, in_src = hg.transpose()
+    src = np.repeat(np.arange(V, dtype=np.uint32), np.diff(hg.offsets.astype(np.int64)))
+    order = np.argsort(hg.columns, kind="stable")
+    want_off = np.zeros(V + 1, np.uint64)
+    np.cumsum(np.bincount(hg.columns, minlength=V), out=want_off[1:])
+    assert np.array_equal(in_off, want_off)
+    assert np.array_equal(in_src, src[order])
+
+
 def host(abi, scale=13, seed=2, wmax=0, kind=0, memory=None):
     return abi.HostGraph(kind=kind, scale=scale, edge_factor=16, seed=seed, weight_min=1 if wmax else 0,
                          weight_max=wmax, memory=abi.HYT_HOST_PINNED if memory is None else memory)
@@ -51,6 +63,8 @@ def test_pull_push_auto_equal_in_hbm(gpu, abi, orc, cfg):
         assert np.array_equal(r, ref), direction
         assert run["converged"] == 1 and run["rounds"] == rst["rounds"]
         assert np.abs(r - same).sum() <= 1e-6
+        if direction == abi.HYT_DIR_PULL:
+            check_transpose(hg, hg.V)
         if direction == abi.HYT_DIR_PUSH:
             assert run["pull_rounds"] == 0
         elif direction == abi.HYT_DIR_PULL:
@@ -87,6 +101,7 @@ def test_long_in_rows_split_over_windows(gpu, abi, orc):
             _say(f"long rows direction {direction}: rounds {st['run']['rounds']} pull {st['run']['pull_rounds']}")
             assert np.array_equal(r, ref), direction
         h.free()
+    check_transpose(hg, V)
     same, _, _ = orc.delta_push_f64(off, col, D, f32(EPS))
     assert np.abs(ref - same).sum() <= 1e-6
 
@@ -98,6 +113,7 @@ def test_pageable_off64_and_er(gpu, abi, orc):
     h = abi.HybridEngine(hp, **SMALL)
     r, st = h.pagerank(D, EPS)
     assert np.array_equal(r, ref) and st["run"]["pull_rounds"] > 0
+    check_transpose(hp, hp.V)
     h64 = abi.HybridEngine(hp, flags=abi.HYT_GRAPH_OFF64, **SMALL)
     r, st = h64.pagerank(D, EPS, direction=abi.HYT_DIR_PULL)
     assert np.array_equal(r, ref)
