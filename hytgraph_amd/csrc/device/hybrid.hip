@@ -1473,8 +1473,11 @@ namespace hyt
             if (!E.in_parts.empty())
                 return false;
             const auto t0 = Clock::now();
-            if (!device_transpose<OffT>(E))
-                ensure_host_transpose(*E.host, E.threads);
+            {
+                std::lock_guard<std::recursive_mutex> lock(E.host->transpose_mu);
+                if (!device_transpose<OffT>(E))
+                    ensure_host_transpose(*E.host, E.threads);
+            }
             E.transpose_ms = ms_since(t0);
             const std::uint64_t V = E.host->V;
             E.in_parts = partition_graph(E.host->in_off, V, E.cfg.partition_edges * 4);

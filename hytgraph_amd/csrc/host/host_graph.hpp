@@ -8,6 +8,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 
 namespace hyt
 {
@@ -20,9 +21,11 @@ namespace hyt
         std::uint32_t *col = nullptr;
         std::uint32_t *w = nullptr;
         // incoming CSR (the transpose, sources of each destination in
-        // ascending order), same memory kind; built on first use
+        // ascending order), same memory kind; built on first use, under
+        // transpose_mu (engines on several threads may share a host graph)
         std::uint64_t *in_off = nullptr;
         std::uint32_t *in_col = nullptr;
+        std::recursive_mutex transpose_mu;
 
         HostGraph(std::uint64_t v, std::uint64_t e, bool weights, int mem);
         ~HostGraph();

@@ -44,6 +44,7 @@ namespace hyt
 
     void ensure_host_transpose(HostGraph &g, int threads)
     {
+        std::lock_guard<std::recursive_mutex> lock(g.transpose_mu);
         if (g.in_off != nullptr && g.in_col != nullptr)
             return;
         const std::uint64_t V = g.V, E = g.E;
