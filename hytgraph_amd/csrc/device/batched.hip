@@ -174,9 +174,9 @@ namespace hyt
             {
                 round_begin<<<1, 64, 0, s>>>(d_ctr, rs, KP, o.lane_form, max_rounds);
                 if (ppr)
-                    pprk_snapshot<<<snapshot_blocks(g.V * static_cast<std::uint64_t>(K)), kBlock, 0, s>>>(
+                    pprk_snapshot<OffT><<<snapshot_blocks(g.V * static_cast<std::uint64_t>(K)), kBlock, 0, s>>>(
                         rows.q, 0, K, rows.qmask, kw.val.as<unsigned long long>(), kw.rank.as<unsigned long long>(),
-                        kw.snap.as<unsigned long long>(), rs);
+                        kw.snap.as<unsigned long long>(), g.off.as<OffT>(), op.damping, rs);
                 expandk_light<QueueRowsK<OffT>, KP, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
                 expandk_light_items<QueueRowsK<OffT>, Op><<<light_grid, kBlock, 0, s>>>(rows, op, a);
                 expandk_heavy<KP, Op><<<kHeavyGrid, kBlock, 0, s>>>(op, a);

@@ -57,7 +57,7 @@ namespace hyt
             ws.ctr.alloc(sizeof(Counters), "hipMalloc(counters)");
             ws.partial.alloc(4096 * sizeof(double), "hipMalloc(partials)");
             ws.active.alloc(Vp / 8 + 64, "hipMalloc(frontier bitmap)");
-            ws.share.alloc(Vp * 4, "hipMalloc(push shares)");
+            ws.share.alloc(Vp * 8, "hipMalloc(push shares)");
             ws.rc.alloc(sizeof(RoundCtl), "hipMalloc(round control)");
             // touch must start (and is kept) all-zero: compaction clears what it reads
             hip_check(hipMemsetAsync(ws.touch.ptr, 0, Vp, g.stream), "clear touch");

@@ -1738,14 +1738,16 @@ namespace hyt
             launch_fill<unsigned long long>(ws.res.as<unsigned long long>(), g.V, m.r0, s);
             launch_fill<unsigned long long>(ws.rank.as<unsigned long long>(), g.V, 0ULL, s);
             hip_check(hipMemsetAsync(d_ctr, 0, sizeof(Counters), s), "reset counters");
-            PrSel<OffT> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(), ws.share.as<float>(),
-                            g.off.as<OffT>(), damping, m.eps};
+            // global PageRank: float shares, as in HBM
+            PrSel<OffT, false> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(),
+                                   ws.share.as<float>(), g.off.as<OffT>(), damping, m.eps};
             Counters c;
             std::uint32_t F = ordered_compaction<OffT>(E, sel, c);
             auto make = [&](const std::uint32_t *col, const std::uint32_t *, const std::uint32_t *qd) {
-                PrOp op;
+                PrOp<false> op;
                 op.col = col;
                 op.res = ws.res.as<unsigned long long>();
+                op.share = nullptr;
                 op.qd = qd;
                 return op;
             };
@@ -2004,10 +2006,10 @@ namespace hyt
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
                 if (ppr)
                 {
-                    lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
-                        g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
-                        L.val.as<unsigned long long>(), L.rank.as<unsigned long long>(),
-                        L.snap.as<unsigned long long>());
+                    lanes::pprk_snapshot<OffT><<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0,
+                                                 s>>>(g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(),
+                                                      L.val.as<unsigned long long>(), L.rank.as<unsigned long long>(),
+                                                      L.snap.as<unsigned long long>(), g.off.as<OffT>(), op.damping);
                     hip_check(hipGetLastError(), "launch pprk_snapshot");
                 }
                 const LaneExpander<OffT, KP, Op> x{E, op, a, lanes::use_items(o.lane_form, pairs, F, KP)};
@@ -2141,9 +2143,9 @@ namespace hyt
             hip_check(hipMemsetAsync(L.lanes.ptr, 0, 8, s), "reset lane mask");
             if (ppr)
             {
-                lanes::pprk_snapshot<<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
+                lanes::pprk_snapshot<OffT><<<lanes::snapshot_blocks(static_cast<std::uint64_t>(F) * K), kBlock, 0, s>>>(
                     g.ws.q.as<std::uint32_t>(), F, K, L.qmask.as<unsigned long long>(), L.val.as<unsigned long long>(),
-                    L.rank.as<unsigned long long>(), L.snap.as<unsigned long long>());
+                    L.rank.as<unsigned long long>(), L.snap.as<unsigned long long>(), g.off.as<OffT>(), op.damping);
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
             lanes::KArgs a{};

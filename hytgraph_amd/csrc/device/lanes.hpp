@@ -718,6 +718,7 @@ namespace hyt
             std::uint32_t *dist;
             std::uint32_t K;
             std::uint32_t *overflow;
+            float damping = 0.0F; // unused (the PPR snapshot's parameter)
 
             __device__ std::uint32_t row_scale(std::uint64_t) const { return 0U; }
             __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
@@ -763,36 +764,32 @@ namespace hyt
         };
 
         // ----------------------------------------------------------- PPR-K
-        // Residuals, ranks and snapshots are mass words (mass.hpp): the add
-        // of lane kl through row i is the share of its snapshot word at the
-        // row's scale, computed as in the single-query form, and the
-        // returning integer atomic tells exactly one adder that the residual
-        // crossed eps.
+        // Residuals, ranks and snapshots are mass words (mass.hpp): the
+        // snapshot of lane kl at row i is already the share word per out-edge
+        // of the residual it moved (pprk_snapshot, computed as in the
+        // single-query form), and the returning integer atomic tells exactly
+        // one adder that the residual crossed eps.
         struct PprKOp
         {
             using Val = unsigned long long; // mass word
             static constexpr bool kRowIsQueueIndex = true;
             const std::uint32_t *__restrict__ col;
             unsigned long long *res;
-            const unsigned long long *snap; // [F][K] snapshot residuals
+            const unsigned long long *snap; // [F][K] share words of the moved residuals
             std::uint32_t K;
             float damping;
             unsigned long long eps; // threshold word
 
-            __device__ std::uint32_t row_scale(std::uint64_t deg) const
-            {
-                return __float_as_uint(row_scale_of(damping, deg));
-            }
+            __device__ std::uint32_t row_scale(std::uint64_t) const { return 0U; }
             __device__ void load(std::uint64_t pos, std::uint32_t &v, std::uint32_t &wt) const
             {
                 v = col[pos];
                 wt = 0;
             }
-            __device__ bool prepare(std::uint32_t i, std::uint32_t scale, std::uint32_t, std::uint32_t, int kl,
+            __device__ bool prepare(std::uint32_t i, std::uint32_t, std::uint32_t, std::uint32_t, int kl,
                                     unsigned long long &cand, unsigned long long &) const
             {
-                const unsigned long long r = snap[static_cast<std::uint64_t>(i) * K + kl];
-                cand = r ? share_word(share_of(r, __uint_as_float(scale))) : 0ULL;
+                cand = snap[static_cast<std::uint64_t>(i) * K + kl];
                 return cand != 0ULL;
             }
             __device__ void commit(std::uint32_t v, int kl, unsigned long long cand, unsigned long long &seen) const
@@ -805,11 +802,13 @@ namespace hyt
             }
         };
 
-        // Moves the active residuals of the frontier rows into rank.
+        // Moves the active residuals of the frontier rows into rank; the
+        // snapshot keeps each moved residual's share word per out-edge.
+        template <class OffT>
         __global__ void pprk_snapshot(const std::uint32_t *q, std::uint32_t F, std::uint32_t K,
                                       const unsigned long long *qmask, unsigned long long *res,
-                                      unsigned long long *rank, unsigned long long *snap,
-                                      const RoundState *rs = nullptr)
+                                      unsigned long long *rank, unsigned long long *snap, const OffT *off,
+                                      float damping, const RoundState *rs = nullptr)
         {
             const std::uint64_t n = static_cast<std::uint64_t>(rs ? rs->F : F) * K;
             const std::uint64_t stride = static_cast<std::uint64_t>(gridDim.x) * blockDim.x;
@@ -819,15 +818,16 @@ namespace hyt
                 const std::uint32_t i = static_cast<std::uint32_t>(t / K);
                 const std::uint32_t k = static_cast<std::uint32_t>(t % K);
                 const std::uint32_t u = q[i];
-                unsigned long long r = 0ULL;
+                unsigned long long sw = 0ULL;
                 if ((qmask[i] >> k) & 1ULL)
                 {
                     const std::uint64_t idx = static_cast<std::uint64_t>(u) * K + k;
-                    r = res[idx];
+                    const unsigned long long r = res[idx];
                     res[idx] = 0ULL;
                     rank[idx] += r;
+                    sw = share_word(r, row_scale_of(damping, static_cast<std::uint64_t>(off[u + 1] - off[u])));
                 }
-                snap[t] = r;
+                snap[t] = sw;
             }
         }
 

@@ -5,18 +5,25 @@
 // holds after a round does not depend on the order in which its incoming
 // pushes land: a query gives the same bits alone, inside a batch or inside a
 // served mix. What is left of rounding is
-//   * each pushing row's per-edge share: its scale d / outdeg(u) rounded to
-//     float, times the residual word, rounded to the float that carries it
-//     through the expand kernels (relative error <= 2^-23 with both roundings),
-//     then to the nearest word, and
+//   * each pushing row's per-edge share: its scale d / outdeg(u) in double,
+//     times the residual word in double (relative error <= 2^-52 with both
+//     roundings), rounded to the nearest word - and, for global PageRank,
+//     that word rounded to float and back (2^-24): its pull rounds gather
+//     4-byte shares, and a row that crosses eps in one form but not in the
+//     other moves eps of mass V, nothing in the normalised ranks;
 //   * the final normalisation x / |x|_1 to float (relative 2^-24 per entry).
 // Every form (push or pull rounds, one query or K lanes, in HBM or through
 // the hybrid engine) computes a share word with the same operations, so a
-// PPR query gives the same words in all of them.
-// With unnormalised delta-push the pushed mass of a whole run is at most
+// query gives the same words in all of them.
+// PPR keeps the words: its shares are then as precise as the FP64
+// restatement's, so the rows that cross eps in a round are the FP64
+// procedure's. With float shares a residual within that margin of eps could
+// be pushed in one and not in the other, and each such flip moved about eps
+// of one seed's mass (seen: L1 1.7e-4 at eps 1e-6 on a scale-13 graph). With
+// unnormalised delta-push the pushed mass of a whole run is at most
 // |x|_1 * d / (1 - d), so the share rounding moves the normalised L1 result
-// by at most d / (1 - d) * 2^-23 (6.8e-7 at d = 0.85), against 1e-6 allowed;
-// the roundings are unbiased, and the measured error is far below the bound.
+// by at most d / (1 - d) * 2^-52 (words) or d / (1 - d) * 2^-24 (float
+// shares, 3.4e-7 at d = 0.85), plus half a word per push.
 //
 // frac: a run seeded with n vertices of residual (1 - d) never holds more
 // than n in total (each push of r adds at most d * r to the residuals and r
@@ -54,19 +61,16 @@ namespace hyt
         return m;
     }
 
-    // d / outdeg(u) as a float (the same in every form).
-    __device__ __forceinline__ float row_scale_of(float damping, std::uint64_t deg)
+    // d / outdeg(u) in double (the same in every form; 0 for no out-edges).
+    __device__ __forceinline__ double row_scale_of(float damping, std::uint64_t deg)
     {
-        return deg ? __fdiv_rn(damping, static_cast<float>(deg)) : 0.0F;
+        return deg ? __ddiv_rn(static_cast<double>(damping), static_cast<double>(deg)) : 0.0;
     }
 
-    // Share per out-edge, in words, of a pushing residual word r with the
-    // row's scale (float: what the expand kernels carry per row).
-    __device__ __forceinline__ float share_of(unsigned long long r, float scale)
+    // Word of the share per out-edge of a pushing residual word r at the
+    // row's scale: r * scale in double, rounded to the nearest word.
+    __device__ __forceinline__ unsigned long long share_word(unsigned long long r, double scale)
     {
-        return static_cast<float>(static_cast<double>(r) * static_cast<double>(scale));
+        return __double2ull_rn(__dmul_rn(static_cast<double>(r), scale));
     }
-
-    // Word of a share, rounded to nearest.
-    __device__ __forceinline__ unsigned long long share_word(float share) { return __float2ull_rn(share); }
 } // namespace hyt

@@ -3,6 +3,8 @@
 // and the hybrid transfer engine.
 #pragma once
 
+#include <type_traits>
+
 #include "compact.hpp"
 #include "expand.hpp"
 #include "mass.hpp"
@@ -304,19 +306,34 @@ namespace hyt
         }
     };
 
-    // Delta-push of PageRank / PPR mass (mass.hpp): a row carries its share
-    // per out-edge (float bits, in mass words, written by PrSel into the
-    // queue values), and every edge adds the share's word to the residual of
-    // its destination with an integer atomic (exact, order-independent).
+    // Share words per out-edge, as PrSel stores them by vertex: PPR keeps the
+    // words themselves (8 bytes), global PageRank the words rounded to float
+    // (4 bytes per gather in pull rounds; mass.hpp).
+    template <bool WORDS>
+    using ShareT = std::conditional_t<WORDS, unsigned long long, float>;
+    __device__ __forceinline__ unsigned long long word_of(unsigned long long w) { return w; }
+    __device__ __forceinline__ unsigned long long word_of(float f) { return __float2ull_rn(f); }
+
+    // Delta-push of PageRank / PPR mass (mass.hpp): every edge adds its row's
+    // share word to the residual of its destination with an integer atomic
+    // (exact, order-independent). PPR (WORDS): a row carries its vertex id
+    // and the word is read from share[] (the rows of one wavefront step share
+    // few words, so the per-edge read is a cache hit); global PageRank: a row
+    // carries its float share as the queue value.
+    template <bool WORDS>
     struct PrOp
     {
         const std::uint32_t *__restrict__ col;
         unsigned long long *res;
-        const std::uint32_t *qd; // share of each queue row (float bits)
+        const unsigned long long *__restrict__ share; // WORDS: share word per out-edge, by vertex
+        const std::uint32_t *qd;                      // !WORDS: float share of each queue row
 
-        __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t, std::uint64_t deg) const
+        __device__ std::uint32_t row_value(std::uint32_t i, std::uint32_t u, std::uint64_t deg) const
         {
-            return deg ? qd[i] : 0U;
+            if constexpr (WORDS)
+                return u;
+            else
+                return deg ? qd[i] : 0U;
         }
         __device__ void expanded(std::uint32_t, std::uint32_t, bool) const {}
         __device__ void expanded_chunked(std::uint32_t, bool) const {}
@@ -331,23 +348,29 @@ namespace hyt
         __device__ std::uint32_t peek(std::uint32_t) const { return 0U; }
         __device__ void apply(std::uint32_t v, std::uint32_t c, std::uint32_t) const
         {
-            const unsigned long long w = share_word(__uint_as_float(c));
+            unsigned long long w;
+            if constexpr (WORDS)
+                w = share[c];
+            else
+                w = word_of(__uint_as_float(c));
             if (w != 0ULL)
                 atomicAdd(&res[v], w);
         }
     };
 
     // Selects the rows whose residual word is above eps, moves the residual
-    // into rank and zeroes it, and writes the row's share per out-edge
-    // d * r_u / outdeg(u) (float, in words) both as the queue value (push
-    // rounds) and by vertex into share[] (pull rounds; 0 outside the frontier).
-    template <class OffT>
+    // into rank and zeroes it, and writes the row's share word per out-edge
+    // d * r_u / outdeg(u) (mass.hpp) by vertex into share[] (ShareT<WORDS>;
+    // 0 outside the frontier); the queue value is that word as a float (the
+    // push rounds of global PageRank, the hybrid engine's pushed-mass
+    // priorities).
+    template <class OffT, bool WORDS>
     struct PrSel
     {
         using Aux = NoAux;
         unsigned long long *res;
         unsigned long long *rank;
-        float *share;
+        ShareT<WORDS> *share;
         const OffT *off;
         float damping;
         unsigned long long eps;
@@ -381,7 +404,7 @@ namespace hyt
             for (int j = 0; j < 4; ++j)
                 if (r[j] > eps)
                     m |= 1U << j;
-            float sh[4] = {0.0F, 0.0F, 0.0F, 0.0F};
+            ShareT<WORDS> sh[4] = {};
             if (m)
             {
                 // offsets of the 4 rows (+1); a full group reads them as one
@@ -419,8 +442,14 @@ namespace hyt
                     if ((m >> j) & 1U)
                     {
                         k[j] += r[j];
-                        sh[j] = share_of(r[j], row_scale_of(damping, static_cast<std::uint64_t>(o[j + 1] - o[j])));
-                        vals[j] = __float_as_uint(sh[j]);
+                        const unsigned long long sw =
+                            share_word(r[j], row_scale_of(damping, static_cast<std::uint64_t>(o[j + 1] - o[j])));
+                        const float sf = __ull2float_rn(sw);
+                        if constexpr (WORDS)
+                            sh[j] = sw;
+                        else
+                            sh[j] = sf;
+                        vals[j] = __float_as_uint(sf);
                         r[j] = 0ULL;
                     }
                 }
@@ -445,7 +474,15 @@ namespace hyt
             // every row's share is rewritten (0 outside the frontier), so a
             // pull round needs no frontier test: one gather per in-edge
             if (n == 4)
-                *reinterpret_cast<float4 *>(share + row0) = make_float4(sh[0], sh[1], sh[2], sh[3]);
+            {
+                if constexpr (WORDS)
+                {
+                    *reinterpret_cast<ulonglong2 *>(share + row0) = make_ulonglong2(sh[0], sh[1]);
+                    *reinterpret_cast<ulonglong2 *>(share + row0 + 2) = make_ulonglong2(sh[2], sh[3]);
+                }
+                else
+                    *reinterpret_cast<float4 *>(share + row0) = make_float4(sh[0], sh[1], sh[2], sh[3]);
+            }
             else
             {
 #pragma unroll

@@ -29,16 +29,17 @@ namespace hyt
 {
     namespace
     {
+        template <bool WORDS>
         struct PrPullOp
         {
             const std::uint32_t *__restrict__ in_col;
-            const float *share;
+            const ShareT<WORDS> *share;
             unsigned long long *res;
 
             __device__ unsigned long long identity() const { return 0ULL; }
             __device__ bool edge(std::uint64_t pos, unsigned long long &c) const
             {
-                c = share_word(share[in_col[pos]]);
+                c = word_of(share[in_col[pos]]);
                 return c != 0ULL;
             }
             __device__ unsigned long long fold(unsigned long long a, unsigned long long b) const { return a + b; }
@@ -144,12 +145,12 @@ namespace hyt
 
         constexpr unsigned kFoldBlocks = 1024; // partial holds 2*kFoldBlocks+2 words
 
-        template <class OffT>
-        Counters compact_and_read(DeviceGraph &g, const PrSel<OffT> &sel)
+        template <class OffT, bool WORDS>
+        Counters compact_and_read(DeviceGraph &g, const PrSel<OffT, WORDS> &sel)
         {
             Workspace &ws = g.ws;
             Counters *d_ctr = ws.ctr.as<Counters>();
-            compact_rows<std::uint32_t, OffT, PrSel<OffT>><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
+            compact_rows<std::uint32_t, OffT, PrSel<OffT, WORDS>><<<compact_blocks(g.V), kBlock, 0, g.stream>>>(
                 g.V, sel, ws.q.as<std::uint32_t>(), ws.qd.as<std::uint32_t>(), d_ctr, g.off.as<OffT>(),
                 static_cast<std::uint16_t *>(nullptr));
             hip_check(hipGetLastError(), "launch pagerank compaction");
@@ -159,7 +160,8 @@ namespace hyt
             return *h_ctr;
         }
 
-        template <class OffT>
+        // WORDS: PPR (share words by vertex); global PageRank: float shares
+        template <class OffT, bool WORDS>
         void run_push(DeviceGraph &g, const hyt_run_options &o, float damping, const MassScale &m,
                       hyt_run_stats &st, std::uint32_t light_max, std::uint32_t chunk, bool resume = false,
                       std::uint64_t limit = 0)
@@ -169,11 +171,12 @@ namespace hyt
             Counters *d_ctr = ws.ctr.as<Counters>();
             const std::uint64_t max_rounds = limit ? limit : (o.max_rounds ? o.max_rounds : 100000);
 
-            PrSel<OffT> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(), ws.share.as<float>(),
-                            g.off.as<OffT>(), damping, m.eps};
-            PrOp op;
+            PrSel<OffT, WORDS> sel{ws.res.as<unsigned long long>(), ws.rank.as<unsigned long long>(),
+                                   ws.share.as<ShareT<WORDS>>(), g.off.as<OffT>(), damping, m.eps};
+            PrOp<WORDS> op;
             op.col = g.col.as<std::uint32_t>();
             op.res = ws.res.as<unsigned long long>();
+            op.share = ws.share.as<unsigned long long>();
             op.qd = ws.qd.as<std::uint32_t>();
 
             ExpandArgs a;
@@ -190,7 +193,8 @@ namespace hyt
                 ensure_transpose(g, light_max, chunk);
             const bool can_pull = g.in_col.ptr != nullptr;
             require(dir != HYT_DIR_PULL || can_pull || g.E == 0, "pull rounds need fewer than 2^31 edges");
-            PrPullOp pop{g.in_col.as<std::uint32_t>(), ws.share.as<float>(), ws.res.as<unsigned long long>()};
+            PrPullOp<WORDS> pop{g.in_col.as<std::uint32_t>(), ws.share.as<ShareT<WORDS>>(),
+                                ws.res.as<unsigned long long>()};
 
             Counters c0{};
             if (resume)
@@ -204,7 +208,7 @@ namespace hyt
             else
             {
                 clear_counters(d_ctr, s);
-                c0 = compact_and_read<OffT>(g, sel);
+                c0 = compact_and_read(g, sel);
             }
             std::uint32_t F = c0.next_count;
             unsigned long long Fe = c0.frontier_edges;
@@ -219,10 +223,10 @@ namespace hyt
                     hip_check(hipEventRecord(g.ev0, s), "hipEventRecord");
                 if (pull)
                 {
-                    pull_light<OffT, PrPullOp><<<blocks_for(g.V), kBlock, 0, s>>>(g.in_off.as<OffT>(), pop, g.V,
+                    pull_light<OffT, PrPullOp<WORDS>><<<blocks_for(g.V), kBlock, 0, s>>>(g.in_off.as<OffT>(), pop, g.V,
                                                                                    light_max);
                     hip_check(hipGetLastError(), "launch pr_pull_light");
-                    pull_heavy<PrPullOp><<<kSingleHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
+                    pull_heavy<PrPullOp<WORDS>><<<kSingleHeavyGrid, kBlock, 0, s>>>(pop, g.in_heavy.as<HeavyChunk>(),
                                                                       g.in_heavy_count);
                     hip_check(hipGetLastError(), "launch pr_pull_heavy");
                     st.pull_rounds += 1;
@@ -230,15 +234,15 @@ namespace hyt
                 }
                 else
                 {
-                    expand_light<GraphRows<OffT>, PrOp><<<blocks_for(F), kBlock, 0, s>>>(whole_graph(g.off.as<OffT>(), a.q),
+                    expand_light<GraphRows<OffT>, PrOp<WORDS>><<<blocks_for(F), kBlock, 0, s>>>(whole_graph(g.off.as<OffT>(), a.q),
                                                                                      op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_light");
-                    expand_heavy<PrOp><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a);
+                    expand_heavy<PrOp<WORDS>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a);
                     hip_check(hipGetLastError(), "launch pr_expand_heavy");
                 }
                 if (o.time_kernels)
                     hip_check(hipEventRecord(g.ev1, s), "hipEventRecord");
-                const Counters c = compact_and_read<OffT>(g, sel);
+                const Counters c = compact_and_read(g, sel);
                 if (o.time_kernels)
                 {
                     float ms = 0;
@@ -336,10 +340,20 @@ namespace hyt
             pr_seed<<<1, 64, 0, s>>>(res, seed, m.r0);
             hip_check(hipGetLastError(), "launch pr_seed");
         }
+        // PPR keeps share words (its normalised mass is one seed's, so a row
+        // flipping at eps would move eps of it); global PageRank float shares
+        // (a flip moves eps of V, and pull rounds gather 4 bytes)
         if (g.off64)
-            run_push<std::uint64_t>(g, o, damping, m, st, light_max, chunk, resume, rounds);
+        {
+            if (global)
+                run_push<std::uint64_t, false>(g, o, damping, m, st, light_max, chunk, resume, rounds);
+            else
+                run_push<std::uint64_t, true>(g, o, damping, m, st, light_max, chunk, resume, rounds);
+        }
+        else if (global)
+            run_push<std::uint32_t, false>(g, o, damping, m, st, light_max, chunk, resume, rounds);
         else
-            run_push<std::uint32_t>(g, o, damping, m, st, light_max, chunk, resume, rounds);
+            run_push<std::uint32_t, true>(g, o, damping, m, st, light_max, chunk, resume, rounds);
     }
 
     void ppr_batched(DeviceGraph &g, const std::uint32_t *seeds, std::uint32_t K, float damping, float eps,

@@ -14,14 +14,14 @@
 //     segment's index in `value`), which pr_pull_seg_heavy reduces one
 //     wavefront per chunk, finishing with one atomic per chunk.
 // Every row's share is rewritten by the compaction before a round (0 outside
-// the frontier), so an in-edge adds share_word(share[source]) whatever its
-// source. The sums are 64-bit integer words, so the totals equal the push
+// the frontier), so an in-edge adds the word of share[source] (global
+// PageRank: the float share of ops.hpp) whatever its source. The sums are 64-bit integer words, so the totals equal the push
 // rounds' bit for bit. Segments of one launch hold disjoint rows, and
 // launches on one stream run in order, so the single store is safe.
 #pragma once
 
 #include "kern_common.hpp"
-#include "mass.hpp"
+#include "ops.hpp"
 
 namespace hyt
 {
@@ -125,7 +125,7 @@ namespace hyt
                 {
                     const int j = owner_of(ex, e);
                     jj[k] = j;
-                    cc[k] = share_word(share[col[s_begin[wid][j] + (e - ex[j]) - lo]]);
+                    cc[k] = word_of(share[col[s_begin[wid][j] + (e - ex[j]) - lo]]);
                 }
             }
 #pragma unroll
@@ -179,7 +179,7 @@ namespace hyt
                 for (int k = 0; k < kUnroll; ++k)
                 {
                     const std::uint32_t e = base + static_cast<std::uint32_t>(k * kWave + lane);
-                    cc[k] = e < h.count ? share_word(share[col[h.begin + e - sg.lo]]) : 0ULL;
+                    cc[k] = e < h.count ? word_of(share[col[h.begin + e - sg.lo]]) : 0ULL;
                 }
 #pragma unroll
                 for (int k = 0; k < kUnroll; ++k)

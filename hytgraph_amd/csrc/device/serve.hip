@@ -438,9 +438,9 @@ namespace hyt
             if (S.kind == HYT_QUERY_PPR)
             {
                 const std::uint64_t n = static_cast<std::uint64_t>(S.F) * S.K;
-                pprk_snapshot<<<snapshot_blocks(n), kBlock, 0, s>>>(
+                pprk_snapshot<OffT><<<snapshot_blocks(n), kBlock, 0, s>>>(
                     rows.q, S.F, S.K, rows.qmask, S.val.as<unsigned long long>(), S.rank.as<unsigned long long>(),
-                    S.snap.as<unsigned long long>());
+                    S.snap.as<unsigned long long>(), g.off.as<OffT>(), op.damping);
                 hip_check(hipGetLastError(), "launch pprk_snapshot");
             }
             if (use_items(S.form, S.pairs, S.F, S.KP))
