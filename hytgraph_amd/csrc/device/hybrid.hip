@@ -508,7 +508,8 @@ namespace hyt
         {
             const std::uint64_t Vp = dg.V + kPad;
             return edge_area_bytes() + dg.off.bytes + part_vbegin.bytes + part_stats_d.bytes + block_counts.bytes +
-                   block_offsets.bytes + scan_temp.bytes + stage_q.bytes + seg_d.bytes + Vp * 4 + sizeof(Counters);
+                   block_offsets.bytes + scan_temp.bytes + stage_q.bytes + seg_d.bytes + Vp * 4 + sizeof(Counters) +
+                   in_off_d.bytes + pull_seg_d.bytes + pull_heavy.bytes + pull_ctr.bytes;
         }
 
         // A lane session uses only the queue and the counters of the
