@@ -18,7 +18,10 @@ namespace hyt
     struct Weights<kWU32>
     {
         static constexpr std::uint32_t kInf = 0xFFFFFFFFU;
-        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos)
+        {
+            return __builtin_nontemporal_load(w + pos);
+        }
         __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
         {
             const std::uint64_t s = static_cast<std::uint64_t>(d) + w;
@@ -35,7 +38,10 @@ namespace hyt
     struct Weights<kWF32>
     {
         static constexpr std::uint32_t kInf = 0x7F800000U;
-        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos) { return w[pos]; }
+        __device__ static std::uint32_t load(const std::uint32_t *w, std::uint64_t pos)
+        {
+            return __builtin_nontemporal_load(w + pos);
+        }
         __device__ static bool add(std::uint32_t d, std::uint32_t w, std::uint32_t &out)
         {
             out = __float_as_uint(__fadd_rn(__uint_as_float(d), __uint_as_float(w)));
@@ -105,9 +111,11 @@ namespace hyt
             }
             return __shfl(v, 0, kWave);
         }
+        // columns and weights stream (each edge is read once per expansion):
+        // nontemporal loads leave the L2 to the destinations' distances
         __device__ bool load(std::uint64_t pos, std::uint32_t du, std::uint32_t &v, std::uint32_t &c) const
         {
-            v = col[pos];
+            v = __builtin_nontemporal_load(col + pos);
             if (Weights<WM>::add(du, Weights<WM>::load(w, pos), c))
                 return true;
             atomicOr(overflow, 1U);
