@@ -16,7 +16,9 @@
 // Optionally (non-null pointers) the pass also writes the frontier bitmap
 // (16 rows per thread = one u16 word, every word rewritten) and adds the
 // out-degree sum of the selected rows to ctr->frontier_edges, which the host
-// uses to choose a push or a pull round.
+// uses to choose a push or a pull round; and (HubOut) lists the selected
+// rows that a per-vertex bitmap marks (rows above the expansion's light_max)
+// apart as well, so the pass that queues their chunks reads them alone.
 #pragma once
 
 #include "kern_common.hpp"
@@ -26,6 +28,16 @@ namespace hyt
     // Selector side information for selectors that have none.
     struct NoAux
     {
+    };
+
+    // bits[t]: bit j = row 16t + j is listed (the same 16 rows per thread as
+    // the compaction); list / count: the list and its length (appended, one
+    // atomic per workgroup that has such rows; count on a line of its own).
+    struct HubOut
+    {
+        const std::uint16_t *bits = nullptr;
+        std::uint32_t *list = nullptr;
+        std::uint32_t *count = nullptr;
     };
 
 
@@ -49,16 +61,19 @@ namespace hyt
     __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
                                                            Counters *ctr, const OffT *off, std::uint16_t *active,
                                                            std::uint32_t *block_counts = nullptr,
-                                                           const std::uint32_t *gate = nullptr)
+                                                           const std::uint32_t *gate = nullptr, HubOut hubs = {})
     {
         if (gate != nullptr && *gate == 0)
             return; // uniform: every thread reads the same word
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
         __shared__ unsigned long long s_dsum[kWavesPerBlock];
-        __shared__ std::uint32_t s_base;
+        __shared__ std::uint32_t s_base, s_hbase;
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint64_t row0 = (static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x) * kCompactRows;
+        const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+        // the listed-row bits of the thread's 16 rows, read beside the selection's loads
+        const std::uint32_t hbits = hubs.bits != nullptr && row0 < V ? hubs.bits[tid] : 0U;
 
         // Static indexing only (fully unrolled), so everything stays in VGPRs.
         Val gv[kCompactGroups][4];
@@ -94,9 +109,11 @@ namespace hyt
         }
 
         sel.publish(aux, ctr);
-        const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+        const std::uint32_t picked = gm[0] | (gm[1] << 4) | (gm[2] << 8) | (gm[3] << 12);
         if (active != nullptr && row0 < V)
-            active[tid] = static_cast<std::uint16_t>(gm[0] | (gm[1] << 4) | (gm[2] << 8) | (gm[3] << 12));
+            active[tid] = static_cast<std::uint16_t>(picked);
+        std::uint32_t hm = picked & hbits;
+        const std::uint32_t nh = static_cast<std::uint32_t>(__popc(hm));
         if (off != nullptr)
         {
             unsigned long long dsum = 0;
@@ -116,9 +133,13 @@ namespace hyt
                 s_dsum[wid] = dsum;
         }
 
-        const std::uint32_t incl = wave_incl_scan(n);
+        // rows (low 16 bits) and listed rows (high 16 bits) in one scan: a
+        // workgroup selects at most kCompactRowsPerBlock = 4096 rows
+        static_assert(kCompactRowsPerBlock < (1U << 16), "packed row counts");
+        const std::uint32_t incl2 = wave_incl_scan(n | (nh << 16));
+        const std::uint32_t incl = incl2 & 0xFFFFU;
         if (lane == kWave - 1)
-            s_wsum[wid] = incl;
+            s_wsum[wid] = incl2;
         __syncthreads();
         if (off != nullptr && threadIdx.x == 0)
         {
@@ -130,14 +151,16 @@ namespace hyt
             if (dsum)
                 atomicAdd(&ctr->frontier_edges, dsum);
         }
-        std::uint32_t wbase = 0, total = 0;
+        std::uint32_t wbase2 = 0, total2 = 0;
 #pragma unroll
         for (int w = 0; w < kWavesPerBlock; ++w)
         {
             if (w < wid)
-                wbase += s_wsum[w];
-            total += s_wsum[w];
+                wbase2 += s_wsum[w];
+            total2 += s_wsum[w];
         }
+        const std::uint32_t wbase = wbase2 & 0xFFFFU, total = total2 & 0xFFFFU;
+        const std::uint32_t htotal = total2 >> 16;
         if (threadIdx.x == 0)
         {
             if (block_counts != nullptr)
@@ -147,8 +170,15 @@ namespace hyt
             }
             else
                 s_base = total ? atomicAdd(&ctr->next_count, total) : 0U;
+            s_hbase = htotal ? atomicAdd(hubs.count, htotal) : 0U;
         }
         __syncthreads();
+        if (hm != 0U)
+        {
+            std::uint32_t at = s_hbase + (wbase2 >> 16) + (incl2 >> 16) - nh;
+            for (; hm != 0U; hm &= hm - 1U)
+                hubs.list[at++] = static_cast<std::uint32_t>(row0) + static_cast<std::uint32_t>(__ffs(hm) - 1);
+        }
         std::uint32_t pos = s_base + wbase + incl - n;
 #pragma unroll
         for (int g = 0; g < kCompactGroups; ++g)

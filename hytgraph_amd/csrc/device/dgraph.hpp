@@ -66,6 +66,9 @@ namespace hyt
         std::uint32_t pending;             // improved rows held back (at/above the bucket bound)
         std::uint32_t pending_inv_max;     // max of ~dist over held-back rows (= ~min dist)
         EdgeSlot slot[kEdgeSlots];         // edges expanded this round, spread
+        // rows above light_max the compaction listed apart (a line of its own:
+        // next_count takes every workgroup's atomic)
+        alignas(256) std::uint32_t next_hubs;
         std::uint32_t overflow;            // a u32 sum overflowed
         std::uint32_t heavy_overflow;      // heavy chunk queue overflowed
 
@@ -117,6 +120,7 @@ namespace hyt
         std::uint32_t done;      // a round found the queue empty, or the round limit
         std::uint32_t truncated; // stopped by the round limit with work left
         std::uint32_t heavy_bound; // rows above light_max with a larger distance wait (this round)
+        std::uint32_t H;           // rows of the current round's hub list (rows above light_max)
         std::uint64_t rounds, frontier_total, edges_total, updates_total;
         std::uint64_t heavy_edges_total; // of edges_total: walked by the chunk kernel
         // timed runs: device wall clock before and after the expand kernels
@@ -130,6 +134,12 @@ namespace hyt
         DeviceBuffer res, rank; // PageRank mass words u64[V] (ensure_mass_state)
         DeviceBuffer active; // frontier bitmap (bit v = row v selected by the last compaction)
         DeviceBuffer share;  // PageRank pull: d * r_u / outdeg(u) of the frontier rows, by vertex
+        // device-driven ASYNC SSSP: bit j of hub_bits[t] = row 16t + j has
+        // more than hub_light_max edges; the compaction lists the selected
+        // ones in hub_list (hub_rows entries at most)
+        DeviceBuffer hub_bits, hub_list;
+        std::uint32_t hub_light_max = 0;
+        std::uint64_t hub_rows = 0;
         std::uint64_t heavy_cap = 0;
         bool ready = false;
     };
@@ -160,7 +170,8 @@ namespace hyt
         {
             return off.bytes + col.bytes + w.bytes + in_off.bytes + in_col.bytes + in_w.bytes + in_heavy.bytes +
                    ws.dist.bytes + ws.res.bytes + ws.rank.bytes + ws.prev.bytes + ws.touch.bytes + ws.q.bytes + ws.qd.bytes +
-                   ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes + ws.rc.bytes;
+                   ws.heavy.bytes + ws.ctr.bytes + ws.partial.bytes + ws.active.bytes + ws.share.bytes + ws.rc.bytes +
+                   ws.hub_bits.bytes + ws.hub_list.bytes;
         }
         int weight_mode() const
         {
@@ -281,6 +292,12 @@ namespace hyt
         Counters *ctr;
         // non-null: push-only rounds run device-driven from a captured graph
         RoundCtl *rc = nullptr;
+        // non-null (device-driven ASYNC rounds): rows above light_max by bit
+        // (hub_bits, for this light_max) and the list the compaction writes
+        // them to, so the chunk-queuing pass reads those rows alone
+        const std::uint16_t *hub_bits = nullptr;
+        std::uint32_t *hub_list = nullptr;
+        std::uint64_t hub_rows = 0;
     };
     // The single-source work split (options, else the size-dependent default).
     void sssp_split(const DeviceGraph &g, const hyt_run_options &o, std::uint32_t &light_max, std::uint32_t &chunk);

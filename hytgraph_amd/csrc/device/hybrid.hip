@@ -519,9 +519,10 @@ namespace hyt
         {
             Workspace &ws = dg.ws;
             for (DeviceBuffer *b : {&ws.dist, &ws.res, &ws.rank, &ws.prev, &ws.touch, &ws.qd, &ws.partial, &ws.active,
-                                    &ws.share, &ws.heavy, &stage_val})
+                                    &ws.share, &ws.heavy, &ws.hub_bits, &ws.hub_list, &stage_val})
                 b->release();
             ws.heavy_cap = 0;
+            ws.hub_rows = 0;
             ws.ready = false; // the next ensure_workspace re-creates every array
         }
 

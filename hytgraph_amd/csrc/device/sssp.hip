@@ -254,6 +254,57 @@ namespace hyt
         static_assert(kGraphRounds <= kMaxGraphRounds, "round stamps");
         constexpr unsigned kDeviceLightGrid = 16384; // tiles of 256 rows per sweep
 
+        // hub_bits[t]: bit j = row 16t + j has more than light_max edges;
+        // *count += the rows marked.
+        template <class OffT>
+        __global__ __launch_bounds__(kBlock) void hub_bits_kernel(const OffT *off, std::uint64_t V,
+                                                                  std::uint32_t light_max, std::uint16_t *bits,
+                                                                  unsigned long long *count)
+        {
+            const std::uint64_t t = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+            const std::uint64_t r0 = t * 16;
+            std::uint32_t m = 0;
+            if (r0 < V)
+            {
+                for (std::uint32_t j = 0; j < 16 && r0 + j < V; ++j)
+                    if (static_cast<std::uint64_t>(off[r0 + j + 1] - off[r0 + j]) > light_max)
+                        m |= 1U << j;
+                bits[t] = static_cast<std::uint16_t>(m);
+            }
+            unsigned long long n = static_cast<unsigned long long>(__popc(m));
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1)
+                n += __shfl_xor(n, d, kWave);
+            if (lane_id() == 0 && n != 0ULL)
+                atomicAdd(count, n);
+        }
+
+        // The hub list of a queue the compaction did not list (the first
+        // round of a call): the marked rows among q[0, F), appended at *count.
+        __global__ __launch_bounds__(kBlock) void hub_filter(const std::uint32_t *q, std::uint32_t F,
+                                                             const std::uint16_t *bits, std::uint32_t *list,
+                                                             std::uint32_t *count)
+        {
+            const std::uint64_t i = static_cast<std::uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+            const int lane = lane_id();
+            std::uint32_t u = 0;
+            bool hub = false;
+            if (i < F)
+            {
+                u = q[i];
+                hub = ((bits[u >> 4] >> (u & 15U)) & 1U) != 0U;
+            }
+            const unsigned long long m = __ballot(hub);
+            if (m == 0ULL)
+                return; // uniform across the wavefront
+            std::uint32_t base = 0;
+            if (lane == 0)
+                base = atomicAdd(count, static_cast<std::uint32_t>(__popcll(m)));
+            base = __shfl(base, 0, kWave);
+            if (hub)
+                list[base + static_cast<std::uint32_t>(__popcll(m & ((1ULL << lane) - 1ULL)))] = u;
+        }
+
         __global__ void rc_reset(RoundCtl *rc)
         {
             if (threadIdx.x == 0 && blockIdx.x == 0)
@@ -271,10 +322,14 @@ namespace hyt
                 // the round limit: the counters keep the queue for a resumed run
                 rc->truncated = 1;
                 rc->F = 0;
+                rc->H = 0;
                 rc->done = 1;
                 return;
             }
             rc->F = F;
+            // the rows above light_max among them, listed by the compaction
+            rc->H = F != 0 ? ctr->next_hubs : 0U;
+            ctr->next_hubs = 0;
             rc->heavy_bound = heavy_bound_of(ctr->kept_min(), heavy_delta, float_dist);
             if (F != 0)
             {
@@ -363,7 +418,7 @@ namespace hyt
 
         template <class OffT, int WM, bool SYNC>
         void run_device_driven(Launch &L, const SsspSel &sel, const SsspOp<WM, SYNC> &op, ExpandArgs a,
-                               std::uint64_t max_rounds, hyt_run_stats &st)
+                               std::uint64_t max_rounds, std::uint32_t F0, hyt_run_stats &st)
         {
             DeviceGraph &g = L.g;
             const SsspBuffers &B = L.b;
@@ -376,6 +431,13 @@ namespace hyt
             const unsigned light_grid = std::min(blocks_for(g.V), kDeviceLightGrid);
             const unsigned nb = compact_blocks(g.V);
             const GraphRows<OffT> rows = whole_graph(g.off.as<OffT>(), B.q);
+            // ASYNC with the hub bitmap: the chunk-queuing pass reads the
+            // compaction's list of rows above light_max instead of the whole
+            // queue (the light pass still reads the queue and skips them)
+            const bool hub_list = !SYNC && B.hub_bits != nullptr;
+            const GraphRows<OffT> hub_rows = whole_graph(g.off.as<OffT>(), B.hub_list);
+            const unsigned hub_grid = std::min(blocks_for(B.hub_rows), kDeviceLightGrid);
+            const HubOut hubs = hub_list ? HubOut{B.hub_bits, B.hub_list, &B.ctr->next_hubs} : HubOut{};
             const std::vector<std::uint64_t> key{
                 static_cast<std::uint64_t>(sizeof(OffT)), static_cast<std::uint64_t>(WM), SYNC ? 1ULL : 0ULL,
                 timed ? 1ULL : 0ULL, max_rounds, L.light_max, L.chunk, B.heavy_cap, heavy_delta, wait_degree,
@@ -383,6 +445,7 @@ namespace hyt
                 reinterpret_cast<std::uint64_t>(B.q), reinterpret_cast<std::uint64_t>(B.qd),
                 reinterpret_cast<std::uint64_t>(B.heavy), reinterpret_cast<std::uint64_t>(B.ctr),
                 reinterpret_cast<std::uint64_t>(B.rc), g.off.ptr ? reinterpret_cast<std::uint64_t>(g.off.ptr) : 0,
+                reinterpret_cast<std::uint64_t>(B.hub_bits), reinterpret_cast<std::uint64_t>(B.hub_list), B.hub_rows,
                 reinterpret_cast<std::uint64_t>(g.col.ptr), reinterpret_cast<std::uint64_t>(g.w.ptr)};
             auto *cached = static_cast<RoundGraph *>(g.round_graph.get());
             if (cached == nullptr || cached->key != key)
@@ -409,7 +472,14 @@ namespace hyt
                         a2.pass = 2;
                         a2.heavy_bound_dev = &B.rc->heavy_bound;
                         a2.wait_degree = wait_degree;
-                        expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a1);
+                        if (hub_list)
+                        {
+                            ExpandArgs ah = a1;
+                            ah.F_dev = &B.rc->H;
+                            expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<hub_grid, kBlock, 0, s>>>(hub_rows, op, ah);
+                        }
+                        else
+                            expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a1);
                         expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a2);
                     }
@@ -417,7 +487,7 @@ namespace hyt
                         stamp<<<1, 64, 0, s>>>(B.rc, 2 * r + 1);
                     compact_rows<std::uint32_t, OffT, SsspSel><<<nb, kBlock, 0, s>>>(
                         g.V, sel, B.q, SYNC ? B.qd : nullptr, B.ctr, static_cast<const OffT *>(nullptr),
-                        static_cast<std::uint16_t *>(nullptr), static_cast<std::uint32_t *>(nullptr), &B.rc->F);
+                        static_cast<std::uint16_t *>(nullptr), static_cast<std::uint32_t *>(nullptr), &B.rc->F, hubs);
                     round_end<<<1, 64, 0, s>>>(B.ctr, B.rc);
                 }
                 const hipError_t launch_err = hipGetLastError();
@@ -430,6 +500,15 @@ namespace hyt
             }
             rc_reset<<<1, 64, 0, s>>>(B.rc);
             hip_check(hipGetLastError(), "launch rc_reset");
+            if (hub_list)
+            {
+                // the queue of the first round came from a compaction that
+                // listed no rows apart (or from a call that stopped at its
+                // round limit): list its rows above light_max again
+                launch_fill<std::uint32_t>(&B.ctr->next_hubs, 1, 0U, s);
+                hub_filter<<<blocks_for(F0), kBlock, 0, s>>>(B.q, F0, B.hub_bits, B.hub_list, &B.ctr->next_hubs);
+                hip_check(hipGetLastError(), "launch hub_filter");
+            }
             double clock_khz = 1.0;
             if (timed)
             {
@@ -585,7 +664,7 @@ namespace hyt
             st.converged = 0;
             if (B.rc != nullptr && !bucketed && !can_pull && F > 0)
             {
-                run_device_driven<OffT, WM, SYNC>(L, sel, op, a, max_rounds, st);
+                run_device_driven<OffT, WM, SYNC>(L, sel, op, a, max_rounds, F, st);
                 return;
             }
             for (std::uint64_t round = 0; round < max_rounds && (F > 0 || pending > 0);)
@@ -729,6 +808,55 @@ namespace hyt
         require(chunk >= 64 && chunk <= (1U << 24), "heavy_chunk must be in [64, 2^24]");
     }
 
+    namespace
+    {
+        // The hub bitmap of the workspace at a split, and a list long
+        // enough for every marked row (built once per graph and light_max).
+        template <class OffT>
+        void build_hub_rows(DeviceGraph &g, std::uint32_t light_max)
+        {
+            Workspace &ws = g.ws;
+            ws.hub_list.release();
+            ws.hub_bits.release();
+            ws.hub_rows = 0;
+            const std::uint64_t words = (g.V + 15) / 16;
+            ws.hub_bits.alloc((words + 1) * sizeof(std::uint16_t), "hipMalloc(hub bitmap)");
+            DeviceBuffer count(sizeof(unsigned long long), "hipMalloc(hub count)");
+            launch_fill<unsigned long long>(count.as<unsigned long long>(), 1, 0ULL, g.stream);
+            if (words > 0)
+            {
+                hub_bits_kernel<OffT><<<blocks_for(words), kBlock, 0, g.stream>>>(
+                    g.off.as<OffT>(), g.V, light_max, ws.hub_bits.as<std::uint16_t>(), count.as<unsigned long long>());
+                hip_check(hipGetLastError(), "launch hub_bits");
+            }
+            unsigned long long n = 0;
+            hip_check(hipMemcpyAsync(&n, count.ptr, sizeof(n), hipMemcpyDeviceToHost, g.stream), "read hub count");
+            hip_check(hipStreamSynchronize(g.stream), "hub bitmap");
+            ws.hub_list.alloc((n + 1) * sizeof(std::uint32_t), "hipMalloc(hub list)");
+            ws.hub_rows = n;
+            ws.hub_light_max = light_max;
+        }
+
+        // Device-driven ASYNC calls list the rows above light_max apart
+        // (SsspBuffers::hub_bits); other calls leave the fields null.
+        void attach_hub_rows(DeviceGraph &g, const hyt_run_options &o, std::uint32_t light_max, SsspBuffers &B)
+        {
+            if (o.mode == HYT_SSSP_SYNC || B.rc == nullptr)
+                return;
+            Workspace &ws = g.ws;
+            if (ws.hub_bits.ptr == nullptr || ws.hub_list.ptr == nullptr || ws.hub_light_max != light_max)
+            {
+                if (g.off64)
+                    build_hub_rows<std::uint64_t>(g, light_max);
+                else
+                    build_hub_rows<std::uint32_t>(g, light_max);
+            }
+            B.hub_bits = ws.hub_bits.as<std::uint16_t>();
+            B.hub_list = ws.hub_list.as<std::uint32_t>();
+            B.hub_rows = ws.hub_rows;
+        }
+    } // namespace
+
     void sssp_rounds(DeviceGraph &g, std::uint32_t src, const SsspBuffers &b, const hyt_run_options &o,
                      std::uint32_t light_max, std::uint32_t chunk, hyt_run_stats &st)
     {
@@ -747,10 +875,11 @@ namespace hyt
         sssp_split(g, o, light_max, chunk);
         ensure_workspace(g, light_max, chunk);
         Workspace &ws = g.ws;
-        const SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
-                            ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
-                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
-                            o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
+        SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
+                      ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
+                      ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
+                      o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
+        attach_hub_rows(g, o, light_max, B);
         Launch L{g, o, light_max, chunk, B, resume, rounds};
         if (g.off64)
             dispatch_weights<std::uint64_t>(L, src, st);
@@ -765,10 +894,10 @@ namespace hyt
         sssp_split(g, o, light_max, chunk);
         ensure_workspace(g, light_max, chunk);
         Workspace &ws = g.ws;
-        const SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
-                            ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
-                            ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
-                            o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
+        SsspBuffers B{ws.dist.as<std::uint32_t>(), ws.prev.as<std::uint32_t>(), ws.q.as<std::uint32_t>(),
+                      ws.qd.as<std::uint32_t>(),   ws.touch.as<std::uint8_t>(), ws.active.as<std::uint16_t>(),
+                      ws.heavy.as<HeavyChunk>(),   ws.heavy_cap,                ws.ctr.as<Counters>(),
+                      o.host_rounds ? nullptr : ws.rc.as<RoundCtl>()};
         if (o.tolerance != 0.0F)
         {
             require(o.tolerance > 0.0F && std::isfinite(o.tolerance), "SSSP tolerance must be finite and non-negative");
@@ -791,6 +920,7 @@ namespace hyt
         {
             try
             {
+                attach_hub_rows(g, o, light_max, B);
                 sssp_rounds(g, src, B, o, light_max, chunk, st);
             }
             catch (...)
