@@ -61,10 +61,15 @@ namespace hyt
     __global__ __launch_bounds__(kBlock) void compact_rows(std::uint64_t V, Sel sel, std::uint32_t *q, Val *qval,
                                                            Counters *ctr, const OffT *off, std::uint16_t *active,
                                                            std::uint32_t *block_counts = nullptr,
-                                                           const std::uint32_t *gate = nullptr, HubOut hubs = {})
+                                                           const std::uint32_t *gate = nullptr, HubOut hubs = {},
+                                                           std::uint64_t *t_start = nullptr)
     {
         if (gate != nullptr && *gate == 0)
             return; // uniform: every thread reads the same word
+        // timed rounds: the device wall clock when the pass starts (the kernels
+        // before it on the stream have finished)
+        if (t_start != nullptr && blockIdx.x == 0 && threadIdx.x == 0)
+            *t_start = static_cast<std::uint64_t>(wall_clock64());
         __shared__ std::uint32_t s_wsum[kWavesPerBlock];
         __shared__ unsigned long long s_dsum[kWavesPerBlock];
         __shared__ std::uint32_t s_base, s_hbase;

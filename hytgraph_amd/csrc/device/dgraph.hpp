@@ -62,13 +62,15 @@ namespace hyt
         std::uint32_t next_count;          // rows written by the compaction pass
         std::uint32_t heavy_count;         // heavy chunks queued this round
         unsigned long long edges;          // edges expanded this round (besides the slots)
-        unsigned long long frontier_edges; // out-degree sum of the compacted rows
         std::uint32_t pending;             // improved rows held back (at/above the bucket bound)
         std::uint32_t pending_inv_max;     // max of ~dist over held-back rows (= ~min dist)
         EdgeSlot slot[kEdgeSlots];         // edges expanded this round, spread
-        // rows above light_max the compaction listed apart (a line of its own:
-        // next_count takes every workgroup's atomic)
-        alignas(256) std::uint32_t next_hubs;
+        // Written by compactions beside next_count, which takes every
+        // workgroup's atomic, so on a line of their own: the out-degree sum
+        // of the compacted rows, and the rows above light_max the SSSP
+        // compaction listed apart.
+        alignas(256) unsigned long long frontier_edges;
+        std::uint32_t next_hubs;
         std::uint32_t overflow;            // a u32 sum overflowed
         std::uint32_t heavy_overflow;      // heavy chunk queue overflowed
 

@@ -311,8 +311,10 @@ namespace hyt
                 *rc = RoundCtl{};
         }
 
+        // t_start (timed runs): the device wall clock when the round's
+        // control is done, the start of its expand kernels
         __global__ void round_begin(Counters *ctr, RoundCtl *rc, std::uint64_t max_rounds, std::uint32_t heavy_delta,
-                                    bool float_dist)
+                                    bool float_dist, std::uint64_t *t_start)
         {
             if (threadIdx.x != 0 || blockIdx.x != 0)
                 return;
@@ -349,14 +351,8 @@ namespace hyt
                 ctr->slot[i].v = ctr->slot[i].h = 0;
                 ctr->slot[i].kinv = 0;
             }
-        }
-
-        // Device wall clock into rc->t[slot] (kernels of a stream run in
-        // order, so two stamps bracket the kernels between them).
-        __global__ void stamp(RoundCtl *rc, int slot)
-        {
-            if (threadIdx.x == 0 && blockIdx.x == 0)
-                rc->t[slot] = static_cast<std::uint64_t>(wall_clock64());
+            if (t_start != nullptr)
+                *t_start = static_cast<std::uint64_t>(wall_clock64());
         }
 
         __global__ void round_end(const Counters *ctr, RoundCtl *rc)
@@ -456,9 +452,8 @@ namespace hyt
                 hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin round capture");
                 for (int r = 0; r < kGraphRounds; ++r)
                 {
-                    round_begin<<<1, 64, 0, s>>>(B.ctr, B.rc, max_rounds, heavy_delta, kFloatDist);
-                    if (timed)
-                        stamp<<<1, 64, 0, s>>>(B.rc, 2 * r);
+                    round_begin<<<1, 64, 0, s>>>(B.ctr, B.rc, max_rounds, heavy_delta, kFloatDist,
+                                                 timed ? &B.rc->t[2 * r] : nullptr);
                     if (SYNC)
                     {
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a);
@@ -483,11 +478,11 @@ namespace hyt
                         expand_heavy<SsspOp<WM, SYNC>><<<kSingleHeavyGrid, kBlock, 0, s>>>(op, a1);
                         expand_light<GraphRows<OffT>, SsspOp<WM, SYNC>><<<light_grid, kBlock, 0, s>>>(rows, op, a2);
                     }
-                    if (timed)
-                        stamp<<<1, 64, 0, s>>>(B.rc, 2 * r + 1);
+                    // the compaction's first workgroup stamps the end of the expand kernels
                     compact_rows<std::uint32_t, OffT, SsspSel><<<nb, kBlock, 0, s>>>(
                         g.V, sel, B.q, SYNC ? B.qd : nullptr, B.ctr, static_cast<const OffT *>(nullptr),
-                        static_cast<std::uint16_t *>(nullptr), static_cast<std::uint32_t *>(nullptr), &B.rc->F, hubs);
+                        static_cast<std::uint16_t *>(nullptr), static_cast<std::uint32_t *>(nullptr), &B.rc->F, hubs,
+                        timed ? &B.rc->t[2 * r + 1] : nullptr);
                     round_end<<<1, 64, 0, s>>>(B.ctr, B.rc);
                 }
                 const hipError_t launch_err = hipGetLastError();
