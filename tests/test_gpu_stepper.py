@@ -67,6 +67,36 @@ def test_async_sssp_steps_reach_dijkstra(gpu, abi, orc):
         s.free()
 
 
+@pytest.mark.parametrize("rounds", [1, 3])
+def test_async_steps_with_hub_rows_in_resumed_queues(gpu, abi, orc, rounds):
+    """Device-driven ASYNC rounds list the queue's rows above light_max apart
+    (the compaction's hub list); a call that resumes a run lists the rows of
+    the queue it inherits again. A small split puts such rows in most queues;
+    stepping 1 or 3 rounds at a time, with and without waiting rows, must
+    still reach Dijkstra's distances, and a whole call must equal it too."""
+    hg = abi.HostGraph(scale=13, edge_factor=16, seed=93, weight_min=1, weight_max=255)
+    g = abi.DeviceGraph.from_host(hg)
+    for src in abi.sample_sources(hg.offsets, 4, 2):
+        want = orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, int(src))
+        for heavy_delta in (0, 1):
+            opt = dict(mode=abi.HYT_SSSP_ASYNC, light_max_degree=8, heavy_chunk=64, heavy_delta=heavy_delta)
+            s = abi.Stepper(g, abi.HYT_STEP_SSSP, int(src), **opt)
+            n = 0
+            while True:
+                _, fin = s.step(rounds)
+                n += 1
+                assert n < 10000
+                if fin:
+                    break
+            d, tot = s.read()
+            s.free()
+            _say(f"src {int(src)} heavy_delta {heavy_delta}: {tot['rounds']} rounds in {n} calls")
+            assert np.array_equal(d, want)
+            assert tot["converged"] == 1
+            whole, _ = g.sssp(int(src), **opt)
+            assert np.array_equal(whole, want)
+
+
 @pytest.mark.parametrize("seed", [-1, 11])
 def test_pagerank_steps_equal_one_call(gpu, abi, orc, seed):
     hg = abi.HostGraph(scale=12, edge_factor=16, seed=93, weight_min=0, weight_max=0)
