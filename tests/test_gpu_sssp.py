@@ -129,7 +129,7 @@ def test_device_driven_rounds_match_host_rounds(gpu, abi, orc, mode):
     control, the default) against one host read-back per round: the same
     distances; in SYNC mode the same round count, frontier, edge and update
     totals, also when truncated by max_rounds; the cached graph serves other
-    sources and is rebuilt after a trim or a split change."""
+    sources and is rebuilt after a trim or a split change (back and forth)."""
     hg = rmat(abi, 15, 81, wmax=255)
     g = abi.DeviceGraph.from_host(hg)
     keys = ("rounds", "converged", "frontier_total", "edges_total", "updates_total")
@@ -150,6 +150,13 @@ def test_device_driven_rounds_match_host_rounds(gpu, abi, orc, mode):
     assert st["expand_launches"] == st["rounds"] and st["expand_ms"] > 0
     d, _ = g.sssp(0, mode=mode, light_max_degree=8, heavy_chunk=64)
     assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
+    # back to the default split without a trim (ASYNC: the rows-above-split
+    # bitmap is rebuilt for the split in use)
+    src = int(abi.sample_sources(hg.offsets, 10, 1)[0])
+    d, _ = g.sssp(src, mode=mode)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src))
+    d, _ = g.sssp(src, mode=mode, light_max_degree=8, heavy_chunk=64)
+    assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, src))
     g.trim()
     d, _ = g.sssp(0, mode=mode)
     assert np.array_equal(d, orc.sssp_dijkstra(hg.offsets, hg.columns, hg.weights, 0))
