@@ -183,18 +183,25 @@ namespace hyt
     }
 
     // Queues the chunks of this workgroup's heavy rows (nch chunks for a
-    // thread's row, 0 for light rows) with one atomic per workgroup: every
-    // thread of the block calls it (s_h: kWavesPerBlock + 1 LDS words). One
-    // atomic per wavefront on the single chunk counter serialised in the
-    // light kernel's chunk-queuing pass (RMAT 24: ~100 K same-address
-    // atomics in a large round).
-    template <class Chunk, class MakeFn>
-    __device__ __forceinline__ void queue_heavy(std::uint32_t nch, Counters *ctr, Chunk *heavy,
-                                                std::uint64_t heavy_cap, std::uint32_t *s_h, MakeFn make)
+    // thread's row, 0 for light rows; chunk c of a row covers its edges
+    // [c * chunk, (c + 1) * chunk) from begin) with one atomic per workgroup:
+    // every thread of the block calls it (s_h: kWavesPerBlock + 1 LDS words;
+    // s_cex: this wavefront's 64 LDS words). One atomic per wavefront on the
+    // single chunk counter serialised in the light kernel's chunk-queuing
+    // pass (RMAT 24: ~100 K same-address atomics in a large round). Each
+    // wavefront writes its rows' chunks together, 64 slots per step (a lane
+    // finds the row of a slot by binary search over the rows' first slots),
+    // so a row of thousands of chunks does not leave one lane writing them.
+    __device__ __forceinline__ void queue_heavy(std::uint32_t nch, Counters *ctr, HeavyChunk *heavy,
+                                                std::uint64_t heavy_cap, std::uint32_t *s_h, std::uint32_t *s_cex,
+                                                std::uint32_t value, std::uint64_t count, std::uint64_t begin,
+                                                std::uint32_t row, std::uint32_t record, std::uint32_t chunk)
     {
         const int lane = lane_id();
         const int wid = static_cast<int>(threadIdx.x) / kWave;
         const std::uint32_t hincl = wave_incl_scan(nch);
+        const std::uint32_t wtotal = __shfl(hincl, kWave - 1, kWave);
+        s_cex[lane] = hincl - nch; // the wavefront's first slot of each row
         if (lane == kWave - 1)
             s_h[wid] = hincl;
         __syncthreads();
@@ -212,24 +219,47 @@ namespace hyt
             s_h[kWavesPerBlock] = atomicAdd(&ctr->heavy_count, total);
         __syncthreads();
         const std::uint32_t hbase = s_h[kWavesPerBlock];
-        const std::uint64_t slot = static_cast<std::uint64_t>(hbase) + before + hincl - nch;
-        if (static_cast<std::uint64_t>(hbase) + total > heavy_cap)
+        // the round fails on the host when the queue is full; until then the
+        // chunk kernel reads min(heavy_count, cap) slots, so the reserved
+        // ones that exist hold empty chunks (no edges, no record) rather
+        // than stale bytes
+        const bool full = static_cast<std::uint64_t>(hbase) + total > heavy_cap;
+        if (full && threadIdx.x == 0)
+            atomicOr(&ctr->heavy_overflow, 1U);
+        const std::uint64_t wbase = static_cast<std::uint64_t>(hbase) + before;
+        // a loop uniform across the wavefront: every lane takes part in the
+        // shuffles that fetch a slot's row
+        for (std::uint32_t base = 0; base < wtotal; base += kWave)
         {
-            // the round fails on the host; until then the chunk kernel reads
-            // min(heavy_count, cap) slots, so the reserved ones that exist
-            // hold empty chunks (no edges, no record) rather than stale bytes
-            if (threadIdx.x == 0)
-                atomicOr(&ctr->heavy_overflow, 1U);
-            for (std::uint32_t c = 0; c < nch; ++c)
-                if (slot + c < heavy_cap)
+            const std::uint32_t e = base + static_cast<std::uint32_t>(lane);
+            const bool ok = e < wtotal;
+            const int j = ok ? owner_of(s_cex, e) : 0;
+            const std::uint32_t jv = __shfl(value, j, kWave);
+            const std::uint64_t jc = __shfl(count, j, kWave);
+            const std::uint64_t jb = __shfl(begin, j, kWave);
+            const std::uint32_t jr = __shfl(row, j, kWave);
+            const std::uint32_t jrec = __shfl(record, j, kWave);
+            if (!ok)
+                continue;
+            const std::uint64_t slot = wbase + e;
+            if (full)
+            {
+                if (slot < heavy_cap)
                 {
-                    Chunk h{};
-                    heavy[slot + c] = h;
+                    HeavyChunk h{};
+                    heavy[slot] = h;
                 }
-            return;
+                continue;
+            }
+            const std::uint64_t lo = static_cast<std::uint64_t>(e - s_cex[j]) * chunk;
+            HeavyChunk h;
+            h.value = jv;
+            h.count = static_cast<std::uint32_t>(jc - lo < chunk ? jc - lo : chunk);
+            h.begin = jb + lo;
+            h.row = jr;
+            h.record = jrec;
+            heavy[slot] = h;
         }
-        for (std::uint32_t c = 0; c < nch; ++c)
-            heavy[slot + c] = make(c);
     }
 
     // One 256-row tile of the queue per workgroup iteration; a grid sized
@@ -292,16 +322,8 @@ namespace hyt
             }
             // the chunk-queuing passes (block-uniform: the pass is a kernel argument)
             if (a.pass != 2)
-                queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, s_h, [&](std::uint32_t c) {
-                    const std::uint64_t lo = static_cast<std::uint64_t>(c) * a.chunk;
-                    HeavyChunk h;
-                    h.value = value;
-                    h.count = static_cast<std::uint32_t>((count - lo < a.chunk) ? (count - lo) : a.chunk);
-                    h.begin = begin + lo;
-                    h.row = row;
-                    h.record = whole ? 1U : 0U;
-                    return h;
-                });
+                queue_heavy(nch, a.ctr, a.heavy, a.heavy_cap, s_h, s_ex[wid], value, count, begin, row,
+                            whole ? 1U : 0U, a.chunk);
 
             const std::uint32_t incl = wave_incl_scan(deg);
             const std::uint32_t total = __shfl(incl, kWave - 1, kWave);
